@@ -1,0 +1,197 @@
+"""Benchmark: batched EM on the 256^3 x 10 mm grid with 1e7 muons (paths of
+80-600 voxels), FP32C (FP32 sums with two-sum compensation, FP64 2x2
+algebra) timed, the plain FP64 GPU path run beside it for the same number
+of iterations, image error of FP32C against FP64 reported.
+
+A step is one EM iteration over every track (forward covariance sums,
+per-track 2x2 algebra, backprojection, voxel update: the mean update of
+src/MutoMLSD.cpp:285-308).  value = track-voxel updates per second.
+
+Prints one JSON line.  One GPU only: --gpus above 1 exits.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+BYTES_PER_UPDATE_FP32C = 8.0  # u32 voxel index + f32 length per entry
+BYTES_PER_UPDATE_FP64 = 12.0  # u32 voxel index + f64 length per entry
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic(workload, kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+    (profiles/<kernel>_<workload>_pmc.txt: 'traffic_bytes: N'), or None."""
+    p = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(g, rays, cfg_kw, n_sample, iters):
+    """The CPU FP64 oracle (reference operation order; serial track loop,
+    OpenMP voxel loop) on a seeded subset of the same muons at the full
+    grid: updates/s."""
+    import oracle
+    from muontomography_amd._abi import mlsd_cfg
+    cfg = mlsd_cfg(g, **cfg_kw)
+    sub = rays[:n_sample]
+    t0 = time.perf_counter()
+    em = oracle.Em(cfg, sub)
+    t_prep = time.perf_counter() - t0
+    lam = np.full(g.n_vox, 1e-7)
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        lam, _, _ = em.iterate(lam)
+    t = time.perf_counter() - t0
+    nnz = em.nnz
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": nnz * iters / t, "unit": "track-voxel updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n_sample} of the same muons (first {n_sample}), full 256^3 grid, "
+                      f"{nnz} entries, {iters} FP64 mean-update iterations in {t:.2f} s "
+                      f"(set-up {t_prep:.2f} s not counted); serial track loop, OpenMP voxel "
+                      f"loop over {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=45)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="em256")
+    ap.add_argument("--n-muons", type=int, default=None)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is designed, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import muontomography_amd as M
+    from muontomography_amd import workloads as W
+    from muontomography_amd._abi import mlsd_cfg
+
+    t0 = time.perf_counter()
+    g, rays, info = W.muons(args.workload, n_muons=args.n_muons)
+    log(f"generated {len(rays)} muons in {time.perf_counter() - t0:.1f} s "
+        f"(acceptance {info['acceptance']:.3f})")
+    cfg_kw = dict(update_with_mean=1, mean_percentile=100, error_xy=0.5, error_dz_outer=500.0,
+                  error_dz_inner=500.0, initialize_method=2, initialize_value=1e-7)
+    cfg = mlsd_cfg(g, **cfg_kw)
+    init = np.full(g.n_vox, 1e-7)
+    ctx = M.Context(0)
+    t0 = time.perf_counter()
+    plan = ctx.em_plan(cfg, rays=rays)
+    c = plan.counters()
+    nnz = c["nnz"]
+    log(f"plan: {c['n_scat_in_grid']} selected tracks, {nnz} entries "
+        f"({nnz / max(c['n_scat_in_grid'], 1):.0f}/track) in {time.perf_counter() - t0:.1f} s")
+
+    # FP32C: warm-up (includes the one-time brick-stream build), then K steps
+    plan.set_image(init)
+    t0 = time.perf_counter()
+    plan.run(args.warmup, M.MUTO_FP32C)
+    log(f"FP32C warm-up {args.warmup} it (incl. brick build) {time.perf_counter() - t0:.1f} s; "
+        f"segments {plan.counters()['n_segments']}")
+    t0 = time.perf_counter()
+    cost32 = plan.run(args.steps, M.MUTO_FP32C)
+    wall = time.perf_counter() - t0
+    tk = plan.timing()
+    img32 = plan.image()
+    ms_per_step = wall * 1e3 / args.steps
+    value = nnz * args.steps / wall
+    log(f"FP32C {args.steps} it: {ms_per_step:.3f} ms/it wall, kernels {tk}")
+
+    # roofline of the dominant kernel
+    kern = max(("forward", "backward"), key=lambda k: tk[k])
+    kname = {"forward": "k_fwd32", "backward": "k_bwd32"}[kern]
+    t_k = tk[kern] * 1e-3
+    achieved = BYTES_PER_UPDATE_FP32C * nnz / t_k
+    traffic = read_traffic(args.workload, kname)
+    roof = {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+            "iteration_frac": BYTES_PER_UPDATE_FP32C * nnz / (tk["iteration"] * 1e-3) / HBM_PEAK}
+
+    extra = {}
+    if not args.no_fp64:
+        plan.set_image(init)
+        t0 = time.perf_counter()
+        cost64 = plan.run(args.warmup + args.steps, M.MUTO_FP64)
+        w64 = time.perf_counter() - t0
+        img64 = plan.image()
+        t64 = plan.timing()
+        # FP32C image after the same number of iterations
+        plan.set_image(init)
+        plan.run(args.warmup + args.steps, M.MUTO_FP32C)
+        img32b = plan.image()
+        mj = plan.voxel_counts()
+        m = (mj > 0) & (img64 > 0)
+        r = np.abs(img32b[m] - img64[m]) / img64[m]
+        extra = {
+            "fp64_ms_per_step": w64 * 1e3 / (args.warmup + args.steps),
+            "fp64_value": nnz * (args.warmup + args.steps) / w64,
+            "fp32c_speedup_vs_fp64": (w64 / (args.warmup + args.steps)) / (wall / args.steps),
+            "image_err_max_rel": float(r.max()) if r.size else None,
+            "image_err_rms_rel": float(np.sqrt(np.mean(r * r))) if r.size else None,
+            "image_err_norm_inf": float(np.abs(img32b - img64).max() / np.abs(img64).max()),
+            "iterations_compared": args.warmup + args.steps,
+            "cost_rel_diff_last": float(abs(cost32[-1] - cost64[args.warmup + args.steps - 1]) /
+                                        abs(cost64[args.warmup + args.steps - 1])),
+            "fp64_kernel_ms": {k: round(v, 4) for k, v in t64.items()},
+        }
+        log(f"FP64 {args.warmup + args.steps} it: {extra['fp64_ms_per_step']:.2f} ms/it; "
+            f"speed-up {extra['fp32c_speedup_vs_fp64']:.2f}x; err max {extra['image_err_max_rel']}")
+    plan.close()
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        cpu = cpu_baseline(g, rays, cfg_kw, min(args.cpu_sample, len(rays)), args.cpu_iters)
+        log(f"cpu baseline {cpu['value']:.3e} upd/s in {time.perf_counter() - t0:.1f} s")
+
+    out = {
+        "metric": "Siddon+EM track-voxel updates/s, FP32c vs FP64 at <=1e-4 rel err; % HBM peak",
+        "value": value,
+        "unit": "track-voxel updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 sums (two-sum) + f64 2x2 algebra",
+        "data": "synthetic (seeded cosmic-muon generator, inputs resident in HBM)",
+        "config": {"workload": "batched EM, 256^3 grid of 10 mm voxels, 1e7 muons with "
+                               "80-600-voxel paths, mean update, FP32C beside FP64",
+                   "grid": [g.nx, g.ny, g.nz], "voxel_mm": g.dx, "muons": int(len(rays)),
+                   "selected_tracks": c["n_scat_in_grid"], "entries": nnz,
+                   "segments": None, "parallelism": "1 GPU"},
+        "kernel_ms": {k: round(v, 4) for k, v in tk.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,227 @@
+/*
+ * muto.h -- C ABI of the MI355X (gfx950) reconstruction hot path for muon
+ * scattering tomography: Siddon voxel tracing, PoCA, MLSD/EM and MAP.
+ *
+ * Every entry point is a plain C function: plain pointers and sizes, no C++
+ * or torch types.  Every function returns MUTO_OK (0) or a negative status;
+ * muto_last_error() returns the message of the last failure on a context.
+ * No exception crosses this boundary.
+ *
+ * Buffers are caller owned.  Functions taking "host" pointers copy in/out;
+ * the plan API (muto_em_plan_*) keeps its inputs resident in device memory
+ * so iterations run without host traffic.
+ *
+ * A context is not thread-safe: use one per host thread, the same rule as
+ * the reference's per-instance reconstruction objects.
+ *
+ * Reference interfaces replaced (paths relative to LasDes/MuonTomography):
+ *   VoxelGrid, RayData, VoxelData ........ include/MutoTypes.h:25-55
+ *   MutoSiddon::getVoxelPath ............. src/MutoSiddon.cpp:9-141
+ *   MutoPoCA::reconstruct ................ src/MutoPoCA.cpp:51-133
+ *   MutoPoCA::getPointOfClosestApproach .. src/MutoPoCA.cpp:142-163
+ *   MutoMLSD::reconstruct ................ src/MutoMLSD.cpp:91-220
+ *   MutoMAP::reconstruct ................. src/MutoMAP.cpp:39-205
+ *   MutoVReconstruction::reconstruct ..... include/MutoVReconstruction.h:11-16
+ */
+#ifndef MUTO_H
+#define MUTO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MUTO_ABI_VERSION 1
+
+/* status codes */
+#define MUTO_OK 0
+#define MUTO_ERR_ARG (-1)     /* invalid argument                      */
+#define MUTO_ERR_HIP (-2)     /* HIP runtime failure                   */
+#define MUTO_ERR_NOMEM (-3)   /* device or host allocation failed      */
+#define MUTO_ERR_NODEV (-4)   /* no usable gfx950 device               */
+#define MUTO_ERR_STATE (-5)   /* call out of order / plan not ready    */
+#define MUTO_ERR_LIMIT (-6)   /* a size exceeds what the kernels take  */
+
+/* arithmetic of the EM / MAP iterations */
+#define MUTO_FP64 0   /* reference order, FP64 throughout                    */
+#define MUTO_FP32C 1  /* FP64 Siddon + 2x2 algebra, FP32 sums with two-sum   */
+
+/* Voxel grid: same field order, types and 64-byte layout as VoxelGrid
+ * (include/MutoTypes.h:25-37).  Linear voxel index j = x + y*nx + z*nx*ny
+ * (include/MutoMLSD.h:76), x fastest. */
+typedef struct muto_grid {
+  uint32_t nx, ny, nz;
+  uint32_t _pad;
+  double x_min, y_min, z_min;
+  double dx, dy, dz;
+} muto_grid;
+
+/* One muon: same fields and 112-byte layout as RayData (MutoTypes.h:48-55):
+ * energies, in/out reference points and in/out directions. */
+typedef struct muto_ray {
+  double ein, eout;
+  double pin[3], pout[3], din[3], dout[3];
+} muto_ray;
+
+/* Keys of the "poca" JSON object (src/MutoPoCA.cpp:24-47). */
+typedef struct muto_poca_cfg {
+  muto_grid grid;
+  int32_t length_weighted, straight_line, has_energy, log_results;
+  int64_t total_num_rays; /* 0 = all */
+  double angle_unit_mag, eps;
+} muto_poca_cfg;
+
+/* Keys of the "mlsd" / "map" JSON objects (src/MutoMLSD.cpp:47-84,
+ * src/MutoMAP.cpp:12-19).  MAP-only fields are ignored by MLSD. */
+typedef struct muto_mlsd_cfg {
+  muto_grid grid;
+  int32_t update_with_mean, straight_line, has_energy, early_stop;
+  int32_t log_results, mean_percentile, num_iteration, initialize_method;
+  int64_t total_num_rays; /* 0 = all */
+  double initialize_value, error_xy, error_dz_outer, error_dz_inner;
+  double angle_unit_mag, eps;
+  double beta, p, delta;               /* MAP prior */
+  int32_t add_object_prior, all_neighbours;
+  int32_t precision;                   /* MUTO_FP64 or MUTO_FP32C */
+  int32_t _pad;
+} muto_mlsd_cfg;
+
+/* Counters and warning tallies.  The reference prints these
+ * (src/MutoPoCA.cpp:176-183, src/MutoMLSD.cpp:535-543) or warns inline
+ * (src/MutoMLSD.cpp:456-463, src/MutoMAP.cpp:356-358); here they are
+ * returned. */
+typedef struct muto_counters {
+  int64_t n_total;          /* rays processed                                 */
+  int64_t n_not_parallel;   /* PoCA lines that were not parallel              */
+  int64_t n_scat_in_grid;   /* PoCA accepted / MLSD selected tracks           */
+  int64_t n_empty_path;     /* selected tracks whose merged path is empty     */
+  int64_t n_out_of_grid;    /* path entries with a voxel index outside grid   */
+  int64_t n_singular;       /* covariances not invertible or det <= 0 (last it)*/
+  int64_t n_cubic_warn;     /* MAP: |p3| < eps in the cubic solver (last it)  */
+  int64_t n_newton_cap;     /* MAP: Newton iteration cap reached (last it)    */
+  int64_t nnz;              /* CSR entries (track-voxel pairs)                */
+  int64_t n_segments;       /* FP32C: brick segments                          */
+  int64_t iterations_done;
+  int64_t _reserved[5];
+} muto_counters;
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+/* ------------------------------------------------------------------ */
+typedef struct muto_ctx muto_ctx;
+
+/* Open device `device` (must be gfx950).  Replaces nothing in the
+ * reference (it is single-process CPU code); holds the HIP stream. */
+int muto_ctx_create(int device, muto_ctx** out);
+int muto_ctx_destroy(muto_ctx* ctx);
+const char* muto_last_error(const muto_ctx* ctx);
+int muto_abi_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Siddon voxel tracing                                                */
+/* ------------------------------------------------------------------ */
+/* Batched MutoSiddon::getVoxelPath (src/MutoSiddon.cpp:9-141) for n
+ * segments p1[i]->p2[i] (host arrays, 3 doubles per point).
+ * Two-call protocol: with vx == NULL only row_ptr (n+1 entries, exclusive
+ * prefix of per-segment counts) is written; then call again with arrays of
+ * row_ptr[n] entries.  vx/vy/vz are the voxel (x,y,z) of each entry
+ * (VoxelData, MutoTypes.h:40-45), len its path length; vy, vz, len may be
+ * NULL to skip them.  Bit-exact with the reference's FP64 arithmetic. */
+int muto_siddon_csr(muto_ctx* ctx, const muto_grid* grid, size_t n,
+                    const double* p1, const double* p2, uint64_t* row_ptr,
+                    uint32_t* vx, uint32_t* vy, uint32_t* vz, double* len);
+
+/* ------------------------------------------------------------------ */
+/* PoCA                                                                */
+/* ------------------------------------------------------------------ */
+/* Per-ray PoCA point (MutoPoCA.cpp:142-163) and voxel assignment.
+ * mode 0: the PoCA accept test (MutoPoCA.cpp:71-82, x86 double->u32 cast);
+ * mode 1: the MLSD selection test (MutoMLSD.cpp:298-301).
+ * zDetMin/zDetMax are taken from rays[0] as the reference does.
+ * Outputs (host, may be NULL): p_scat[3n], vox[3n] (x,y,z casts),
+ * accept[n] (1 = accepted). */
+int muto_poca_points(muto_ctx* ctx, const muto_grid* grid, size_t n,
+                     const muto_ray* rays, double eps, int mode,
+                     double* p_scat, uint32_t* vox, uint8_t* accept,
+                     muto_counters* counters);
+
+/* MutoPoCA::reconstruct (MutoPoCA.cpp:51-133).  img: host, nx*ny*nz FP64. */
+int muto_poca_reconstruct(muto_ctx* ctx, const muto_poca_cfg* cfg, size_t n,
+                          const muto_ray* rays, double* img,
+                          muto_counters* counters);
+
+/* ------------------------------------------------------------------ */
+/* MLSD / MAP one-shot                                                 */
+/* ------------------------------------------------------------------ */
+/* MutoMLSD::reconstruct (MutoMLSD.cpp:91-220).  init_img: the initial
+ * image (initialize_method 0; the caller loads the file), or NULL for
+ * methods 1/2.  cost_per_iter (may be NULL) receives num_iteration values.
+ * img_out: nx*ny*nz FP64. */
+int muto_mlsd_reconstruct(muto_ctx* ctx, const muto_mlsd_cfg* cfg, size_t n,
+                          const muto_ray* rays, const double* init_img,
+                          double* img_out, double* cost_per_iter,
+                          muto_counters* counters);
+
+/* MutoMAP::reconstruct (MutoMAP.cpp:39-205). */
+int muto_map_reconstruct(muto_ctx* ctx, const muto_mlsd_cfg* cfg, size_t n,
+                         const muto_ray* rays, const double* init_img,
+                         double* img_out, double* cost_per_iter,
+                         muto_counters* counters);
+
+/* ------------------------------------------------------------------ */
+/* EM plan: inputs resident in HBM, iterations on demand               */
+/* ------------------------------------------------------------------ */
+/* The plan performs the reference's preparation once (MutoMLSD.cpp:99-166:
+ * selection, angles, displacements, two-leg Siddon paths, tails) and keeps
+ * the track-voxel lists in device memory.  For precision MUTO_FP32C it also
+ * builds the brick-major segment stream.  Iterations then run on device. */
+typedef struct muto_em_plan muto_em_plan;
+
+int muto_em_plan_create(muto_ctx* ctx, const muto_mlsd_cfg* cfg, size_t n,
+                        const muto_ray* rays, muto_em_plan** out);
+/* Same, but from already prepared per-track data (the reference's
+ * gatherInformation output, MutoMLSD.cpp:268-326) and the merged path CSR
+ * (getMuonPathAndTail, MutoMLSD.cpp:382-415): row_ptr[n_trk+1], vox (linear
+ * j), len.  trk4[4*n_trk] = (Sx, Sy, Dx, Dy); p_r[n_trk] or NULL (p_r = 1).
+ * Used to run the iteration on inputs computed elsewhere. */
+int muto_em_plan_create_prepared(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
+                                 size_t n_trk, const double* trk4,
+                                 const double* p_r, const uint64_t* row_ptr,
+                                 const uint32_t* vox, const double* len,
+                                 muto_em_plan** out);
+int muto_em_plan_destroy(muto_em_plan* plan);
+/* Set the image (host FP64, nx*ny*nz) the next iteration starts from. */
+int muto_em_plan_set_image(muto_em_plan* plan, const double* img);
+int muto_em_plan_get_image(muto_em_plan* plan, double* img);
+/* Run `iters` iterations in `precision` (MUTO_FP64 / MUTO_FP32C) with the
+ * plan's update rule (median / mean / trimmed mean, MLSD or MAP).
+ * cost (may be NULL) receives one value per iteration.  Stops early as the
+ * reference does when cfg.early_stop is set. */
+int muto_em_plan_run(muto_em_plan* plan, int iters, int precision,
+                     double* cost);
+/* MAP iterations on the same plan (MutoMAP.cpp:131-204): each iteration
+ * runs the ML step on the current image, then the per-voxel surrogate
+ * step with cfg.beta / p / delta / add_object_prior / all_neighbours. */
+int muto_map_plan_run(muto_em_plan* plan, int iters, int precision,
+                      double* cost);
+int muto_em_plan_counters(const muto_em_plan* plan, muto_counters* out);
+/* Per-voxel entry counts M_j (host, nx*ny*nz uint32). */
+int muto_em_plan_voxel_counts(muto_em_plan* plan, uint32_t* mj);
+/* Copy the plan's prepared data to host arrays (any may be NULL):
+ * row_ptr[n_trk+1], vox/len/tail[nnz] (merged two-leg path, linear voxel
+ * index, length, tail; MutoMLSD.cpp:382-415), trk4[4*n_trk] (Sx, Sy, Dx,
+ * Dy; MutoMLSD.cpp:350-376), p_r[n_trk].  Sizes from muto_em_plan_counters
+ * (n_scat_in_grid = n_trk, nnz). */
+int muto_em_plan_export(muto_em_plan* plan, uint64_t* row_ptr, uint32_t* vox,
+                        double* len, double* tail, double* trk4, double* p_r);
+/* Average kernel time (ms) of the last run, per phase, measured with HIP
+ * events on the plan's stream: t[0] forward, t[1] track, t[2] backward /
+ * S-values, t[3] voxel update, t[4] whole iteration. */
+int muto_em_plan_timing(const muto_em_plan* plan, double* t_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MUTO_H */

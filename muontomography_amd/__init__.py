@@ -1,0 +1,243 @@
+"""MI355X (gfx950) reconstruction hot path of muon scattering tomography.
+
+Thin Python handle over libmuto.so (HIP kernels + C ABI, include/muto.h).
+The library is the product; this module only moves numpy arrays across the
+C ABI.  It raises if the library is missing -- there is no fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import (MUTO_FP32C, MUTO_FP64, RAY_DTYPE, Counters, GenBlock, GenCfg, Grid,
+                   make_grid, mlsd_cfg, poca_cfg, ptr)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmuto.so")
+
+_lib = None
+
+
+def lib():
+    """Load libmuto.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m muontomography_amd.build`")
+        _lib = _abi.bind(C.CDLL(LIB_PATH))
+    return _lib
+
+
+class MutoError(RuntimeError):
+    pass
+
+
+class Context:
+    """One HIP device + stream (muto_ctx)."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        rc = lib().muto_ctx_create(device, C.byref(self._h))
+        if rc != 0:
+            raise MutoError(f"muto_ctx_create({device}) failed with status {rc}")
+
+    def close(self):
+        if self._h:
+            lib().muto_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().muto_last_error(self._h)
+            raise MutoError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+
+    # -- Siddon ---------------------------------------------------------
+    def siddon_csr(self, grid, p1, p2):
+        """Batched Siddon paths; returns (row_ptr, x, y, z, length)."""
+        p1 = np.ascontiguousarray(p1, dtype=np.float64).reshape(-1, 3)
+        p2 = np.ascontiguousarray(p2, dtype=np.float64).reshape(-1, 3)
+        n = p1.shape[0]
+        row = np.zeros(n + 1, dtype=np.uint64)
+        self._check(lib().muto_siddon_csr(self._h, C.byref(grid), n, ptr(p1), ptr(p2), ptr(row),
+                                          None, None, None, None), "muto_siddon_csr(count)")
+        nnz = int(row[-1])
+        vx = np.empty(max(nnz, 1), np.uint32)
+        vy = np.empty_like(vx)
+        vz = np.empty_like(vx)
+        ln = np.empty(max(nnz, 1), np.float64)
+        self._check(lib().muto_siddon_csr(self._h, C.byref(grid), n, ptr(p1), ptr(p2), ptr(row),
+                                          ptr(vx), ptr(vy), ptr(vz), ptr(ln)), "muto_siddon_csr")
+        return row, vx[:nnz], vy[:nnz], vz[:nnz], ln[:nnz]
+
+    # -- PoCA -----------------------------------------------------------
+    def poca_points(self, grid, rays, eps=1e-8, mode=0):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        n = rays.shape[0]
+        P = np.empty((max(n, 1), 3), np.float64)
+        v = np.empty((max(n, 1), 3), np.uint32)
+        a = np.empty(max(n, 1), np.uint8)
+        c = Counters()
+        self._check(lib().muto_poca_points(self._h, C.byref(grid), n, ptr(rays), eps, mode,
+                                           ptr(P), ptr(v), ptr(a), C.byref(c)), "muto_poca_points")
+        return P[:n], v[:n], a[:n].astype(bool), c.as_dict()
+
+    def poca_reconstruct(self, cfg, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        img = np.empty(cfg.grid.n_vox, np.float64)
+        c = Counters()
+        self._check(lib().muto_poca_reconstruct(self._h, C.byref(cfg), rays.shape[0], ptr(rays),
+                                                ptr(img), C.byref(c)), "muto_poca_reconstruct")
+        return img, c.as_dict()
+
+    # -- MLSD / MAP -----------------------------------------------------
+    def _recon(self, fn, cfg, rays, init):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        img = np.empty(cfg.grid.n_vox, np.float64)
+        cost = np.full(max(cfg.num_iteration, 1), np.nan)
+        c = Counters()
+        init = None if init is None else np.ascontiguousarray(init, dtype=np.float64).ravel()
+        self._check(fn(self._h, C.byref(cfg), rays.shape[0], ptr(rays), ptr(init), ptr(img),
+                       ptr(cost), C.byref(c)), fn.__name__)
+        d = c.as_dict()
+        return img, cost[:d["iterations_done"]], d
+
+    def mlsd_reconstruct(self, cfg, rays, init=None):
+        return self._recon(lib().muto_mlsd_reconstruct, cfg, rays, init)
+
+    def map_reconstruct(self, cfg, rays, init=None):
+        return self._recon(lib().muto_map_reconstruct, cfg, rays, init)
+
+    def em_plan(self, cfg, rays=None, prepared=None):
+        return EmPlan(self, cfg, rays=rays, prepared=prepared)
+
+
+class EmPlan:
+    """Device-resident EM problem (muto_em_plan)."""
+
+    def __init__(self, ctx, cfg, rays=None, prepared=None):
+        self.ctx = ctx
+        self.cfg = cfg
+        self._h = C.c_void_p()
+        if prepared is not None:
+            trk4, p, row, vox, ln = prepared
+            self._keep = [np.ascontiguousarray(trk4, np.float64),
+                          None if p is None else np.ascontiguousarray(p, np.float64),
+                          np.ascontiguousarray(row, np.uint64), np.ascontiguousarray(vox, np.uint32),
+                          np.ascontiguousarray(ln, np.float64)]
+            t, pp, r, v, l_ = self._keep
+            ctx._check(lib().muto_em_plan_create_prepared(ctx._h, C.byref(cfg), r.shape[0] - 1,
+                                                          ptr(t), ptr(pp), ptr(r), ptr(v), ptr(l_),
+                                                          C.byref(self._h)),
+                       "muto_em_plan_create_prepared")
+        else:
+            rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+            ctx._check(lib().muto_em_plan_create(ctx._h, C.byref(cfg), rays.shape[0], ptr(rays),
+                                                 C.byref(self._h)), "muto_em_plan_create")
+
+    def close(self):
+        if self._h:
+            lib().muto_em_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_image(self, img):
+        img = np.ascontiguousarray(img, np.float64).ravel()
+        assert img.size == self.cfg.grid.n_vox
+        self.ctx._check(lib().muto_em_plan_set_image(self._h, ptr(img)), "set_image")
+
+    def image(self):
+        img = np.empty(self.cfg.grid.n_vox, np.float64)
+        self.ctx._check(lib().muto_em_plan_get_image(self._h, ptr(img)), "get_image")
+        return img
+
+    def run(self, iters, precision=MUTO_FP64, map_=False):
+        cost = np.full(max(iters, 1), np.nan)
+        f = lib().muto_map_plan_run if map_ else lib().muto_em_plan_run
+        self.ctx._check(f(self._h, iters, precision, ptr(cost)), f.__name__)
+        return cost[: self.counters()["iterations_done"]]
+
+    def counters(self):
+        c = Counters()
+        self.ctx._check(lib().muto_em_plan_counters(self._h, C.byref(c)), "counters")
+        return c.as_dict()
+
+    def voxel_counts(self):
+        m = np.empty(self.cfg.grid.n_vox, np.uint32)
+        self.ctx._check(lib().muto_em_plan_voxel_counts(self._h, ptr(m)), "voxel_counts")
+        return m
+
+    def export(self):
+        """(row_ptr, vox, len, tail, trk4, p_r) of the prepared problem."""
+        c = self.counters()
+        nt, nnz = c["n_scat_in_grid"], c["nnz"]
+        row = np.empty(nt + 1, np.uint64)
+        vox = np.empty(max(nnz, 1), np.uint32)
+        ln = np.empty(max(nnz, 1))
+        tl = np.empty(max(nnz, 1))
+        t4 = np.empty((max(nt, 1), 4))
+        pr = np.empty(max(nt, 1))
+        self.ctx._check(lib().muto_em_plan_export(self._h, ptr(row), ptr(vox), ptr(ln), ptr(tl),
+                                                  ptr(t4), ptr(pr)), "export")
+        return row, vox[:nnz], ln[:nnz], tl[:nnz], t4[:nt], pr[:nt]
+
+    def timing(self):
+        t = np.zeros(5, np.float64)
+        self.ctx._check(lib().muto_em_plan_timing(self._h, ptr(t)), "timing")
+        return {"forward": t[0], "track": t[1], "backward": t[2], "update": t[3], "iteration": t[4]}
+
+
+# -- synthetic muons (host code in libmuto) -------------------------------
+def gen_config(grid, seed, blocks=(), min_voxels=0, max_voxels=0, **kw):
+    cfg = GenCfg()
+    lib().muto_gen_default(C.byref(cfg), C.byref(grid), seed)
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    arr = (GenBlock * max(len(blocks), 1))(*blocks)
+    cfg._blocks_keep = arr
+    cfg.blocks = C.cast(arr, C.POINTER(GenBlock))
+    cfg.n_blocks = len(blocks)
+    cfg.min_voxels = min_voxels
+    cfg.max_voxels = max_voxels
+    return cfg
+
+
+def gen_layout(grid, seed, n, min_mm, max_mm, x0_choices):
+    x0 = np.ascontiguousarray(x0_choices, np.float64)
+    out = (GenBlock * max(n, 1))()
+    rc = lib().muto_gen_layout(C.byref(grid), seed, n, min_mm, max_mm, ptr(x0), len(x0), out)
+    if rc != 0:
+        raise MutoError(f"muto_gen_layout: status {rc}")
+    return list(out)[:n]
+
+
+def generate(cfg, n):
+    rays = np.zeros(n, RAY_DTYPE)
+    tried = C.c_int64(0)
+    rc = lib().muto_generate(C.byref(cfg), n, ptr(rays), C.byref(tried))
+    if rc != 0:
+        raise MutoError(f"muto_generate: status {rc}")
+    return rays, int(tried.value)
+
+
+def gen_truth(cfg):
+    img = np.empty(cfg.grid.n_vox, np.float64)
+    lib().muto_gen_truth(C.byref(cfg), ptr(img))
+    return img

@@ -1,0 +1,43 @@
+// Context: device selection, stream, last error.
+#include <string.h>
+
+#include "muto_internal.hpp"
+
+extern "C" int muto_abi_version(void) { return MUTO_ABI_VERSION; }
+
+extern "C" int muto_ctx_create(int device, muto_ctx** out) {
+  if (!out) return MUTO_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MUTO_ERR_NODEV;
+  if (device < 0 || device >= n) return MUTO_ERR_NODEV;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return MUTO_ERR_HIP;
+  // the code objects are built for gfx950 only
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return MUTO_ERR_NODEV;
+  if (hipSetDevice(device) != hipSuccess) return MUTO_ERR_HIP;
+  auto* c = new muto_ctx();
+  c->device = device;
+  c->n_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return MUTO_ERR_HIP;
+  }
+  *out = c;
+  return MUTO_OK;
+}
+
+extern "C" int muto_ctx_destroy(muto_ctx* ctx) {
+  if (!ctx) return MUTO_OK;
+  if (ctx->stream) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+  return MUTO_OK;
+}
+
+extern "C" const char* muto_last_error(const muto_ctx* ctx) {
+  if (!ctx) return "null context";
+  return ctx->err.c_str();
+}

@@ -1,0 +1,84 @@
+// EM plan: device-resident track-voxel lists and per-iteration buffers.
+#pragma once
+
+#include "muto_internal.hpp"
+
+namespace muto {
+
+// Brick (LDS tile) shape of the FP32C path.
+struct BrickShape {
+  uint32_t bx = 16, by = 16, bz = 16;
+  uint32_t vol() const { return bx * by * bz; }
+};
+
+// A unit of work of the persistent FP32C kernels: a run of brick-major
+// segments [s0, s1) that all lie in brick `brick`.  `owner` is set when
+// the task covers the whole brick (its accumulators are stored, not added).
+struct Task {
+  uint32_t brick;
+  uint32_t owner;
+  uint64_t s0, s1;
+};
+
+struct Fp32cData {
+  bool built = false;
+  BrickShape bs;
+  uint32_t nb[3] = {0, 0, 0};
+  uint64_t n_seg = 0, n_ent = 0;
+  DBuf<uint32_t> seg_trk;   // [n_seg] track rank
+  DBuf<float> seg_tend;     // [n_seg] tail after the segment's last entry
+  DBuf<uint64_t> seg_off;   // [n_seg+1] entry offsets (brick-major)
+  DBuf<uint16_t> ent_vox;   // [n_ent] voxel index inside the brick
+  DBuf<float> ent_len;      // [n_ent] path length
+  DBuf<uint32_t> tseg_ptr;  // [n_trk+1] per rank: segment list offsets
+  DBuf<uint32_t> tseg_pos;  // [n_seg] brick-major positions, rank order
+  DBuf<double> trk_d;       // [4*n_trk] rank order: D00, D01, D11, p_r
+  DBuf<float> seg_part;     // [3*n_seg] forward partial sums (SoA)
+  DBuf<float> trk_g;        // [4*n_trk] p^2 G (g00, g01, g11, -)
+  DBuf<Task> tasks;
+  uint32_t n_tasks = 0;
+  int grid_fwd = 0, grid_bwd = 0, block = 256;
+  hipGraphExec_t graph = nullptr;
+};
+
+}  // namespace muto
+
+struct muto_em_plan {
+  muto_ctx* ctx = nullptr;
+  muto_mlsd_cfg cfg{};
+  bool is_map = false;
+  muto::GridD g{};
+  size_t nv = 0;
+  size_t n_trk = 0;
+  uint64_t nnz = 0;
+  muto_counters counters{};
+
+  // track-major (selection order) inputs of the FP64 path
+  muto::DBuf<double> trk4;      // Sx, Sy, Dx, Dy
+  muto::DBuf<double> pr;        // p_r
+  muto::DBuf<uint64_t> row_ptr; // n_trk+1
+  muto::DBuf<uint32_t> vox;     // linear voxel index
+  muto::DBuf<double> len, tail;
+  muto::DBuf<uint32_t> mj;      // entries per voxel
+
+  // voxel-major slots for median / trimmed mean
+  bool slots_built = false;
+  muto::DBuf<uint64_t> col_ptr;
+  muto::DBuf<uint64_t> slot;
+  muto::DBuf<double> S;
+  uint64_t max_mj = 0;
+  muto::DBuf<double> sel_scratch;
+
+  // images and accumulators
+  muto::DBuf<double> lam64, ml64, tmp64, acc64;
+  muto::DBuf<float> lam32;
+  muto::DBuf<double> cost_acc;     // [1]
+  muto::DBuf<unsigned long long> dcount;  // device counters [8]
+  muto::DBuf<unsigned int> work;   // work-queue counters [4]
+  int cur_prec = -1;               // which image is current
+
+  muto::Fp32cData f32;
+
+  hipEvent_t ev[8] = {};
+  double t_ms[5] = {0, 0, 0, 0, 0};
+};

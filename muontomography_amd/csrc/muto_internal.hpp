@@ -1,0 +1,98 @@
+// Internal declarations shared by the HIP translation units of libmuto.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/muto.h"
+
+#define MUTO_CHECK_HIP(ctx, expr)                                          \
+  do {                                                                     \
+    hipError_t e_ = (expr);                                                \
+    if (e_ != hipSuccess) {                                                \
+      (ctx)->set_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+      return e_ == hipErrorOutOfMemory ? MUTO_ERR_NOMEM : MUTO_ERR_HIP;    \
+    }                                                                      \
+  } while (0)
+
+struct muto_ctx {
+  int device = 0;
+  int n_cu = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  void set_error(const std::string& m) { err = m; }
+};
+
+namespace muto {
+
+// Grid as the kernels see it.
+struct GridD {
+  uint32_t n[3];
+  double mn[3];
+  double d[3];
+};
+
+inline GridD to_dev(const muto_grid& g) {
+  GridD r;
+  r.n[0] = g.nx; r.n[1] = g.ny; r.n[2] = g.nz;
+  r.mn[0] = g.x_min; r.mn[1] = g.y_min; r.mn[2] = g.z_min;
+  r.d[0] = g.dx; r.d[1] = g.dy; r.d[2] = g.dz;
+  return r;
+}
+
+inline size_t n_vox(const muto_grid& g) {
+  return (size_t)g.nx * g.ny * g.nz;
+}
+
+// RAII device buffer
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)&p, count * sizeof(T));
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+inline unsigned grid_for(size_t n, unsigned block) {
+  size_t g = (n + block - 1) / block;
+  if (g == 0) g = 1;
+  if (g > 0x7fffffffull) g = 0x7fffffff;
+  return (unsigned)g;
+}
+
+// Device-wide helpers (scan.hip)
+hipError_t exclusive_scan_u64(const uint64_t* in, uint64_t* out, size_t n,
+                              hipStream_t s);
+hipError_t exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out,
+                                     size_t n, hipStream_t s);
+hipError_t sort_pairs_u64_u32(uint64_t* keys, uint32_t* vals, size_t n,
+                              int begin_bit, int end_bit, hipStream_t s);
+hipError_t sum_f64(const double* in, double* out, size_t n, hipStream_t s);
+
+// Siddon over a batch of segments (siddon.hip)
+hipError_t siddon_count(const GridD& g, const double* p1, const double* p2,
+                        size_t n, uint64_t* counts, hipStream_t s);
+hipError_t siddon_fill(const GridD& g, const double* p1, const double* p2,
+                       size_t n, const uint64_t* row_ptr, uint32_t* vx,
+                       uint32_t* vy, uint32_t* vz, double* len,
+                       hipStream_t s);
+
+}  // namespace muto

@@ -1,0 +1,133 @@
+// Batched Siddon tracing into CSR: count pass, exclusive scan, fill pass.
+// One lane per segment: the alpha merge is a sequential recurrence
+// (repeated addition, src/MutoSiddon.cpp:108-138) and is kept bit-exact.
+#include "geom_dev.hpp"
+
+namespace muto {
+
+namespace {
+
+__global__ void __launch_bounds__(256)
+k_siddon_count(GridD g, const double* __restrict__ p1,
+               const double* __restrict__ p2, size_t n,
+               uint64_t* __restrict__ counts) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a[3] = {p1[3 * i], p1[3 * i + 1], p1[3 * i + 2]};
+  double b[3] = {p2[3 * i], p2[3 * i + 1], p2[3 * i + 2]};
+  counts[i] = siddon_walk(g, a, b, [](uint32_t, uint32_t, uint32_t, double) {});
+}
+
+__global__ void __launch_bounds__(256)
+k_siddon_fill(GridD g, const double* __restrict__ p1,
+              const double* __restrict__ p2, size_t n,
+              const uint64_t* __restrict__ row_ptr, uint32_t* __restrict__ vx,
+              uint32_t* __restrict__ vy, uint32_t* __restrict__ vz,
+              double* __restrict__ len) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a[3] = {p1[3 * i], p1[3 * i + 1], p1[3 * i + 2]};
+  double b[3] = {p2[3 * i], p2[3 * i + 1], p2[3 * i + 2]};
+  uint64_t o = row_ptr[i];
+  const uint64_t end = row_ptr[i + 1];
+  siddon_walk(g, a, b, [&](uint32_t x, uint32_t y, uint32_t z, double L) {
+    if (o < end) {
+      vx[o] = x;
+      if (vy) vy[o] = y;
+      if (vz) vz[o] = z;
+      if (len) len[o] = L;
+    }
+    ++o;
+  });
+}
+
+}  // namespace
+
+hipError_t siddon_count(const GridD& g, const double* p1, const double* p2,
+                        size_t n, uint64_t* counts, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_siddon_count<<<grid_for(n, 256), 256, 0, s>>>(g, p1, p2, n, counts);
+  return hipGetLastError();
+}
+
+hipError_t siddon_fill(const GridD& g, const double* p1, const double* p2,
+                       size_t n, const uint64_t* row_ptr, uint32_t* vx,
+                       uint32_t* vy, uint32_t* vz, double* len,
+                       hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_siddon_fill<<<grid_for(n, 256), 256, 0, s>>>(g, p1, p2, n, row_ptr, vx,
+                                                 vy, vz, len);
+  return hipGetLastError();
+}
+
+}  // namespace muto
+
+using namespace muto;
+
+extern "C" int muto_siddon_csr(muto_ctx* ctx, const muto_grid* grid, size_t n,
+                               const double* p1, const double* p2,
+                               uint64_t* row_ptr, uint32_t* vx, uint32_t* vy,
+                               uint32_t* vz, double* len) {
+  if (!ctx) return MUTO_ERR_ARG;
+  if (!grid || !row_ptr || (n && (!p1 || !p2))) {
+    ctx->set_error("muto_siddon_csr: null argument");
+    return MUTO_ERR_ARG;
+  }
+  if (grid->nx == 0 || grid->ny == 0 || grid->nz == 0) {
+    ctx->set_error("muto_siddon_csr: empty grid");
+    return MUTO_ERR_ARG;
+  }
+  const GridD g = to_dev(*grid);
+  hipStream_t s = ctx->stream;
+  DBuf<double> dp1, dp2;
+  DBuf<uint64_t> dcnt, drp;
+  MUTO_CHECK_HIP(ctx, dp1.alloc(3 * n));
+  MUTO_CHECK_HIP(ctx, dp2.alloc(3 * n));
+  MUTO_CHECK_HIP(ctx, dcnt.alloc(n + 1));
+  MUTO_CHECK_HIP(ctx, drp.alloc(n + 1));
+  if (n) {
+    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(dp1.p, p1, 3 * n * sizeof(double),
+                                       hipMemcpyHostToDevice, s));
+    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(dp2.p, p2, 3 * n * sizeof(double),
+                                       hipMemcpyHostToDevice, s));
+  }
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(dcnt.p, 0, (n + 1) * sizeof(uint64_t), s));
+  MUTO_CHECK_HIP(ctx, siddon_count(g, dp1.p, dp2.p, n, dcnt.p, s));
+  MUTO_CHECK_HIP(ctx, exclusive_scan_u64(dcnt.p, drp.p, n + 1, s));
+  if (!vx) {
+    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(row_ptr, drp.p, (n + 1) * sizeof(uint64_t),
+                                       hipMemcpyDeviceToHost, s));
+    MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    return MUTO_OK;
+  }
+  uint64_t nnz = 0;
+  MUTO_CHECK_HIP(ctx, hipMemcpyAsync(&nnz, drp.p + n, sizeof(uint64_t),
+                                     hipMemcpyDeviceToHost, s));
+  MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+  if (row_ptr[n] != nnz) {
+    ctx->set_error("muto_siddon_csr: row_ptr does not match the count pass");
+    return MUTO_ERR_ARG;
+  }
+  DBuf<uint32_t> dx, dy, dz;
+  DBuf<double> dl;
+  MUTO_CHECK_HIP(ctx, dx.alloc(nnz));
+  if (vy) MUTO_CHECK_HIP(ctx, dy.alloc(nnz));
+  if (vz) MUTO_CHECK_HIP(ctx, dz.alloc(nnz));
+  if (len) MUTO_CHECK_HIP(ctx, dl.alloc(nnz));
+  MUTO_CHECK_HIP(ctx, siddon_fill(g, dp1.p, dp2.p, n, drp.p, dx.p,
+                                  vy ? dy.p : nullptr, vz ? dz.p : nullptr,
+                                  len ? dl.p : nullptr, s));
+  MUTO_CHECK_HIP(ctx, hipMemcpyAsync(vx, dx.p, nnz * sizeof(uint32_t),
+                                     hipMemcpyDeviceToHost, s));
+  if (vy)
+    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(vy, dy.p, nnz * sizeof(uint32_t),
+                                       hipMemcpyDeviceToHost, s));
+  if (vz)
+    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(vz, dz.p, nnz * sizeof(uint32_t),
+                                       hipMemcpyDeviceToHost, s));
+  if (len)
+    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(len, dl.p, nnz * sizeof(double),
+                                       hipMemcpyDeviceToHost, s));
+  MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+  return MUTO_OK;
+}

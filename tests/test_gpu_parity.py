@@ -1,0 +1,255 @@
+"""GPU parity: HIP kernels through the C ABI against the CPU FP64 oracle.
+
+Contract: voxel index lists and PoCA assignments bit-exact; lengths and
+PoCA points bit-exact (the contract allows 1e-12 relative); lambda images
+bit-exact where the reference's order is deterministic (median, trimmed
+mean on identical inputs), else within the tolerance stated per test.
+"""
+import numpy as np
+import pytest
+
+import muontomography_amd as M
+import oracle
+from muontomography_amd import workloads as W
+from muontomography_amd._abi import make_grid, mlsd_cfg, poca_cfg
+
+pytestmark = pytest.mark.gpu
+
+
+def _segments_siddon_test(n=10000):
+    """Segments drawn like the reference's test/SiddonTest.cpp:8-34
+    (uniform in [-500, 500)^3, seeded) -- numpy's generator, same shape."""
+    rng = np.random.default_rng(0)
+    pts = rng.uniform(-500.0, 500.0, size=(n, 6))
+    return pts[:, :3].copy(), pts[:, 3:].copy()
+
+
+def _adversarial_segments(g):
+    """Axis-parallel rays, rays through voxel corners/edges, rays starting or
+    ending inside the grid, outside-parallel rays, degenerate segments."""
+    lo = np.array([g.x_min, g.y_min, g.z_min])
+    d = np.array([g.dx, g.dy, g.dz])
+    n = np.array([g.nx, g.ny, g.nz], dtype=float)
+    hi = lo + d * n
+    P1, P2 = [], []
+    rng = np.random.default_rng(7)
+    for _ in range(300):  # plane-aligned endpoints (exact ties)
+        i1 = rng.integers(-2, n.astype(int) + 3)
+        i2 = rng.integers(-2, n.astype(int) + 3)
+        P1.append(lo + d * i1)
+        P2.append(lo + d * i2)
+    for ax in range(3):  # axis-parallel, inside and outside
+        for _ in range(50):
+            a = lo + (hi - lo) * rng.uniform(-0.2, 1.2, 3)
+            b = a.copy()
+            b[ax] = lo[ax] + (hi[ax] - lo[ax]) * rng.uniform(-0.5, 1.5)
+            P1.append(a)
+            P2.append(b)
+    for _ in range(300):  # start or end inside
+        a = lo + (hi - lo) * rng.uniform(0, 1, 3)
+        b = lo + (hi - lo) * rng.uniform(-1, 2, 3)
+        if rng.uniform() < 0.5:
+            a, b = b, a
+        P1.append(a)
+        P2.append(b)
+    for _ in range(50):  # tiny and zero-length segments
+        a = lo + (hi - lo) * rng.uniform(0, 1, 3)
+        P1.append(a)
+        P2.append(a + rng.normal(0, 1e-7, 3))
+    P1.append(lo.copy()); P2.append(hi.copy())          # main diagonal
+    P1.append(hi.copy()); P2.append(lo.copy())
+    P1.append(lo + 0.5 * d); P2.append(lo + 0.5 * d)    # point
+    return np.array(P1), np.array(P2)
+
+
+def _check_siddon(ctx, g, p1, p2):
+    ro, xo, yo, zo, lo_ = oracle.siddon_csr(g, p1, p2)
+    rg, xg, yg, zg, lg = ctx.siddon_csr(g, p1, p2)
+    assert np.array_equal(ro, rg), "per-segment voxel counts differ"
+    assert np.array_equal(xo, xg) and np.array_equal(yo, yg) and np.array_equal(zo, zg)
+    # lengths: bit-exact (FP64, no contraction, IEEE div/sqrt)
+    assert np.array_equal(lo_.view(np.uint64), lg.view(np.uint64)), \
+        f"max rel diff {np.max(np.abs(lo_ - lg) / np.maximum(np.abs(lo_), 1e-300))}"
+    return int(ro[-1])
+
+
+@pytest.mark.parametrize("grid", [
+    make_grid(100, 2.0, -100.0),                 # test/SiddonTest.cpp:11-14
+    make_grid((40, 30, 20), (25.0, 10.0, 7.5), (-500.0, -150.0, -80.0)),
+    make_grid((7, 1, 3), (1.0, 3.0, 0.5), (0.0, 0.0, 0.0)),
+])
+def test_siddon_bitexact(ctx, grid):
+    p1, p2 = _segments_siddon_test(5000)
+    assert _check_siddon(ctx, grid, p1, p2) > 0 or grid.nx == 7
+    a1, a2 = _adversarial_segments(grid)
+    _check_siddon(ctx, grid, a1, a2)
+
+
+def test_siddon_empty_and_outside(ctx):
+    g = make_grid(10, 1.0, 0.0)
+    p1 = np.array([[20.0, 20.0, 20.0], [-5.0, 5.0, 5.0], [5.0, 5.0, 5.0]])
+    p2 = np.array([[30.0, 25.0, 21.0], [-5.0, 5.0, 15.0], [5.0, 5.0, 5.0]])
+    row, x, y, z, l = ctx.siddon_csr(g, p1, p2)
+    ro, *_ = oracle.siddon_csr(g, p1, p2)
+    assert np.array_equal(row, ro)
+    assert row[1] == 0 and row[2] == 0
+    r0, *_ = ctx.siddon_csr(g, np.zeros((0, 3)), np.zeros((0, 3)))
+    assert r0.tolist() == [0]
+
+
+def test_poca_points_bitexact(ctx, small_set):
+    g, rays, _ = small_set
+    for mode in (0, 1):
+        Po, vo, ao, co = oracle.poca_points(g, rays, 1e-8, mode)
+        Pg, vg, ag, cg = ctx.poca_points(g, rays, 1e-8, mode)
+        assert np.array_equal(Po.view(np.uint64), Pg.view(np.uint64))
+        assert np.array_equal(vo, vg)
+        assert np.array_equal(ao, ag)
+        assert co["n_not_parallel"] == cg["n_not_parallel"]
+        assert co["n_scat_in_grid"] == cg["n_scat_in_grid"]
+
+
+def test_poca_x86_cast_edges(ctx):
+    """PoCA voxel casts below the grid wrap like x86 (B4): -0.5 -> 0,
+    -1.5 -> 4294967295; both paths agree."""
+    g = make_grid(10, 1.0, 0.0)
+    rays = np.zeros(4, M.RAY_DTYPE)
+    for i, x in enumerate([-0.5, -1.5, 9.99, 10.0]):
+        rays[i]["pin"] = (x, 5.0, 9.0)
+        rays[i]["pout"] = (x, 5.0, 1.0)
+        rays[i]["din"] = (0.0, 0.1, 1.0)
+        rays[i]["dout"] = (0.0, -0.1, 1.0)
+    Po, vo, ao, _ = oracle.poca_points(g, rays)
+    Pg, vg, ag, _ = ctx.poca_points(g, rays)
+    assert np.array_equal(vo, vg) and np.array_equal(ao, ag)
+    assert vo[0, 0] == 0 and vo[1, 0] == 4294967295
+
+
+@pytest.mark.parametrize("lw,sl", [(1, 0), (0, 0), (1, 1), (0, 1)])
+def test_poca_image(ctx, small_set, lw, sl):
+    g, rays, _ = small_set
+    cfg = poca_cfg(g, length_weighted=lw, straight_line=sl)
+    io, co = oracle.poca_reconstruct(cfg, rays)
+    ig, cg = ctx.poca_reconstruct(cfg, rays)
+    assert co["n_scat_in_grid"] == cg["n_scat_in_grid"]
+    assert np.array_equal(io == 0, ig == 0)
+    # FP64 atomics reorder the per-voxel sums; atan2 may differ by 1 ulp
+    np.testing.assert_allclose(ig, io, rtol=1e-12, atol=0)
+
+
+def _em_cfg(g, **kw):
+    base = dict(error_xy=0.5, error_dz_outer=500.0, error_dz_inner=500.0,
+                initialize_method=2, initialize_value=1e-7, num_iteration=5)
+    base.update(kw)
+    return mlsd_cfg(g, **base)
+
+
+def test_em_plan_paths_match_oracle(ctx, small_set):
+    g, rays, _ = small_set
+    cfg = _em_cfg(g)
+    em = oracle.Em(cfg, rays)
+    plan = ctx.em_plan(cfg, rays=rays)
+    row, vox, ln, tl, t4, pr = plan.export()
+    assert np.array_equal(row, em.row_ptr)
+    assert np.array_equal(vox, em.vox)
+    assert np.array_equal(ln.view(np.uint64), em.len.view(np.uint64))
+    assert np.array_equal(tl.view(np.uint64), em.tail.view(np.uint64))
+    # angles and displacements go through libm atan2/tan/cos: ulp-level only
+    np.testing.assert_allclose(t4, em.trk4, rtol=1e-13, atol=1e-15)
+    np.testing.assert_array_equal(pr, em.p)
+    mo = em.mj()
+    assert np.array_equal(plan.voxel_counts(), mo)
+    c = plan.counters()
+    assert c["n_scat_in_grid"] == em.n_sel and c["nnz"] == em.nnz
+
+
+@pytest.mark.parametrize("mode", ["median", "trimmed", "mean"])
+def test_em_fp64_prepared(ctx, small_set, mode):
+    """Same prepared inputs: per-track Sigma, det, inverse and S_ij are
+    bit-identical, so median and trimmed-mean images match bit for bit; the
+    mean's per-voxel FP64 sum runs in atomic order (rel <= 1e-13)."""
+    g, rays, _ = small_set
+    kw = {"median": dict(update_with_mean=0), "trimmed": dict(update_with_mean=1, mean_percentile=98),
+          "mean": dict(update_with_mean=1, mean_percentile=100)}[mode]
+    cfg = _em_cfg(g, **kw)
+    em = oracle.Em(cfg, rays)
+    plan = ctx.em_plan(cfg, prepared=em.prepared())
+    img = np.full(g.n_vox, 1e-7)
+    plan.set_image(img)
+    ref = img
+    costs = []
+    for _ in range(4):
+        ref, cost, _ns = em.iterate(ref)
+        costs.append(cost)
+    cg = plan.run(4)
+    out = plan.image()
+    np.testing.assert_allclose(cg, costs, rtol=1e-12)
+    if mode == "mean":
+        np.testing.assert_allclose(out, ref, rtol=1e-13, atol=0)
+    else:
+        assert np.array_equal(out.view(np.uint64), ref.view(np.uint64)), \
+            f"max abs diff {np.max(np.abs(out - ref))}"
+
+
+def test_mlsd_one_shot(ctx, small_set):
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=0, num_iteration=3)
+    io, co, no = oracle.mlsd_reconstruct(cfg, rays)
+    ig, cg, ng = ctx.mlsd_reconstruct(cfg, rays)
+    assert no["n_scat_in_grid"] == ng["n_scat_in_grid"] and no["nnz"] == ng["nnz"]
+    assert ng["iterations_done"] == 3
+    # inputs differ from the oracle only by libm ulps in the angles
+    np.testing.assert_allclose(ig, io, rtol=1e-9, atol=1e-22)
+    np.testing.assert_allclose(cg, co, rtol=1e-9)
+
+
+def _rel_stats(a, ref, mj):
+    m = (mj > 0) & (ref > 0)
+    r = np.abs(a[m] - ref[m]) / ref[m]
+    return float(r.max()), float(np.sqrt(np.mean(r * r))), float(np.abs(a - ref).max() / np.abs(ref).max())
+
+
+def test_fp32c_matches_fp64(ctx, small_set):
+    """FP32C (FP32 sums with two-sum, FP64 2x2 algebra) stays within 1e-4
+    relative of FP64 after 20 iterations, mean update."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100)
+    p64 = ctx.em_plan(cfg, rays=rays)
+    p32 = ctx.em_plan(cfg, rays=rays)
+    img = np.full(g.n_vox, 1e-7)
+    p64.set_image(img)
+    p32.set_image(img)
+    c64 = p64.run(20, M.MUTO_FP64)
+    c32 = p32.run(20, M.MUTO_FP32C)
+    a64, a32 = p64.image(), p32.image()
+    mx, rms, norm = _rel_stats(a32, a64, p64.voxel_counts())
+    assert mx <= 1e-4, (mx, rms, norm)
+    np.testing.assert_allclose(c32, c64, rtol=1e-5)
+    assert p32.counters()["n_segments"] > 0
+
+
+@pytest.mark.parametrize("nb,obj,p", [(0, 0, 1.0), (1, 1, 2.0)])
+def test_map_fp64(ctx, small_set, nb, obj, p):
+    """MAP (MutoMAP.cpp): ML step + per-voxel cubic surrogate step.  On the
+    same prepared inputs the ML part is bit-exact; the voxel step uses pow()
+    and log(), which differ from glibc by ulps, so rtol 1e-9."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=0, beta=1.0, p=p, delta=0.1, all_neighbours=nb,
+                  add_object_prior=obj, num_iteration=3)
+    em = oracle.Em(cfg, rays)
+    plan = ctx.em_plan(cfg, prepared=em.prepared())
+    img = np.full(g.n_vox, 1e-7)
+    plan.set_image(img)
+    mj = em.mj()
+    ref = img.copy()
+    for _ in range(3):
+        ml, _c, _ = em.iterate(ref)
+        ref, _w = oracle.map_voxel_step(cfg, mj, ref, ml)
+    plan.run(3, M.MUTO_FP64, map_=True)
+    out = plan.image()
+    np.testing.assert_allclose(out, ref, rtol=1e-9, atol=1e-22)
+    # one-shot MAP from rays
+    io, co, no = oracle.mlsd_reconstruct(cfg, rays, map_=True)
+    ig, cg, ng = ctx.map_reconstruct(cfg, rays)
+    np.testing.assert_allclose(ig, io, rtol=1e-8, atol=1e-20)
+    assert ng["n_cubic_warn"] == no["n_cubic_warn"]

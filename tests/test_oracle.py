@@ -1,0 +1,446 @@
+"""Pins the CPU oracle (the checker) against what the mathematics fixes:
+exact-rational brute force, closed forms, invariants and finite
+differences.  The reference cannot be compiled here (it needs Eigen), and
+its tests hold no golden vectors, so these are the checks that make a
+dropped term, a wrong sign or index, or a transposed operand fail."""
+from fractions import Fraction as F
+
+import numpy as np
+import pytest
+
+import oracle
+from muontomography_amd import workloads as W
+from muontomography_amd._abi import make_grid, mlsd_cfg, poca_cfg
+import muontomography_amd as M
+
+EPS_A = 1e-6  # Siddon's alpha eps (src/MutoSiddon.cpp:19)
+
+
+# ---------------------------------------------------------------- casts
+def test_x86_u32_cast():
+    assert oracle.x86_u32(-0.5) == 0
+    assert oracle.x86_u32(-1.5) == 4294967295
+    assert oracle.x86_u32(3.99) == 3
+    assert oracle.x86_u32(2.0 ** 32 + 5.7) == 5
+    assert oracle.x86_u32(float("nan")) == 0
+    assert oracle.x86_u32(1e300) == 0
+    assert oracle.x86_u32(-1e300) == 0
+
+
+# ---------------------------------------------------------------- Siddon
+def _brute_siddon(g, p1, p2):
+    """Exact rational Siddon: all plane crossings inside [amin, amax], voxel of
+    each interval from its midpoint, sub-eps intervals merged into the next
+    recorded voxel as the reference does (MutoSiddon.cpp:131-137).  Returns
+    (list of (x,y,z), list of exact alpha widths) and the smallest gap
+    between distinct crossings (to skip near-tie rays)."""
+    mn = [F(g.x_min), F(g.y_min), F(g.z_min)]
+    d = [F(g.dx), F(g.dy), F(g.dz)]
+    n = [g.nx, g.ny, g.nz]
+    a = [F(v) for v in p1]
+    b = [F(v) for v in p2]
+    dv = [b[k] - a[k] for k in range(3)]
+    amin, amax = F(0), F(1)
+    for k in range(3):
+        if abs(dv[k]) > F(1e-6):
+            t0 = (mn[k] - a[k]) / dv[k]
+            t1 = (mn[k] + d[k] * n[k] - a[k]) / dv[k]
+            amin = max(amin, min(t0, t1))
+            amax = min(amax, max(t0, t1))
+        elif not (mn[k] <= a[k] < mn[k] + d[k] * n[k]):
+            return [], [], 1.0
+    if amax <= amin:
+        return [], [], 1.0
+    cuts = {amin, amax}
+    for k in range(3):
+        if abs(dv[k]) <= F(1e-6):
+            continue
+        for i in range(n[k] + 1):
+            t = (mn[k] + d[k] * i - a[k]) / dv[k]
+            if amin < t < amax:
+                cuts.add(t)
+    cuts = sorted(cuts)
+    gaps = [float(cuts[i + 1] - cuts[i]) for i in range(len(cuts) - 1)]
+    vox, wid = [], []
+    acc = F(0)
+    for i in range(len(cuts) - 1):
+        mid = (cuts[i] + cuts[i + 1]) / 2
+        idx = tuple(int(((a[k] + mid * dv[k]) - mn[k]) // d[k]) for k in range(3))
+        acc += cuts[i + 1] - cuts[i]
+        if acc > F(EPS_A):
+            vox.append(idx)
+            wid.append(acc)
+            acc = F(0)
+    min_gap = min(gaps) if gaps else 1.0
+    near_eps = min(abs(gp - EPS_A) for gp in gaps) if gaps else 1.0
+    return vox, wid, min(min_gap, near_eps)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_siddon_matches_exact_rational_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    g = make_grid((int(rng.integers(1, 8)), int(rng.integers(1, 8)), int(rng.integers(1, 8))),
+                  tuple(rng.uniform(0.5, 3.0, 3)), tuple(rng.uniform(-5, 5, 3)))
+    checked = 0
+    for _ in range(150):
+        hi = np.array([g.x_min + g.dx * g.nx, g.y_min + g.dy * g.ny, g.z_min + g.dz * g.nz])
+        lo = np.array([g.x_min, g.y_min, g.z_min])
+        p1 = lo + (hi - lo) * rng.uniform(-0.5, 1.5, 3)
+        p2 = lo + (hi - lo) * rng.uniform(-0.5, 1.5, 3)
+        vox, wid, gap = _brute_siddon(g, p1, p2)
+        if gap < 1e-9:
+            continue  # exact or near tie: covered by the tie tests below
+        row, x, y, z, ln = oracle.siddon_csr(g, p1[None], p2[None])
+        got = list(zip(x.tolist(), y.tolist(), z.tolist()))
+        assert got == vox, (p1, p2)
+        L = float(np.linalg.norm(p2 - p1))
+        np.testing.assert_allclose(ln, [float(w) * L for w in wid], rtol=1e-9, atol=1e-12)
+        checked += 1
+    assert checked > 50
+
+
+def test_siddon_axis_aligned_closed_form():
+    g = make_grid(10, 2.0, 0.0)
+    p1 = np.array([[-5.0, 3.0, 7.0]])   # through y-voxel 1, z-voxel 3
+    p2 = np.array([[25.0, 3.0, 7.0]])
+    row, x, y, z, ln = oracle.siddon_csr(g, p1, p2)
+    assert x.tolist() == list(range(10))
+    assert set(y.tolist()) == {1} and set(z.tolist()) == {3}
+    np.testing.assert_allclose(ln, 2.0, rtol=1e-12)
+    # reversed direction: descending indices
+    row, x, y, z, ln = oracle.siddon_csr(g, p2, p1)
+    assert x.tolist() == list(range(9, -1, -1))
+
+
+def test_siddon_total_length_invariant():
+    g = make_grid((13, 9, 11), (1.5, 2.5, 2.0), (-7.0, -10.0, -3.0))
+    rng = np.random.default_rng(11)
+    lo = np.array([g.x_min, g.y_min, g.z_min])
+    hi = lo + np.array([g.dx * g.nx, g.dy * g.ny, g.dz * g.nz])
+    p1 = lo + (hi - lo) * rng.uniform(-0.3, 1.3, (500, 3))
+    p2 = lo + (hi - lo) * rng.uniform(-0.3, 1.3, (500, 3))
+    row, x, y, z, ln = oracle.siddon_csr(g, p1, p2)
+    for i in range(500):
+        seg = ln[row[i]:row[i + 1]]
+        if seg.size == 0:
+            continue
+        # chord inside the box, exact in rationals
+        vox, wid, _ = _brute_siddon(g, p1[i], p2[i])
+        chord = float(sum(wid)) * np.linalg.norm(p2[i] - p1[i])
+        # the last sub-eps sliver may be dropped: within eps * |p2-p1|
+        assert abs(seg.sum() - chord) <= 2e-6 * np.linalg.norm(p2[i] - p1[i]) + 1e-9
+        assert np.all(seg > 0)
+        # consecutive voxels are face neighbours unless a sliver was merged
+        v = np.stack([x[row[i]:row[i + 1]], y[row[i]:row[i + 1]], z[row[i]:row[i + 1]]], 1).astype(int)
+        steps = np.abs(np.diff(v, axis=0)).sum(1)
+        assert np.all(steps >= 1) and np.all(steps <= 3)
+
+
+def test_siddon_parallel_outside_and_degenerate():
+    g = make_grid(4, 1.0, 0.0)
+    # parallel to x, outside in y
+    r, *_ = oracle.siddon_csr(g, np.array([[-1.0, 5.0, 1.0]]), np.array([[6.0, 5.0, 1.0]]))
+    assert r[-1] == 0
+    # parallel to x, on the y = max plane (half-open grid): outside
+    r, *_ = oracle.siddon_csr(g, np.array([[-1.0, 4.0, 1.0]]), np.array([[6.0, 4.0, 1.0]]))
+    assert r[-1] == 0
+    # on the y = min plane: inside
+    r, x, y, z, l = oracle.siddon_csr(g, np.array([[-1.0, 0.0, 1.5]]), np.array([[6.0, 0.0, 1.5]]))
+    assert r[-1] == 4 and set(y.tolist()) == {0}
+    # a point: zero-length segment inside records its voxel with length 0*..
+    r, x, y, z, l = oracle.siddon_csr(g, np.array([[1.5, 1.5, 1.5]]), np.array([[1.5, 1.5, 1.5]]))
+    assert r[-1] == 1 and (x[0], y[0], z[0]) == (1, 1, 1) and l[0] == 0.0
+
+
+def test_siddon_corner_tie_breaking():
+    """Through a voxel corner: ties resolve x only if strictly smaller than
+    both, then y if ay < az, else z (MutoSiddon.cpp:116-128); the zero-width
+    voxels in between are dropped (:131-137)."""
+    g = make_grid(4, 1.0, 0.0)
+    r, x, y, z, l = oracle.siddon_csr(g, np.array([[-1.0, -1.0, -1.0]]), np.array([[5.0, 5.0, 5.0]]))
+    got = list(zip(x.tolist(), y.tolist(), z.tolist()))
+    assert got == [(0, 0, 0), (1, 1, 1), (2, 2, 2), (3, 3, 3)]
+    np.testing.assert_allclose(l, np.sqrt(3.0), rtol=1e-12)
+
+
+# ---------------------------------------------------------------- PoCA
+def _ray(pin, din, pout, dout, e=3.0):
+    r = np.zeros(1, M.RAY_DTYPE)
+    r["pin"], r["din"], r["pout"], r["dout"] = pin, din, pout, dout
+    r["ein"] = r["eout"] = e
+    return r
+
+
+def test_poca_skew_lines_closed_form():
+    g = make_grid(10, 10.0, -50.0)
+    # L1: (0,0,100) + s (1,0,1); L2: (0,5,-100) + t (0,1,1).  Closest points:
+    # minimise |(s, -5 - t, 200 + s - t)|^2 -> s = -200/3 - ..., solved exactly
+    a = np.array([0.0, 0.0, 100.0]); u = np.array([1.0, 0.0, 1.0])
+    b = np.array([0.0, 5.0, -100.0]); v = np.array([0.0, 1.0, 1.0])
+    A = np.array([[u @ u, -u @ v], [u @ v, -v @ v]])
+    rhs = np.array([(b - a) @ u, (b - a) @ v])
+    s, t = np.linalg.solve(A, rhs)
+    P = 0.5 * ((a + s * u) + (b + t * v))
+    Po, *_ = oracle.poca_points(g, _ray(a, u, b, v))
+    np.testing.assert_allclose(Po[0], P, rtol=1e-12, atol=1e-12)
+    # the connecting segment is perpendicular to both lines
+    w = (a + s * u) - (b + t * v)
+    assert abs(w @ u) < 1e-9 and abs(w @ v) < 1e-9
+
+
+def test_poca_intersecting_and_parallel():
+    g = make_grid(10, 10.0, -50.0)
+    X = np.array([3.0, -4.0, 7.0])
+    u = np.array([0.1, 0.2, 1.0]); v = np.array([-0.3, 0.05, 1.0])
+    Po, *_ = oracle.poca_points(g, _ray(X + 200 * u, u, X - 300 * v, v))
+    np.testing.assert_allclose(Po[0], X, atol=1e-9)
+    # parallel: midpoint of pin/pout
+    pin = np.array([1.0, 2.0, 100.0]); pout = np.array([4.0, -2.0, -100.0])
+    Po, _, _, c = oracle.poca_points(g, _ray(pin, u, pout, u))
+    np.testing.assert_array_equal(Po[0], 0.5 * (pin + pout))
+    assert c["n_not_parallel"] == 0
+
+
+def test_poca_image_single_track_closed_form():
+    """One straight vertical track through voxel column (2,3,*), scattered
+    by a known angle at z = 5: length-weighted straight-line PoCA gives
+    theta^2 / dz in the scattering voxel and 0 elsewhere."""
+    g = make_grid(6, 10.0, -30.0)
+    th = 0.01
+    pin = np.array([-5.0, 5.0, 30.0]); din = np.array([0.0, 0.0, 1.0])
+    dout = np.array([np.sin(th), 0.0, np.cos(th)])
+    P = np.array([-5.0, 5.0, 5.0])  # middle of voxel layer z = 3
+    pout = P - 35.0 / dout[2] * dout
+    img, c = oracle.poca_reconstruct(poca_cfg(g, length_weighted=1, straight_line=1),
+                                     _ray(pin, din, pout, dout))
+    assert c["n_scat_in_grid"] == 1
+    a2 = (0.0 - np.arctan2(dout[0], dout[2])) ** 2 / 2.0  # (axz^2 + ayz^2)/2
+    j = 2 + 6 * (3 + 6 * 3)
+    nz = np.nonzero(img)[0]
+    assert nz.tolist() == [j]
+    # weight of that voxel: the straight chord pin->pout inside it
+    row, x, y, zz, ln = oracle.siddon_csr(g, pin[None], pout[None])
+    w = sum(l for xx, yy, z_, l in zip(x, y, zz, ln) if (xx, yy, z_) == (2, 3, 3))
+    np.testing.assert_allclose(img[j], a2 / w, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- EM algebra
+def _single_voxel_problem(K=7, seed=0):
+    """1x1x1 grid, K tracks with one entry each: every S_ij in closed form."""
+    rng = np.random.default_rng(seed)
+    g = make_grid(1, 10.0, 0.0)
+    trk4 = rng.normal(0, [3e-3, 3e-3, 1.0, 1.0], (K, 4))
+    p = rng.uniform(0.5, 2.0, K)
+    ln = rng.uniform(1.0, 14.0, K)
+    row = np.arange(K + 1, dtype=np.uint64)
+    vox = np.zeros(K, np.uint32)
+    return g, (trk4, p, row, vox, ln)
+
+
+def _closed_form_S(lam, prep, e=0.5, o=500.0, i=500.0):
+    trk4, p, row, vox, ln = prep
+    E = np.array([[4 * e * e / o ** 2, 2 * e * e * i / o ** 2],
+                  [2 * e * e * i / o ** 2, 2 * e * e * (1 + (i / o + 1) * i / o)]])
+    S, cost = [], []
+    for k in range(len(ln)):
+        L, T = ln[k], 0.0
+        Wm = np.array([[L, L * T + L * L / 2], [L * T + L * L / 2, L ** 3 / 3 + (L + T) * L * T]])
+        Sig = p[k] ** 2 * lam * Wm + E
+        Si = np.linalg.inv(Sig)
+        Sx, Sy, Dx, Dy = trk4[k]
+        D = np.array([[Sx * Sx + Sy * Sy, Sx * Dx + Sy * Dy], [Sx * Dx + Sy * Dy, Dx * Dx + Dy * Dy]])
+        xy = np.trace(D @ (Si @ Wm @ Si))
+        tr = np.trace(Si @ Wm)
+        S.append(2 * lam + (0.5 * xy - tr) * p[k] ** 2 * lam ** 2)
+        cost.append(np.log(np.linalg.det(Sig)) + 0.5 * np.trace(Si @ D))
+    return np.array(S), np.array(cost)
+
+
+@pytest.mark.parametrize("mode", ["median_odd", "median_even", "mean", "trimmed"])
+def test_em_update_rules_closed_form(mode):
+    K = 8 if mode == "median_even" else 7
+    g, prep = _single_voxel_problem(K)
+    kw = dict(error_xy=0.5, error_dz_outer=500.0, error_dz_inner=500.0)
+    if mode == "mean":
+        kw.update(update_with_mean=1, mean_percentile=100)
+    elif mode == "trimmed":
+        kw.update(update_with_mean=1, mean_percentile=60)
+    cfg = mlsd_cfg(g, **kw)
+    em = oracle.Em(cfg, prepared=prep)
+    lam = 3e-6
+    out, cost, ns = em.iterate(np.array([lam]))
+    S, c = _closed_form_S(lam, prep)
+    if mode.startswith("median"):
+        want = 0.5 * np.median(S)
+    elif mode == "mean":
+        want = 0.5 * S.mean()
+    else:
+        k = max(int(0.6 * K), 1)
+        want = 0.5 * np.sort(S)[:k].mean()
+    np.testing.assert_allclose(out[0], want, rtol=1e-12)
+    np.testing.assert_allclose(cost, c.mean(), rtol=1e-12)
+    assert ns == 0
+
+
+def test_em_tail_and_weights_closed_form():
+    """Two entries in one track: tails T = (L2, 0), W from (L, T)."""
+    g = make_grid((2, 1, 1), 10.0, 0.0)
+    trk4 = np.array([[2e-3, -1e-3, 0.7, 0.3]])
+    prep = (trk4, np.array([1.3]), np.array([0, 2], np.uint64), np.array([0, 1], np.uint32),
+            np.array([4.0, 7.0]))
+    cfg = mlsd_cfg(g, update_with_mean=1, error_xy=0.5, error_dz_outer=500.0, error_dz_inner=500.0)
+    em = oracle.Em(cfg, prepared=prep)
+    np.testing.assert_array_equal(em.tail, [7.0, 0.0])
+    lam = np.array([2e-6, 5e-6])
+    out, cost, _ = em.iterate(lam)
+    e, o, i = 0.5, 500.0, 500.0
+    E = np.array([[4 * e * e / o ** 2, 2 * e * e * i / o ** 2],
+                  [2 * e * e * i / o ** 2, 2 * e * e * (1 + (i / o + 1) * i / o)]])
+
+    def Wm(L, T):
+        return np.array([[L, L * T + L * L / 2], [L * T + L * L / 2, L ** 3 / 3 + (L + T) * L * T]])
+    W1, W2 = Wm(4.0, 7.0), Wm(7.0, 0.0)
+    Sig = 1.3 ** 2 * (lam[0] * W1 + lam[1] * W2) + E
+    Si = np.linalg.inv(Sig)
+    Sx, Sy, Dx, Dy = trk4[0]
+    D = np.array([[Sx * Sx + Sy * Sy, Sx * Dx + Sy * Dy], [Sx * Dx + Sy * Dy, Dx * Dx + Dy * Dy]])
+    for j, Wj in enumerate((W1, W2)):
+        S = 2 * lam[j] + (0.5 * np.trace(D @ Si @ Wj @ Si) - np.trace(Si @ Wj)) * 1.3 ** 2 * lam[j] ** 2
+        np.testing.assert_allclose(out[j], 0.5 * S, rtol=1e-12)
+
+
+def test_em_compact_form_identity(small_prepared):
+    """S_ij = 2 lambda_j + c_ij lambda_j^2 with c_ij = p^2 <W_ij, G_i>: the
+    mean update equals lambda + lambda^2/(2 M) sum_i c_ij."""
+    cfg, em, lam = small_prepared
+    out, _, _ = em.iterate(lam)
+    c, _ = em.cij(lam)
+    mj = em.mj()
+    sc = np.bincount(em.vox, weights=c, minlength=cfg.grid.n_vox)
+    m = mj > 0
+    want = np.zeros_like(lam)
+    want[m] = lam[m] + lam[m] ** 2 / (2 * mj[m]) * sc[m]
+    np.testing.assert_allclose(out[m], want[m], rtol=1e-9, atol=1e-22)
+    assert np.all(out[~m] == 0)
+
+
+@pytest.fixture(scope="module")
+def small_prepared():
+    g, rays, _ = W.muons("fd32", n_muons=3000)
+    cfg = mlsd_cfg(g, update_with_mean=1, mean_percentile=100, error_xy=0.5,
+                   error_dz_outer=500.0, error_dz_inner=500.0, beta=1.0, p=1.0, delta=0.1)
+    em = oracle.Em(cfg, rays)
+    rng = np.random.default_rng(3)
+    lam = rng.uniform(0.5e-6, 2e-6, g.n_vox)
+    return cfg, em, lam
+
+
+def test_cost_gradient_is_minus_c(small_prepared):
+    """d cost_i / d lambda_j = -c_ij (central differences)."""
+    cfg, em, lam = small_prepared
+    c, ci = em.cij(lam)
+    G = -np.bincount(em.vox, weights=c, minlength=cfg.grid.n_vox)
+    js = np.argsort(-np.abs(G))[:6]
+    for j in js:
+        h = 1e-6 * lam[j]
+        lp, lm = lam.copy(), lam.copy()
+        lp[j] += h
+        lm[j] -= h
+        fd = (em.cij(lp)[1].sum() - em.cij(lm)[1].sum()) / (2 * h)
+        np.testing.assert_allclose(fd, G[j], rtol=1e-5)
+
+
+@pytest.mark.parametrize("p,obj,nb", [(1.0, 0, 0), (2.0, 1, 1)])
+def test_map_phi_gradient_fd(small_prepared, p, obj, nb):
+    """grad Phi against FP64 central differences, h = 1e-6 lambda_j."""
+    cfg0, em, lam = small_prepared
+    cfg = mlsd_cfg(cfg0.grid, update_with_mean=1, error_xy=0.5, error_dz_outer=500.0,
+                   error_dz_inner=500.0, beta=1.0, p=p, delta=0.1, add_object_prior=obj,
+                   all_neighbours=nb)
+    g = em.grad(lam, cfg)
+    rng = np.random.default_rng(5)
+    js = rng.choice(np.nonzero(em.mj() > 0)[0], 12, replace=False)
+    for j in js:
+        h = 1e-6 * lam[j]
+        lp, lm = lam.copy(), lam.copy()
+        lp[j] += h
+        lm[j] -= h
+        fd = (em.phi(lp, cfg) - em.phi(lm, cfg)) / (2 * h)
+        np.testing.assert_allclose(fd, g[j], rtol=2e-4, atol=1e-6 * np.abs(g).max())
+
+
+def test_map_surrogate_tangent_to_phi(small_prepared):
+    """In mean mode the reference's surrogate (MutoMAP.cpp:261-281) has
+    the gradient of Phi at l_n: the reference's MAP gradient."""
+    cfg0, em, lam = small_prepared
+    for p, obj, nb in [(1.0, 0, 0), (2.0, 1, 1), (1.2, 0, 1)]:
+        cfg = mlsd_cfg(cfg0.grid, update_with_mean=1, error_xy=0.5, error_dz_outer=500.0,
+                       error_dz_inner=500.0, beta=1.0, p=p, delta=0.1, add_object_prior=obj,
+                       all_neighbours=nb)
+        g = em.grad(lam, cfg)
+        sg = em.surrogate_grad(lam, cfg)
+        m = em.mj() > 0
+        np.testing.assert_allclose(sg[m], g[m], rtol=1e-4, atol=1e-6 * np.abs(g[m]).max())
+
+
+# ---------------------------------------------------------------- MAP solver
+def test_cubic_solver_constructed_roots():
+    for r in [(1.0, 2.0, 3.0), (-5.0, 0.5, 40.0), (1e-3, 2e-3, 7.0)]:
+        p3 = 2.0
+        c = np.poly(r) * p3  # p3 x^3 + p2 x^2 + p1 x + p0
+        roots, w = oracle.cubic_solve(c[0], c[1], c[2], c[3], 1e-12)
+        assert len(roots) == 3 and w == 0
+        np.testing.assert_allclose(np.sort(roots), np.sort(r), rtol=1e-6, atol=1e-9)
+
+
+def test_cubic_solver_one_real_root_matches_numpy():
+    rng = np.random.default_rng(1)
+    n1 = 0
+    for _ in range(200):
+        c = rng.normal(size=4)
+        c[0] = abs(c[0]) + 0.1
+        roots, _ = oracle.cubic_solve(c[0], c[1], c[2], c[3], 1e-12)
+        ref = np.roots(c)
+        real = np.sort(ref[np.abs(ref.imag) < 1e-7].real)
+        if len(real) == 1:
+            n1 += 1
+            assert len(roots) == 1
+        np.testing.assert_allclose(np.sort(roots)[-len(real):] if len(roots) >= len(real) else roots,
+                                   real[-len(roots):] if len(roots) <= len(real) else real,
+                                   rtol=1e-6, atol=1e-8)
+    assert n1 > 20
+
+
+def test_map_voxel_step_minimises_surrogate_p2():
+    """p = 2 (quadratic prior): the voxel step returns the minimiser of the
+    surrogate over l > 0, checked by a dense 1-D search."""
+    g = make_grid(4, 10.0, 0.0)
+    cfg = mlsd_cfg(g, beta=0.5, p=2.0, delta=0.1)
+    rng = np.random.default_rng(2)
+    last = rng.uniform(0.5e-7, 3e-7, g.n_vox)
+    ml = last * rng.uniform(0.7, 1.4, g.n_vox)
+    mj = rng.integers(1, 50, g.n_vox).astype(np.uint32)
+    out, _ = oracle.map_voxel_step(cfg, mj, last, ml)
+    s = 1e7
+    for j in rng.choice(g.n_vox, 10, replace=False):
+        w, lm = oracle.map_neighbours(cfg, last, int(j))
+        ln, lml = last[j] * s, ml[j] * s
+        grid_l = np.linspace(0.2 * min(ln, lml), 3 * max(ln, lml), 20001)
+        q = mj[j] * (np.log(grid_l) + lml / grid_l) + sum(
+            0.5 * cfg.beta * cfg.p * wk * (2 * grid_l - ln - lk) ** 2 for wk, lk in zip(w, lm))
+        best = grid_l[np.argmin(q)] / s
+        np.testing.assert_allclose(out[j], best, rtol=2e-4)
+
+
+# ---------------------------------------------------------------- end to end
+def test_mlsd_reconstruct_recovers_dense_blocks():
+    """Sanity: MLSD from the seeded generator puts lead and iron above air."""
+    g, rays, info = W.muons("small", n_muons=40000)
+    cfg = mlsd_cfg(g, update_with_mean=1, error_xy=0.5, error_dz_outer=500.0,
+                   error_dz_inner=500.0, initialize_method=2, initialize_value=1e-7,
+                   num_iteration=5)
+    img, cost, c = oracle.mlsd_reconstruct(cfg, rays)
+    truth = info["truth"]
+    dense = truth > 1e-7
+    assert img[dense].mean() > 3 * img[~dense & (img > 0)].mean()
+    assert np.all(np.diff(cost) < 0)
