@@ -1,28 +1,40 @@
 // FP32C EM iteration: brick-major segment stream, LDS-staged gathers and
-// backprojection, FP32 sums with two-sum compensation, FP64 2x2 algebra.
+// backprojection, FP32 entry arithmetic, FP64 2x2 algebra.
 //
 // Layout (built once per plan from the track-major FP64 lists):
 //   - the grid is cut into bricks (LDS tiles) of bx*by*bz voxels;
 //   - each track's merged path is cut into segments: maximal runs of
 //     consecutive entries inside one brick;
-//   - segments are sorted by (brick, track rank), tracks ranked by a
-//     spatial key (first brick, last brick) so that neighbouring lanes
-//     handle neighbouring, nearly parallel tracks;
+//   - tracks are ranked by a spatial key (direction bin, then the Morton
+//     code of where the track crosses the top plane), segments sorted by
+//     (brick, track rank): the tracks a block sees together are nearly
+//     parallel neighbours;
+//   - each brick's segment list is cut into tasks (the persistent kernels'
+//     work units) and each task into chunks of at most 256 segments and
+//     ECAP entries; each chunk carries a lane -> segment permutation that
+//     orders its segments by length, longest first, so the 64 lanes of a
+//     wave walk segments of nearly equal length while the segments stay in
+//     track order in memory;
 //   - entries are stored brick-major as (u16 voxel-in-brick, f32 length):
 //     6 bytes per track-voxel entry.
 // Per iteration (src/MutoMLSD.cpp:285-308, mean update):
 //   k_fwd32   per segment: partial covariance sums sum lambda_j W_ij with
 //             lambda staged in LDS, W from (L, tail) in registers;
 //   k_track32 per track: Sigma_i in FP64 from its segment partials, Eigen's
-//             2x2 det/inverse in FP64, cost_i, G_i = p^2 (Si^-1 D Si^-1/2
-//             - Si^-1) in FP32;
-//   k_bwd32   per segment: c_ij = <W_ij, G_i> accumulated per voxel in LDS
-//             with atomic two-sum (hi, lo), flushed once per task;
+//             2x2 det/inverse in FP64, cost_i, G_i in FP32;
+//   k_bwd32   per segment: c_ij = <W_ij, G_i> accumulated per voxel in LDS,
+//             flushed once per task;
 //   k_upd32   per voxel: lambda += lambda^2/(2 M_j) sum_i c_ij, which is
 //             half the mean of S_ij = 2 lambda_j + c_ij lambda_j^2.
-// The persistent kernels take tasks (brick, segment range) from a queue
-// sorted by size, largest first, through an atomic counter.
+// Accumulation (acc_mode): 1 (default) FP64 -- registers in the forward
+// walk, one FP64 LDS atomic per entry in the backprojection; 0 FP32 with
+// two-sum compensation -- error-free transformation in the forward walk,
+// atomic two-sum (returned old value + error word) in the backprojection.
+// The persistent kernels take tasks from a queue sorted by size, largest
+// first, through an atomic counter; each block stages the next chunk into
+// registers while it walks the current one.
 #include <math.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <numeric>
@@ -40,6 +52,12 @@ ErrM error_matrix(const muto_mlsd_cfg& c);
 int map_step(muto_em_plan* pl, const double* last, const double* ml, double* out);
 
 namespace {
+
+constexpr uint32_t BLOCK = 256;          // threads per block (4 wavefronts)
+constexpr uint32_t ECAP = 4096;          // staged entries per chunk
+constexpr uint32_t PF = ECAP / BLOCK;    // prefetched entries per thread
+static_assert((ECAP * (sizeof(float) + sizeof(uint16_t))) % 16 == 0,
+              "static LDS must keep the dynamic region 16-byte aligned");
 
 struct BrickGeo {
   uint32_t n[3];   // grid
@@ -59,7 +77,21 @@ __device__ __forceinline__ uint32_t brick_of(const BrickGeo& G, uint32_t j,
   return bx + G.nb[0] * (by + G.nb[1] * bz);
 }
 
+__device__ __forceinline__ uint32_t spread12(uint32_t v) {
+  // interleave the 12 low bits of v with zeros (Morton)
+  v &= 0xFFFu;
+  v = (v | (v << 8)) & 0x00FF00FFu;
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
 // ---------------- build ------------------------------------------------
+// Segments per track and the track's spatial key: direction bin of the
+// chord from its first to its last voxel (9 x 9 bins of dx/|dz|, dy/|dz|
+// over [-2, 2]) above the Morton code of where that chord crosses the top
+// plane of the grid.
 __global__ void __launch_bounds__(256)
 k_seg_count(BrickGeo G, size_t n_trk, size_t nv,
             const uint64_t* __restrict__ row_ptr,
@@ -68,21 +100,33 @@ k_seg_count(BrickGeo G, size_t n_trk, size_t nv,
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_trk) return;
   const uint64_t b = row_ptr[t], e = row_ptr[t + 1];
-  uint32_t cnt = 0, prev = 0xFFFFFFFFu, first = 0, last = 0;
+  uint32_t cnt = 0, prev = 0xFFFFFFFFu, jf = 0, jl = 0;
   bool have = false;
   for (uint64_t k = b; k < e; ++k) {
     const uint32_t j = vox[k];
     if (j >= nv) { prev = 0xFFFFFFFFu; continue; }
     const uint32_t br = brick_of(G, j, nullptr);
     if (br != prev) ++cnt;
-    if (!have) { first = br; have = true; }
-    last = br;
+    if (!have) { jf = j; have = true; }
+    jl = j;
     prev = br;
   }
   nseg[t] = cnt;
-  const uint64_t nbk = (uint64_t)G.nb[0] * G.nb[1] * G.nb[2];
-  key[t] = have ? (uint64_t)first * nbk + last : ~0ull;
   val[t] = (uint32_t)t;
+  if (!have) { key[t] = ~0ull; return; }
+  const float x0 = (float)(jf % G.n[0]), y0 = (float)((jf / G.n[0]) % G.n[1]);
+  const float z0 = (float)(jf / G.n[0] / G.n[1]);
+  const float x1 = (float)(jl % G.n[0]), y1 = (float)((jl / G.n[0]) % G.n[1]);
+  const float z1 = (float)(jl / G.n[0] / G.n[1]);
+  const float dz = fmaxf(fabsf(z1 - z0), 1.0f);
+  const float sx = fminf(fmaxf((x1 - x0) / dz, -2.0f), 2.0f);
+  const float sy = fminf(fmaxf((y1 - y0) / dz, -2.0f), 2.0f);
+  const uint32_t dbin = (uint32_t)((sx + 2.0f) * 2.0f + 0.5f) * 9u + (uint32_t)((sy + 2.0f) * 2.0f + 0.5f);
+  const float s = (z1 != z0) ? ((float)G.n[2] - z0) / (z1 - z0) : 0.0f;
+  const float xt = fminf(fmaxf(x0 + s * (x1 - x0) + 1024.0f, 0.0f), 4095.0f);
+  const float yt = fminf(fmaxf(y0 + s * (y1 - y0) + 1024.0f, 0.0f), 4095.0f);
+  const uint32_t mort = spread12((uint32_t)xt) | (spread12((uint32_t)yt) << 1);
+  key[t] = ((uint64_t)dbin << 24) | mort;
 }
 
 __global__ void k_invert(size_t n, const uint32_t* __restrict__ order,
@@ -91,6 +135,8 @@ __global__ void k_invert(size_t n, const uint32_t* __restrict__ order,
   if (r < n) rank[order[r]] = (uint32_t)r;
 }
 
+// Track-major segments: key (brick, rank), first entry, length, tail after
+// the last entry (FP64), track.
 __global__ void __launch_bounds__(256)
 k_seg_fill(BrickGeo G, size_t n_trk, size_t nv,
            const uint64_t* __restrict__ row_ptr,
@@ -98,7 +144,7 @@ k_seg_fill(BrickGeo G, size_t n_trk, size_t nv,
            const uint64_t* __restrict__ sstart,
            const uint32_t* __restrict__ rank, uint64_t* __restrict__ skey,
            uint32_t* __restrict__ sval, uint64_t* __restrict__ st_start,
-           uint32_t* __restrict__ st_len, float* __restrict__ st_tend,
+           uint32_t* __restrict__ st_len, double* __restrict__ st_tend,
            uint32_t* __restrict__ st_trk) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_trk) return;
@@ -106,7 +152,7 @@ k_seg_fill(BrickGeo G, size_t n_trk, size_t nv,
   uint64_t s = sstart[t];
   uint32_t prev = 0xFFFFFFFFu;
   const uint32_t r = rank[t];
-  uint64_t cur = ~0ull;  // current segment id
+  uint64_t cur = ~0ull;
   for (uint64_t k = b; k < e; ++k) {
     const uint32_t j = vox[k];
     if (j >= nv) { prev = 0xFFFFFFFFu; continue; }
@@ -120,26 +166,69 @@ k_seg_fill(BrickGeo G, size_t n_trk, size_t nv,
       st_trk[cur] = (uint32_t)t;
     }
     st_len[cur] += 1;
-    st_tend[cur] = (float)tail[k];
+    st_tend[cur] = tail[k];
     prev = br;
   }
 }
 
+// lengths in brick-major order and the first segment of each brick
+__global__ void __launch_bounds__(256)
+k_seg_len(uint64_t n_seg, const uint32_t* __restrict__ sval,
+          const uint32_t* __restrict__ st_len, const uint64_t* __restrict__ skey,
+          uint32_t* __restrict__ lenq, uint64_t* __restrict__ bstart) {
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_seg) return;
+  lenq[q] = st_len[sval[q]];
+  const uint32_t b = (uint32_t)(skey[q] >> 32);
+  if (q == 0 || (uint32_t)(skey[q - 1] >> 32) != b) bstart[b] = q;
+}
+
+// Lane -> segment permutation of each chunk: its segments by length,
+// longest first (stable); one block per chunk.
+__global__ void __launch_bounds__(256)
+k_chunk_sort(const Chunk* __restrict__ chunks, const uint32_t* __restrict__ lenq,
+             uint8_t* __restrict__ perm) {
+  __shared__ uint32_t key[BLOCK];
+  const Chunk ch = chunks[blockIdx.x];
+  const uint32_t t = threadIdx.x;
+  // key: (ECAP - len) << 8 | position; ascending = length desc, stable
+  key[t] = t < ch.cnt ? ((ECAP - lenq[ch.q0 + t]) << 8) | t : 0xFFFFFFFFu;
+  __syncthreads();
+  for (uint32_t k = 2; k <= BLOCK; k <<= 1) {
+    for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+      const uint32_t l = t ^ jj;
+      if (l > t) {
+        const uint32_t a = key[t], b = key[l];
+        const bool up = (t & k) == 0;
+        if (up ? (b < a) : (a < b)) { key[t] = b; key[l] = a; }
+      }
+      __syncthreads();
+    }
+  }
+  if (t < ch.cnt) perm[ch.q0 + t] = (uint8_t)(key[t] & 0xFFu);
+}
+
 __global__ void __launch_bounds__(256)
 k_seg_gather(uint64_t n_seg, const uint32_t* __restrict__ sval,
-             const uint32_t* __restrict__ st_len,
-             const float* __restrict__ st_tend,
+             const double* __restrict__ st_tend,
              const uint32_t* __restrict__ st_trk,
-             const uint32_t* __restrict__ rank, const uint64_t* __restrict__ skey,
-             uint32_t* __restrict__ seg_trk, float* __restrict__ seg_tend,
-             uint32_t* __restrict__ lenq, uint32_t* __restrict__ bcount) {
+             const uint32_t* __restrict__ rank,
+             uint32_t* __restrict__ seg_trk, double* __restrict__ seg_tend) {
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_seg) return;
   const uint32_t s = sval[q];
   seg_trk[q] = rank[st_trk[s]];
   seg_tend[q] = st_tend[s];
-  lenq[q] = st_len[s];
-  atomicAdd(&bcount[(uint32_t)(skey[q] >> 32)], 1u);
+}
+
+__global__ void k_chunk_final(uint32_t n, Chunk* __restrict__ chunks,
+                              const uint64_t* __restrict__ seg_off) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  Chunk ch = chunks[c];
+  ch.base = seg_off[ch.q0];
+  ch.ne = (uint32_t)(seg_off[ch.q0 + ch.cnt] - ch.base);
+  chunks[c] = ch;
 }
 
 __global__ void __launch_bounds__(256)
@@ -154,7 +243,7 @@ k_ent_scatter(BrickGeo G, uint64_t n_seg, const uint32_t* __restrict__ sval,
   const uint32_t s = sval[q];
   const uint64_t src = st_start[s];
   const uint32_t n = st_len[s];
-  uint64_t dst = seg_off[q];
+  const uint64_t dst = seg_off[q];
   // entries with j >= nv end segments, so a segment is contiguous in the
   // track-major list
   for (uint32_t k = 0; k < n; ++k) {
@@ -165,6 +254,7 @@ k_ent_scatter(BrickGeo G, uint64_t n_seg, const uint32_t* __restrict__ sval,
   }
 }
 
+// Brick-major position of each track's segments, listed in rank order.
 __global__ void __launch_bounds__(256)
 k_tseg_fill(uint64_t n_seg, const uint32_t* __restrict__ sval,
             const uint32_t* __restrict__ st_trk,
@@ -179,15 +269,23 @@ k_tseg_fill(uint64_t n_seg, const uint32_t* __restrict__ sval,
   tseg_pos[tseg_ptr[rank[t]] + (uint32_t)(s - sstart[t])] = (uint32_t)q;
 }
 
+// Per-track data in rank order.  The initial basis offset is half the
+// track's path length (the tail of its first entry plus that entry).
 __global__ void k_trk_d(size_t n_trk, const uint32_t* __restrict__ order,
                         const double* __restrict__ trk4,
                         const double* __restrict__ pr,
                         const uint32_t* __restrict__ nseg,
+                        const uint64_t* __restrict__ row_ptr,
+                        const double* __restrict__ len,
+                        const double* __restrict__ tail,
                         double* __restrict__ trk_d,
-                        uint32_t* __restrict__ nseg_rank) {
+                        uint32_t* __restrict__ nseg_rank,
+                        float* __restrict__ trk_tc) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_trk) return;
   const uint32_t t = order[r];
+  const uint64_t b = row_ptr[t];
+  trk_tc[r] = row_ptr[t + 1] > b ? (float)(0.5 * (tail[b] + len[b])) : 0.0f;
   const double Sx = trk4[4 * t], Sy = trk4[4 * t + 1];
   const double Dx = trk4[4 * t + 2], Dy = trk4[4 * t + 3];
   trk_d[4 * r] = Sx * Sx + Sy * Sy;
@@ -195,15 +293,6 @@ __global__ void k_trk_d(size_t n_trk, const uint32_t* __restrict__ order,
   trk_d[4 * r + 2] = Dx * Dx + Dy * Dy;
   trk_d[4 * r + 3] = pr[t];
   nseg_rank[r] = nseg[t];
-}
-
-__global__ void k_brick_entries(uint64_t n_seg, const uint64_t* __restrict__ skey,
-                                const uint64_t* __restrict__ seg_off,
-                                unsigned long long* __restrict__ bent) {
-  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n_seg) return;
-  atomicAdd(&bent[(uint32_t)(skey[q] >> 32)],
-            (unsigned long long)(seg_off[q + 1] - seg_off[q]));
 }
 
 // ---------------- iteration --------------------------------------------
@@ -215,16 +304,18 @@ __device__ __forceinline__ void two_sum_acc(float& hi, float& lo, float x) {
   lo += e;
 }
 
-
 struct IterArgs {
   BrickGeo G;
   const Task* tasks;
+  const Chunk* chunks;
   uint32_t n_tasks;
   unsigned int* work;
   uint64_t n_seg;
   const uint64_t* seg_off;
-  const float* seg_tend;
+  const double* seg_tend;
   const uint32_t* seg_trk;
+  const float* trk_tc;
+  const uint8_t* perm;
   const uint16_t* ent_vox;
   const float* ent_len;
 };
@@ -256,66 +347,167 @@ __device__ __forceinline__ void load_brick(const BrickGeo& G, uint32_t brick,
   }
 }
 
-// Forward: per segment, sum_j lambda_j W_ij (three components), walking
-// the segment backwards so the tail T grows by each entry's length.
-__global__ void __launch_bounds__(256)
-k_fwd32(IterArgs a, const float* __restrict__ lam, float* __restrict__ part) {
-  extern __shared__ float lds_lam[];
-  __shared__ uint32_t task_s;
+// Register staging of one chunk's entries (PF per thread, coalesced).
+struct Stage {
+  uint16_t v[PF];
+  float l[PF];
+  __device__ __forceinline__ void fetch(const IterArgs& a, const Chunk& ch) {
+#pragma unroll
+    for (uint32_t i = 0; i < PF; ++i) {
+      const uint32_t k = threadIdx.x + BLOCK * i;
+      if (k < ch.ne) {
+        v[i] = a.ent_vox[ch.base + k];
+        l[i] = a.ent_len[ch.base + k];
+      }
+    }
+  }
+  __device__ __forceinline__ void commit(const Chunk& ch, uint16_t* sv, float* sl) {
+#pragma unroll
+    for (uint32_t i = 0; i < PF; ++i) {
+      const uint32_t k = threadIdx.x + BLOCK * i;
+      if (k < ch.ne) {
+        sv[k] = v[i];
+        sl[k] = l[i];
+      }
+    }
+  }
+};
+
+// Forward: per segment, sum_j lambda_j W_ij (three components) with
+// lambda staged in LDS; the walk goes backwards so the tail grows by each
+// entry's length.  Tails are in the track's shifted basis t' = t - t_c.
+template <int ACC>
+__global__ void __launch_bounds__(BLOCK)
+k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
+  // static LDS is a multiple of 16 bytes and the dynamic region is declared
+  // as 16-byte aligned doubles, so the brick arrays start 16-byte aligned
+  extern __shared__ __align__(16) double lds_dyn[];
+  float* lds_lam = reinterpret_cast<float*>(lds_dyn);
+  __shared__ float sl[ECAP];
+  __shared__ uint16_t sv[ECAP];
+  __shared__ uint32_t sh_[4];
+  uint32_t& sh = sh_[0];
+  Stage st;
   for (;;) {
-    if (threadIdx.x == 0) task_s = atomicAdd(&a.work[0], 1u);
+    if (threadIdx.x == 0) sh = atomicAdd(&a.work[0], 1u);
     __syncthreads();
-    const uint32_t ti = task_s;
+    const uint32_t ti = sh;
     if (ti >= a.n_tasks) break;
     const Task tk = a.tasks[ti];
     load_brick(a.G, tk.brick, lam, lds_lam);
-    __syncthreads();
-    for (uint64_t q = tk.s0 + threadIdx.x; q < tk.s1; q += blockDim.x) {
-      const uint64_t o0 = a.seg_off[q], o1 = a.seg_off[q + 1];
-      float T = a.seg_tend[q];
-      float h0 = 0.f, l0 = 0.f, h1 = 0.f, l1 = 0.f, h2 = 0.f, l2 = 0.f;
-      for (uint64_t k = o1; k > o0; --k) {
-        const float L = a.ent_len[k - 1];
-        const float lj = lds_lam[a.ent_vox[k - 1]];
-        const float LT = L * T;
-        const float w01 = LT + L * L * 0.5f;
-        const float w11 = L * L * L * (1.0f / 3.0f) + (L + T) * LT;
-        two_sum_acc(h0, l0, lj * L);
-        two_sum_acc(h1, l1, lj * w01);
-        two_sum_acc(h2, l2, lj * w11);
-        T += L;
+    Chunk cur = a.chunks[tk.c0];
+    st.fetch(a, cur);
+    for (uint32_t c = tk.c0; c < tk.c1; ++c) {
+      st.commit(cur, sv, sl);
+      // this lane's segment metadata, read before the next prefetch is
+      // issued so the walk does not wait on it
+      const bool act = threadIdx.x < cur.cnt;
+      const uint64_t q = cur.q0 + (act ? a.perm[cur.q0 + threadIdx.x] : 0u);
+      uint32_t lo = 0, hi = 0;
+      float T = 0.f;
+      if (act) {
+        lo = (uint32_t)(a.seg_off[q] - cur.base);
+        hi = (uint32_t)(a.seg_off[q + 1] - cur.base);
+        T = (float)(a.seg_tend[q] - (double)a.trk_tc[a.seg_trk[q]]);
       }
-      part[q] = h0 + l0;
-      part[a.n_seg + q] = h1 + l1;
-      part[2 * a.n_seg + q] = h2 + l2;
+      Chunk nxt = cur;
+      if (c + 1 < tk.c1) {
+        nxt = a.chunks[c + 1];
+        st.fetch(a, nxt);
+      }
+      __syncthreads();
+      if (act) {
+        if (ACC == 1) {
+          double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+          auto step = [&](float L, float lj) {
+            const float LT = L * T;
+            const float w01 = fmaf(0.5f * L, L, LT);
+            const float w11 = fmaf(L * L, L * (1.0f / 3.0f), (L + T) * LT);
+            a0 += (double)(lj * L);
+            a1 += (double)(lj * w01);
+            a2 += (double)(lj * w11);
+            T += L;
+          };
+          uint32_t k = hi;
+          for (; k >= lo + 4; k -= 4) {
+            const float La = sl[k - 1], Lb = sl[k - 2], Lc = sl[k - 3], Ld = sl[k - 4];
+            const float xa = lds_lam[sv[k - 1]], xb = lds_lam[sv[k - 2]];
+            const float xc = lds_lam[sv[k - 3]], xd = lds_lam[sv[k - 4]];
+            step(La, xa);
+            step(Lb, xb);
+            step(Lc, xc);
+            step(Ld, xd);
+          }
+          for (; k > lo; --k) step(sl[k - 1], lds_lam[sv[k - 1]]);
+          part[q] = make_float4((float)a0, (float)a1, (float)a2, 0.0f);
+        } else {
+          float h0 = 0.f, l0 = 0.f, h1 = 0.f, l1 = 0.f, h2 = 0.f, l2 = 0.f;
+          for (uint32_t k = hi; k > lo; --k) {
+            const float L = sl[k - 1];
+            const float lj = lds_lam[sv[k - 1]];
+            const float LT = L * T;
+            const float w01 = LT + L * L * 0.5f;
+            const float w11 = L * L * L * (1.0f / 3.0f) + (L + T) * LT;
+            two_sum_acc(h0, l0, lj * L);
+            two_sum_acc(h1, l1, lj * w01);
+            two_sum_acc(h2, l2, lj * w11);
+            T += L;
+          }
+          part[q] = make_float4(h0 + l0, h1 + l1, h2 + l2, 0.0f);
+        }
+      }
+      cur = nxt;
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
 // Per track (rank order): Sigma from segment partials, 2x2 algebra in FP64.
+// The partials were summed in the track's shifted basis u'(t) = (1, t - t_c):
+// with M = [[1, 0], [t_c, 1]], Sigma = M Sigma' M^T, so E and D enter as
+// M^-1 E M^-T and M^-1 D M^-T; det Sigma = det Sigma', cost and
+// <W, G> = <W', G'> are unchanged.  Choosing t_c at the track's scattering
+// centroid makes Sigma' nearly diagonal, so FP32 partial sums keep the
+// determinant and G' accurate where Sigma itself is strongly correlated.
+// The next iteration's t_c is the centroid of this one: t_c + s01 / s00.
 __global__ void __launch_bounds__(256)
 k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
-          const uint32_t* __restrict__ tseg_pos, uint64_t n_seg,
-          const float* __restrict__ part, const double* __restrict__ trk_d,
-          ErrM E, float* __restrict__ trk_g, double* __restrict__ cost_acc,
-          unsigned long long* __restrict__ cnt) {
+          const uint32_t* __restrict__ tseg_pos,
+          const float4* __restrict__ part, const double* __restrict__ trk_d,
+          ErrM E, float* __restrict__ trk_tc, float* __restrict__ trk_g,
+          double* __restrict__ cost_acc, unsigned long long* __restrict__ cnt) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   double cost = 0.0;
   bool sing = false;
   if (r < n_trk) {
+    // this track's segment partials: brick-major positions in track order;
+    // neighbouring tracks (adjacent ranks) read neighbouring positions
     double s00 = 0.0, s01 = 0.0, s11 = 0.0;
-    for (uint32_t k = tseg_ptr[r]; k < tseg_ptr[r + 1]; ++k) {
-      const uint32_t q = tseg_pos[k];
-      s00 += (double)part[q];
-      s01 += (double)part[n_seg + q];
-      s11 += (double)part[2 * n_seg + q];
+    const uint32_t k0 = tseg_ptr[r], k1 = tseg_ptr[r + 1];
+    uint32_t k = k0;
+    for (; k + 4 <= k1; k += 4) {
+      const uint32_t qa = tseg_pos[k], qb = tseg_pos[k + 1], qc = tseg_pos[k + 2], qd = tseg_pos[k + 3];
+      const float4 pa = part[qa], pb = part[qb], pc = part[qc], pd = part[qd];
+      s00 += ((double)pa.x + (double)pb.x) + ((double)pc.x + (double)pd.x);
+      s01 += ((double)pa.y + (double)pb.y) + ((double)pc.y + (double)pd.y);
+      s11 += ((double)pa.z + (double)pb.z) + ((double)pc.z + (double)pd.z);
     }
+    for (; k < k1; ++k) {
+      const float4 pa = part[tseg_pos[k]];
+      s00 += (double)pa.x;
+      s01 += (double)pa.y;
+      s11 += (double)pa.z;
+    }
+    const float tcf = trk_tc[r];
+    const double tc = (double)tcf;
     const double D00 = trk_d[4 * r], D01 = trk_d[4 * r + 1];
     const double D11 = trk_d[4 * r + 2], p = trk_d[4 * r + 3];
+    // shifted-basis measurement and error matrices
+    const double d00 = D00, d01 = D01 - tc * D00, d11 = D11 - 2.0 * tc * D01 + tc * tc * D00;
+    const double e00 = E.e00, e01 = E.e01 - tc * E.e00;
+    const double e11 = E.e11 - 2.0 * tc * E.e01 + tc * tc * E.e00;
     const double pp = p * p;
-    const double m00 = pp * s00 + E.e00, m01 = pp * s01 + E.e01;
-    const double m11 = pp * s11 + E.e11;
+    const double m00 = pp * s00 + e00, m01 = pp * s01 + e01, m11 = pp * s11 + e11;
     const double det = m00 * m11 - m01 * m01;
     double i00 = 0, i01 = 0, i11 = 0;
     const bool ok = fabs(det) > 1e-12;
@@ -328,18 +520,21 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
     if (!ok || det <= 0) {
       sing = true;
     } else {
-      cost = log(det) + (D00 * i00 + D11 * i11 + D01 * 2.0 * i01) * 0.5;
+      cost = log(det) + (d00 * i00 + d11 * i11 + d01 * 2.0 * i01) * 0.5;
     }
-    // G = 1/2 Si^-1 D Si^-1 - Si^-1  (symmetric)
-    const double A00 = i00 * D00 + i01 * D01, A01 = i00 * D01 + i01 * D11;
-    const double A10 = i01 * D00 + i11 * D01, A11 = i01 * D01 + i11 * D11;
+    // G' = 1/2 Si'^-1 D' Si'^-1 - Si'^-1  (symmetric)
+    const double A00 = i00 * d00 + i01 * d01, A01 = i00 * d01 + i01 * d11;
+    const double A10 = i01 * d00 + i11 * d01, A11 = i01 * d01 + i11 * d11;
     const double M00 = A00 * i00 + A01 * i01;
     const double M01 = A00 * i01 + A01 * i11;
     const double M11 = A10 * i01 + A11 * i11;
-    trk_g[4 * r] = (float)(pp * (0.5 * M00 - i00));
-    trk_g[4 * r + 1] = (float)(pp * (0.5 * M01 - i01));
-    trk_g[4 * r + 2] = (float)(pp * (0.5 * M11 - i11));
-    trk_g[4 * r + 3] = 0.0f;
+    float4 g;
+    g.x = (float)(pp * (0.5 * M00 - i00));
+    g.y = (float)(pp * (0.5 * M01 - i01));
+    g.z = (float)(pp * (0.5 * M11 - i11));
+    g.w = tcf;
+    *reinterpret_cast<float4*>(trk_g + 4 * r) = g;
+    if (s00 > 0.0) trk_tc[r] = (float)(tc + s01 / s00);
   }
   double v = cost;
 #pragma unroll
@@ -350,45 +545,94 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
     atomicAdd(&cnt[C_SINGULAR], (unsigned long long)__popcll(m));
 }
 
-// Backprojection: c_ij = <W_ij, G_i> accumulated per voxel in LDS with an
-// atomic two-sum: the returned old value o and the stored o + c give the
-// exact rounding error, which goes into the low word.
-__global__ void __launch_bounds__(256)
+// Backprojection: c_ij = <W_ij, G_i> accumulated per voxel in LDS, then
+// flushed once per task (stored when the task owns the brick, else added).
+// ACC = 1: one FP64 LDS atomic per entry;
+// ACC = 0: FP32 atomic two-sum -- the returned old value o and the stored
+//          o + c give the exact rounding error, added into a low word.
+template <int ACC>
+__global__ void __launch_bounds__(BLOCK)
 k_bwd32(IterArgs a, const float* __restrict__ trk_g,
         double* __restrict__ acc) {
-  extern __shared__ float lds_acc[];
-  __shared__ uint32_t task_s;
+  // static LDS is a multiple of 16 bytes and the dynamic region is declared
+  // as 16-byte aligned doubles: ds_add_f64 needs 8-byte aligned addresses
+  extern __shared__ __align__(16) double lds_dyn[];
+  float* lds_acc = reinterpret_cast<float*>(lds_dyn);
+  __shared__ float sl[ECAP];
+  __shared__ uint16_t sv[ECAP];
+  __shared__ uint32_t sh_[4];
+  uint32_t& sh = sh_[0];
   const uint32_t vol = a.G.b[0] * a.G.b[1] * a.G.b[2];
-  float* hi = lds_acc;
-  float* lo = lds_acc + vol;
+  float* hi_s = lds_acc;
+  float* lo_s = lds_acc + vol;
+  double* d_s = lds_dyn;
+  Stage st;
   for (;;) {
-    if (threadIdx.x == 0) task_s = atomicAdd(&a.work[1], 1u);
+    if (threadIdx.x == 0) sh = atomicAdd(&a.work[1], 1u);
     for (uint32_t v = threadIdx.x; v < 2 * vol; v += blockDim.x) lds_acc[v] = 0.0f;
     __syncthreads();
-    const uint32_t ti = task_s;
+    const uint32_t ti = sh;
     if (ti >= a.n_tasks) break;
     const Task tk = a.tasks[ti];
-    for (uint64_t q = tk.s0 + threadIdx.x; q < tk.s1; q += blockDim.x) {
-      const uint64_t o0 = a.seg_off[q], o1 = a.seg_off[q + 1];
-      const uint32_t r = a.seg_trk[q];
-      const float4 g = *reinterpret_cast<const float4*>(trk_g + 4 * (size_t)r);
-      float T = a.seg_tend[q];
-      for (uint64_t k = o1; k > o0; --k) {
-        const float L = a.ent_len[k - 1];
-        const uint32_t v = a.ent_vox[k - 1];
-        const float LT = L * T;
-        const float w01 = LT + L * L * 0.5f;
-        const float w11 = L * L * L * (1.0f / 3.0f) + (L + T) * LT;
-        const float c = L * g.x + 2.0f * w01 * g.y + w11 * g.z;
-        const float old = atomicAdd(&hi[v], c);
-        const float s = old + c;
-        const float bp = s - old;
-        const float e = (old - (s - bp)) + (c - bp);
-        if (e != 0.0f) atomicAdd(&lo[v], e);
-        T += L;
+    Chunk cur = a.chunks[tk.c0];
+    st.fetch(a, cur);
+    for (uint32_t c = tk.c0; c < tk.c1; ++c) {
+      st.commit(cur, sv, sl);
+      const bool act = threadIdx.x < cur.cnt;
+      const uint64_t q = cur.q0 + (act ? a.perm[cur.q0 + threadIdx.x] : 0u);
+      uint32_t lo = 0, hi = 0;
+      float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+      double tend = 0.0;
+      if (act) {
+        lo = (uint32_t)(a.seg_off[q] - cur.base);
+        hi = (uint32_t)(a.seg_off[q + 1] - cur.base);
+        tend = a.seg_tend[q];
+        g = *reinterpret_cast<const float4*>(trk_g + 4 * (size_t)a.seg_trk[q]);
       }
+      Chunk nxt = cur;
+      if (c + 1 < tk.c1) {
+        nxt = a.chunks[c + 1];
+        st.fetch(a, nxt);
+      }
+      __syncthreads();
+      if (act) {
+        float T = (float)(tend - (double)g.w);
+        auto coef = [&](float L) {
+          const float LT = L * T;
+          const float w01 = fmaf(0.5f * L, L, LT);
+          const float w11 = fmaf(L * L, L * (1.0f / 3.0f), (L + T) * LT);
+          T += L;
+          return fmaf(L, g.x, fmaf(2.0f * w01, g.y, w11 * g.z));
+        };
+        uint32_t k = hi;
+        if (ACC == 1) {
+          for (; k >= lo + 4; k -= 4) {
+            const float La = sl[k - 1], Lb = sl[k - 2], Lc = sl[k - 3], Ld = sl[k - 4];
+            const uint32_t va = sv[k - 1], vb = sv[k - 2], vc = sv[k - 3], vd = sv[k - 4];
+            const float ca = coef(La), cb = coef(Lb), cc = coef(Lc), cd = coef(Ld);
+            atomicAdd(&d_s[va], (double)ca);
+            atomicAdd(&d_s[vb], (double)cb);
+            atomicAdd(&d_s[vc], (double)cc);
+            atomicAdd(&d_s[vd], (double)cd);
+          }
+          for (; k > lo; --k) atomicAdd(&d_s[sv[k - 1]], (double)coef(sl[k - 1]));
+        } else {
+          auto add = [&](uint32_t v, float cv, float old) {
+            const float sum = old + cv;
+            const float bp = sum - old;
+            const float e = (old - (sum - bp)) + (cv - bp);
+            if (e != 0.0f) atomicAdd(&lo_s[v], e);
+          };
+          for (; k > lo; --k) {
+            const uint32_t v = sv[k - 1];
+            const float cv = coef(sl[k - 1]);
+            add(v, cv, atomicAdd(&hi_s[v], cv));
+          }
+        }
+      }
+      cur = nxt;
+      __syncthreads();
     }
-    __syncthreads();
     uint32_t o[3];
     brick_origin(a.G, tk.brick, o);
     for (uint32_t v = threadIdx.x; v < vol; v += blockDim.x) {
@@ -397,7 +641,7 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
       const uint32_t x = o[0] + lx, y = o[1] + ly, z = o[2] + lz;
       if (x >= a.G.n[0] || y >= a.G.n[1] || z >= a.G.n[2]) continue;
       const size_t j = (size_t)x + (size_t)a.G.n[0] * ((size_t)y + (size_t)a.G.n[1] * z);
-      const double val = (double)hi[v] + (double)lo[v];
+      const double val = ACC == 1 ? d_s[v] : (double)hi_s[v] + (double)lo_s[v];
       if (tk.owner)
         acc[j] = val;
       else if (val != 0.0)
@@ -454,12 +698,15 @@ static IterArgs iter_args(muto_em_plan* pl) {
   IterArgs a;
   a.G = brick_geo(pl);
   a.tasks = f.tasks.p;
+  a.chunks = f.chunks.p;
   a.n_tasks = f.n_tasks;
   a.work = pl->work.p;
   a.n_seg = f.n_seg;
   a.seg_off = f.seg_off.p;
   a.seg_tend = f.seg_tend.p;
   a.seg_trk = f.seg_trk.p;
+  a.trk_tc = f.trk_tc.p;
+  a.perm = f.perm.p;
   a.ent_vox = f.ent_vox.p;
   a.ent_len = f.ent_len.p;
   return a;
@@ -478,11 +725,23 @@ static int bits_for(uint64_t v) {
   return b;
 }
 
+template <int ACC>
+static hipError_t occupancy(int* f, int* b, size_t lds_f, size_t lds_b) {
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(f, k_fwd32<ACC>, BLOCK, lds_f);
+  if (e != hipSuccess) return e;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(b, k_bwd32<ACC>, BLOCK, lds_b);
+}
+
 int fp32c_build(muto_em_plan* pl) {
   Fp32cData& f = pl->f32;
   if (f.built) return MUTO_OK;
   muto_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream;
+  {
+    const char* ev = getenv("MUTO_FP32C_ACC");
+    if (ev) f.acc_mode = atoi(ev);
+    if (f.acc_mode != 0 && f.acc_mode != 1) f.acc_mode = 1;
+  }
   const BrickGeo G = brick_geo(pl);
   f.nb[0] = G.nb[0]; f.nb[1] = G.nb[1]; f.nb[2] = G.nb[2];
   const uint64_t nbk = (uint64_t)G.nb[0] * G.nb[1] * G.nb[2];
@@ -507,10 +766,19 @@ int fp32c_build(muto_em_plan* pl) {
   MUTO_CHECK_HIP(ctx, hipMemcpy(&n_seg, sstart.p + nt, sizeof n_seg, hipMemcpyDeviceToHost));
   if (n_seg >= 0xFFFFFFFFull) { ctx->set_error("more than 2^32 segments"); return MUTO_ERR_LIMIT; }
   f.n_seg = n_seg;
+  // occupancy-derived persistent grids
+  const size_t lds_f = f.bs.vol() * sizeof(float), lds_b = 2 * f.bs.vol() * sizeof(float);
+  int occ_f = 1, occ_b = 1;
+  if (f.acc_mode == 1)
+    MUTO_CHECK_HIP(ctx, occupancy<1>(&occ_f, &occ_b, lds_f, lds_b));
+  else
+    MUTO_CHECK_HIP(ctx, occupancy<0>(&occ_f, &occ_b, lds_f, lds_b));
+  f.grid_fwd = std::max(1, occ_f) * ctx->n_cu;
+  f.grid_bwd = std::max(1, occ_b) * ctx->n_cu;
   {
-    DBuf<uint64_t> skey, st_start;
-    DBuf<uint32_t> sval, st_len, st_trk, lenq, bcount;
-    DBuf<float> st_tend;
+    DBuf<uint64_t> skey, st_start, bstart;
+    DBuf<uint32_t> sval, st_len, st_trk, lenq;
+    DBuf<double> st_tend;
     MUTO_CHECK_HIP(ctx, skey.alloc(n_seg));
     MUTO_CHECK_HIP(ctx, sval.alloc(n_seg));
     MUTO_CHECK_HIP(ctx, st_start.alloc(n_seg));
@@ -522,38 +790,125 @@ int fp32c_build(muto_em_plan* pl) {
                                                  st_start.p, st_len.p, st_tend.p, st_trk.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     MUTO_CHECK_HIP(ctx, sort_pairs_u64_u32(skey.p, sval.p, n_seg, 0, 32 + bits_for(nbk), s));
+    // lengths (brick-major) and brick starts
+    MUTO_CHECK_HIP(ctx, lenq.alloc(n_seg + 1));
+    MUTO_CHECK_HIP(ctx, bstart.alloc(nbk + 1));
+    MUTO_CHECK_HIP(ctx, hipMemsetAsync(bstart.p, 0xFF, (nbk + 1) * sizeof(uint64_t), s));
+    if (n_seg) {
+      k_seg_len<<<grid_for(n_seg, 256), 256, 0, s>>>(n_seg, sval.p, st_len.p, skey.p, lenq.p, bstart.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    }
+    std::vector<uint32_t> hl(n_seg);
+    std::vector<uint64_t> hb(nbk + 1);
+    MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    if (n_seg) MUTO_CHECK_HIP(ctx, hipMemcpy(hl.data(), lenq.p, n_seg * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    MUTO_CHECK_HIP(ctx, hipMemcpy(hb.data(), bstart.p, (nbk + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    hb[nbk] = n_seg;
+    for (uint64_t b = nbk; b-- > 0;)
+      if (hb[b] == ~0ull) hb[b] = hb[b + 1];  // empty brick
+    uint64_t n_ent = 0;
+    uint32_t max_len = 0;
+    for (uint32_t v : hl) { n_ent += v; max_len = std::max(max_len, v); }
+    if (max_len > ECAP) {
+      ctx->set_error("a brick segment is longer than the staging capacity");
+      return MUTO_ERR_LIMIT;
+    }
+    // tasks: bricks cut into runs of about `target` entries (at least 16 per
+    // brick voxel so the flush is amortised; small enough for the queue to
+    // balance); tasks into chunks of <= BLOCK segments and <= ECAP entries
+    const uint64_t workers = (uint64_t)std::max(f.grid_fwd, f.grid_bwd);
+    const uint64_t target = std::max<uint64_t>(16ull * f.bs.vol(), n_ent / (4 * workers) + 1);
+    std::vector<Task> tasks;
+    std::vector<uint64_t> tent;
+    std::vector<Chunk> chunks;
+    for (uint64_t b = 0; b < nbk; ++b) {
+      const uint64_t s0 = hb[b], s1 = hb[b + 1];
+      if (s1 == s0) continue;
+      uint64_t eb = 0;
+      for (uint64_t q = s0; q < s1; ++q) eb += hl[q];
+      const uint64_t k = std::max<uint64_t>(1, (eb + target - 1) / target);
+      const uint64_t per = (eb + k - 1) / k;
+      uint64_t q = s0;
+      std::vector<Task> bt;
+      while (q < s1) {
+        Task tk;
+        tk.brick = (uint32_t)b;
+        tk.c0 = (uint32_t)chunks.size();
+        uint64_t te = 0;
+        while (q < s1 && (te < per || bt.size() + 1 == k)) {
+          Chunk ch{};
+          ch.q0 = q;
+          uint32_t ce = 0, cn = 0;
+          while (q < s1 && cn < BLOCK && ce + hl[q] <= ECAP) { ce += hl[q]; ++cn; ++q; }
+          ch.cnt = cn;
+          ch.ne = ce;
+          chunks.push_back(ch);
+          te += ce;
+        }
+        tk.c1 = (uint32_t)chunks.size();
+        bt.push_back(tk);
+        tent.push_back(te);
+      }
+      for (Task& tk : bt) {
+        tk.owner = bt.size() == 1 ? 1u : 0u;
+        tasks.push_back(tk);
+      }
+    }
+    if (chunks.size() >= 0xFFFFFFFFull) { ctx->set_error("too many chunks"); return MUTO_ERR_LIMIT; }
+    std::vector<size_t> idx(tasks.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return tent[x] > tent[y]; });
+    std::vector<Task> sorted(tasks.size());
+    for (size_t i = 0; i < idx.size(); ++i) sorted[i] = tasks[idx[i]];
+    f.n_tasks = (uint32_t)sorted.size();
+    f.n_chunks = (uint32_t)chunks.size();
+    MUTO_CHECK_HIP(ctx, f.tasks.alloc(sorted.size()));
+    MUTO_CHECK_HIP(ctx, f.chunks.alloc(chunks.size()));
+    if (!sorted.empty())
+      MUTO_CHECK_HIP(ctx, hipMemcpy(f.tasks.p, sorted.data(), sorted.size() * sizeof(Task), hipMemcpyHostToDevice));
+    if (!chunks.empty())
+      MUTO_CHECK_HIP(ctx, hipMemcpy(f.chunks.p, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice));
+    // lane -> segment permutation of every chunk (length order)
+    MUTO_CHECK_HIP(ctx, f.perm.alloc(n_seg));
+    if (f.n_chunks) {
+      k_chunk_sort<<<f.n_chunks, BLOCK, 0, s>>>(f.chunks.p, lenq.p, f.perm.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    }
+    // brick-major segment arrays in final order
     MUTO_CHECK_HIP(ctx, f.seg_trk.alloc(n_seg));
     MUTO_CHECK_HIP(ctx, f.seg_tend.alloc(n_seg));
     MUTO_CHECK_HIP(ctx, f.seg_off.alloc(n_seg + 1));
-    MUTO_CHECK_HIP(ctx, lenq.alloc(n_seg + 1));
-    MUTO_CHECK_HIP(ctx, bcount.alloc(nbk + 1));
-    MUTO_CHECK_HIP(ctx, hipMemsetAsync(lenq.p, 0, (n_seg + 1) * sizeof(uint32_t), s));
-    MUTO_CHECK_HIP(ctx, hipMemsetAsync(bcount.p, 0, (nbk + 1) * sizeof(uint32_t), s));
     if (n_seg) {
-      k_seg_gather<<<grid_for(n_seg, 256), 256, 0, s>>>(n_seg, sval.p, st_len.p, st_tend.p, st_trk.p,
-                                                        rank.p, skey.p, f.seg_trk.p, f.seg_tend.p,
-                                                        lenq.p, bcount.p);
+      k_seg_gather<<<grid_for(n_seg, 256), 256, 0, s>>>(n_seg, sval.p, st_tend.p, st_trk.p, rank.p,
+                                                        f.seg_trk.p, f.seg_tend.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
+    MUTO_CHECK_HIP(ctx, hipMemsetAsync(lenq.p + n_seg, 0, sizeof(uint32_t), s));
     MUTO_CHECK_HIP(ctx, exclusive_scan_u32_to_u64(lenq.p, f.seg_off.p, n_seg + 1, s));
-    MUTO_CHECK_HIP(ctx, hipMemcpy(&f.n_ent, f.seg_off.p + n_seg, sizeof(uint64_t), hipMemcpyDeviceToHost));
-    MUTO_CHECK_HIP(ctx, f.ent_vox.alloc(f.n_ent));
-    MUTO_CHECK_HIP(ctx, f.ent_len.alloc(f.n_ent));
+    f.n_ent = n_ent;
+    if (f.n_chunks) {
+      k_chunk_final<<<grid_for(f.n_chunks, 256), 256, 0, s>>>(f.n_chunks, f.chunks.p, f.seg_off.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    }
+    MUTO_CHECK_HIP(ctx, f.ent_vox.alloc(n_ent));
+    MUTO_CHECK_HIP(ctx, f.ent_len.alloc(n_ent));
     if (n_seg) {
       k_ent_scatter<<<grid_for(n_seg, 256), 256, 0, s>>>(G, n_seg, sval.p, st_start.p, st_len.p,
                                                          f.seg_off.p, pl->vox.p, pl->len.p,
                                                          f.ent_vox.p, f.ent_len.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
-    // per-rank segment lists
+    // per-rank segment lists and per-track data
     DBuf<uint32_t> nseg_rank;
     DBuf<uint64_t> tptr64;
     MUTO_CHECK_HIP(ctx, nseg_rank.alloc(nt + 1));
     MUTO_CHECK_HIP(ctx, tptr64.alloc(nt + 1));
     MUTO_CHECK_HIP(ctx, f.trk_d.alloc(4 * nt));
+    MUTO_CHECK_HIP(ctx, f.trk_tc.alloc(nt));
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(nseg_rank.p, 0, (nt + 1) * sizeof(uint32_t), s));
     k_trk_d<<<grid_for(nt, 256), 256, 0, s>>>(nt, order.p, pl->trk4.p, pl->pr.p, nseg.p,
-                                              f.trk_d.p, nseg_rank.p);
+                                              pl->row_ptr.p, pl->len.p, pl->tail.p,
+                                              f.trk_d.p, nseg_rank.p, f.trk_tc.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     MUTO_CHECK_HIP(ctx, exclusive_scan_u32_to_u64(nseg_rank.p, tptr64.p, nt + 1, s));
     std::vector<uint64_t> h64(nt + 1);
@@ -567,66 +922,9 @@ int fp32c_build(muto_em_plan* pl) {
                                                        f.tseg_ptr.p, f.tseg_pos.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
-    // bricks -> tasks
-    DBuf<uint64_t> bstart;
-    DBuf<unsigned long long> bent;
-    MUTO_CHECK_HIP(ctx, bstart.alloc(nbk + 1));
-    MUTO_CHECK_HIP(ctx, bent.alloc(nbk));
-    MUTO_CHECK_HIP(ctx, exclusive_scan_u32_to_u64(bcount.p, bstart.p, nbk + 1, s));
-    MUTO_CHECK_HIP(ctx, hipMemsetAsync(bent.p, 0, nbk * sizeof(unsigned long long), s));
-    if (n_seg) {
-      k_brick_entries<<<grid_for(n_seg, 256), 256, 0, s>>>(n_seg, skey.p, f.seg_off.p, bent.p);
-      MUTO_CHECK_HIP(ctx, hipGetLastError());
-    }
-    MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
-    std::vector<uint64_t> hb(nbk + 1);
-    std::vector<unsigned long long> he(nbk);
-    MUTO_CHECK_HIP(ctx, hipMemcpy(hb.data(), bstart.p, (nbk + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
-    MUTO_CHECK_HIP(ctx, hipMemcpy(he.data(), bent.p, nbk * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    // occupancy-derived persistent grid
-    int occ_f = 1, occ_b = 1;
-    const size_t lds_f = f.bs.vol() * sizeof(float), lds_b = 2 * f.bs.vol() * sizeof(float);
-    MUTO_CHECK_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, k_fwd32, f.block, lds_f));
-    MUTO_CHECK_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_b, k_bwd32, f.block, lds_b));
-    f.grid_fwd = std::max(1, occ_f) * ctx->n_cu;
-    f.grid_bwd = std::max(1, occ_b) * ctx->n_cu;
-    // task size: at least 16 entries per brick voxel (flush amortised),
-    // at most total / (4 * workers) so the queue balances
-    const uint64_t workers = (uint64_t)std::max(f.grid_fwd, f.grid_bwd);
-    uint64_t target = std::max<uint64_t>(16ull * f.bs.vol(), f.n_ent / (4 * workers) + 1);
-    std::vector<Task> tasks;
-    std::vector<uint64_t> tent;
-    for (uint64_t b = 0; b < nbk; ++b) {
-      const uint64_t s0 = hb[b], s1 = hb[b + 1];
-      if (s1 == s0) continue;
-      const uint64_t e = he[b];
-      const uint64_t k = std::max<uint64_t>(1, (e + target - 1) / target);
-      const uint64_t ns = s1 - s0;
-      const uint64_t kk = std::min(k, ns);
-      for (uint64_t t = 0; t < kk; ++t) {
-        Task tk;
-        tk.brick = (uint32_t)b;
-        tk.owner = kk == 1 ? 1u : 0u;
-        tk.s0 = s0 + ns * t / kk;
-        tk.s1 = s0 + ns * (t + 1) / kk;
-        tasks.push_back(tk);
-        tent.push_back(e / kk);
-      }
-    }
-    std::vector<size_t> idx(tasks.size());
-    std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return tent[x] > tent[y]; });
-    std::vector<Task> sorted(tasks.size());
-    for (size_t i = 0; i < idx.size(); ++i) sorted[i] = tasks[idx[i]];
-    f.n_tasks = (uint32_t)sorted.size();
-    MUTO_CHECK_HIP(ctx, f.tasks.alloc(sorted.size()));
-    if (!sorted.empty())
-      MUTO_CHECK_HIP(ctx, hipMemcpy(f.tasks.p, sorted.data(), sorted.size() * sizeof(Task), hipMemcpyHostToDevice));
-    // bricks that no task owns completely get their accumulators by atomics
-    // into acc64, which is zero between iterations
     MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
   }
-  MUTO_CHECK_HIP(ctx, f.seg_part.alloc(3 * n_seg));
+  MUTO_CHECK_HIP(ctx, f.seg_part.alloc(4 * n_seg));
   MUTO_CHECK_HIP(ctx, f.trk_g.alloc(4 * nt));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->work.p, 0, 4 * sizeof(unsigned int), s));
   MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
@@ -645,16 +943,27 @@ int fp32c_iterate(muto_em_plan* pl, bool time_it) {
   const bool map = pl->is_map;
   if (map && !pl->tmp64.p) MUTO_CHECK_HIP(ctx, pl->tmp64.alloc(pl->nv));
   if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[0], s));
-  k_fwd32<<<f.grid_fwd, f.block, lds_f, s>>>(a, pl->lam32.p, f.seg_part.p);
-  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  if (f.n_tasks) {
+    if (f.acc_mode == 1)
+      k_fwd32<1><<<f.grid_fwd, BLOCK, lds_f, s>>>(a, pl->lam32.p, reinterpret_cast<float4*>(f.seg_part.p));
+    else
+      k_fwd32<0><<<f.grid_fwd, BLOCK, lds_f, s>>>(a, pl->lam32.p, reinterpret_cast<float4*>(f.seg_part.p));
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  }
   if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[1], s));
-  k_track32<<<grid_for(pl->n_trk, 256), 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p, f.n_seg,
-                                                      f.seg_part.p, f.trk_d.p, E, f.trk_g.p,
+  k_track32<<<grid_for(pl->n_trk, 256), 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p,
+                                                      reinterpret_cast<const float4*>(f.seg_part.p),
+                                                      f.trk_d.p, E, f.trk_tc.p, f.trk_g.p,
                                                       pl->cost_acc.p, pl->dcount.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[2], s));
-  k_bwd32<<<f.grid_bwd, f.block, lds_b, s>>>(a, f.trk_g.p, pl->acc64.p);
-  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  if (f.n_tasks) {
+    if (f.acc_mode == 1)
+      k_bwd32<1><<<f.grid_bwd, BLOCK, lds_b, s>>>(a, f.trk_g.p, pl->acc64.p);
+    else
+      k_bwd32<0><<<f.grid_bwd, BLOCK, lds_b, s>>>(a, f.trk_g.p, pl->acc64.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  }
   if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[3], s));
   k_upd32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->acc64.p, pl->lam32.p,
                                                 map ? pl->ml64.p : nullptr,

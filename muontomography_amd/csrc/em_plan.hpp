@@ -11,13 +11,22 @@ struct BrickShape {
   uint32_t vol() const { return bx * by * bz; }
 };
 
-// A unit of work of the persistent FP32C kernels: a run of brick-major
-// segments [s0, s1) that all lie in brick `brick`.  `owner` is set when
-// the task covers the whole brick (its accumulators are stored, not added).
+// Chunk: up to 256 consecutive brick-major segments [q0, q0 + cnt) whose
+// ne entries start at entry `base`; staged into LDS as one unit.
+struct Chunk {
+  uint64_t q0;
+  uint64_t base;
+  uint32_t cnt;
+  uint32_t ne;
+};
+
+// A unit of work of the persistent FP32C kernels: chunks [c0, c1) that all
+// lie in brick `brick`.  `owner` is set when the task covers the whole
+// brick (its accumulators are stored, not added).
 struct Task {
   uint32_t brick;
   uint32_t owner;
-  uint64_t s0, s1;
+  uint32_t c0, c1;
 };
 
 struct Fp32cData {
@@ -26,18 +35,22 @@ struct Fp32cData {
   uint32_t nb[3] = {0, 0, 0};
   uint64_t n_seg = 0, n_ent = 0;
   DBuf<uint32_t> seg_trk;   // [n_seg] track rank
-  DBuf<float> seg_tend;     // [n_seg] tail after the segment's last entry
+  DBuf<double> seg_tend;    // [n_seg] tail after the segment's last entry
   DBuf<uint64_t> seg_off;   // [n_seg+1] entry offsets (brick-major)
   DBuf<uint16_t> ent_vox;   // [n_ent] voxel index inside the brick
   DBuf<float> ent_len;      // [n_ent] path length
   DBuf<uint32_t> tseg_ptr;  // [n_trk+1] per rank: segment list offsets
   DBuf<uint32_t> tseg_pos;  // [n_seg] brick-major positions, rank order
+  DBuf<uint8_t> perm;       // [n_seg] per chunk: lane -> segment (length order)
   DBuf<double> trk_d;       // [4*n_trk] rank order: D00, D01, D11, p_r
-  DBuf<float> seg_part;     // [3*n_seg] forward partial sums (SoA)
+  DBuf<float> trk_tc;       // [n_trk] tail offset of the track's basis
+  DBuf<float> seg_part;     // [4*n_seg] forward partial sums (float4, brick-major)
   DBuf<float> trk_g;        // [4*n_trk] p^2 G (g00, g01, g11, -)
   DBuf<Task> tasks;
-  uint32_t n_tasks = 0;
+  DBuf<Chunk> chunks;
+  uint32_t n_tasks = 0, n_chunks = 0;
   int grid_fwd = 0, grid_bwd = 0, block = 256;
+  int acc_mode = 1;  // accumulation: 1 FP64, 0 FP32 with two-sum
   hipGraphExec_t graph = nullptr;
 };
 

@@ -133,8 +133,10 @@ def test_poca_image(ctx, small_set, lw, sl):
     ig, cg = ctx.poca_reconstruct(cfg, rays)
     assert co["n_scat_in_grid"] == cg["n_scat_in_grid"]
     assert np.array_equal(io == 0, ig == 0)
-    # FP64 atomics reorder the per-voxel sums; atan2 may differ by 1 ulp
-    np.testing.assert_allclose(ig, io, rtol=1e-12, atol=0)
+    # FP64 atomics reorder the per-voxel sums, and the scattering angle is a
+    # difference of two atan2 values (ocml vs glibc differ by an ulp), which
+    # cancellation amplifies for small angles: rtol 1e-9
+    np.testing.assert_allclose(ig, io, rtol=1e-9, atol=0)
 
 
 def _em_cfg(g, **kw):
