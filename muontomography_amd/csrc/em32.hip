@@ -208,27 +208,35 @@ k_chunk_sort(const Chunk* __restrict__ chunks, const uint32_t* __restrict__ lenq
   if (t < ch.cnt) perm[ch.q0 + t] = (uint8_t)(key[t] & 0xFFu);
 }
 
+// Track rank and tail after the segment, as unsigned fixed point with
+// 2^-tail_bits mm resolution (tails are >= 0 and below the fixed-point
+// range, which the host sizes from the grid diagonal).
 __global__ void __launch_bounds__(256)
 k_seg_gather(uint64_t n_seg, const uint32_t* __restrict__ sval,
              const double* __restrict__ st_tend,
              const uint32_t* __restrict__ st_trk,
-             const uint32_t* __restrict__ rank,
-             uint32_t* __restrict__ seg_trk, double* __restrict__ seg_tend) {
+             const uint32_t* __restrict__ rank, double tail_scale,
+             uint32_t* __restrict__ seg_trk, uint32_t* __restrict__ seg_tfx) {
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_seg) return;
   const uint32_t s = sval[q];
   seg_trk[q] = rank[st_trk[s]];
-  seg_tend[q] = st_tend[s];
+  seg_tfx[q] = (uint32_t)llrint(st_tend[s] * tail_scale);
 }
 
+// chunk entry base and count; per segment its first entry relative to the
+// chunk base (u16: a chunk holds at most ECAP entries)
 __global__ void k_chunk_final(uint32_t n, Chunk* __restrict__ chunks,
-                              const uint64_t* __restrict__ seg_off) {
+                              const uint64_t* __restrict__ seg_off,
+                              uint16_t* __restrict__ seg_lo) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   Chunk ch = chunks[c];
   ch.base = seg_off[ch.q0];
   ch.ne = (uint32_t)(seg_off[ch.q0 + ch.cnt] - ch.base);
   chunks[c] = ch;
+  for (uint32_t t = 0; t < ch.cnt; ++t)
+    seg_lo[ch.q0 + t] = (uint16_t)(seg_off[ch.q0 + t] - ch.base);
 }
 
 __global__ void __launch_bounds__(256)
@@ -269,23 +277,16 @@ k_tseg_fill(uint64_t n_seg, const uint32_t* __restrict__ sval,
   tseg_pos[tseg_ptr[rank[t]] + (uint32_t)(s - sstart[t])] = (uint32_t)q;
 }
 
-// Per-track data in rank order.  The initial basis offset is half the
-// track's path length (the tail of its first entry plus that entry).
+// Per-track data in rank order.
 __global__ void k_trk_d(size_t n_trk, const uint32_t* __restrict__ order,
                         const double* __restrict__ trk4,
                         const double* __restrict__ pr,
                         const uint32_t* __restrict__ nseg,
-                        const uint64_t* __restrict__ row_ptr,
-                        const double* __restrict__ len,
-                        const double* __restrict__ tail,
                         double* __restrict__ trk_d,
-                        uint32_t* __restrict__ nseg_rank,
-                        float* __restrict__ trk_tc) {
+                        uint32_t* __restrict__ nseg_rank) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_trk) return;
   const uint32_t t = order[r];
-  const uint64_t b = row_ptr[t];
-  trk_tc[r] = row_ptr[t + 1] > b ? (float)(0.5 * (tail[b] + len[b])) : 0.0f;
   const double Sx = trk4[4 * t], Sy = trk4[4 * t + 1];
   const double Dx = trk4[4 * t + 2], Dy = trk4[4 * t + 3];
   trk_d[4 * r] = Sx * Sx + Sy * Sy;
@@ -311,10 +312,10 @@ struct IterArgs {
   uint32_t n_tasks;
   unsigned int* work;
   uint64_t n_seg;
-  const uint64_t* seg_off;
-  const double* seg_tend;
+  const uint16_t* seg_lo;
+  const uint32_t* seg_tfx;
+  double inv_scale;
   const uint32_t* seg_trk;
-  const float* trk_tc;
   const uint8_t* perm;
   const uint16_t* ent_vox;
   const float* ent_len;
@@ -375,7 +376,10 @@ struct Stage {
 
 // Forward: per segment, sum_j lambda_j W_ij (three components) with
 // lambda staged in LDS; the walk goes backwards so the tail grows by each
-// entry's length.  Tails are in the track's shifted basis t' = t - t_c.
+// entry's length.  Tails are measured from the segment's own end (at most a
+// brick diagonal), so the FP32 terms carry no large offsets; the track
+// kernel moves the partials to the track's basis in FP64.  The partial's
+// fourth word is the segment's end tail (fixed point bits).
 template <int ACC>
 __global__ void __launch_bounds__(BLOCK)
 k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
@@ -403,13 +407,14 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
       // issued so the walk does not wait on it
       const bool act = threadIdx.x < cur.cnt;
       const uint64_t q = cur.q0 + (act ? a.perm[cur.q0 + threadIdx.x] : 0u);
-      uint32_t lo = 0, hi = 0;
-      float T = 0.f;
+      uint32_t lo = 0, hi = 0, tfx = 0;
       if (act) {
-        lo = (uint32_t)(a.seg_off[q] - cur.base);
-        hi = (uint32_t)(a.seg_off[q + 1] - cur.base);
-        T = (float)(a.seg_tend[q] - (double)a.trk_tc[a.seg_trk[q]]);
+        const uint32_t li = (uint32_t)(q - cur.q0);
+        lo = a.seg_lo[q];
+        hi = li + 1 < cur.cnt ? a.seg_lo[q + 1] : cur.ne;
+        tfx = a.seg_tfx[q];
       }
+      float T = 0.f;
       Chunk nxt = cur;
       if (c + 1 < tk.c1) {
         nxt = a.chunks[c + 1];
@@ -439,7 +444,7 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
             step(Ld, xd);
           }
           for (; k > lo; --k) step(sl[k - 1], lds_lam[sv[k - 1]]);
-          part[q] = make_float4((float)a0, (float)a1, (float)a2, 0.0f);
+          part[q] = make_float4((float)a0, (float)a1, (float)a2, __uint_as_float(tfx));
         } else {
           float h0 = 0.f, l0 = 0.f, h1 = 0.f, l1 = 0.f, h2 = 0.f, l2 = 0.f;
           for (uint32_t k = hi; k > lo; --k) {
@@ -453,7 +458,7 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
             two_sum_acc(h2, l2, lj * w11);
             T += L;
           }
-          part[q] = make_float4(h0 + l0, h1 + l1, h2 + l2, 0.0f);
+          part[q] = make_float4(h0 + l0, h1 + l1, h2 + l2, __uint_as_float(tfx));
         }
       }
       cur = nxt;
@@ -463,43 +468,56 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
 }
 
 // Per track (rank order): Sigma from segment partials, 2x2 algebra in FP64.
-// The partials were summed in the track's shifted basis u'(t) = (1, t - t_c):
-// with M = [[1, 0], [t_c, 1]], Sigma = M Sigma' M^T, so E and D enter as
-// M^-1 E M^-T and M^-1 D M^-T; det Sigma = det Sigma', cost and
-// <W, G> = <W', G'> are unchanged.  Choosing t_c at the track's scattering
-// centroid makes Sigma' nearly diagonal, so FP32 partial sums keep the
-// determinant and G' accurate where Sigma itself is strongly correlated.
-// The next iteration's t_c is the centroid of this one: t_c + s01 / s00.
+// A segment's partial P_q is summed in its local basis u(t) = (1, t - tau_q)
+// (tau_q: tail after the segment); in a basis at t_c it becomes
+// A P_q A^T with A = [[1, 0], [tau_q - t_c, 1]].  The track's basis t_c is
+// its scattering centroid for the current image, rounded to the tail
+// fixed-point grid so that tau_q - t_c is exact; there Sigma is nearly
+// diagonal and the determinant and G keep their accuracy where Sigma itself
+// is strongly correlated.  With M = [[1, 0], [t_c, 1]], Sigma = M Sigma' M^T:
+// E and D enter as M^-1 E M^-T and M^-1 D M^-T; det, cost and <W, G> are
+// basis-free.  Written per track: G' (FP32) and t_c (fixed-point bits).
 __global__ void __launch_bounds__(256)
 k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
           const uint32_t* __restrict__ tseg_pos,
           const float4* __restrict__ part, const double* __restrict__ trk_d,
-          ErrM E, float* __restrict__ trk_tc, float* __restrict__ trk_g,
+          ErrM E, double tail_scale, float* __restrict__ trk_g,
           double* __restrict__ cost_acc, unsigned long long* __restrict__ cnt) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   double cost = 0.0;
   bool sing = false;
   if (r < n_trk) {
     // this track's segment partials: brick-major positions in track order;
-    // neighbouring tracks (adjacent ranks) read neighbouring positions
-    double s00 = 0.0, s01 = 0.0, s11 = 0.0;
+    // neighbouring tracks (adjacent ranks) read neighbouring positions.
+    // Moments about the track end (t = 0) in FP64, exact up to FP64 rounding.
+    const double inv_scale = 1.0 / tail_scale;
+    double S00 = 0.0, S01 = 0.0, S11 = 0.0;
+    auto add = [&](const float4 pq) {
+      const double tau = (double)__float_as_uint(pq.w) * inv_scale;
+      const double p0 = pq.x, p1 = pq.y, p2 = pq.z;
+      S00 += p0;
+      S01 += fma(tau, p0, p1);
+      S11 += fma(tau, fma(tau, p0, 2.0 * p1), p2);
+    };
     const uint32_t k0 = tseg_ptr[r], k1 = tseg_ptr[r + 1];
     uint32_t k = k0;
     for (; k + 4 <= k1; k += 4) {
       const uint32_t qa = tseg_pos[k], qb = tseg_pos[k + 1], qc = tseg_pos[k + 2], qd = tseg_pos[k + 3];
       const float4 pa = part[qa], pb = part[qb], pc = part[qc], pd = part[qd];
-      s00 += ((double)pa.x + (double)pb.x) + ((double)pc.x + (double)pd.x);
-      s01 += ((double)pa.y + (double)pb.y) + ((double)pc.y + (double)pd.y);
-      s11 += ((double)pa.z + (double)pb.z) + ((double)pc.z + (double)pd.z);
+      add(pa);
+      add(pb);
+      add(pc);
+      add(pd);
     }
-    for (; k < k1; ++k) {
-      const float4 pa = part[tseg_pos[k]];
-      s00 += (double)pa.x;
-      s01 += (double)pa.y;
-      s11 += (double)pa.z;
-    }
-    const float tcf = trk_tc[r];
-    const double tc = (double)tcf;
+    for (; k < k1; ++k) add(part[tseg_pos[k]]);
+    // basis at the centroid, on the fixed-point grid (clamped to its range:
+    // any basis is exact, the centroid is only the best-conditioned one)
+    double xc = S00 > 0.0 ? S01 / S00 * tail_scale : 0.0;
+    xc = xc >= 0.0 ? fmin(xc, 4294967295.0) : 0.0;
+    const uint32_t tcx = (uint32_t)llrint(xc);
+    const double tc = (double)tcx * inv_scale;
+    const double s00 = S00, s01 = S01 - tc * S00;
+    const double s11 = S11 - tc * (2.0 * S01 - tc * S00);
     const double D00 = trk_d[4 * r], D01 = trk_d[4 * r + 1];
     const double D11 = trk_d[4 * r + 2], p = trk_d[4 * r + 3];
     // shifted-basis measurement and error matrices
@@ -532,9 +550,8 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
     g.x = (float)(pp * (0.5 * M00 - i00));
     g.y = (float)(pp * (0.5 * M01 - i01));
     g.z = (float)(pp * (0.5 * M11 - i11));
-    g.w = tcf;
+    g.w = __uint_as_float(tcx);
     *reinterpret_cast<float4*>(trk_g + 4 * r) = g;
-    if (s00 > 0.0) trk_tc[r] = (float)(tc + s01 / s00);
   }
   double v = cost;
 #pragma unroll
@@ -582,12 +599,13 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
       const uint64_t q = cur.q0 + (act ? a.perm[cur.q0 + threadIdx.x] : 0u);
       uint32_t lo = 0, hi = 0;
       float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
-      double tend = 0.0;
+      uint32_t tfx = 0;
       if (act) {
-        lo = (uint32_t)(a.seg_off[q] - cur.base);
-        hi = (uint32_t)(a.seg_off[q + 1] - cur.base);
-        tend = a.seg_tend[q];
+        const uint32_t li = (uint32_t)(q - cur.q0);
+        lo = a.seg_lo[q];
+        hi = li + 1 < cur.cnt ? a.seg_lo[q + 1] : cur.ne;
         g = *reinterpret_cast<const float4*>(trk_g + 4 * (size_t)a.seg_trk[q]);
+        tfx = a.seg_tfx[q];
       }
       Chunk nxt = cur;
       if (c + 1 < tk.c1) {
@@ -596,7 +614,13 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
       }
       __syncthreads();
       if (act) {
-        float T = (float)(tend - (double)g.w);
+        // G' moved from the track's basis to the segment's local one,
+        // A^T G' A with A = [[1, 0], [s, 1]], s = tau_q - t_c (exact), in FP64
+        const double sh = (double)((int64_t)tfx - (int64_t)__float_as_uint(g.w)) * a.inv_scale;
+        const double gz = g.z, gy = fma(sh, gz, (double)g.y);
+        g.x = (float)fma(sh, gy + (double)g.y, (double)g.x);
+        g.y = (float)gy;
+        float T = 0.f;
         auto coef = [&](float L) {
           const float LT = L * T;
           const float w01 = fmaf(0.5f * L, L, LT);
@@ -702,10 +726,10 @@ static IterArgs iter_args(muto_em_plan* pl) {
   a.n_tasks = f.n_tasks;
   a.work = pl->work.p;
   a.n_seg = f.n_seg;
-  a.seg_off = f.seg_off.p;
-  a.seg_tend = f.seg_tend.p;
+  a.seg_lo = f.seg_lo.p;
+  a.seg_tfx = f.seg_tfx.p;
+  a.inv_scale = 1.0 / f.tail_scale;
   a.seg_trk = f.seg_trk.p;
-  a.trk_tc = f.trk_tc.p;
   a.perm = f.perm.p;
   a.ent_vox = f.ent_vox.p;
   a.ent_len = f.ent_len.p;
@@ -874,27 +898,39 @@ int fp32c_build(muto_em_plan* pl) {
       k_chunk_sort<<<f.n_chunks, BLOCK, 0, s>>>(f.chunks.p, lenq.p, f.perm.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
-    // brick-major segment arrays in final order
+    // brick-major segment arrays in final order; tails as fixed point with
+    // the finest power-of-two step that keeps twice the grid diagonal (the
+    // longest two-leg path inside the grid) below 2^32
+    {
+      const double ex = pl->g.d[0] * pl->g.n[0], ey = pl->g.d[1] * pl->g.n[1],
+                   ez = pl->g.d[2] * pl->g.n[2];
+      const double bound = 2.0 * sqrt(ex * ex + ey * ey + ez * ez) + 1.0;
+      int bits = (int)floor(log2(4294967295.0 / bound));
+      if (bits > 40) bits = 40;
+      f.tail_scale = ldexp(1.0, bits);
+    }
     MUTO_CHECK_HIP(ctx, f.seg_trk.alloc(n_seg));
-    MUTO_CHECK_HIP(ctx, f.seg_tend.alloc(n_seg));
-    MUTO_CHECK_HIP(ctx, f.seg_off.alloc(n_seg + 1));
+    MUTO_CHECK_HIP(ctx, f.seg_tfx.alloc(n_seg));
+    MUTO_CHECK_HIP(ctx, f.seg_lo.alloc(n_seg));
+    DBuf<uint64_t> seg_off;
+    MUTO_CHECK_HIP(ctx, seg_off.alloc(n_seg + 1));
     if (n_seg) {
       k_seg_gather<<<grid_for(n_seg, 256), 256, 0, s>>>(n_seg, sval.p, st_tend.p, st_trk.p, rank.p,
-                                                        f.seg_trk.p, f.seg_tend.p);
+                                                        f.tail_scale, f.seg_trk.p, f.seg_tfx.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(lenq.p + n_seg, 0, sizeof(uint32_t), s));
-    MUTO_CHECK_HIP(ctx, exclusive_scan_u32_to_u64(lenq.p, f.seg_off.p, n_seg + 1, s));
+    MUTO_CHECK_HIP(ctx, exclusive_scan_u32_to_u64(lenq.p, seg_off.p, n_seg + 1, s));
     f.n_ent = n_ent;
     if (f.n_chunks) {
-      k_chunk_final<<<grid_for(f.n_chunks, 256), 256, 0, s>>>(f.n_chunks, f.chunks.p, f.seg_off.p);
+      k_chunk_final<<<grid_for(f.n_chunks, 256), 256, 0, s>>>(f.n_chunks, f.chunks.p, seg_off.p, f.seg_lo.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
     MUTO_CHECK_HIP(ctx, f.ent_vox.alloc(n_ent));
     MUTO_CHECK_HIP(ctx, f.ent_len.alloc(n_ent));
     if (n_seg) {
       k_ent_scatter<<<grid_for(n_seg, 256), 256, 0, s>>>(G, n_seg, sval.p, st_start.p, st_len.p,
-                                                         f.seg_off.p, pl->vox.p, pl->len.p,
+                                                         seg_off.p, pl->vox.p, pl->len.p,
                                                          f.ent_vox.p, f.ent_len.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
@@ -904,11 +940,9 @@ int fp32c_build(muto_em_plan* pl) {
     MUTO_CHECK_HIP(ctx, nseg_rank.alloc(nt + 1));
     MUTO_CHECK_HIP(ctx, tptr64.alloc(nt + 1));
     MUTO_CHECK_HIP(ctx, f.trk_d.alloc(4 * nt));
-    MUTO_CHECK_HIP(ctx, f.trk_tc.alloc(nt));
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(nseg_rank.p, 0, (nt + 1) * sizeof(uint32_t), s));
     k_trk_d<<<grid_for(nt, 256), 256, 0, s>>>(nt, order.p, pl->trk4.p, pl->pr.p, nseg.p,
-                                              pl->row_ptr.p, pl->len.p, pl->tail.p,
-                                              f.trk_d.p, nseg_rank.p, f.trk_tc.p);
+                                              f.trk_d.p, nseg_rank.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     MUTO_CHECK_HIP(ctx, exclusive_scan_u32_to_u64(nseg_rank.p, tptr64.p, nt + 1, s));
     std::vector<uint64_t> h64(nt + 1);
@@ -953,7 +987,7 @@ int fp32c_iterate(muto_em_plan* pl, bool time_it) {
   if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[1], s));
   k_track32<<<grid_for(pl->n_trk, 256), 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p,
                                                       reinterpret_cast<const float4*>(f.seg_part.p),
-                                                      f.trk_d.p, E, f.trk_tc.p, f.trk_g.p,
+                                                      f.trk_d.p, E, f.tail_scale, f.trk_g.p,
                                                       pl->cost_acc.p, pl->dcount.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[2], s));

@@ -35,15 +35,15 @@ struct Fp32cData {
   uint32_t nb[3] = {0, 0, 0};
   uint64_t n_seg = 0, n_ent = 0;
   DBuf<uint32_t> seg_trk;   // [n_seg] track rank
-  DBuf<double> seg_tend;    // [n_seg] tail after the segment's last entry
-  DBuf<uint64_t> seg_off;   // [n_seg+1] entry offsets (brick-major)
+  DBuf<uint32_t> seg_tfx;   // [n_seg] tail after the segment, fixed point
+  DBuf<uint16_t> seg_lo;    // [n_seg] first entry relative to the chunk base
+  double tail_scale = 1.0;  // fixed-point scale of seg_tfx (a power of two)
   DBuf<uint16_t> ent_vox;   // [n_ent] voxel index inside the brick
   DBuf<float> ent_len;      // [n_ent] path length
   DBuf<uint32_t> tseg_ptr;  // [n_trk+1] per rank: segment list offsets
   DBuf<uint32_t> tseg_pos;  // [n_seg] brick-major positions, rank order
   DBuf<uint8_t> perm;       // [n_seg] per chunk: lane -> segment (length order)
   DBuf<double> trk_d;       // [4*n_trk] rank order: D00, D01, D11, p_r
-  DBuf<float> trk_tc;       // [n_trk] tail offset of the track's basis
   DBuf<float> seg_part;     // [4*n_seg] forward partial sums (float4, brick-major)
   DBuf<float> trk_g;        // [4*n_trk] p^2 G (g00, g01, g11, -)
   DBuf<Task> tasks;
