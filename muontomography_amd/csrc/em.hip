@@ -421,11 +421,6 @@ __global__ void k_copy_f64_to_f32(size_t n, const double* __restrict__ a,
   size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j < n) b[j] = (float)a[j];
 }
-__global__ void k_copy_f32_to_f64(size_t n, const float* __restrict__ a,
-                                  double* __restrict__ b) {
-  size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) b[j] = (double)a[j];
-}
 
 }  // namespace
 
@@ -782,8 +777,10 @@ extern "C" int muto_em_plan_set_image(muto_em_plan* pl, const double* img) {
   return MUTO_OK;
 }
 
-// make the image of `prec` current (FP64 master <-> FP32 working copy);
-// cur_prec: -1 no image, MUTO_FP64 lam64 current, MUTO_FP32C lam32 current
+// lam64 is the image in both precisions (FP32C updates it in FP64 too);
+// FP32C's forward pass reads a float mirror, lam32, refreshed here when
+// FP32C takes over.  cur_prec: -1 no image, else the precision of the last
+// run (MUTO_FP32C: the mirror is current).
 static int sync_image(muto_em_plan* pl, int prec) {
   muto_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream;
@@ -791,10 +788,8 @@ static int sync_image(muto_em_plan* pl, int prec) {
   if (prec == MUTO_FP32C) {
     if (!pl->lam32.p) MUTO_CHECK_HIP(ctx, pl->lam32.alloc(pl->nv));
     k_copy_f64_to_f32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->lam64.p, pl->lam32.p);
-  } else {
-    k_copy_f32_to_f64<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->lam32.p, pl->lam64.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
-  MUTO_CHECK_HIP(ctx, hipGetLastError());
   pl->cur_prec = prec;
   return MUTO_OK;
 }
@@ -802,8 +797,6 @@ static int sync_image(muto_em_plan* pl, int prec) {
 extern "C" int muto_em_plan_get_image(muto_em_plan* pl, double* img) {
   if (!pl || !img) return MUTO_ERR_ARG;
   muto_ctx* ctx = pl->ctx;
-  if (pl->cur_prec == MUTO_FP32C)
-    if (int rc = sync_image(pl, MUTO_FP64)) return rc;
   MUTO_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
   MUTO_CHECK_HIP(ctx, hipMemcpy(img, pl->lam64.p, pl->nv * sizeof(double), hipMemcpyDeviceToHost));
   return MUTO_OK;

@@ -11,18 +11,24 @@
 //     parallel neighbours;
 //   - each brick's segment list is cut into tasks (the persistent kernels'
 //     work units) and each task into chunks of at most 256 segments and
-//     ECAP entries; each chunk carries a lane -> segment permutation that
-//     orders its segments by length, longest first, so the 64 lanes of a
-//     wave walk segments of nearly equal length while the segments stay in
-//     track order in memory;
+//     ECAP entries;
 //   - entries are stored brick-major as (u16 voxel-in-brick, f32 length):
-//     6 bytes per track-voxel entry.
+//     6 bytes per track-voxel entry; each chunk starts on a 16-entry
+//     boundary and is padded to whole 16-entry groups with zero-length
+//     entries, and each group has a descriptor (segment-start mask, the
+//     segment holding its first entry, the length of its last segment after
+//     the group): 0.5 bytes per entry.
+// The passes over entries are entry-parallel: thread t of a block takes the
+// chunk's group t (16-byte vector loads) and walks it backwards, the tail of
+// each entry inside its segment following from the group's carry; every
+// thread does the same work, whatever the segment lengths.
 // Per iteration (src/MutoMLSD.cpp:285-308, mean update):
 //   k_fwd32   per segment: partial covariance sums sum lambda_j W_ij with
-//             lambda staged in LDS, W from (L, tail) in registers;
+//             lambda staged in LDS, W from (L, tail) in registers, in the
+//             segment's local tail basis;
 //   k_track32 per track: Sigma_i in FP64 from its segment partials, Eigen's
 //             2x2 det/inverse in FP64, cost_i, G_i in FP32;
-//   k_bwd32   per segment: c_ij = <W_ij, G_i> accumulated per voxel in LDS,
+//   k_bwd32   per entry: c_ij = <W_ij, G_i> accumulated per voxel in LDS,
 //             flushed once per task;
 //   k_upd32   per voxel: lambda += lambda^2/(2 M_j) sum_i c_ij, which is
 //             half the mean of S_ij = 2 lambda_j + c_ij lambda_j^2.
@@ -31,8 +37,7 @@
 // two-sum compensation -- error-free transformation in the forward walk,
 // atomic two-sum (returned old value + error word) in the backprojection.
 // The persistent kernels take tasks from a queue sorted by size, largest
-// first, through an atomic counter; each block stages the next chunk into
-// registers while it walks the current one.
+// first, through an atomic counter.
 #include <math.h>
 #include <stdlib.h>
 
@@ -54,10 +59,12 @@ int map_step(muto_em_plan* pl, const double* last, const double* ml, double* out
 namespace {
 
 constexpr uint32_t BLOCK = 256;          // threads per block (4 wavefronts)
-constexpr uint32_t ECAP = 4096;          // staged entries per chunk
-constexpr uint32_t PF = ECAP / BLOCK;    // prefetched entries per thread
-static_assert((ECAP * (sizeof(float) + sizeof(uint16_t))) % 16 == 0,
-              "static LDS must keep the dynamic region 16-byte aligned");
+constexpr int GE = 16;                   // entries per thread and chunk
+constexpr uint32_t ECAP = BLOCK * GE;    // entries per chunk, padding included
+// static LDS of both passes is a multiple of 16 bytes, so the dynamic region
+// after it is 16-byte aligned
+static_assert((2 * 3 * BLOCK * 8 + 16) % 16 == 0, "forward static LDS");
+static_assert((2 * BLOCK * 16 + 16) % 16 == 0, "backward static LDS");
 
 struct BrickGeo {
   uint32_t n[3];   // grid
@@ -183,31 +190,6 @@ k_seg_len(uint64_t n_seg, const uint32_t* __restrict__ sval,
   if (q == 0 || (uint32_t)(skey[q - 1] >> 32) != b) bstart[b] = q;
 }
 
-// Lane -> segment permutation of each chunk: its segments by length,
-// longest first (stable); one block per chunk.
-__global__ void __launch_bounds__(256)
-k_chunk_sort(const Chunk* __restrict__ chunks, const uint32_t* __restrict__ lenq,
-             uint8_t* __restrict__ perm) {
-  __shared__ uint32_t key[BLOCK];
-  const Chunk ch = chunks[blockIdx.x];
-  const uint32_t t = threadIdx.x;
-  // key: (ECAP - len) << 8 | position; ascending = length desc, stable
-  key[t] = t < ch.cnt ? ((ECAP - lenq[ch.q0 + t]) << 8) | t : 0xFFFFFFFFu;
-  __syncthreads();
-  for (uint32_t k = 2; k <= BLOCK; k <<= 1) {
-    for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-      const uint32_t l = t ^ jj;
-      if (l > t) {
-        const uint32_t a = key[t], b = key[l];
-        const bool up = (t & k) == 0;
-        if (up ? (b < a) : (a < b)) { key[t] = b; key[l] = a; }
-      }
-      __syncthreads();
-    }
-  }
-  if (t < ch.cnt) perm[ch.q0 + t] = (uint8_t)(key[t] & 0xFFu);
-}
-
 // Track rank and tail after the segment, as unsigned fixed point with
 // 2^-tail_bits mm resolution (tails are >= 0 and below the fixed-point
 // range, which the host sizes from the grid diagonal).
@@ -224,19 +206,48 @@ k_seg_gather(uint64_t n_seg, const uint32_t* __restrict__ sval,
   seg_tfx[q] = (uint32_t)llrint(st_tend[s] * tail_scale);
 }
 
-// chunk entry base and count; per segment its first entry relative to the
-// chunk base (u16: a chunk holds at most ECAP entries)
-__global__ void k_chunk_final(uint32_t n, Chunk* __restrict__ chunks,
-                              const uint64_t* __restrict__ seg_off,
-                              uint16_t* __restrict__ seg_lo) {
+// Entry offset of every segment: its chunk's base (16-entry aligned, set by
+// the host) plus its offset inside the chunk.
+__global__ void k_chunk_offsets(uint32_t n, const Chunk* __restrict__ chunks,
+                                const uint64_t* __restrict__ seg_off,
+                                uint64_t* __restrict__ seg_off_p) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  Chunk ch = chunks[c];
-  ch.base = seg_off[ch.q0];
-  ch.ne = (uint32_t)(seg_off[ch.q0 + ch.cnt] - ch.base);
-  chunks[c] = ch;
-  for (uint32_t t = 0; t < ch.cnt; ++t)
-    seg_lo[ch.q0 + t] = (uint16_t)(seg_off[ch.q0 + t] - ch.base);
+  const Chunk ch = chunks[c];
+  const uint64_t b = seg_off[ch.q0];
+  for (uint32_t t = 0; t < ch.cnt; ++t) seg_off_p[ch.q0 + t] = ch.base + (seg_off[ch.q0 + t] - b);
+}
+
+// Descriptor of every 16-entry group (one block per chunk, one thread per
+// group): segment-start mask, the chunk segment holding its first entry and
+// the length of its last segment after the group (summed in FP64).
+__global__ void __launch_bounds__(BLOCK)
+k_group_desc(const Chunk* __restrict__ chunks, const uint64_t* __restrict__ seg_off_p,
+             const float* __restrict__ ent_len, uint32_t* __restrict__ gdesc,
+             float* __restrict__ gcarry) {
+  __shared__ uint32_t so[BLOCK + 1];
+  const Chunk ch = chunks[blockIdx.x];
+  const uint32_t t = threadIdx.x;
+  if (t < ch.cnt) so[t] = (uint32_t)(seg_off_p[ch.q0 + t] - ch.base);
+  if (t == 0) so[ch.cnt] = ch.ne;
+  __syncthreads();
+  const uint32_t e0 = t * GE;
+  if (e0 >= ch.ne) return;
+  uint32_t lo = 0, hi = ch.cnt;  // so[lo] <= e0 < so[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (so[mid] <= e0) lo = mid; else hi = mid;
+  }
+  uint32_t mask = 0, s = lo;
+  for (uint32_t k = lo; k < ch.cnt && so[k] < e0 + GE; ++k) {
+    if (so[k] >= e0) mask |= 1u << (so[k] - e0);
+    s = k;
+  }
+  double cr = 0.0;
+  for (uint32_t e = e0 + GE; e < so[s + 1]; ++e) cr += (double)ent_len[ch.base + e];
+  const uint64_t gi = ch.base / GE + t;
+  gdesc[gi] = mask | (lo << 16);
+  gcarry[gi] = (float)cr;
 }
 
 __global__ void __launch_bounds__(256)
@@ -312,11 +323,11 @@ struct IterArgs {
   uint32_t n_tasks;
   unsigned int* work;
   uint64_t n_seg;
-  const uint16_t* seg_lo;
   const uint32_t* seg_tfx;
+  const uint32_t* gdesc;
+  const float* gcarry;
   double inv_scale;
   const uint32_t* seg_trk;
-  const uint8_t* perm;
   const uint16_t* ent_vox;
   const float* ent_len;
 };
@@ -348,124 +359,138 @@ __device__ __forceinline__ void load_brick(const BrickGeo& G, uint32_t brick,
   }
 }
 
-// Register staging of one chunk's entries (PF per thread, coalesced).
-struct Stage {
-  uint16_t v[PF];
-  float l[PF];
-  __device__ __forceinline__ void fetch(const IterArgs& a, const Chunk& ch) {
+// One thread's group of 16 consecutive entries of a chunk (chunks start on
+// 16-entry boundaries, so the loads are 16-byte vectors), with the group's
+// descriptor: bit i of `mask` is set when entry i starts a segment, `s0` is
+// the chunk segment holding entry 0, and `carry` is the length of that
+// segment after the group (the tail of entry 15 inside its segment).
+struct Group {
+  uint32_t vw[GE / 2];  // voxel-in-brick indices, two u16 per word
+  float l[GE];          // lengths
+  uint32_t mask, s0;
+  float carry;
+  __device__ __forceinline__ void load(const IterArgs& a, uint64_t gi) {
+    const uint4* pv = reinterpret_cast<const uint4*>(a.ent_vox + gi * GE);
+    const float4* pl = reinterpret_cast<const float4*>(a.ent_len + gi * GE);
 #pragma unroll
-    for (uint32_t i = 0; i < PF; ++i) {
-      const uint32_t k = threadIdx.x + BLOCK * i;
-      if (k < ch.ne) {
-        v[i] = a.ent_vox[ch.base + k];
-        l[i] = a.ent_len[ch.base + k];
-      }
+    for (int k = 0; k < GE / 8; ++k) {
+      const uint4 w = pv[k];
+      vw[4 * k] = w.x; vw[4 * k + 1] = w.y; vw[4 * k + 2] = w.z; vw[4 * k + 3] = w.w;
     }
+#pragma unroll
+    for (int k = 0; k < GE / 4; ++k) {
+      const float4 w = pl[k];
+      l[4 * k] = w.x; l[4 * k + 1] = w.y; l[4 * k + 2] = w.z; l[4 * k + 3] = w.w;
+    }
+    const uint32_t d = a.gdesc[gi];
+    mask = d & 0xFFFFu;
+    s0 = d >> 16;
+    carry = a.gcarry[gi];
   }
-  __device__ __forceinline__ void commit(const Chunk& ch, uint16_t* sv, float* sl) {
-#pragma unroll
-    for (uint32_t i = 0; i < PF; ++i) {
-      const uint32_t k = threadIdx.x + BLOCK * i;
-      if (k < ch.ne) {
-        sv[k] = v[i];
-        sl[k] = l[i];
-      }
-    }
+  __device__ __forceinline__ uint32_t vox(int i) const {
+    return (i & 1) ? (vw[i >> 1] >> 16) : (vw[i >> 1] & 0xFFFFu);
+  }
+  // chunk segment of entry i: s0 plus the segment starts in 1..i
+  __device__ __forceinline__ uint32_t seg(int i) const {
+    return s0 + (uint32_t)__popc(mask & ((2u << i) - 2u));
   }
 };
 
 // Forward: per segment, sum_j lambda_j W_ij (three components) with
-// lambda staged in LDS; the walk goes backwards so the tail grows by each
-// entry's length.  Tails are measured from the segment's own end (at most a
-// brick diagonal), so the FP32 terms carry no large offsets; the track
-// kernel moves the partials to the track's basis in FP64.  The partial's
-// fourth word is the segment's end tail (fixed point bits).
+// lambda staged in LDS.  Entry-parallel: thread t takes the chunk's 16-entry
+// group t and walks it backwards, so an entry's tail inside its segment
+// grows by each length from the group's carry and restarts at segment
+// starts.  Tails are measured from the segment's own end (at most a brick
+// diagonal), so the FP32 terms carry no large offsets; the track kernel
+// moves the partials to the track's basis in FP64.  A thread adds each
+// segment piece's sums into LDS (FP64 atomics; a segment is split over at
+// most a few threads) and one thread per segment writes the partial, with
+// the segment's end tail (fixed-point bits) as its fourth word.  The LDS
+// sums alternate between two buffers, so one barrier per chunk suffices.
 template <int ACC>
 __global__ void __launch_bounds__(BLOCK)
 k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
   // static LDS is a multiple of 16 bytes and the dynamic region is declared
-  // as 16-byte aligned doubles, so the brick arrays start 16-byte aligned
+  // as 16-byte aligned doubles, so the brick array starts 16-byte aligned
   extern __shared__ __align__(16) double lds_dyn[];
   float* lds_lam = reinterpret_cast<float*>(lds_dyn);
-  __shared__ float sl[ECAP];
-  __shared__ uint16_t sv[ECAP];
+  __shared__ double sacc[2 * 3 * BLOCK];
   __shared__ uint32_t sh_[4];
   uint32_t& sh = sh_[0];
-  Stage st;
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < 2 * 3 * BLOCK; i += BLOCK) sacc[i] = 0.0;
+  uint32_t buf = 0;
   for (;;) {
-    if (threadIdx.x == 0) sh = atomicAdd(&a.work[0], 1u);
+    if (t == 0) sh = atomicAdd(&a.work[0], 1u);
     __syncthreads();
     const uint32_t ti = sh;
     if (ti >= a.n_tasks) break;
     const Task tk = a.tasks[ti];
     load_brick(a.G, tk.brick, lam, lds_lam);
-    Chunk cur = a.chunks[tk.c0];
-    st.fetch(a, cur);
+    __syncthreads();
     for (uint32_t c = tk.c0; c < tk.c1; ++c) {
-      st.commit(cur, sv, sl);
-      // this lane's segment metadata, read before the next prefetch is
-      // issued so the walk does not wait on it
-      const bool act = threadIdx.x < cur.cnt;
-      const uint64_t q = cur.q0 + (act ? a.perm[cur.q0 + threadIdx.x] : 0u);
-      uint32_t lo = 0, hi = 0, tfx = 0;
-      if (act) {
-        const uint32_t li = (uint32_t)(q - cur.q0);
-        lo = a.seg_lo[q];
-        hi = li + 1 < cur.cnt ? a.seg_lo[q + 1] : cur.ne;
-        tfx = a.seg_tfx[q];
-      }
-      float T = 0.f;
-      Chunk nxt = cur;
-      if (c + 1 < tk.c1) {
-        nxt = a.chunks[c + 1];
-        st.fetch(a, nxt);
-      }
-      __syncthreads();
-      if (act) {
-        if (ACC == 1) {
-          double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-          auto step = [&](float L, float lj) {
-            const float LT = L * T;
-            const float w01 = fmaf(0.5f * L, L, LT);
-            const float w11 = fmaf(L * L, L * (1.0f / 3.0f), (L + T) * LT);
+      const Chunk ch = a.chunks[c];
+      double* sa = sacc + buf * 3 * BLOCK;
+      uint32_t tfx = 0;
+      if (t < ch.cnt) tfx = a.seg_tfx[ch.q0 + t];
+      if (t * GE < ch.ne) {
+        Group gr;
+        gr.load(a, (ch.base / GE) + t);
+        uint32_t s = gr.seg(GE - 1);
+        float T = gr.carry;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;                               // ACC 1
+        float h0 = 0.f, l0 = 0.f, h1 = 0.f, l1 = 0.f, h2 = 0.f, l2 = 0.f;  // ACC 0
+        auto flush = [&]() {
+          if (ACC == 1) {
+            atomicAdd(&sa[3 * s], a0);
+            atomicAdd(&sa[3 * s + 1], a1);
+            atomicAdd(&sa[3 * s + 2], a2);
+            a0 = a1 = a2 = 0.0;
+          } else {
+            atomicAdd(&sa[3 * s], (double)h0 + (double)l0);
+            atomicAdd(&sa[3 * s + 1], (double)h1 + (double)l1);
+            atomicAdd(&sa[3 * s + 2], (double)h2 + (double)l2);
+            h0 = l0 = h1 = l1 = h2 = l2 = 0.f;
+          }
+        };
+#pragma unroll
+        for (int i = GE - 1; i >= 0; --i) {
+          const float L = gr.l[i];
+          const float lj = lds_lam[gr.vox(i)];
+          const float LT = L * T;
+          const float w01 = fmaf(0.5f * L, L, LT);
+          const float w11 = fmaf(L * L, L * (1.0f / 3.0f), (L + T) * LT);
+          if (ACC == 1) {
             a0 += (double)(lj * L);
             a1 += (double)(lj * w01);
             a2 += (double)(lj * w11);
-            T += L;
-          };
-          uint32_t k = hi;
-          for (; k >= lo + 4; k -= 4) {
-            const float La = sl[k - 1], Lb = sl[k - 2], Lc = sl[k - 3], Ld = sl[k - 4];
-            const float xa = lds_lam[sv[k - 1]], xb = lds_lam[sv[k - 2]];
-            const float xc = lds_lam[sv[k - 3]], xd = lds_lam[sv[k - 4]];
-            step(La, xa);
-            step(Lb, xb);
-            step(Lc, xc);
-            step(Ld, xd);
-          }
-          for (; k > lo; --k) step(sl[k - 1], lds_lam[sv[k - 1]]);
-          part[q] = make_float4((float)a0, (float)a1, (float)a2, __uint_as_float(tfx));
-        } else {
-          float h0 = 0.f, l0 = 0.f, h1 = 0.f, l1 = 0.f, h2 = 0.f, l2 = 0.f;
-          for (uint32_t k = hi; k > lo; --k) {
-            const float L = sl[k - 1];
-            const float lj = lds_lam[sv[k - 1]];
-            const float LT = L * T;
-            const float w01 = LT + L * L * 0.5f;
-            const float w11 = L * L * L * (1.0f / 3.0f) + (L + T) * LT;
+          } else {
             two_sum_acc(h0, l0, lj * L);
             two_sum_acc(h1, l1, lj * w01);
             two_sum_acc(h2, l2, lj * w11);
-            T += L;
           }
-          part[q] = make_float4(h0 + l0, h1 + l1, h2 + l2, __uint_as_float(tfx));
+          const bool st = (gr.mask >> i) & 1u;
+          if (i > 0 && st) {  // entry i starts segment s: its piece is complete
+            flush();
+            --s;
+          }
+          T = st ? 0.0f : T + L;
         }
+        flush();  // the piece holding entry 0 (segment s0)
       }
-      cur = nxt;
       __syncthreads();
+      if (t < ch.cnt) {
+        part[ch.q0 + t] = make_float4((float)sa[3 * t], (float)sa[3 * t + 1],
+                                      (float)sa[3 * t + 2], __uint_as_float(tfx));
+        sa[3 * t] = 0.0;
+        sa[3 * t + 1] = 0.0;
+        sa[3 * t + 2] = 0.0;
+      }
+      buf ^= 1u;
     }
   }
 }
+
 
 // Per track (rank order): Sigma from segment partials, 2x2 algebra in FP64.
 // A segment's partial P_q is summed in its local basis u(t) = (1, t - tau_q)
@@ -564,6 +589,9 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
 
 // Backprojection: c_ij = <W_ij, G_i> accumulated per voxel in LDS, then
 // flushed once per task (stored when the task owns the brick, else added).
+// One thread per segment first moves G' from the track's basis to the
+// segment's local one (FP64) into an LDS table (two alternating tables, one
+// barrier per chunk); then entry-parallel like the forward pass.
 // ACC = 1: one FP64 LDS atomic per entry;
 // ACC = 0: FP32 atomic two-sum -- the returned old value o and the stored
 //          o + c give the exact rounding error, added into a low word.
@@ -575,91 +603,67 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
   // as 16-byte aligned doubles: ds_add_f64 needs 8-byte aligned addresses
   extern __shared__ __align__(16) double lds_dyn[];
   float* lds_acc = reinterpret_cast<float*>(lds_dyn);
-  __shared__ float sl[ECAP];
-  __shared__ uint16_t sv[ECAP];
+  __shared__ float4 sg[2 * BLOCK];
   __shared__ uint32_t sh_[4];
   uint32_t& sh = sh_[0];
+  const uint32_t t = threadIdx.x;
   const uint32_t vol = a.G.b[0] * a.G.b[1] * a.G.b[2];
   float* hi_s = lds_acc;
   float* lo_s = lds_acc + vol;
   double* d_s = lds_dyn;
-  Stage st;
+  uint32_t buf = 0;
   for (;;) {
-    if (threadIdx.x == 0) sh = atomicAdd(&a.work[1], 1u);
-    for (uint32_t v = threadIdx.x; v < 2 * vol; v += blockDim.x) lds_acc[v] = 0.0f;
+    if (t == 0) sh = atomicAdd(&a.work[1], 1u);
+    for (uint32_t v = t; v < 2 * vol; v += blockDim.x) lds_acc[v] = 0.0f;
     __syncthreads();
     const uint32_t ti = sh;
     if (ti >= a.n_tasks) break;
     const Task tk = a.tasks[ti];
-    Chunk cur = a.chunks[tk.c0];
-    st.fetch(a, cur);
     for (uint32_t c = tk.c0; c < tk.c1; ++c) {
-      st.commit(cur, sv, sl);
-      const bool act = threadIdx.x < cur.cnt;
-      const uint64_t q = cur.q0 + (act ? a.perm[cur.q0 + threadIdx.x] : 0u);
-      uint32_t lo = 0, hi = 0;
-      float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
-      uint32_t tfx = 0;
-      if (act) {
-        const uint32_t li = (uint32_t)(q - cur.q0);
-        lo = a.seg_lo[q];
-        hi = li + 1 < cur.cnt ? a.seg_lo[q + 1] : cur.ne;
-        g = *reinterpret_cast<const float4*>(trk_g + 4 * (size_t)a.seg_trk[q]);
-        tfx = a.seg_tfx[q];
+      const Chunk ch = a.chunks[c];
+      float4* sgb = sg + buf * BLOCK;
+      if (t < ch.cnt) {
+        const uint64_t q = ch.q0 + t;
+        const uint32_t tfx = a.seg_tfx[q];
+        const float4 g = *reinterpret_cast<const float4*>(trk_g + 4 * (size_t)a.seg_trk[q]);
+        // A^T G' A with A = [[1, 0], [s, 1]], s = tau_q - t_c (exact)
+        const double shf = (double)((int64_t)tfx - (int64_t)__float_as_uint(g.w)) * a.inv_scale;
+        const double gy = fma(shf, (double)g.z, (double)g.y);
+        sgb[t] = make_float4((float)fma(shf, gy + (double)g.y, (double)g.x), (float)gy, g.z, 0.0f);
       }
-      Chunk nxt = cur;
-      if (c + 1 < tk.c1) {
-        nxt = a.chunks[c + 1];
-        st.fetch(a, nxt);
-      }
+      const bool act = t * GE < ch.ne;
+      Group gr;
+      if (act) gr.load(a, (ch.base / GE) + t);
       __syncthreads();
       if (act) {
-        // G' moved from the track's basis to the segment's local one,
-        // A^T G' A with A = [[1, 0], [s, 1]], s = tau_q - t_c (exact), in FP64
-        const double sh = (double)((int64_t)tfx - (int64_t)__float_as_uint(g.w)) * a.inv_scale;
-        const double gz = g.z, gy = fma(sh, gz, (double)g.y);
-        g.x = (float)fma(sh, gy + (double)g.y, (double)g.x);
-        g.y = (float)gy;
-        float T = 0.f;
-        auto coef = [&](float L) {
+        float T = gr.carry;
+#pragma unroll
+        for (int i = GE - 1; i >= 0; --i) {
+          const float4 g = sgb[gr.seg(i)];
+          const float L = gr.l[i];
           const float LT = L * T;
           const float w01 = fmaf(0.5f * L, L, LT);
           const float w11 = fmaf(L * L, L * (1.0f / 3.0f), (L + T) * LT);
-          T += L;
-          return fmaf(L, g.x, fmaf(2.0f * w01, g.y, w11 * g.z));
-        };
-        uint32_t k = hi;
-        if (ACC == 1) {
-          for (; k >= lo + 4; k -= 4) {
-            const float La = sl[k - 1], Lb = sl[k - 2], Lc = sl[k - 3], Ld = sl[k - 4];
-            const uint32_t va = sv[k - 1], vb = sv[k - 2], vc = sv[k - 3], vd = sv[k - 4];
-            const float ca = coef(La), cb = coef(Lb), cc = coef(Lc), cd = coef(Ld);
-            atomicAdd(&d_s[va], (double)ca);
-            atomicAdd(&d_s[vb], (double)cb);
-            atomicAdd(&d_s[vc], (double)cc);
-            atomicAdd(&d_s[vd], (double)cd);
-          }
-          for (; k > lo; --k) atomicAdd(&d_s[sv[k - 1]], (double)coef(sl[k - 1]));
-        } else {
-          auto add = [&](uint32_t v, float cv, float old) {
+          const float cv = fmaf(L, g.x, fmaf(2.0f * w01, g.y, w11 * g.z));
+          const uint32_t v = gr.vox(i);
+          if (ACC == 1) {
+            atomicAdd(&d_s[v], (double)cv);
+          } else {
+            const float old = atomicAdd(&hi_s[v], cv);
             const float sum = old + cv;
             const float bp = sum - old;
             const float e = (old - (sum - bp)) + (cv - bp);
             if (e != 0.0f) atomicAdd(&lo_s[v], e);
-          };
-          for (; k > lo; --k) {
-            const uint32_t v = sv[k - 1];
-            const float cv = coef(sl[k - 1]);
-            add(v, cv, atomicAdd(&hi_s[v], cv));
           }
+          T = ((gr.mask >> i) & 1u) ? 0.0f : T + L;
         }
       }
-      cur = nxt;
-      __syncthreads();
+      buf ^= 1u;
     }
+    __syncthreads();
     uint32_t o[3];
     brick_origin(a.G, tk.brick, o);
-    for (uint32_t v = threadIdx.x; v < vol; v += blockDim.x) {
+    for (uint32_t v = t; v < vol; v += blockDim.x) {
       const uint32_t lx = v % a.G.b[0], rr = v / a.G.b[0];
       const uint32_t ly = rr % a.G.b[1], lz = rr / a.G.b[1];
       const uint32_t x = o[0] + lx, y = o[1] + ly, z = o[2] + lz;
@@ -675,31 +679,39 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
   }
 }
 
-// lambda_j <- lambda_j + lambda_j^2 / (2 M_j) * sum_i c_ij; clears the
-// accumulator and the work counters for the next iteration.
+// lambda_j <- lambda_j + lambda_j^2 / (2 M_j) * sum_i c_ij from the FP64
+// image, which stays the iteration's state (the float mirror only feeds the
+// forward pass); clears the accumulator and the work counters for the next
+// iteration.  MAP: lambda_ML goes to ml and the image is left as
+// lambda_last for the voxel step.
 __global__ void __launch_bounds__(256)
 k_upd32(size_t nv, const uint32_t* __restrict__ mj, double* __restrict__ acc,
-        float* __restrict__ lam, double* __restrict__ ml,
-        double* __restrict__ last, unsigned int* __restrict__ work) {
+        double* __restrict__ lam, float* __restrict__ lam32,
+        double* __restrict__ ml, unsigned int* __restrict__ work) {
   size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j == 0) { work[0] = 0; work[1] = 0; }
   if (j >= nv) return;
   const uint32_t m = mj[j];
-  const double l = (double)lam[j];
+  const double l = lam[j];
   const double out = m == 0 ? 0.0 : l + l * l / (2.0 * (double)m) * acc[j];
   acc[j] = 0.0;
-  if (ml) {  // MAP: keep lambda_last (FP64) and lambda_ML for the voxel step
+  if (ml) {
     ml[j] = out;
-    last[j] = l;
   } else {
-    lam[j] = (float)out;
+    lam[j] = out;
+    lam32[j] = (float)out;
   }
 }
 
-__global__ void k_f64_to_f32(size_t n, const double* __restrict__ a,
-                             float* __restrict__ b) {
+// MAP: the voxel step's result becomes the image and its float mirror
+__global__ void k_commit(size_t n, const double* __restrict__ a, double* __restrict__ lam,
+                         float* __restrict__ lam32) {
   size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) b[j] = (float)a[j];
+  if (j < n) {
+    const double v = a[j];
+    lam[j] = v;
+    lam32[j] = (float)v;
+  }
 }
 
 }  // namespace
@@ -726,11 +738,11 @@ static IterArgs iter_args(muto_em_plan* pl) {
   a.n_tasks = f.n_tasks;
   a.work = pl->work.p;
   a.n_seg = f.n_seg;
-  a.seg_lo = f.seg_lo.p;
   a.seg_tfx = f.seg_tfx.p;
+  a.gdesc = f.gdesc.p;
+  a.gcarry = f.gcarry.p;
   a.inv_scale = 1.0 / f.tail_scale;
   a.seg_trk = f.seg_trk.p;
-  a.perm = f.perm.p;
   a.ent_vox = f.ent_vox.p;
   a.ent_len = f.ent_len.p;
   return a;
@@ -839,12 +851,15 @@ int fp32c_build(muto_em_plan* pl) {
     }
     // tasks: bricks cut into runs of about `target` entries (at least 16 per
     // brick voxel so the flush is amortised; small enough for the queue to
-    // balance); tasks into chunks of <= BLOCK segments and <= ECAP entries
+    // balance); tasks into chunks of <= BLOCK segments and <= ECAP entries.
+    // Every chunk starts on a 16-entry boundary: its entries are padded to
+    // whole groups with zero-length entries.
     const uint64_t workers = (uint64_t)std::max(f.grid_fwd, f.grid_bwd);
     const uint64_t target = std::max<uint64_t>(16ull * f.bs.vol(), n_ent / (4 * workers) + 1);
     std::vector<Task> tasks;
     std::vector<uint64_t> tent;
     std::vector<Chunk> chunks;
+    uint64_t n_ent_pad = 0;
     for (uint64_t b = 0; b < nbk; ++b) {
       const uint64_t s0 = hb[b], s1 = hb[b + 1];
       if (s1 == s0) continue;
@@ -866,6 +881,8 @@ int fp32c_build(muto_em_plan* pl) {
           while (q < s1 && cn < BLOCK && ce + hl[q] <= ECAP) { ce += hl[q]; ++cn; ++q; }
           ch.cnt = cn;
           ch.ne = ce;
+          ch.base = n_ent_pad;
+          n_ent_pad += (ce + GE - 1) / GE * GE;
           chunks.push_back(ch);
           te += ce;
         }
@@ -892,12 +909,6 @@ int fp32c_build(muto_em_plan* pl) {
       MUTO_CHECK_HIP(ctx, hipMemcpy(f.tasks.p, sorted.data(), sorted.size() * sizeof(Task), hipMemcpyHostToDevice));
     if (!chunks.empty())
       MUTO_CHECK_HIP(ctx, hipMemcpy(f.chunks.p, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice));
-    // lane -> segment permutation of every chunk (length order)
-    MUTO_CHECK_HIP(ctx, f.perm.alloc(n_seg));
-    if (f.n_chunks) {
-      k_chunk_sort<<<f.n_chunks, BLOCK, 0, s>>>(f.chunks.p, lenq.p, f.perm.p);
-      MUTO_CHECK_HIP(ctx, hipGetLastError());
-    }
     // brick-major segment arrays in final order; tails as fixed point with
     // the finest power-of-two step that keeps twice the grid diagonal (the
     // longest two-leg path inside the grid) below 2^32
@@ -911,9 +922,9 @@ int fp32c_build(muto_em_plan* pl) {
     }
     MUTO_CHECK_HIP(ctx, f.seg_trk.alloc(n_seg));
     MUTO_CHECK_HIP(ctx, f.seg_tfx.alloc(n_seg));
-    MUTO_CHECK_HIP(ctx, f.seg_lo.alloc(n_seg));
-    DBuf<uint64_t> seg_off;
+    DBuf<uint64_t> seg_off, seg_off_p;
     MUTO_CHECK_HIP(ctx, seg_off.alloc(n_seg + 1));
+    MUTO_CHECK_HIP(ctx, seg_off_p.alloc(n_seg + 1));
     if (n_seg) {
       k_seg_gather<<<grid_for(n_seg, 256), 256, 0, s>>>(n_seg, sval.p, st_tend.p, st_trk.p, rank.p,
                                                         f.tail_scale, f.seg_trk.p, f.seg_tfx.p);
@@ -922,16 +933,29 @@ int fp32c_build(muto_em_plan* pl) {
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(lenq.p + n_seg, 0, sizeof(uint32_t), s));
     MUTO_CHECK_HIP(ctx, exclusive_scan_u32_to_u64(lenq.p, seg_off.p, n_seg + 1, s));
     f.n_ent = n_ent;
+    f.n_ent_pad = n_ent_pad;
     if (f.n_chunks) {
-      k_chunk_final<<<grid_for(f.n_chunks, 256), 256, 0, s>>>(f.n_chunks, f.chunks.p, seg_off.p, f.seg_lo.p);
+      k_chunk_offsets<<<grid_for(f.n_chunks, 256), 256, 0, s>>>(f.n_chunks, f.chunks.p, seg_off.p,
+                                                                seg_off_p.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
-    MUTO_CHECK_HIP(ctx, f.ent_vox.alloc(n_ent));
-    MUTO_CHECK_HIP(ctx, f.ent_len.alloc(n_ent));
+    // entries with their padding (zero-length entries of voxel 0)
+    const uint64_t n_grp = n_ent_pad / GE;
+    MUTO_CHECK_HIP(ctx, f.ent_vox.alloc(n_ent_pad + GE));
+    MUTO_CHECK_HIP(ctx, f.ent_len.alloc(n_ent_pad + GE));
+    MUTO_CHECK_HIP(ctx, f.gdesc.alloc(n_grp + 1));
+    MUTO_CHECK_HIP(ctx, f.gcarry.alloc(n_grp + 1));
+    MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.ent_vox.p, 0, (n_ent_pad + GE) * sizeof(uint16_t), s));
+    MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.ent_len.p, 0, (n_ent_pad + GE) * sizeof(float), s));
     if (n_seg) {
       k_ent_scatter<<<grid_for(n_seg, 256), 256, 0, s>>>(G, n_seg, sval.p, st_start.p, st_len.p,
-                                                         seg_off.p, pl->vox.p, pl->len.p,
+                                                         seg_off_p.p, pl->vox.p, pl->len.p,
                                                          f.ent_vox.p, f.ent_len.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    }
+    if (f.n_chunks) {
+      k_group_desc<<<f.n_chunks, BLOCK, 0, s>>>(f.chunks.p, seg_off_p.p, f.ent_len.p, f.gdesc.p,
+                                                f.gcarry.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
     // per-rank segment lists and per-track data
@@ -999,13 +1023,13 @@ int fp32c_iterate(muto_em_plan* pl, bool time_it) {
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
   if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[3], s));
-  k_upd32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->acc64.p, pl->lam32.p,
-                                                map ? pl->ml64.p : nullptr,
-                                                map ? pl->lam64.p : nullptr, pl->work.p);
+  k_upd32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->acc64.p, pl->lam64.p,
+                                                pl->lam32.p, map ? pl->ml64.p : nullptr,
+                                                pl->work.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (map) {
     if (int rc = map_step(pl, pl->lam64.p, pl->ml64.p, pl->tmp64.p)) return rc;
-    k_f64_to_f32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->tmp64.p, pl->lam32.p);
+    k_commit<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->tmp64.p, pl->lam64.p, pl->lam32.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
   if (time_it) {

@@ -12,7 +12,8 @@ struct BrickShape {
 };
 
 // Chunk: up to 256 consecutive brick-major segments [q0, q0 + cnt) whose
-// ne entries start at entry `base`; staged into LDS as one unit.
+// ne entries start at entry `base` (a multiple of 16; the entries are
+// padded to whole 16-entry groups); one block processes it as one unit.
 struct Chunk {
   uint64_t q0;
   uint64_t base;
@@ -33,19 +34,19 @@ struct Fp32cData {
   bool built = false;
   BrickShape bs;
   uint32_t nb[3] = {0, 0, 0};
-  uint64_t n_seg = 0, n_ent = 0;
+  uint64_t n_seg = 0, n_ent = 0, n_ent_pad = 0;
   DBuf<uint32_t> seg_trk;   // [n_seg] track rank
   DBuf<uint32_t> seg_tfx;   // [n_seg] tail after the segment, fixed point
-  DBuf<uint16_t> seg_lo;    // [n_seg] first entry relative to the chunk base
   double tail_scale = 1.0;  // fixed-point scale of seg_tfx (a power of two)
-  DBuf<uint16_t> ent_vox;   // [n_ent] voxel index inside the brick
-  DBuf<float> ent_len;      // [n_ent] path length
+  DBuf<uint16_t> ent_vox;   // [n_ent_pad] voxel index inside the brick
+  DBuf<float> ent_len;      // [n_ent_pad] path length
+  DBuf<uint32_t> gdesc;     // [groups] segment-start mask | first segment << 16
+  DBuf<float> gcarry;       // [groups] length of the last segment after the group
   DBuf<uint32_t> tseg_ptr;  // [n_trk+1] per rank: segment list offsets
   DBuf<uint32_t> tseg_pos;  // [n_seg] brick-major positions, rank order
-  DBuf<uint8_t> perm;       // [n_seg] per chunk: lane -> segment (length order)
   DBuf<double> trk_d;       // [4*n_trk] rank order: D00, D01, D11, p_r
   DBuf<float> seg_part;     // [4*n_seg] forward partial sums (float4, brick-major)
-  DBuf<float> trk_g;        // [4*n_trk] p^2 G (g00, g01, g11, -)
+  DBuf<float> trk_g;        // [4*n_trk] p^2 G' (g00, g01, g11, basis t_c bits)
   DBuf<Task> tasks;
   DBuf<Chunk> chunks;
   uint32_t n_tasks = 0, n_chunks = 0;
