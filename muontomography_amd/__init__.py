@@ -5,6 +5,7 @@ The library is the product; this module only moves numpy arrays across the
 C ABI.  It raises if the library is missing -- there is no fallback.
 """
 import ctypes as C
+import weakref
 import os
 
 import numpy as np
@@ -38,12 +39,17 @@ class Context:
 
     def __init__(self, device=0):
         self._h = C.c_void_p()
+        self._plans = weakref.WeakSet()
         rc = lib().muto_ctx_create(device, C.byref(self._h))
         if rc != 0:
             raise MutoError(f"muto_ctx_create({device}) failed with status {rc}")
 
     def close(self):
+        """Destroys the context; plans still open on it are closed first
+        (a plan must not outlive its context)."""
         if self._h:
+            for p in list(self._plans):
+                p.close()
             lib().muto_ctx_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -146,6 +152,7 @@ class EmPlan:
             rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
             ctx._check(lib().muto_em_plan_create(ctx._h, C.byref(cfg), rays.shape[0], ptr(rays),
                                                  C.byref(self._h)), "muto_em_plan_create")
+        ctx._plans.add(self)
 
     def close(self):
         if self._h:

@@ -60,6 +60,7 @@ namespace {
 
 constexpr uint32_t BLOCK = 256;          // threads per block (4 wavefronts)
 constexpr int GE = 16;                   // entries per thread and chunk
+constexpr uint32_t DIR_BINS = 16;        // track-order direction bins per unit slope
 constexpr uint32_t ECAP = BLOCK * GE;    // entries per chunk, padding included
 // static LDS of both passes is a multiple of 16 bytes, so the dynamic region
 // after it is 16-byte aligned
@@ -96,9 +97,9 @@ __device__ __forceinline__ uint32_t spread12(uint32_t v) {
 
 // ---------------- build ------------------------------------------------
 // Segments per track and the track's spatial key: direction bin of the
-// chord from its first to its last voxel (9 x 9 bins of dx/|dz|, dy/|dz|
-// over [-2, 2]) above the Morton code of where that chord crosses the top
-// plane of the grid.
+// chord from its first to its last voxel (bins of 1/DIR_BINS in dx/|dz| and
+// dy/|dz| over [-2, 2]) above the Morton code of where that chord crosses
+// the top plane of the grid.
 __global__ void __launch_bounds__(256)
 k_seg_count(BrickGeo G, size_t n_trk, size_t nv,
             const uint64_t* __restrict__ row_ptr,
@@ -128,7 +129,11 @@ k_seg_count(BrickGeo G, size_t n_trk, size_t nv,
   const float dz = fmaxf(fabsf(z1 - z0), 1.0f);
   const float sx = fminf(fmaxf((x1 - x0) / dz, -2.0f), 2.0f);
   const float sy = fminf(fmaxf((y1 - y0) / dz, -2.0f), 2.0f);
-  const uint32_t dbin = (uint32_t)((sx + 2.0f) * 2.0f + 0.5f) * 9u + (uint32_t)((sy + 2.0f) * 2.0f + 0.5f);
+  // slope bins of 1/DIR_BINS: over the grid height the tracks of one bin
+  // spread by about a brick, so the tracks crossing a brick in one bin are
+  // a dense run of ranks
+  const uint32_t dbin = (uint32_t)((sx + 2.0f) * DIR_BINS + 0.5f) * (4u * DIR_BINS + 1u) +
+                        (uint32_t)((sy + 2.0f) * DIR_BINS + 0.5f);
   const float s = (z1 != z0) ? ((float)G.n[2] - z0) / (z1 - z0) : 0.0f;
   const float xt = fminf(fmaxf(x0 + s * (x1 - x0) + 1024.0f, 0.0f), 4095.0f);
   const float yt = fminf(fmaxf(y0 + s * (y1 - y0) + 1024.0f, 0.0f), 4095.0f);
