@@ -61,6 +61,10 @@ namespace {
 constexpr uint32_t BLOCK = 256;          // threads per block (4 wavefronts)
 constexpr int GE = 16;                   // entries per thread and chunk
 constexpr uint32_t DIR_BINS = 16;        // track-order direction bins per unit slope
+// backprojection LDS sums (ACC 1) are int64 fixed point: an entry is at
+// most 2^FX_BITS and a task holds at most MAX_TASK_SEG segments
+constexpr int FX_BITS = 46;
+constexpr uint32_t MAX_TASK_SEG = 1u << 16;
 constexpr uint32_t ECAP = BLOCK * GE;    // entries per chunk, padding included
 // static LDS of both passes is a multiple of 16 bytes, so the dynamic region
 // after it is 16-byte aligned
@@ -511,10 +515,11 @@ __global__ void __launch_bounds__(256)
 k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
           const uint32_t* __restrict__ tseg_pos,
           const float4* __restrict__ part, const double* __restrict__ trk_d,
-          ErrM E, double tail_scale, float* __restrict__ trk_g,
+          ErrM E, double tail_scale, double lmax, double ubound,
+          float* __restrict__ trk_g, unsigned long long* __restrict__ cmax,
           double* __restrict__ cost_acc, unsigned long long* __restrict__ cnt) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double cost = 0.0;
+  double cost = 0.0, bnd = 0.0;
   bool sing = false;
   if (r < n_trk) {
     // this track's segment partials: brick-major positions in track order;
@@ -522,8 +527,12 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
     // Moments about the track end (t = 0) in FP64, exact up to FP64 rounding.
     const double inv_scale = 1.0 / tail_scale;
     double S00 = 0.0, S01 = 0.0, S11 = 0.0;
+    uint32_t tmin = 0xFFFFFFFFu, tmax = 0u;
     auto add = [&](const float4 pq) {
-      const double tau = (double)__float_as_uint(pq.w) * inv_scale;
+      const uint32_t tb = __float_as_uint(pq.w);
+      tmin = min(tmin, tb);
+      tmax = max(tmax, tb);
+      const double tau = (double)tb * inv_scale;
       const double p0 = pq.x, p1 = pq.y, p2 = pq.z;
       S00 += p0;
       S01 += fma(tau, p0, p1);
@@ -582,6 +591,21 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
     g.z = (float)(pp * (0.5 * M11 - i11));
     g.w = __uint_as_float(tcx);
     *reinterpret_cast<float4*>(trk_g + 4 * r) = g;
+    // bound on |c_ij| over the track's entries for the backprojection's
+    // fixed-point sums: an entry lies at most R = max|tau_q - t_c| + ubound
+    // from t_c and is at most lmax long, so |<W, G'>| <= lmax (|g00| +
+    // 2 R |g01| + R^2 |g11|)
+    if (k1 > k0) {
+      const double S = fmax((double)tmax * inv_scale - tc, tc - (double)tmin * inv_scale);
+      const double R = fmax(S, 0.0) + ubound;
+      bnd = lmax * (fabs((double)g.x) + 2.0 * R * fabs((double)g.y) + R * R * fabs((double)g.z)) * 1.001;
+    }
+  }
+  {
+    double m = bnd;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(cmax, (unsigned long long)__double_as_longlong(m));
   }
   double v = cost;
 #pragma unroll
@@ -603,7 +627,7 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
 template <int ACC>
 __global__ void __launch_bounds__(BLOCK)
 k_bwd32(IterArgs a, const float* __restrict__ trk_g,
-        double* __restrict__ acc) {
+        const unsigned long long* __restrict__ cmax, double* __restrict__ acc) {
   // static LDS is a multiple of 16 bytes and the dynamic region is declared
   // as 16-byte aligned doubles: ds_add_f64 needs 8-byte aligned addresses
   extern __shared__ __align__(16) double lds_dyn[];
@@ -615,7 +639,13 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
   const uint32_t vol = a.G.b[0] * a.G.b[1] * a.G.b[2];
   float* hi_s = lds_acc;
   float* lo_s = lds_acc + vol;
-  double* d_s = lds_dyn;
+  long long* d_s = reinterpret_cast<long long*>(lds_dyn);
+  // fixed-point scale of the LDS sums: every |c_ij| is below the bound the
+  // track kernel found, so a scaled entry is below 2^FX_BITS and a task's
+  // sum per voxel (at most one entry per segment) stays inside int64
+  const double cb = __longlong_as_double((long long)cmax[0]);
+  const double fx = cb > 0.0 ? ldexp(1.0, FX_BITS) / cb : 1.0;
+  const double fx_inv = 1.0 / fx;
   uint32_t buf = 0;
   for (;;) {
     if (t == 0) sh = atomicAdd(&a.work[1], 1u);
@@ -652,7 +682,11 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
           const float cv = fmaf(L, g.x, fmaf(2.0f * w01, g.y, w11 * g.z));
           const uint32_t v = gr.vox(i);
           if (ACC == 1) {
-            atomicAdd(&d_s[v], (double)cv);
+            // round(cv * fx) as int64: adding 1.5 * 2^52 leaves the integer
+            // in the low mantissa bits (|cv * fx| < 2^51)
+            const double m = fma((double)cv, fx, 6755399441055744.0);
+            const long long q = __double_as_longlong(m) - 0x4338000000000000LL;
+            atomicAdd(reinterpret_cast<unsigned long long*>(&d_s[v]), (unsigned long long)q);
           } else {
             const float old = atomicAdd(&hi_s[v], cv);
             const float sum = old + cv;
@@ -674,7 +708,7 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
       const uint32_t x = o[0] + lx, y = o[1] + ly, z = o[2] + lz;
       if (x >= a.G.n[0] || y >= a.G.n[1] || z >= a.G.n[2]) continue;
       const size_t j = (size_t)x + (size_t)a.G.n[0] * ((size_t)y + (size_t)a.G.n[1] * z);
-      const double val = ACC == 1 ? d_s[v] : (double)hi_s[v] + (double)lo_s[v];
+      const double val = ACC == 1 ? (double)d_s[v] * fx_inv : (double)hi_s[v] + (double)lo_s[v];
       if (tk.owner)
         acc[j] = val;
       else if (val != 0.0)
@@ -692,9 +726,10 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
 __global__ void __launch_bounds__(256)
 k_upd32(size_t nv, const uint32_t* __restrict__ mj, double* __restrict__ acc,
         double* __restrict__ lam, float* __restrict__ lam32,
-        double* __restrict__ ml, unsigned int* __restrict__ work) {
+        double* __restrict__ ml, unsigned int* __restrict__ work,
+        unsigned long long* __restrict__ cmax) {
   size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j == 0) { work[0] = 0; work[1] = 0; }
+  if (j == 0) { work[0] = 0; work[1] = 0; cmax[0] = 0ull; }
   if (j >= nv) return;
   const uint32_t m = mj[j];
   const double l = lam[j];
@@ -879,7 +914,8 @@ int fp32c_build(muto_em_plan* pl) {
         tk.brick = (uint32_t)b;
         tk.c0 = (uint32_t)chunks.size();
         uint64_t te = 0;
-        while (q < s1 && (te < per || bt.size() + 1 == k)) {
+        const uint64_t tq0 = q;
+        while (q < s1 && (te < per || bt.size() + 1 == k) && q - tq0 + BLOCK <= MAX_TASK_SEG) {
           Chunk ch{};
           ch.q0 = q;
           uint32_t ce = 0, cn = 0;
@@ -989,6 +1025,14 @@ int fp32c_build(muto_em_plan* pl) {
   }
   MUTO_CHECK_HIP(ctx, f.seg_part.alloc(4 * n_seg));
   MUTO_CHECK_HIP(ctx, f.trk_g.alloc(4 * nt));
+  MUTO_CHECK_HIP(ctx, f.cmax.alloc(1));
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.cmax.p, 0, sizeof(unsigned long long), s));
+  {
+    const double dv = sqrt(pl->g.d[0] * pl->g.d[0] + pl->g.d[1] * pl->g.d[1] + pl->g.d[2] * pl->g.d[2]);
+    const double bx = f.bs.bx * pl->g.d[0], by = f.bs.by * pl->g.d[1], bz = f.bs.bz * pl->g.d[2];
+    f.lmax = 2.0 * dv;  // a merged entry holds both legs
+    f.ubound = 2.0 * sqrt(bx * bx + by * by + bz * bz) + f.lmax;
+  }
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->work.p, 0, 4 * sizeof(unsigned int), s));
   MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
   f.built = true;
@@ -1016,21 +1060,21 @@ int fp32c_iterate(muto_em_plan* pl, bool time_it) {
   if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[1], s));
   k_track32<<<grid_for(pl->n_trk, 256), 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p,
                                                       reinterpret_cast<const float4*>(f.seg_part.p),
-                                                      f.trk_d.p, E, f.tail_scale, f.trk_g.p,
-                                                      pl->cost_acc.p, pl->dcount.p);
+                                                      f.trk_d.p, E, f.tail_scale, f.lmax, f.ubound,
+                                                      f.trk_g.p, f.cmax.p, pl->cost_acc.p, pl->dcount.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[2], s));
   if (f.n_tasks) {
     if (f.acc_mode == 1)
-      k_bwd32<1><<<f.grid_bwd, BLOCK, lds_b, s>>>(a, f.trk_g.p, pl->acc64.p);
+      k_bwd32<1><<<f.grid_bwd, BLOCK, lds_b, s>>>(a, f.trk_g.p, f.cmax.p, pl->acc64.p);
     else
-      k_bwd32<0><<<f.grid_bwd, BLOCK, lds_b, s>>>(a, f.trk_g.p, pl->acc64.p);
+      k_bwd32<0><<<f.grid_bwd, BLOCK, lds_b, s>>>(a, f.trk_g.p, f.cmax.p, pl->acc64.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
   if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[3], s));
   k_upd32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->acc64.p, pl->lam64.p,
                                                 pl->lam32.p, map ? pl->ml64.p : nullptr,
-                                                pl->work.p);
+                                                pl->work.p, f.cmax.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (map) {
     if (int rc = map_step(pl, pl->lam64.p, pl->ml64.p, pl->tmp64.p)) return rc;

@@ -47,6 +47,8 @@ struct Fp32cData {
   DBuf<double> trk_d;       // [4*n_trk] rank order: D00, D01, D11, p_r
   DBuf<float> seg_part;     // [4*n_seg] forward partial sums (float4, brick-major)
   DBuf<float> trk_g;        // [4*n_trk] p^2 G' (g00, g01, g11, basis t_c bits)
+  DBuf<unsigned long long> cmax;  // [1] bound on |c_ij| (FP64 bits), per iteration
+  double lmax = 0.0, ubound = 0.0;  // entry length bound; segment extent bound
   DBuf<Task> tasks;
   DBuf<Chunk> chunks;
   uint32_t n_tasks = 0, n_chunks = 0;

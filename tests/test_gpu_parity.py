@@ -230,6 +230,29 @@ def test_fp32c_matches_fp64(ctx, small_set):
     assert p32.counters()["n_segments"] > 0
 
 
+def test_fp32c_precision_em256(ctx):
+    """FP32C against FP64 on the 256^3 x 10 mm grid with 2e6 muons (about
+    3.6e8 entries, 2.3e7 brick segments): 40 mean-update iterations from a
+    uniform start, max relative image error <= 1e-4 over voxels with
+    entries (the FP32C accuracy target), costs within 1e-6."""
+    from muontomography_amd import workloads as W
+    g, rays, _ = W.muons("em256", n_muons=2_000_000)
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100)
+    plan = ctx.em_plan(cfg, rays=rays)
+    img = np.full(g.n_vox, 1e-7)
+    plan.set_image(img)
+    c64 = plan.run(40, M.MUTO_FP64)
+    a64 = plan.image()
+    plan.set_image(img)
+    c32 = plan.run(40, M.MUTO_FP32C)
+    a32 = plan.image()
+    mx, rms, norm = _rel_stats(a32, a64, plan.voxel_counts())
+    print(f"fp32c em256 2e6: max {mx:.3e} rms {rms:.3e} norm {norm:.3e}", flush=True)
+    assert mx <= 1e-4, (mx, rms, norm)
+    np.testing.assert_allclose(c32, c64, rtol=1e-6)
+    plan.close()
+
+
 @pytest.mark.parametrize("nb,obj,p", [(0, 0, 1.0), (1, 1, 2.0)])
 def test_map_fp64(ctx, small_set, nb, obj, p):
     """MAP (MutoMAP.cpp): ML step + per-voxel cubic surrogate step.  On the
