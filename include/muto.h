@@ -206,6 +206,18 @@ int muto_em_plan_run(muto_em_plan* plan, int iters, int precision,
  * step with cfg.beta / p / delta / add_object_prior / all_neighbours. */
 int muto_map_plan_run(muto_em_plan* plan, int iters, int precision,
                       double* cost);
+/* Plan options (value 0/1 unless stated).
+ *   MUTO_OPT_GRAPH    FP32C iterations replay one captured HIP graph (default 1)
+ *   MUTO_OPT_TIMING   record per-kernel times for muto_em_plan_timing; runs
+ *                     without the graph (default 0)
+ *   MUTO_OPT_ACC_MODE FP32C accumulation: 1 FP64 registers + int64 fixed-point
+ *                     LDS sums (default), 0 FP32 with two-sum compensation
+ * No counterpart in the reference (which has no device state). */
+#define MUTO_OPT_GRAPH 1
+#define MUTO_OPT_TIMING 2
+#define MUTO_OPT_ACC_MODE 3
+int muto_em_plan_set_option(muto_em_plan* plan, int option, int value);
+
 int muto_em_plan_counters(const muto_em_plan* plan, muto_counters* out);
 /* Per-voxel entry counts M_j (host, nx*ny*nz uint32). */
 int muto_em_plan_voxel_counts(muto_em_plan* plan, uint32_t* mj);

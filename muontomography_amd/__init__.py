@@ -30,6 +30,11 @@ def lib():
     return _lib
 
 
+MUTO_OPT_GRAPH = 1
+MUTO_OPT_TIMING = 2
+MUTO_OPT_ACC_MODE = 3
+
+
 class MutoError(RuntimeError):
     pass
 
@@ -204,6 +209,11 @@ class EmPlan:
         self.ctx._check(lib().muto_em_plan_export(self._h, ptr(row), ptr(vox), ptr(ln), ptr(tl),
                                                   ptr(t4), ptr(pr)), "export")
         return row, vox[:nnz], ln[:nnz], tl[:nnz], t4[:nt], pr[:nt]
+
+    def set_option(self, option, value):
+        """MUTO_OPT_GRAPH, MUTO_OPT_TIMING or MUTO_OPT_ACC_MODE (include/muto.h)."""
+        self.ctx._check(lib().muto_em_plan_set_option(self._h, int(option), int(value)),
+                        "set_option")
 
     def timing(self):
         t = np.zeros(5, np.float64)

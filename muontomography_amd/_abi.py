@@ -155,6 +155,7 @@ PROTOS = {
     "muto_em_plan_counters": (C.c_int, [P, C.POINTER(Counters)]),
     "muto_em_plan_voxel_counts": (C.c_int, [P, P]),
     "muto_em_plan_timing": (C.c_int, [P, P]),
+    "muto_em_plan_set_option": (C.c_int, [P, C.c_int, C.c_int]),
     "muto_em_plan_export": (C.c_int, [P, P, P, P, P, P, P]),
     "muto_gen_default": (None, [C.POINTER(GenCfg), C.POINTER(Grid), C.c_uint64]),
     "muto_gen_layout": (C.c_int, [C.POINTER(Grid), C.c_uint64, C.c_int, C.c_double, C.c_double,

@@ -569,7 +569,7 @@ MapCfgD map_cfg(const muto_em_plan* pl) {
 
 // FP32C iteration (em32.hip)
 int fp32c_build(muto_em_plan* pl);
-int fp32c_iterate(muto_em_plan* pl, bool time_it);
+int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done);
 int fp32c_release(muto_em_plan* pl);
 
 }  // namespace muto
@@ -939,6 +939,20 @@ static int plan_run(muto_em_plan* pl, int iters, int precision, double* cost) {
   int done = 0;
   double tsum[5] = {0, 0, 0, 0, 0};
   const bool time_it = true;
+  if (precision == MUTO_FP32C) {
+    std::vector<double> sums(iters, 0.0);
+    if (int rc = fp32c_run(pl, iters, sums.data(), &done)) return rc;
+    for (int it = 0; it < done; ++it) {
+      const double cst = sums[it] / (double)pl->n_trk;  // MutoMLSD.cpp:202
+      if (c.log_results) printf("Iteration #%d finished.\n\tcost function = %f\n", it, cst);
+      if (c.early_stop && it + 1 == done && it > 0 && costs[it - 1] < cst && c.log_results)
+        printf("Early stop at iteration#%d. Minimum cost function = %f\n", it, costs[it - 1]);
+      costs[it] = cst;
+    }
+    if (cost) for (int k = 0; k < done; ++k) cost[k] = costs[k];
+    pl->counters.iterations_done = done;
+    return read_counters(pl);
+  }
   for (int it = 0; it < iters; ++it) {
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->cost_acc.p, 0, sizeof(double), s));
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->dcount.p, 0, 3 * sizeof(unsigned long long), s));
@@ -961,9 +975,6 @@ static int plan_run(muto_em_plan* pl, int iters, int precision, double* cost) {
         tsum[3] += d - a;  // voxel update (+ MAP step)
         tsum[4] += d;
       }
-    } else {
-      if (int rc = fp32c_iterate(pl, time_it)) return rc;
-      for (int k = 0; k < 5; ++k) tsum[k] += pl->t_ms[k];
     }
     double csum = 0.0;
     MUTO_CHECK_HIP(ctx, hipMemcpyAsync(&csum, pl->cost_acc.p, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -983,6 +994,31 @@ static int plan_run(muto_em_plan* pl, int iters, int precision, double* cost) {
   pl->counters.iterations_done = done;
   if (int rc = read_counters(pl)) return rc;
   return MUTO_OK;
+}
+
+extern "C" int muto_em_plan_set_option(muto_em_plan* pl, int option, int value) {
+  if (!pl) return MUTO_ERR_ARG;
+  switch (option) {
+    case MUTO_OPT_GRAPH:
+      pl->use_graph = value != 0;
+      return MUTO_OK;
+    case MUTO_OPT_TIMING:
+      pl->timing = value != 0;
+      return MUTO_OK;
+    case MUTO_OPT_ACC_MODE:
+      if (value != 0 && value != 1) break;
+      if (pl->f32.acc_mode != value && pl->f32.graph) {
+        (void)hipGraphExecDestroy(pl->f32.graph);
+        pl->f32.graph = nullptr;
+      }
+      pl->f32.acc_mode = value;
+      pl->f32.acc_mode_set = true;
+      return MUTO_OK;
+    default:
+      break;
+  }
+  pl->ctx->set_error("muto_em_plan_set_option: unknown option or value");
+  return MUTO_ERR_ARG;
 }
 
 extern "C" int muto_em_plan_counters(const muto_em_plan* pl, muto_counters* out) {

@@ -743,6 +743,22 @@ k_upd32(size_t nv, const uint32_t* __restrict__ mj, double* __restrict__ acc,
   }
 }
 
+// clears the iteration's cost sum and per-iteration counters
+__global__ void k_iter_begin(double* __restrict__ cost_acc, unsigned long long* __restrict__ cnt) {
+  if (threadIdx.x == 0) cost_acc[0] = 0.0;
+  if (threadIdx.x < 3) cnt[threadIdx.x] = 0ull;
+}
+
+// appends the iteration's cost sum to the history
+__global__ void k_cost_push(const double* __restrict__ cost_acc, double* __restrict__ hist,
+                            unsigned int* __restrict__ n) {
+  if (threadIdx.x == 0) {
+    const unsigned int k = n[0];
+    hist[k] = cost_acc[0];
+    n[0] = k + 1u;
+  }
+}
+
 // MAP: the voxel step's result becomes the image and its float mirror
 __global__ void k_commit(size_t n, const double* __restrict__ a, double* __restrict__ lam,
                          float* __restrict__ lam32) {
@@ -815,7 +831,7 @@ int fp32c_build(muto_em_plan* pl) {
   hipStream_t s = ctx->stream;
   {
     const char* ev = getenv("MUTO_FP32C_ACC");
-    if (ev) f.acc_mode = atoi(ev);
+    if (ev && !f.acc_mode_set) f.acc_mode = atoi(ev);
     if (f.acc_mode != 0 && f.acc_mode != 1) f.acc_mode = 1;
   }
   const BrickGeo G = brick_geo(pl);
@@ -1040,7 +1056,12 @@ int fp32c_build(muto_em_plan* pl) {
   return MUTO_OK;
 }
 
-int fp32c_iterate(muto_em_plan* pl, bool time_it) {
+// One FP32C iteration enqueued on the plan's stream with no host
+// synchronisation and no memset, so it can be captured into a graph: a
+// kernel clears the per-iteration sums and counters, the last kernel appends
+// the iteration's cost sum to the device cost history.  With events (5),
+// kernel boundaries are recorded for per-kernel timing.
+static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
   Fp32cData& f = pl->f32;
   muto_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream;
@@ -1048,8 +1069,9 @@ int fp32c_iterate(muto_em_plan* pl, bool time_it) {
   const size_t lds_f = f.bs.vol() * sizeof(float), lds_b = 2 * f.bs.vol() * sizeof(float);
   const ErrM E = error_matrix(pl->cfg);
   const bool map = pl->is_map;
-  if (map && !pl->tmp64.p) MUTO_CHECK_HIP(ctx, pl->tmp64.alloc(pl->nv));
-  if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[0], s));
+  k_iter_begin<<<1, 64, 0, s>>>(pl->cost_acc.p, pl->dcount.p);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[0], s));
   if (f.n_tasks) {
     if (f.acc_mode == 1)
       k_fwd32<1><<<f.grid_fwd, BLOCK, lds_f, s>>>(a, pl->lam32.p, reinterpret_cast<float4*>(f.seg_part.p));
@@ -1057,13 +1079,13 @@ int fp32c_iterate(muto_em_plan* pl, bool time_it) {
       k_fwd32<0><<<f.grid_fwd, BLOCK, lds_f, s>>>(a, pl->lam32.p, reinterpret_cast<float4*>(f.seg_part.p));
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
-  if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[1], s));
+  if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[1], s));
   k_track32<<<grid_for(pl->n_trk, 256), 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p,
                                                       reinterpret_cast<const float4*>(f.seg_part.p),
                                                       f.trk_d.p, E, f.tail_scale, f.lmax, f.ubound,
                                                       f.trk_g.p, f.cmax.p, pl->cost_acc.p, pl->dcount.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
-  if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[2], s));
+  if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[2], s));
   if (f.n_tasks) {
     if (f.acc_mode == 1)
       k_bwd32<1><<<f.grid_bwd, BLOCK, lds_b, s>>>(a, f.trk_g.p, f.cmax.p, pl->acc64.p);
@@ -1071,7 +1093,7 @@ int fp32c_iterate(muto_em_plan* pl, bool time_it) {
       k_bwd32<0><<<f.grid_bwd, BLOCK, lds_b, s>>>(a, f.trk_g.p, f.cmax.p, pl->acc64.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
-  if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[3], s));
+  if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[3], s));
   k_upd32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->acc64.p, pl->lam64.p,
                                                 pl->lam32.p, map ? pl->ml64.p : nullptr,
                                                 pl->work.p, f.cmax.p);
@@ -1081,18 +1103,88 @@ int fp32c_iterate(muto_em_plan* pl, bool time_it) {
     k_commit<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->tmp64.p, pl->lam64.p, pl->lam32.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
-  if (time_it) {
-    MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[4], s));
-    MUTO_CHECK_HIP(ctx, hipEventSynchronize(pl->ev[4]));
-    float t[4];
-    for (int k = 0; k < 4; ++k)
-      MUTO_CHECK_HIP(ctx, hipEventElapsedTime(&t[k], pl->ev[k], pl->ev[k + 1]));
-    pl->t_ms[0] = t[0];
-    pl->t_ms[1] = t[1];
-    pl->t_ms[2] = t[2];
-    pl->t_ms[3] = t[3];
-    pl->t_ms[4] = t[0] + t[1] + t[2] + t[3];
+  k_cost_push<<<1, 64, 0, s>>>(pl->cost_acc.p, f.hist.p, f.hist_n.p);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[4], s));
+  return MUTO_OK;
+}
+
+// The iteration as a graph, captured once per plan (again when the cost
+// history grows or the MAP flag changes).
+static int fp32c_graph(muto_em_plan* pl) {
+  Fp32cData& f = pl->f32;
+  muto_ctx* ctx = pl->ctx;
+  if (f.graph && f.graph_map == pl->is_map) return MUTO_OK;
+  if (f.graph) (void)hipGraphExecDestroy(f.graph);
+  f.graph = nullptr;
+  hipGraph_t g = nullptr;
+  MUTO_CHECK_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = fp32c_enqueue(pl, nullptr);
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (rc != MUTO_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
   }
+  MUTO_CHECK_HIP(ctx, e);
+  const hipError_t ei = hipGraphInstantiate(&f.graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  MUTO_CHECK_HIP(ctx, ei);
+  f.graph_map = pl->is_map;
+  return MUTO_OK;
+}
+
+// `iters` FP32C iterations; cost sums per iteration into sums[], *done the
+// iterations run.  Early stop (MutoMLSD.cpp:203-210) needs each cost before
+// the next iteration starts: then every iteration is synchronised,
+// otherwise all are enqueued back to back and the costs read once.
+// Per-kernel timing (pl->timing) runs without the graph, with events.
+int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done) {
+  Fp32cData& f = pl->f32;
+  muto_ctx* ctx = pl->ctx;
+  hipStream_t s = ctx->stream;
+  *done = 0;
+  if (pl->is_map && !pl->tmp64.p) MUTO_CHECK_HIP(ctx, pl->tmp64.alloc(pl->nv));
+  if (f.hist_cap < (size_t)iters) {
+    if (f.graph) (void)hipGraphExecDestroy(f.graph);
+    f.graph = nullptr;
+    f.hist.release();
+    MUTO_CHECK_HIP(ctx, f.hist.alloc((size_t)iters));
+    f.hist_cap = (size_t)iters;
+  }
+  if (!f.hist_n.p) MUTO_CHECK_HIP(ctx, f.hist_n.alloc(1));
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.hist_n.p, 0, sizeof(unsigned int), s));
+  const bool graph = pl->use_graph && !pl->timing;
+  if (graph)
+    if (int rc = fp32c_graph(pl)) return rc;
+  const bool each = pl->cfg.early_stop || pl->timing;
+  double tsum[5] = {0, 0, 0, 0, 0};
+  double prev = INFINITY;
+  for (int it = 0; it < iters; ++it) {
+    if (graph)
+      MUTO_CHECK_HIP(ctx, hipGraphLaunch(f.graph, s));
+    else if (int rc = fp32c_enqueue(pl, pl->timing ? pl->ev : nullptr))
+      return rc;
+    if (!each) continue;
+    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(&sums[it], f.hist.p + it, sizeof(double), hipMemcpyDeviceToHost, s));
+    MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    *done = it + 1;
+    if (pl->timing) {
+      float t[4];
+      for (int k = 0; k < 4; ++k) MUTO_CHECK_HIP(ctx, hipEventElapsedTime(&t[k], pl->ev[k], pl->ev[k + 1]));
+      for (int k = 0; k < 4; ++k) tsum[k] += t[k];
+      tsum[4] += t[0] + t[1] + t[2] + t[3];
+    }
+    const double c = sums[it] / (double)pl->n_trk;
+    if (pl->cfg.early_stop && prev < c) break;
+    prev = c;
+  }
+  if (!each && iters > 0) {
+    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(sums, f.hist.p, iters * sizeof(double), hipMemcpyDeviceToHost, s));
+    MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    *done = iters;
+  }
+  if (pl->timing && *done)
+    for (int k = 0; k < 5; ++k) pl->t_ms[k] = tsum[k] / *done;
   return MUTO_OK;
 }
 

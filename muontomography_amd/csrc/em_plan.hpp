@@ -53,8 +53,13 @@ struct Fp32cData {
   DBuf<Chunk> chunks;
   uint32_t n_tasks = 0, n_chunks = 0;
   int grid_fwd = 0, grid_bwd = 0, block = 256;
-  int acc_mode = 1;  // accumulation: 1 FP64, 0 FP32 with two-sum
-  hipGraphExec_t graph = nullptr;
+  int acc_mode = 1;  // accumulation: 1 FP64 / int64 fixed point, 0 FP32 two-sum
+  bool acc_mode_set = false;  // set through the option call (else environment)
+  hipGraphExec_t graph = nullptr;  // one iteration, captured
+  bool graph_map = false;          // the MAP flag the graph was captured with
+  DBuf<double> hist;               // [hist_cap] cost sum per iteration of a run
+  DBuf<unsigned int> hist_n;       // [1] entries in hist
+  size_t hist_cap = 0;
 };
 
 }  // namespace muto
@@ -92,6 +97,8 @@ struct muto_em_plan {
   muto::DBuf<unsigned long long> dcount;  // device counters [8]
   muto::DBuf<unsigned int> work;   // work-queue counters [4]
   int cur_prec = -1;               // which image is current
+  int use_graph = 1;               // FP32C: replay the captured iteration
+  int timing = 0;                  // per-kernel events (no graph)
 
   muto::Fp32cData f32;
 

@@ -230,6 +230,32 @@ def test_fp32c_matches_fp64(ctx, small_set):
     assert p32.counters()["n_segments"] > 0
 
 
+def test_fp32c_graph_replays_match_eager(ctx, small_set):
+    """The captured FP32C iteration graph, replayed one iteration at a time
+    with host reads in between and then several iterations back to back,
+    gives the images and costs of the eager launches (up to the order of
+    floating-point atomics: rtol 1e-9)."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100)
+    img = np.full(g.n_vox, 1e-7)
+    out = {}
+    for graph in (0, 1):
+        plan = ctx.em_plan(cfg, rays=rays)
+        plan.set_option(M.MUTO_OPT_GRAPH, graph)
+        plan.set_image(img)
+        steps = []
+        for _ in range(3):
+            c = plan.run(1, M.MUTO_FP32C)
+            steps.append((plan.image(), c))
+        c4 = plan.run(4, M.MUTO_FP32C)
+        steps.append((plan.image(), c4))
+        out[graph] = steps
+        plan.close()
+    for (a_e, c_e), (a_g, c_g) in zip(out[0], out[1]):
+        np.testing.assert_allclose(a_g, a_e, rtol=1e-9, atol=1e-30)
+        np.testing.assert_allclose(c_g, c_e, rtol=1e-9)
+
+
 def test_fp32c_precision_em256(ctx):
     """FP32C against FP64 on the 256^3 x 10 mm grid with 2e6 muons (about
     3.6e8 entries, 2.3e7 brick segments): 40 mean-update iterations from a
