@@ -206,6 +206,41 @@ int muto_em_plan_run(muto_em_plan* plan, int iters, int precision,
  * step with cfg.beta / p / delta / add_object_prior / all_neighbours. */
 int muto_map_plan_run(muto_em_plan* plan, int iters, int precision,
                       double* cost);
+/* As muto_em_plan_create, with a budget for the track-major path lists
+ * (20 bytes per track-voxel entry: u32 voxel, f64 length, f64 tail).  Above
+ * it the plan keeps no lists: muto_em_plan_run refuses it, and the MAP
+ * objective operator below walks every path again by Siddon (FP64).  No
+ * counterpart in the reference, which always stores its paths
+ * (MutoMLSD.cpp:382-415). */
+int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg, size_t n,
+                           const muto_ray* rays, uint64_t list_budget_bytes,
+                           muto_em_plan** out);
+
+/* ---- MAP objective operator (forward / backward with saved context) ----
+ *   Phi(lambda) = 1/2 sum_i cost_i(lambda) + beta R(lambda)
+ * cost_i: the EM iteration's per-track cost (MutoMLSD.cpp:456-468; 0 for a
+ * singular or non-positive Sigma_i); R: the MAP prior of MutoMAP.cpp:207-237
+ * (psi(t) = (t^2 + delta^2)^(p/2) over the 6 or 26 clamp-to-edge
+ * neighbours with weights 1, 1/sqrt2, 1/sqrt3, plus psi(l_j) with
+ * add_object_prior; l = 1e7 lambda), summed over all voxels.  The reference
+ * never evaluates Phi or its gradient; in mean mode its MAP step
+ * (MutoMAP.cpp:162-184) minimises a per-voxel surrogate tangent to Phi, so
+ * this gradient is the reference's MAP gradient there.
+ * muto_map_forward evaluates Phi at lambda (host, n_vox doubles) on the
+ * plan's tracks and returns a saved context; muto_map_backward writes
+ * grad_lambda = grad_phi dPhi/dlambda (host, n_vox doubles) and may be
+ * called several times.  precision: MUTO_FP64 (track-major, FP64) or
+ * MUTO_FP32C (brick stream, FP32 entry terms).  flags:
+ * MUTO_MAP_RECOMPUTE_SIDDON walks the paths again instead of reading the
+ * stored lists (FP64; implied for plans without lists).  A saved context
+ * must be destroyed before its plan. */
+typedef struct muto_map_saved muto_map_saved;
+#define MUTO_MAP_RECOMPUTE_SIDDON 1
+int muto_map_forward(muto_em_plan* plan, const double* lambda, int precision, int flags,
+                     double* phi, muto_map_saved** saved);
+int muto_map_backward(muto_map_saved* saved, double grad_phi, double* grad_lambda);
+int muto_map_saved_destroy(muto_map_saved* saved);
+
 /* Plan options (value 0/1 unless stated).
  *   MUTO_OPT_GRAPH    FP32C iterations replay one captured HIP graph (default 1)
  *   MUTO_OPT_TIMING   record per-kernel times for muto_em_plan_timing; runs

@@ -30,6 +30,7 @@ def lib():
     return _lib
 
 
+MUTO_MAP_RECOMPUTE_SIDDON = 1
 MUTO_OPT_GRAPH = 1
 MUTO_OPT_TIMING = 2
 MUTO_OPT_ACC_MODE = 3
@@ -131,17 +132,21 @@ class Context:
     def map_reconstruct(self, cfg, rays, init=None):
         return self._recon(lib().muto_map_reconstruct, cfg, rays, init)
 
-    def em_plan(self, cfg, rays=None, prepared=None):
-        return EmPlan(self, cfg, rays=rays, prepared=prepared)
+    def em_plan(self, cfg, rays=None, prepared=None, list_budget=None):
+        """list_budget: bytes allowed for the track-major path lists (20 per
+        entry); above it the plan keeps none and only the MAP objective
+        operator runs, walking the paths again."""
+        return EmPlan(self, cfg, rays=rays, prepared=prepared, list_budget=list_budget)
 
 
 class EmPlan:
     """Device-resident EM problem (muto_em_plan)."""
 
-    def __init__(self, ctx, cfg, rays=None, prepared=None):
+    def __init__(self, ctx, cfg, rays=None, prepared=None, list_budget=None):
         self.ctx = ctx
         self.cfg = cfg
         self._h = C.c_void_p()
+        self._saved = weakref.WeakSet()
         if prepared is not None:
             trk4, p, row, vox, ln = prepared
             self._keep = [np.ascontiguousarray(trk4, np.float64),
@@ -155,12 +160,16 @@ class EmPlan:
                        "muto_em_plan_create_prepared")
         else:
             rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
-            ctx._check(lib().muto_em_plan_create(ctx._h, C.byref(cfg), rays.shape[0], ptr(rays),
-                                                 C.byref(self._h)), "muto_em_plan_create")
+            budget = (1 << 64) - 1 if list_budget is None else int(list_budget)
+            ctx._check(lib().muto_em_plan_create_ex(ctx._h, C.byref(cfg), rays.shape[0], ptr(rays),
+                                                    budget, C.byref(self._h)),
+                       "muto_em_plan_create_ex")
         ctx._plans.add(self)
 
     def close(self):
         if self._h:
+            for sv in list(self._saved):
+                sv.close()
             lib().muto_em_plan_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -210,6 +219,20 @@ class EmPlan:
                                                   ptr(t4), ptr(pr)), "export")
         return row, vox[:nnz], ln[:nnz], tl[:nnz], t4[:nt], pr[:nt]
 
+    def map_forward(self, lam, precision=MUTO_FP64, recompute=False):
+        """Phi(lam) of the MAP objective (muto_map_forward); returns a
+        MapSaved whose backward(grad_phi) gives grad_phi * dPhi/dlam."""
+        lam = np.ascontiguousarray(lam, np.float64).ravel()
+        assert lam.size == self.cfg.grid.n_vox
+        phi = C.c_double()
+        h = C.c_void_p()
+        self.ctx._check(lib().muto_map_forward(self._h, ptr(lam), int(precision),
+                                               MUTO_MAP_RECOMPUTE_SIDDON if recompute else 0,
+                                               C.byref(phi), C.byref(h)), "muto_map_forward")
+        sv = MapSaved(self, h, phi.value)
+        self._saved.add(sv)
+        return sv
+
     def set_option(self, option, value):
         """MUTO_OPT_GRAPH, MUTO_OPT_TIMING or MUTO_OPT_ACC_MODE (include/muto.h)."""
         self.ctx._check(lib().muto_em_plan_set_option(self._h, int(option), int(value)),
@@ -219,6 +242,33 @@ class EmPlan:
         t = np.zeros(5, np.float64)
         self.ctx._check(lib().muto_em_plan_timing(self._h, ptr(t)), "timing")
         return {"forward": t[0], "track": t[1], "backward": t[2], "update": t[3], "iteration": t[4]}
+
+
+class MapSaved:
+    """Saved context of one muto_map_forward call (autograd-Function style):
+    phi, and backward(grad_phi) -> grad_phi * dPhi/dlambda (n_vox doubles)."""
+
+    def __init__(self, plan, h, phi):
+        self.plan = plan
+        self._h = h
+        self.phi = phi
+
+    def backward(self, grad_phi=1.0):
+        g = np.empty(self.plan.cfg.grid.n_vox, np.float64)
+        self.plan.ctx._check(lib().muto_map_backward(self._h, float(grad_phi), ptr(g)),
+                             "muto_map_backward")
+        return g
+
+    def close(self):
+        if self._h:
+            lib().muto_map_saved_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # -- synthetic muons (host code in libmuto) -------------------------------

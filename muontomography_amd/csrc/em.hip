@@ -19,45 +19,15 @@
 #include <algorithm>
 #include <vector>
 
+#include "em_dev.hpp"
 #include "em_plan.hpp"
 #include "geom_dev.hpp"
 
 namespace muto {
 
 // ------------------------------------------------------------------ //
-// shared device helpers                                              //
+// host helpers shared with the other kernels' files                  //
 // ------------------------------------------------------------------ //
-struct Mat2 {
-  double m00, m01, m10, m11;
-};
-
-// Eigen computeInverseAndDetWithCheck for 2x2: det = m00*m11 - m10*m01,
-// invertible iff |det| > 1e-12, inverse = adj * (1/det), untouched (zero)
-// otherwise.
-__device__ __forceinline__ bool inv_det(const Mat2& m, Mat2& inv,
-                                        double& det) {
-  det = m.m00 * m.m11 - m.m10 * m.m01;
-  inv.m00 = inv.m01 = inv.m10 = inv.m11 = 0.0;
-  if (!(fabs(det) > 1e-12)) return false;
-  const double invdet = 1.0 / det;
-  inv.m00 = m.m11 * invdet;
-  inv.m10 = -m.m10 * invdet;
-  inv.m01 = -m.m01 * invdet;
-  inv.m11 = m.m00 * invdet;
-  return true;
-}
-
-__device__ __forceinline__ void weight(double L, double T, Mat2& W) {
-  const double LT = L * T;  // generateWeightMatrix, MutoMLSD.cpp:417-430
-  W.m00 = L;
-  W.m01 = W.m10 = LT + L * L * 0.5;
-  W.m11 = L * L * L / 3.0 + (L + T) * LT;
-}
-
-struct ErrM {
-  double e00, e01, e11;
-};
-
 ErrM error_matrix(const muto_mlsd_cfg& c) {  // MutoMLSD.cpp:168-170
   const double e = c.error_xy, o = c.error_dz_outer, i = c.error_dz_inner;
   ErrM E;
@@ -66,27 +36,6 @@ ErrM error_matrix(const muto_mlsd_cfg& c) {  // MutoMLSD.cpp:168-170
   E.e01 = 2.0 * e * e * i / (o * o);
   return E;
 }
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// Block-level sum of one double per thread into *dst (one atomic per wave).
-__device__ __forceinline__ void atomic_block_sum(double* dst, double v) {
-  v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(dst, v);
-}
-
-__device__ __forceinline__ void wave_count(unsigned long long* dst, bool v) {
-  unsigned long long m = __ballot(v);
-  if (m && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)m) - 1))
-    atomicAdd(dst, (unsigned long long)__popcll(m));
-}
-
-// device counter slots
-enum { C_SINGULAR = 0, C_CUBIC = 1, C_NEWTON = 2, C_OUT = 3, C_EMPTY = 4 };
 
 namespace {
 
@@ -115,7 +64,7 @@ k_compact(GridD g, const muto_ray* __restrict__ rays, size_t n,
           const uint64_t* __restrict__ pos, const double* __restrict__ pscat,
           int has_energy, uint32_t* __restrict__ sel,
           double* __restrict__ sel_p, double* __restrict__ trk4,
-          double* __restrict__ pr) {
+          double* __restrict__ pr, double* __restrict__ geo) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || !flag[i]) return;
   const size_t s = pos[i];
@@ -125,6 +74,11 @@ k_compact(GridD g, const muto_ray* __restrict__ rays, size_t n,
   sel_p[3 * s + 2] = pscat[3 * i + 2];
   muto_ray r;
   load_ray(rays, i, r);
+  for (int k = 0; k < 3; ++k) {  // path endpoints for recomputing the walk
+    geo[9 * s + k] = r.pin[k];
+    geo[9 * s + 3 + k] = r.pout[k];
+    geo[9 * s + 6 + k] = pscat[3 * i + k];
+  }
   // getScatteringAngle (MutoMLSD.cpp:350-354)
   trk4[4 * s] = (atan2(r.din[0], r.din[2]) - atan2(r.dout[0], r.dout[2]));
   trk4[4 * s + 1] = (atan2(r.din[1], r.din[2]) - atan2(r.dout[1], r.dout[2]));
@@ -231,6 +185,27 @@ k_tails(size_t n_trk, const uint64_t* __restrict__ row_ptr,
     }
   }
   wave_count(&cnt[C_EMPTY], empty);
+}
+
+// Per-voxel entry counts and the empty / out-of-grid counters from a walk
+// of each path, for plans that keep no path lists.
+__global__ void __launch_bounds__(256)
+k_mj_walk(GridD g, const double* __restrict__ geo, size_t n_trk, int straight,
+          uint32_t* __restrict__ mj, size_t nv, unsigned long long* __restrict__ cnt) {
+  size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool empty = false;
+  uint32_t nout = 0;
+  if (s < n_trk) {
+    uint32_t ne = 0;
+    walk_path(g, geo + 9 * s, straight, [&](uint32_t j, double, double, bool oob) {
+      ++ne;
+      if (oob) ++nout;
+      if (j < nv) atomicAdd(&mj[j], 1u);
+    });
+    empty = ne == 0;
+  }
+  wave_count(&cnt[C_EMPTY], empty);
+  if (nout) atomicAdd(&cnt[C_OUT], (unsigned long long)nout);
 }
 
 // voxel-major slots for the order statistics (allocateSValueMatrix,
@@ -427,42 +402,6 @@ __global__ void k_copy_f64_to_f32(size_t n, const double* __restrict__ a,
 // ------------------------------------------------------------------ //
 // MAP voxel step (src/MutoMAP.cpp:162-184, 207-407), one lane / voxel //
 // ------------------------------------------------------------------ //
-struct MapCfgD {
-  GridD g;
-  double beta, p, delta, eps;
-  int obj_prior, all_nb;
-};
-
-#define MAP_SCALE 1.0e7
-
-__device__ int map_neighbours(const MapCfgD& c, const double* __restrict__ img,
-                              uint32_t j, double* w, double* lm) {
-  const uint32_t nx = c.g.n[0], ny = c.g.n[1], nz = c.g.n[2];
-  const uint32_t x = j % (nx * ny) % nx;
-  const uint32_t y = j % (nx * ny) / nx;
-  const uint32_t z = j / (nx * ny);
-  const uint32_t xp = x < nx - 1 ? x + 1 : nx - 1;
-  const uint32_t yp = y < ny - 1 ? y + 1 : ny - 1;
-  const uint32_t zp = z < nz - 1 ? z + 1 : nz - 1;
-  const uint32_t xm = 0 < x ? x - 1 : 0;
-  const uint32_t ym = 0 < y ? y - 1 : 0;
-  const uint32_t zm = 0 < z ? z - 1 : 0;
-  const uint32_t nb[26][3] = {
-      {xp, y, z},   {xm, y, z},   {x, yp, z},   {x, ym, z},   {x, y, zp},
-      {x, y, zm},   {xp, yp, z},  {xm, yp, z},  {xp, ym, z},  {xm, ym, z},
-      {x, yp, zp},  {x, ym, zp},  {x, yp, zm},  {x, ym, zm},  {xp, y, zp},
-      {xm, y, zp},  {xp, y, zm},  {xm, y, zm},  {xp, yp, zp}, {xm, yp, zp},
-      {xp, ym, zp}, {xm, ym, zp}, {xp, yp, zm}, {xm, yp, zm}, {xp, ym, zm},
-      {xm, ym, zm}};
-  const int nn = c.all_nb ? 26 : 6;
-  const double w2 = 1.0 / sqrt(2.0), w3 = 1.0 / sqrt(3.0);
-  for (int k = 0; k < nn; ++k) {
-    w[k] = k < 6 ? 1.0 : (k < 18 ? w2 : w3);
-    lm[k] = img[(size_t)nb[k][0] + (size_t)nx * ((size_t)nb[k][1] + (size_t)ny * nb[k][2])] * MAP_SCALE;
-  }
-  return nn;
-}
-
 __device__ double map_surrogate(const MapCfgD& c, uint32_t mj, double ljn,
                                 double ljml, double lj, const double* w,
                                 const double* lm, int nn) {
@@ -624,6 +563,12 @@ static int read_counters(muto_em_plan* pl) {
 extern "C" int muto_em_plan_create(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
                                    size_t n, const muto_ray* rays,
                                    muto_em_plan** out) {
+  return muto_em_plan_create_ex(ctx, cfg, n, rays, UINT64_MAX, out);
+}
+
+extern "C" int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
+                                      size_t n, const muto_ray* rays, uint64_t list_budget,
+                                      muto_em_plan** out) {
   if (!ctx) return MUTO_ERR_ARG;
   if (!out) { ctx->set_error("muto_em_plan_create: null out"); return MUTO_ERR_ARG; }
   *out = nullptr;
@@ -675,8 +620,10 @@ extern "C" int muto_em_plan_create(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
     MUTO_CHECK_HIP(ctx, sel_p.alloc(3 * nsel));
     MUTO_CHECK_HIP(ctx, pl->trk4.alloc(4 * nsel));
     MUTO_CHECK_HIP(ctx, pl->pr.alloc(nsel));
+    MUTO_CHECK_HIP(ctx, pl->geo.alloc(9 * nsel));
     k_compact<<<grid_for(ntot, 256), 256, 0, s>>>(pl->g, dr.p, ntot, flag.p, pos.p, pscat.p,
-                                                  cfg->has_energy, sel.p, sel_p.p, pl->trk4.p, pl->pr.p);
+                                                  cfg->has_energy, sel.p, sel_p.p, pl->trk4.p, pl->pr.p,
+                                                  pl->geo.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     MUTO_CHECK_HIP(ctx, counts.alloc(nsel + 1));
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(counts.p, 0, (nsel + 1) * sizeof(uint64_t), s));
@@ -686,6 +633,15 @@ extern "C" int muto_em_plan_create(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
     MUTO_CHECK_HIP(ctx, pl->row_ptr.alloc(nsel + 1));
     MUTO_CHECK_HIP(ctx, exclusive_scan_u64(counts.p, pl->row_ptr.p, nsel + 1, s));
     MUTO_CHECK_HIP(ctx, hipMemcpy(&pl->nnz, pl->row_ptr.p + nsel, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if ((double)pl->nnz * 20.0 > (double)list_budget) {
+      // over the budget: no lists; per-voxel counts from a walk, paths
+      // recomputed by the MAP objective operator
+      pl->has_lists = false;
+      k_mj_walk<<<grid_for(nsel, 256), 256, 0, s>>>(pl->g, pl->geo.p, nsel, cfg->straight_line,
+                                                    pl->mj.p, pl->nv, pl->dcount.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+      MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    } else {
     MUTO_CHECK_HIP(ctx, pl->vox.alloc(pl->nnz));
     MUTO_CHECK_HIP(ctx, pl->len.alloc(pl->nnz));
     MUTO_CHECK_HIP(ctx, pl->tail.alloc(pl->nnz));
@@ -694,6 +650,7 @@ extern "C" int muto_em_plan_create(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
         pl->dcount.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    }
   }
   pl->counters.nnz = (int64_t)pl->nnz;
   if (int rc2 = read_counters(pl)) { delete pl; return rc2; }
@@ -916,6 +873,11 @@ static int plan_run(muto_em_plan* pl, int iters, int precision, double* cost) {
     return MUTO_ERR_ARG;
   }
   if (pl->cur_prec < 0) { ctx->set_error("muto_em_plan_run: no image set"); return MUTO_ERR_STATE; }
+  if (!pl->has_lists) {
+    ctx->set_error("muto_em_plan_run: the plan keeps no path lists (they exceeded its list budget); "
+                   "only muto_map_forward/backward run on it, recomputing the paths");
+    return MUTO_ERR_STATE;
+  }
   hipStream_t s = ctx->stream;
   if (pl->n_trk == 0) {
     // no selected track: every voxel has M_j = 0 and becomes 0; the mean
@@ -1040,6 +1002,10 @@ extern "C" int muto_em_plan_export(muto_em_plan* pl, uint64_t* row_ptr, uint32_t
   muto_ctx* ctx = pl->ctx;
   const size_t nt = pl->n_trk;
   const uint64_t nnz = pl->nnz;
+  if (!pl->has_lists && (vox || len || tail)) {
+    ctx->set_error("muto_em_plan_export: the plan keeps no path lists");
+    return MUTO_ERR_STATE;
+  }
   if (row_ptr) MUTO_CHECK_HIP(ctx, hipMemcpy(row_ptr, pl->row_ptr.p, (nt + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
   if (nnz && vox) MUTO_CHECK_HIP(ctx, hipMemcpy(vox, pl->vox.p, nnz * sizeof(uint32_t), hipMemcpyDeviceToHost));
   if (nnz && len) MUTO_CHECK_HIP(ctx, hipMemcpy(len, pl->len.p, nnz * sizeof(double), hipMemcpyDeviceToHost));

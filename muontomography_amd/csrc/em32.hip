@@ -515,7 +515,7 @@ __global__ void __launch_bounds__(256)
 k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
           const uint32_t* __restrict__ tseg_pos,
           const float4* __restrict__ part, const double* __restrict__ trk_d,
-          ErrM E, double tail_scale, double lmax, double ubound,
+          ErrM E, double tail_scale, double lmax, double ubound, int grad_mode,
           float* __restrict__ trk_g, unsigned long long* __restrict__ cmax,
           double* __restrict__ cost_acc, unsigned long long* __restrict__ cnt) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -589,6 +589,7 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
     g.x = (float)(pp * (0.5 * M00 - i00));
     g.y = (float)(pp * (0.5 * M01 - i01));
     g.z = (float)(pp * (0.5 * M11 - i11));
+    if (grad_mode && sing) g.x = g.y = g.z = 0.0f;  // cost_i = 0: no gradient
     g.w = __uint_as_float(tcx);
     *reinterpret_cast<float4*>(trk_g + 4 * r) = g;
     // bound on |c_ij| over the track's entries for the backprojection's
@@ -757,6 +758,11 @@ __global__ void k_cost_push(const double* __restrict__ cost_acc, double* __restr
     hist[k] = cost_acc[0];
     n[0] = k + 1u;
   }
+}
+
+__global__ void k_mirror32(size_t n, const double* __restrict__ a, float* __restrict__ b) {
+  size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) b[j] = (float)a[j];
 }
 
 // MAP: the voxel step's result becomes the image and its float mirror
@@ -1082,7 +1088,7 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[1], s));
   k_track32<<<grid_for(pl->n_trk, 256), 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p,
                                                       reinterpret_cast<const float4*>(f.seg_part.p),
-                                                      f.trk_d.p, E, f.tail_scale, f.lmax, f.ubound,
+                                                      f.trk_d.p, E, f.tail_scale, f.lmax, f.ubound, 0,
                                                       f.trk_g.p, f.cmax.p, pl->cost_acc.p, pl->dcount.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[2], s));
@@ -1185,6 +1191,68 @@ int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done) {
   }
   if (pl->timing && *done)
     for (int k = 0; k < 5; ++k) pl->t_ms[k] = tsum[k] / *done;
+  return MUTO_OK;
+}
+
+// MAP objective operator, FP32C: the forward pass of the EM iteration on
+// `lam` (FP64 on the device; its float mirror is rewritten) and the track
+// kernel with the gradient's convention (G = 0 where cost_i = 0); G' per
+// track and the |c| bound are copied out for the backward call, the cost
+// sum returned.
+int fp32c_map_forward(muto_em_plan* pl, const double* lam, float* g_out,
+                      unsigned long long* cmax_out, double* cost_sum) {
+  Fp32cData& f = pl->f32;
+  muto_ctx* ctx = pl->ctx;
+  hipStream_t s = ctx->stream;
+  if (!pl->lam32.p) MUTO_CHECK_HIP(ctx, pl->lam32.alloc(pl->nv));
+  k_mirror32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, lam, pl->lam32.p);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  pl->cur_prec = pl->cur_prec < 0 ? -1 : MUTO_FP64;  // the EM's float mirror is stale now
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->work.p, 0, 4 * sizeof(unsigned int), s));
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.cmax.p, 0, sizeof(unsigned long long), s));
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->cost_acc.p, 0, sizeof(double), s));
+  const IterArgs a = iter_args(pl);
+  const size_t lds_f = f.bs.vol() * sizeof(float);
+  const ErrM E = error_matrix(pl->cfg);
+  if (f.n_tasks) {
+    if (f.acc_mode == 1)
+      k_fwd32<1><<<f.grid_fwd, BLOCK, lds_f, s>>>(a, pl->lam32.p, reinterpret_cast<float4*>(f.seg_part.p));
+    else
+      k_fwd32<0><<<f.grid_fwd, BLOCK, lds_f, s>>>(a, pl->lam32.p, reinterpret_cast<float4*>(f.seg_part.p));
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  }
+  k_track32<<<grid_for(pl->n_trk, 256), 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p,
+                                                      reinterpret_cast<const float4*>(f.seg_part.p),
+                                                      f.trk_d.p, E, f.tail_scale, f.lmax, f.ubound, 1,
+                                                      f.trk_g.p, f.cmax.p, pl->cost_acc.p, pl->dcount.p);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  MUTO_CHECK_HIP(ctx, hipMemcpyAsync(g_out, f.trk_g.p, 4 * pl->n_trk * sizeof(float), hipMemcpyDeviceToDevice, s));
+  MUTO_CHECK_HIP(ctx, hipMemcpyAsync(cmax_out, f.cmax.p, sizeof(unsigned long long), hipMemcpyDeviceToDevice, s));
+  MUTO_CHECK_HIP(ctx, hipMemcpyAsync(cost_sum, pl->cost_acc.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->work.p, 0, 4 * sizeof(unsigned int), s));
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.cmax.p, 0, sizeof(unsigned long long), s));
+  MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+  return MUTO_OK;
+}
+
+// ... and the backprojection of that G' into acc (sum_i <W_ij, G_i>, FP64).
+int fp32c_map_backward(muto_em_plan* pl, const float* g, const unsigned long long* cmax,
+                       double* acc) {
+  Fp32cData& f = pl->f32;
+  muto_ctx* ctx = pl->ctx;
+  hipStream_t s = ctx->stream;
+  const IterArgs a = iter_args(pl);
+  const size_t lds_b = 2 * f.bs.vol() * sizeof(float);
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(acc, 0, pl->nv * sizeof(double), s));
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->work.p, 0, 4 * sizeof(unsigned int), s));
+  if (f.n_tasks) {
+    if (f.acc_mode == 1)
+      k_bwd32<1><<<f.grid_bwd, BLOCK, lds_b, s>>>(a, g, cmax, acc);
+    else
+      k_bwd32<0><<<f.grid_bwd, BLOCK, lds_b, s>>>(a, g, cmax, acc);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  }
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->work.p, 0, 4 * sizeof(unsigned int), s));
   return MUTO_OK;
 }
 

@@ -81,6 +81,8 @@ struct muto_em_plan {
   muto::DBuf<uint32_t> vox;     // linear voxel index
   muto::DBuf<double> len, tail;
   muto::DBuf<uint32_t> mj;      // entries per voxel
+  muto::DBuf<double> geo;       // [9*n_trk] pin, pout, PoCA point (path recompute)
+  bool has_lists = true;        // row_ptr/vox/len/tail hold the paths
 
   // voxel-major slots for median / trimmed mean
   bool slots_built = false;

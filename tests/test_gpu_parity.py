@@ -304,3 +304,121 @@ def test_map_fp64(ctx, small_set, nb, obj, p):
     ig, cg, ng = ctx.map_reconstruct(cfg, rays)
     np.testing.assert_allclose(ig, io, rtol=1e-8, atol=1e-20)
     assert ng["n_cubic_warn"] == no["n_cubic_warn"]
+
+
+def _lam_field(n, seed=7):
+    rng = np.random.default_rng(seed)
+    return 1e-7 * (0.5 + rng.random(n))
+
+
+@pytest.mark.parametrize("nb,obj,p", [(0, 0, 1.0), (1, 1, 2.0)])
+def test_map_objective_fp64_matches_oracle(ctx, small_set, nb, obj, p):
+    """The MAP objective operator (muto_map_forward/backward), FP64, from
+    the stored lists and with the paths walked again: Phi and grad Phi equal
+    the oracle's (whose gradient is pinned by central differences and by
+    tangency with the reference's MAP surrogate) to 1e-10 relative (FP64
+    atomics and the recomputed tails change the order of sums)."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, beta=1.0, p=p, delta=0.1,
+                  all_neighbours=nb, add_object_prior=obj)
+    em = oracle.Em(cfg, rays)
+    lam = _lam_field(g.n_vox)
+    phi_ref = em.phi(lam, cfg)
+    g_ref = em.grad(lam, cfg)
+    plan = ctx.em_plan(cfg, rays=rays)
+    for recompute in (False, True):
+        sv = plan.map_forward(lam, M.MUTO_FP64, recompute=recompute)
+        assert abs(sv.phi - phi_ref) <= 1e-10 * abs(phi_ref), (recompute, sv.phi, phi_ref)
+        gr = sv.backward(1.0)
+        np.testing.assert_allclose(gr, g_ref, rtol=1e-9, atol=1e-10 * np.abs(g_ref).max())
+        gr2 = sv.backward(-2.5)  # the saved context is reusable and linear in grad_phi
+        np.testing.assert_allclose(gr2, -2.5 * gr, rtol=1e-11, atol=1e-12 * np.abs(gr).max())
+        sv.close()
+    plan.close()
+
+
+@pytest.fixture(scope="module")
+def fd32_set():
+    return W.muons("fd32")
+
+
+@pytest.mark.parametrize("recompute", [False, True])
+def test_map_gradient_central_differences_gpu(ctx, fd32_set, recompute):
+    """grad Phi from the backward call against FP64 central differences of
+    the forward call, h = 1e-6 lambda_j, on the 32^3 x 5 mm grid with 1e4
+    muons, on voxels with and without entries (Phi's sums are block-ordered,
+    so the forward value is the same in every run)."""
+    g, rays, _ = fd32_set
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, beta=0.5, p=1.5, delta=0.2,
+                  all_neighbours=1, add_object_prior=1)
+    plan = ctx.em_plan(cfg, rays=rays)
+    lam = _lam_field(g.n_vox, 11)
+    sv = plan.map_forward(lam, M.MUTO_FP64, recompute=recompute)
+    gr = sv.backward(1.0)
+    again = plan.map_forward(lam, M.MUTO_FP64, recompute=recompute)
+    assert again.phi == sv.phi
+    sv.close()
+    again.close()
+    mj = plan.voxel_counts()
+    rng = np.random.default_rng(3)
+    hit = np.nonzero(mj > 0)[0]
+    miss = np.nonzero(mj == 0)[0]
+    picks = list(rng.choice(hit, 8, replace=False)) + list(rng.choice(miss, 2, replace=False))
+    for j in picks:
+        h = 1e-6 * lam[j]
+        lp, lm = lam.copy(), lam.copy()
+        lp[j] += h
+        lm[j] -= h
+        sp = plan.map_forward(lp, M.MUTO_FP64, recompute=recompute)
+        sm = plan.map_forward(lm, M.MUTO_FP64, recompute=recompute)
+        fd = (sp.phi - sm.phi) / (2 * h)
+        sp.close()
+        sm.close()
+        assert abs(fd - gr[j]) <= 1e-5 * abs(gr[j]) + 1e-7 * np.abs(gr).max(), (j, mj[j], fd, gr[j])
+    plan.close()
+
+
+def test_map_objective_fp32c_close_to_fp64(ctx, small_set):
+    """FP32C objective operator against FP64: Phi within 1e-6 relative, the
+    gradient within 1e-4 of its largest component."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, beta=1.0, p=1.2, delta=0.1)
+    plan = ctx.em_plan(cfg, rays=rays)
+    lam = _lam_field(g.n_vox, 5)
+    s64 = plan.map_forward(lam, M.MUTO_FP64)
+    s32 = plan.map_forward(lam, M.MUTO_FP32C)
+    g64, g32 = s64.backward(1.0), s32.backward(1.0)
+    print(f"fp32c map: phi rel {abs(s32.phi - s64.phi) / abs(s64.phi):.3e} "
+          f"grad {np.abs(g32 - g64).max() / np.abs(g64).max():.3e}", flush=True)
+    assert abs(s32.phi - s64.phi) <= 1e-6 * abs(s64.phi)
+    assert np.abs(g32 - g64).max() <= 1e-4 * np.abs(g64).max()
+    s64.close()
+    s32.close()
+    plan.close()
+
+
+def test_plan_without_lists_recomputes(ctx, small_set):
+    """A plan over its list budget keeps no path lists: the same per-voxel
+    counts from a walk, EM runs refused, and the MAP operator (paths walked
+    again) equal to the list plan's recomputed one."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, beta=1.0, p=2.0, delta=0.1)
+    full = ctx.em_plan(cfg, rays=rays)
+    lean = ctx.em_plan(cfg, rays=rays, list_budget=0)
+    assert lean.counters()["nnz"] == full.counters()["nnz"]
+    np.testing.assert_array_equal(lean.voxel_counts(), full.voxel_counts())
+    lean.set_image(np.full(g.n_vox, 1e-7))
+    with pytest.raises(M.MutoError):
+        lean.run(1, M.MUTO_FP64)
+    lam = _lam_field(g.n_vox, 9)
+    a = full.map_forward(lam, M.MUTO_FP64, recompute=True)
+    b = lean.map_forward(lam, M.MUTO_FP64)
+    assert abs(a.phi - b.phi) <= 1e-12 * abs(a.phi)
+    np.testing.assert_allclose(b.backward(1.0), a.backward(1.0), rtol=1e-11,
+                               atol=1e-12 * np.abs(a.backward(1.0)).max())
+    with pytest.raises(M.MutoError):
+        lean.map_forward(lam, M.MUTO_FP32C)
+    a.close()
+    b.close()
+    full.close()
+    lean.close()
