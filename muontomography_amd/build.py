@@ -63,6 +63,50 @@ def build(verbose=False):
     return LIB
 
 
+JSON_DIRS = ["/opt/conda/include", "/usr/include", "/usr/local/include"]
+CXX_LIB = os.path.join(HERE, "libmuto_cxx.so")
+CLI = os.path.join(HERE, "muto_cli")
+
+
+def _json_dir():
+    for d in JSON_DIRS:
+        if os.path.exists(os.path.join(d, "json.hpp")):
+            return d
+        if os.path.exists(os.path.join(d, "nlohmann", "json.hpp")):
+            return os.path.join(d, "nlohmann")
+    return None
+
+
+def build_cxx():
+    """The reference-compatible C++ classes (MutoPoCA, MutoMLSD, MutoMAP,
+    MutoFile) over libmuto, and the muto_cli driver.  Needs the single-header
+    nlohmann json.hpp; without it the C++ layer is skipped with a message."""
+    jd = _json_dir()
+    if jd is None:
+        print("json.hpp not found: the C++ drop-in layer (libmuto_cxx.so, muto_cli) is not built",
+              file=sys.stderr)
+        return None
+    inc = ["-I", os.path.join(ROOT, "include", "muto_cxx"), "-I", os.path.join(ROOT, "include"),
+           "-I", jd]
+    link = ["-L", HERE, "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    src = os.path.join(HERE, "cxx")
+    tmp = CXX_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", os.path.join(src, "MutoCxx.cpp"),
+          "-o", tmp] + inc + link + ["-lmuto"])
+    os.replace(tmp, CXX_LIB)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(src, "muto_cli.cpp"), "-o", CLI + ".tmp"]
+         + inc + link + ["-lmuto_cxx", "-lmuto"])
+    os.replace(CLI + ".tmp", CLI)
+    # the C++ layer's own check program (tests/cxx), linked like a user's
+    chk = os.path.join(ROOT, "tests", "cxx", "dropin_check")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(ROOT, "tests", "cxx", "dropin_check.cpp"),
+          "-o", chk + ".tmp"] + inc +
+         ["-L", HERE, "-Wl,-rpath,$ORIGIN/../../muontomography_amd", "-Wl,-rpath-link,/opt/rocm/lib",
+          "-lmuto_cxx", "-lmuto"])
+    os.replace(chk + ".tmp", chk)
+    return CXX_LIB
+
+
 def build_oracle():
     """Compile the CPU oracle (test infrastructure) with its Makefile."""
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
@@ -71,4 +115,5 @@ def build_oracle():
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_cxx())
     print(build_oracle())

@@ -5,6 +5,8 @@ PoCA points bit-exact (the contract allows 1e-12 relative); lambda images
 bit-exact where the reference's order is deterministic (median, trimmed
 mean on identical inputs), else within the tolerance stated per test.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -422,3 +424,69 @@ def test_plan_without_lists_recomputes(ctx, small_set):
     b.close()
     full.close()
     lean.close()
+
+
+def _records_from_rays(rays, zl):
+    """Float32 tracker records (e_in, (x, y) per layer, e_out) of the rays'
+    in and out lines evaluated at the layer heights (3 above, 3 below)."""
+    n = rays.shape[0]
+    rec = np.empty((n, 2 * len(zl) + 2), np.float32)
+    rec[:, 0] = rays["ein"]
+    rec[:, -1] = rays["eout"]
+    for i, z in enumerate(zl):
+        p, d = (rays["pin"], rays["din"]) if i < 3 else (rays["pout"], rays["dout"])
+        t = (z - p[:, 2]) / d[:, 2]
+        rec[:, 1 + 2 * i] = p[:, 0] + t * d[:, 0]
+        rec[:, 2 + 2 * i] = p[:, 1] + t * d[:, 1]
+    return rec
+
+
+def test_cli_poca_then_mlsd_matches_library(ctx, small_set, tmp_path):
+    """The reference-style driver (muto_cli: JSON config, record files,
+    image files) end to end: PoCA, then MLSD started from the PoCA image
+    file, equal to the library called on the rays the C++ reader parsed
+    (images compared as the float32 values the files hold)."""
+    import json
+    import subprocess
+    from muontomography_amd.imagefile import read_image
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    g, rays, _ = small_set
+    rays = rays[:20000]
+    zi, zo = float(rays["pin"][0, 2]), float(rays["pout"][0, 2])
+    zl = [zi + 1000.0, zi + 500.0, zi, zo, zo - 500.0, zo - 1000.0]
+    recf = tmp_path / "muons.bin"
+    recf.write_bytes(_records_from_rays(rays, zl).tobytes())
+    rf = tmp_path / "rays.bin"
+    r = subprocess.run([os.path.join(root, "tests", "cxx", "dropin_check"), "rays", str(rf), "4", "3"]
+                       + [repr(z) for z in zl] + ["--", str(recf)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    rc = np.frombuffer(rf.read_bytes(), M.RAY_DTYPE)
+    assert rc.shape[0] == rays.shape[0]
+    grid = dict(nx=g.nx, ny=g.ny, nz=g.nz, x_min=g.x_min, y_min=g.y_min, z_min=g.z_min,
+                dx=g.dx, dy=g.dy, dz=g.dz)
+    common = dict(data_files=[str(recf)], detector_height=zl, detector_middle=3)
+    poca_out = tmp_path / "poca.bin"
+    cfgp = dict(common, output_file=str(poca_out),
+                poca=dict(grid=grid, length_weighted=True, straight_line=False, eps=1e-8))
+    (tmp_path / "poca.json").write_text(json.dumps(cfgp))
+    cli = os.path.join(root, "muontomography_amd", "muto_cli")
+    r = subprocess.run([cli, str(tmp_path / "poca.json")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    pimg, ph = read_image(poca_out)
+    assert ph["method"] == "poca" and ph["n_ray"] == rays.shape[0] and ph["nx"] == g.nx
+    ref, _ = ctx.poca_reconstruct(poca_cfg(g, length_weighted=1), rc)
+    np.testing.assert_allclose(pimg, ref.astype(np.float32), rtol=1e-6, atol=0)
+    mlsd_out = tmp_path / "mlsd.bin"
+    mk = dict(grid=grid, update_with_mean=True, mean_percentile=100, num_iteration=5,
+              initialize_method=0, initial_image_file=str(poca_out), error_xy=0.5,
+              error_dz_outer=500.0, error_dz_inner=500.0)
+    cfgm = dict(common, output_file=str(mlsd_out), mlsd=mk)
+    (tmp_path / "mlsd.json").write_text(json.dumps(cfgm))
+    r = subprocess.run([cli, str(tmp_path / "mlsd.json")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    mimg, mh = read_image(mlsd_out)
+    assert mh["method"] == "mlsd" and mh["n_iter"] == 5
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, num_iteration=5, initialize_method=0)
+    ref, _, _ = ctx.mlsd_reconstruct(cfg, rc, init=pimg)
+    np.testing.assert_allclose(mimg, ref.astype(np.float32), rtol=1e-5, atol=0)
