@@ -241,6 +241,22 @@ int muto_map_forward(muto_em_plan* plan, const double* lambda, int precision, in
 int muto_map_backward(muto_map_saved* saved, double grad_phi, double* grad_lambda);
 int muto_map_saved_destroy(muto_map_saved* saved);
 
+/* ---- many independent scans (one launch per pass for all of them) ----
+ * A batch combines plans (each with its own grid, voxel size and tracks;
+ * sharing the EM configuration, mean update) into one problem: their grids
+ * side by side in one brick-aligned grid, their track lists concatenated,
+ * one size-sorted task queue for the persistent FP32C kernels and one
+ * captured graph per iteration.  muto_batch_run runs `iters` mean-update
+ * iterations of every scan in lock step, from and back to the scans' plan
+ * images; costs (may be NULL): iters x n values, scan fastest.  The plans
+ * must outlive the batch.  No counterpart in the reference (one scan per
+ * process). */
+typedef struct muto_batch muto_batch;
+int muto_batch_create(muto_ctx* ctx, size_t n, muto_em_plan* const* plans, muto_batch** out);
+int muto_batch_run(muto_batch* batch, int iters, double* costs);
+int muto_batch_stats(const muto_batch* batch, double* wall_ms, muto_counters* counters);
+int muto_batch_destroy(muto_batch* batch);
+
 /* Plan options (value 0/1 unless stated).
  *   MUTO_OPT_GRAPH    FP32C iterations replay one captured HIP graph (default 1)
  *   MUTO_OPT_TIMING   record per-kernel times for muto_em_plan_timing; runs

@@ -46,6 +46,7 @@ class Context:
     def __init__(self, device=0):
         self._h = C.c_void_p()
         self._plans = weakref.WeakSet()
+        self._batches = weakref.WeakSet()
         rc = lib().muto_ctx_create(device, C.byref(self._h))
         if rc != 0:
             raise MutoError(f"muto_ctx_create({device}) failed with status {rc}")
@@ -54,6 +55,8 @@ class Context:
         """Destroys the context; plans still open on it are closed first
         (a plan must not outlive its context)."""
         if self._h:
+            for b in list(self._batches):
+                b.close()
             for p in list(self._plans):
                 p.close()
             lib().muto_ctx_destroy(self._h)
@@ -132,6 +135,11 @@ class Context:
     def map_reconstruct(self, cfg, rays, init=None):
         return self._recon(lib().muto_map_reconstruct, cfg, rays, init)
 
+    def batch(self, plans):
+        """A Batch over EmPlans (mean-update configs sharing the measurement
+        errors); the plans must outlive it."""
+        return Batch(self, plans)
+
     def em_plan(self, cfg, rays=None, prepared=None, list_budget=None):
         """list_budget: bytes allowed for the track-major path lists (20 per
         entry); above it the plan keeps none and only the MAP objective
@@ -168,6 +176,9 @@ class EmPlan:
 
     def close(self):
         if self._h:
+            for b in list(self.ctx._batches):  # a batch must not outlive its plans
+                if any(p is self for p in b.plans):
+                    b.close()
             for sv in list(self._saved):
                 sv.close()
             lib().muto_em_plan_destroy(self._h)
@@ -242,6 +253,44 @@ class EmPlan:
         t = np.zeros(5, np.float64)
         self.ctx._check(lib().muto_em_plan_timing(self._h, ptr(t)), "timing")
         return {"forward": t[0], "track": t[1], "backward": t[2], "update": t[3], "iteration": t[4]}
+
+
+class Batch:
+    """Many scans' FP32C mean-update iterations in lock step (muto_batch):
+    one launch per pass and one captured graph per iteration for all."""
+
+    def __init__(self, ctx, plans):
+        self.ctx = ctx
+        self.plans = list(plans)
+        self._h = C.c_void_p()
+        arr = (C.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
+        ctx._check(lib().muto_batch_create(ctx._h, len(self.plans), arr, C.byref(self._h)),
+                   "muto_batch_create")
+        ctx._batches.add(self)
+
+    def run(self, iters):
+        """Runs `iters` iterations from the plans' images (written back);
+        returns the costs, shape (iters, n_scans)."""
+        cost = np.full((max(iters, 1), len(self.plans)), np.nan)
+        self.ctx._check(lib().muto_batch_run(self._h, int(iters), ptr(cost)), "muto_batch_run")
+        return cost[:iters]
+
+    def stats(self):
+        ms = C.c_double()
+        c = Counters()
+        self.ctx._check(lib().muto_batch_stats(self._h, C.byref(ms), C.byref(c)), "muto_batch_stats")
+        return ms.value, c.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().muto_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class MapSaved:

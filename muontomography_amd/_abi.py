@@ -161,6 +161,10 @@ PROTOS = {
     "muto_map_forward": (C.c_int, [P, P, C.c_int, C.c_int, DP, C.POINTER(P)]),
     "muto_map_backward": (C.c_int, [P, C.c_double, P]),
     "muto_map_saved_destroy": (C.c_int, [P]),
+    "muto_batch_create": (C.c_int, [P, C.c_size_t, P, C.POINTER(P)]),
+    "muto_batch_run": (C.c_int, [P, C.c_int, P]),
+    "muto_batch_stats": (C.c_int, [P, DP, C.POINTER(Counters)]),
+    "muto_batch_destroy": (C.c_int, [P]),
     "muto_em_plan_export": (C.c_int, [P, P, P, P, P, P, P]),
     "muto_gen_default": (None, [C.POINTER(GenCfg), C.POINTER(Grid), C.c_uint64]),
     "muto_gen_layout": (C.c_int, [C.POINTER(Grid), C.c_uint64, C.c_int, C.c_double, C.c_double,

@@ -23,6 +23,9 @@
 #include "em_plan.hpp"
 #include "geom_dev.hpp"
 
+int muto_plan_alloc_common(muto_em_plan* pl);  // shared with batch.hip
+int muto_read_counters(muto_em_plan* pl);
+
 namespace muto {
 
 // ------------------------------------------------------------------ //
@@ -518,7 +521,7 @@ using namespace muto;
 // ------------------------------------------------------------------ //
 // plan creation                                                      //
 // ------------------------------------------------------------------ //
-static int plan_alloc_common(muto_em_plan* pl) {
+int muto_plan_alloc_common(muto_em_plan* pl) {
   muto_ctx* ctx = pl->ctx;
   MUTO_CHECK_HIP(ctx, pl->lam64.alloc(pl->nv));
   MUTO_CHECK_HIP(ctx, pl->ml64.alloc(pl->nv));
@@ -548,7 +551,7 @@ static int validate_cfg(muto_ctx* ctx, const muto_mlsd_cfg* cfg) {
   return MUTO_OK;
 }
 
-static int read_counters(muto_em_plan* pl) {
+int muto_read_counters(muto_em_plan* pl) {
   unsigned long long c[8];
   MUTO_CHECK_HIP(pl->ctx, hipMemcpyAsync(c, pl->dcount.p, sizeof c, hipMemcpyDeviceToHost, pl->ctx->stream));
   MUTO_CHECK_HIP(pl->ctx, hipStreamSynchronize(pl->ctx->stream));
@@ -579,7 +582,7 @@ extern "C" int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
   pl->cfg = *cfg;
   pl->g = to_dev(cfg->grid);
   pl->nv = n_vox(cfg->grid);
-  int rc = plan_alloc_common(pl);
+  int rc = muto_plan_alloc_common(pl);
   if (rc) { delete pl; return rc; }
   hipStream_t s = ctx->stream;
   size_t ntot = cfg->total_num_rays > 0 ? (size_t)cfg->total_num_rays : n;
@@ -611,8 +614,8 @@ extern "C" int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
     MUTO_CHECK_HIP(ctx, exclusive_scan_u32_to_u64(flag.p, pos.p, ntot, s));
     uint64_t last_pos = 0;
     uint32_t last_flag = 0;
-    MUTO_CHECK_HIP(ctx, hipMemcpy(&last_pos, pos.p + ntot - 1, sizeof last_pos, hipMemcpyDeviceToHost));
-    MUTO_CHECK_HIP(ctx, hipMemcpy(&last_flag, flag.p + ntot - 1, sizeof last_flag, hipMemcpyDeviceToHost));
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, &last_pos, pos.p + ntot - 1, sizeof last_pos, hipMemcpyDeviceToHost));
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, &last_flag, flag.p + ntot - 1, sizeof last_flag, hipMemcpyDeviceToHost));
     const size_t nsel = (size_t)(last_pos + last_flag);
     pl->n_trk = nsel;
     pl->counters.n_scat_in_grid = (int64_t)nsel;
@@ -632,7 +635,7 @@ extern "C" int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     MUTO_CHECK_HIP(ctx, pl->row_ptr.alloc(nsel + 1));
     MUTO_CHECK_HIP(ctx, exclusive_scan_u64(counts.p, pl->row_ptr.p, nsel + 1, s));
-    MUTO_CHECK_HIP(ctx, hipMemcpy(&pl->nnz, pl->row_ptr.p + nsel, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, &pl->nnz, pl->row_ptr.p + nsel, sizeof(uint64_t), hipMemcpyDeviceToHost));
     if ((double)pl->nnz * 20.0 > (double)list_budget) {
       // over the budget: no lists; per-voxel counts from a walk, paths
       // recomputed by the MAP objective operator
@@ -653,7 +656,7 @@ extern "C" int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
     }
   }
   pl->counters.nnz = (int64_t)pl->nnz;
-  if (int rc2 = read_counters(pl)) { delete pl; return rc2; }
+  if (int rc2 = muto_read_counters(pl)) { delete pl; return rc2; }
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->dcount.p, 0, 8 * sizeof(unsigned long long), s));
   *out = pl;
   return MUTO_OK;
@@ -677,7 +680,7 @@ extern "C" int muto_em_plan_create_prepared(
   pl->cfg = *cfg;
   pl->g = to_dev(cfg->grid);
   pl->nv = n_vox(cfg->grid);
-  int rc = plan_alloc_common(pl);
+  int rc = muto_plan_alloc_common(pl);
   if (rc) { delete pl; return rc; }
   hipStream_t s = ctx->stream;
   pl->n_trk = n_trk;
@@ -696,13 +699,13 @@ extern "C" int muto_em_plan_create_prepared(
   std::vector<double> ph(n_trk, 1.0);
   if (p_r) std::copy(p_r, p_r + n_trk, ph.begin());
   if (n_trk) {
-    MUTO_CHECK_HIP(ctx, hipMemcpy(pl->trk4.p, trk4, 4 * n_trk * sizeof(double), hipMemcpyHostToDevice));
-    MUTO_CHECK_HIP(ctx, hipMemcpy(pl->pr.p, ph.data(), n_trk * sizeof(double), hipMemcpyHostToDevice));
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, pl->trk4.p, trk4, 4 * n_trk * sizeof(double), hipMemcpyHostToDevice));
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, pl->pr.p, ph.data(), n_trk * sizeof(double), hipMemcpyHostToDevice));
   }
-  MUTO_CHECK_HIP(ctx, hipMemcpy(pl->row_ptr.p, row_ptr, (n_trk + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+  MUTO_CHECK_HIP(ctx, copy_sync(ctx, pl->row_ptr.p, row_ptr, (n_trk + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
   if (nnz) {
-    MUTO_CHECK_HIP(ctx, hipMemcpy(pl->vox.p, vox, nnz * sizeof(uint32_t), hipMemcpyHostToDevice));
-    MUTO_CHECK_HIP(ctx, hipMemcpy(pl->len.p, len, nnz * sizeof(double), hipMemcpyHostToDevice));
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, pl->vox.p, vox, nnz * sizeof(uint32_t), hipMemcpyHostToDevice));
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, pl->len.p, len, nnz * sizeof(double), hipMemcpyHostToDevice));
   }
   if (n_trk) {
     k_tails<<<grid_for(n_trk, 256), 256, 0, s>>>(n_trk, pl->row_ptr.p, pl->vox.p, pl->len.p,
@@ -710,7 +713,7 @@ extern "C" int muto_em_plan_create_prepared(
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
   MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
-  if (int rc2 = read_counters(pl)) { delete pl; return rc2; }
+  if (int rc2 = muto_read_counters(pl)) { delete pl; return rc2; }
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->dcount.p, 0, 8 * sizeof(unsigned long long), s));
   *out = pl;
   return MUTO_OK;
@@ -729,7 +732,7 @@ extern "C" int muto_em_plan_destroy(muto_em_plan* pl) {
 extern "C" int muto_em_plan_set_image(muto_em_plan* pl, const double* img) {
   if (!pl || !img) return MUTO_ERR_ARG;
   muto_ctx* ctx = pl->ctx;
-  MUTO_CHECK_HIP(ctx, hipMemcpy(pl->lam64.p, img, pl->nv * sizeof(double), hipMemcpyHostToDevice));
+  MUTO_CHECK_HIP(ctx, copy_sync(ctx, pl->lam64.p, img, pl->nv * sizeof(double), hipMemcpyHostToDevice));
   pl->cur_prec = MUTO_FP64;
   return MUTO_OK;
 }
@@ -755,7 +758,7 @@ extern "C" int muto_em_plan_get_image(muto_em_plan* pl, double* img) {
   if (!pl || !img) return MUTO_ERR_ARG;
   muto_ctx* ctx = pl->ctx;
   MUTO_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  MUTO_CHECK_HIP(ctx, hipMemcpy(img, pl->lam64.p, pl->nv * sizeof(double), hipMemcpyDeviceToHost));
+  MUTO_CHECK_HIP(ctx, copy_sync(ctx, img, pl->lam64.p, pl->nv * sizeof(double), hipMemcpyDeviceToHost));
   return MUTO_OK;
 }
 
@@ -913,7 +916,7 @@ static int plan_run(muto_em_plan* pl, int iters, int precision, double* cost) {
     }
     if (cost) for (int k = 0; k < done; ++k) cost[k] = costs[k];
     pl->counters.iterations_done = done;
-    return read_counters(pl);
+    return muto_read_counters(pl);
   }
   for (int it = 0; it < iters; ++it) {
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->cost_acc.p, 0, sizeof(double), s));
@@ -954,7 +957,7 @@ static int plan_run(muto_em_plan* pl, int iters, int precision, double* cost) {
   if (cost) for (int k = 0; k < done; ++k) cost[k] = costs[k];
   for (int k = 0; k < 5; ++k) pl->t_ms[k] = done ? tsum[k] / done : 0.0;
   pl->counters.iterations_done = done;
-  if (int rc = read_counters(pl)) return rc;
+  if (int rc = muto_read_counters(pl)) return rc;
   return MUTO_OK;
 }
 
@@ -992,7 +995,7 @@ extern "C" int muto_em_plan_counters(const muto_em_plan* pl, muto_counters* out)
 
 extern "C" int muto_em_plan_voxel_counts(muto_em_plan* pl, uint32_t* mj) {
   if (!pl || !mj) return MUTO_ERR_ARG;
-  MUTO_CHECK_HIP(pl->ctx, hipMemcpy(mj, pl->mj.p, pl->nv * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  MUTO_CHECK_HIP(pl->ctx, copy_sync(pl->ctx, mj, pl->mj.p, pl->nv * sizeof(uint32_t), hipMemcpyDeviceToHost));
   return MUTO_OK;
 }
 
@@ -1006,12 +1009,12 @@ extern "C" int muto_em_plan_export(muto_em_plan* pl, uint64_t* row_ptr, uint32_t
     ctx->set_error("muto_em_plan_export: the plan keeps no path lists");
     return MUTO_ERR_STATE;
   }
-  if (row_ptr) MUTO_CHECK_HIP(ctx, hipMemcpy(row_ptr, pl->row_ptr.p, (nt + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
-  if (nnz && vox) MUTO_CHECK_HIP(ctx, hipMemcpy(vox, pl->vox.p, nnz * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  if (nnz && len) MUTO_CHECK_HIP(ctx, hipMemcpy(len, pl->len.p, nnz * sizeof(double), hipMemcpyDeviceToHost));
-  if (nnz && tail) MUTO_CHECK_HIP(ctx, hipMemcpy(tail, pl->tail.p, nnz * sizeof(double), hipMemcpyDeviceToHost));
-  if (nt && trk4) MUTO_CHECK_HIP(ctx, hipMemcpy(trk4, pl->trk4.p, 4 * nt * sizeof(double), hipMemcpyDeviceToHost));
-  if (nt && p_r) MUTO_CHECK_HIP(ctx, hipMemcpy(p_r, pl->pr.p, nt * sizeof(double), hipMemcpyDeviceToHost));
+  if (row_ptr) MUTO_CHECK_HIP(ctx, copy_sync(ctx, row_ptr, pl->row_ptr.p, (nt + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  if (nnz && vox) MUTO_CHECK_HIP(ctx, copy_sync(ctx, vox, pl->vox.p, nnz * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (nnz && len) MUTO_CHECK_HIP(ctx, copy_sync(ctx, len, pl->len.p, nnz * sizeof(double), hipMemcpyDeviceToHost));
+  if (nnz && tail) MUTO_CHECK_HIP(ctx, copy_sync(ctx, tail, pl->tail.p, nnz * sizeof(double), hipMemcpyDeviceToHost));
+  if (nt && trk4) MUTO_CHECK_HIP(ctx, copy_sync(ctx, trk4, pl->trk4.p, 4 * nt * sizeof(double), hipMemcpyDeviceToHost));
+  if (nt && p_r) MUTO_CHECK_HIP(ctx, copy_sync(ctx, p_r, pl->pr.p, nt * sizeof(double), hipMemcpyDeviceToHost));
   return MUTO_OK;
 }
 

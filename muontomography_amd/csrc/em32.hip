@@ -40,6 +40,7 @@
 // first, through an atomic counter.
 #include <math.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include <algorithm>
 #include <numeric>
@@ -103,12 +104,13 @@ __device__ __forceinline__ uint32_t spread12(uint32_t v) {
 // Segments per track and the track's spatial key: direction bin of the
 // chord from its first to its last voxel (bins of 1/DIR_BINS in dx/|dz| and
 // dy/|dz| over [-2, 2]) above the Morton code of where that chord crosses
-// the top plane of the grid.
+// the top plane of the grid; in a batch of scans the scan comes first, so
+// each scan's tracks form one run of ranks.
 __global__ void __launch_bounds__(256)
 k_seg_count(BrickGeo G, size_t n_trk, size_t nv,
             const uint64_t* __restrict__ row_ptr,
-            const uint32_t* __restrict__ vox, uint32_t* __restrict__ nseg,
-            uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+            const uint32_t* __restrict__ vox, const uint16_t* __restrict__ scan,
+            uint32_t* __restrict__ nseg, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_trk) return;
   const uint64_t b = row_ptr[t], e = row_ptr[t + 1];
@@ -142,7 +144,7 @@ k_seg_count(BrickGeo G, size_t n_trk, size_t nv,
   const float xt = fminf(fmaxf(x0 + s * (x1 - x0) + 1024.0f, 0.0f), 4095.0f);
   const float yt = fminf(fmaxf(y0 + s * (y1 - y0) + 1024.0f, 0.0f), 4095.0f);
   const uint32_t mort = spread12((uint32_t)xt) | (spread12((uint32_t)yt) << 1);
-  key[t] = ((uint64_t)dbin << 24) | mort;
+  key[t] = ((scan ? (uint64_t)scan[t] : 0ull) << 48) | ((uint64_t)dbin << 24) | mort;
 }
 
 __global__ void k_invert(size_t n, const uint32_t* __restrict__ order,
@@ -297,16 +299,32 @@ k_tseg_fill(uint64_t n_seg, const uint32_t* __restrict__ sval,
   tseg_pos[tseg_ptr[rank[t]] + (uint32_t)(s - sstart[t])] = (uint32_t)q;
 }
 
+// longest path over the tracks (tail of the first entry plus its length),
+// as the bit pattern of a non-negative double (ordered like the values)
+__global__ void __launch_bounds__(256)
+k_max_path(size_t n_trk, const uint64_t* __restrict__ row_ptr, const double* __restrict__ len,
+           const double* __restrict__ tail, unsigned long long* __restrict__ out) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (t < n_trk && row_ptr[t + 1] > row_ptr[t]) v = fmax(tail[row_ptr[t]] + len[row_ptr[t]], 0.0);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
+}
+
 // Per-track data in rank order.
 __global__ void k_trk_d(size_t n_trk, const uint32_t* __restrict__ order,
                         const double* __restrict__ trk4,
                         const double* __restrict__ pr,
                         const uint32_t* __restrict__ nseg,
                         double* __restrict__ trk_d,
-                        uint32_t* __restrict__ nseg_rank) {
+                        uint32_t* __restrict__ nseg_rank,
+                        const uint16_t* __restrict__ scan_in,
+                        uint16_t* __restrict__ scan_rank) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_trk) return;
   const uint32_t t = order[r];
+  if (scan_rank) scan_rank[r] = scan_in[t];
   const double Sx = trk4[4 * t], Sy = trk4[4 * t + 1];
   const double Dx = trk4[4 * t + 2], Dy = trk4[4 * t + 3];
   trk_d[4 * r] = Sx * Sx + Sy * Sy;
@@ -517,6 +535,7 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
           const float4* __restrict__ part, const double* __restrict__ trk_d,
           ErrM E, double tail_scale, double lmax, double ubound, int grad_mode,
           float* __restrict__ trk_g, unsigned long long* __restrict__ cmax,
+          const uint16_t* __restrict__ trk_scan,
           double* __restrict__ cost_acc, unsigned long long* __restrict__ cnt) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   double cost = 0.0, bnd = 0.0;
@@ -608,10 +627,17 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(cmax, (unsigned long long)__double_as_longlong(m));
   }
-  double v = cost;
+  // cost per scan: one atomic per wave when the wave's tracks share a scan
+  const uint32_t sc = (trk_scan && r < n_trk) ? trk_scan[r] : 0u;
+  const uint32_t s0 = __shfl(sc, 0, 64);
+  if (__all(sc == s0)) {
+    double v = cost;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(cost_acc, v);
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(&cost_acc[s0], v);
+  } else if (cost != 0.0) {
+    atomicAdd(&cost_acc[sc], cost);
+  }
   unsigned long long m = __ballot(sing);
   if (m && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)m) - 1))
     atomicAdd(&cnt[C_SINGULAR], (unsigned long long)__popcll(m));
@@ -745,19 +771,19 @@ k_upd32(size_t nv, const uint32_t* __restrict__ mj, double* __restrict__ acc,
 }
 
 // clears the iteration's cost sum and per-iteration counters
-__global__ void k_iter_begin(double* __restrict__ cost_acc, unsigned long long* __restrict__ cnt) {
-  if (threadIdx.x == 0) cost_acc[0] = 0.0;
+__global__ void k_iter_begin(uint32_t n_scans, double* __restrict__ cost_acc,
+                             unsigned long long* __restrict__ cnt) {
+  for (uint32_t k = threadIdx.x; k < n_scans; k += blockDim.x) cost_acc[k] = 0.0;
   if (threadIdx.x < 3) cnt[threadIdx.x] = 0ull;
 }
 
-// appends the iteration's cost sum to the history
-__global__ void k_cost_push(const double* __restrict__ cost_acc, double* __restrict__ hist,
-                            unsigned int* __restrict__ n) {
-  if (threadIdx.x == 0) {
-    const unsigned int k = n[0];
-    hist[k] = cost_acc[0];
-    n[0] = k + 1u;
-  }
+// appends the iteration's cost sums (one per scan) to the history
+__global__ void k_cost_push(uint32_t n_scans, const double* __restrict__ cost_acc,
+                            double* __restrict__ hist, unsigned int* __restrict__ n) {
+  const unsigned int k = n[0];
+  for (uint32_t q = threadIdx.x; q < n_scans; q += blockDim.x) hist[(size_t)k * n_scans + q] = cost_acc[q];
+  __syncthreads();
+  if (threadIdx.x == 0) n[0] = k + 1u;
 }
 
 __global__ void k_mirror32(size_t n, const double* __restrict__ a, float* __restrict__ b) {
@@ -854,14 +880,14 @@ int fp32c_build(muto_em_plan* pl) {
   MUTO_CHECK_HIP(ctx, sstart.alloc(nt + 1));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(nseg.p, 0, (nt + 1) * sizeof(uint32_t), s));
   k_seg_count<<<grid_for(nt, 256), 256, 0, s>>>(G, nt, pl->nv, pl->row_ptr.p, pl->vox.p,
-                                                nseg.p, tkey.p, order.p);
+                                                pl->trk_scan.p, nseg.p, tkey.p, order.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   MUTO_CHECK_HIP(ctx, sort_pairs_u64_u32(tkey.p, order.p, nt, 0, 64, s));
   k_invert<<<grid_for(nt, 256), 256, 0, s>>>(nt, order.p, rank.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   MUTO_CHECK_HIP(ctx, exclusive_scan_u32_to_u64(nseg.p, sstart.p, nt + 1, s));
   uint64_t n_seg = 0;
-  MUTO_CHECK_HIP(ctx, hipMemcpy(&n_seg, sstart.p + nt, sizeof n_seg, hipMemcpyDeviceToHost));
+  MUTO_CHECK_HIP(ctx, copy_sync(ctx, &n_seg, sstart.p + nt, sizeof n_seg, hipMemcpyDeviceToHost));
   if (n_seg >= 0xFFFFFFFFull) { ctx->set_error("more than 2^32 segments"); return MUTO_ERR_LIMIT; }
   f.n_seg = n_seg;
   // occupancy-derived persistent grids
@@ -899,8 +925,8 @@ int fp32c_build(muto_em_plan* pl) {
     std::vector<uint32_t> hl(n_seg);
     std::vector<uint64_t> hb(nbk + 1);
     MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
-    if (n_seg) MUTO_CHECK_HIP(ctx, hipMemcpy(hl.data(), lenq.p, n_seg * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    MUTO_CHECK_HIP(ctx, hipMemcpy(hb.data(), bstart.p, (nbk + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (n_seg) MUTO_CHECK_HIP(ctx, copy_sync(ctx, hl.data(), lenq.p, n_seg * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, hb.data(), bstart.p, (nbk + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
     hb[nbk] = n_seg;
     for (uint64_t b = nbk; b-- > 0;)
       if (hb[b] == ~0ull) hb[b] = hb[b + 1];  // empty brick
@@ -969,16 +995,22 @@ int fp32c_build(muto_em_plan* pl) {
     MUTO_CHECK_HIP(ctx, f.tasks.alloc(sorted.size()));
     MUTO_CHECK_HIP(ctx, f.chunks.alloc(chunks.size()));
     if (!sorted.empty())
-      MUTO_CHECK_HIP(ctx, hipMemcpy(f.tasks.p, sorted.data(), sorted.size() * sizeof(Task), hipMemcpyHostToDevice));
+      MUTO_CHECK_HIP(ctx, copy_sync(ctx, f.tasks.p, sorted.data(), sorted.size() * sizeof(Task), hipMemcpyHostToDevice));
     if (!chunks.empty())
-      MUTO_CHECK_HIP(ctx, hipMemcpy(f.chunks.p, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice));
+      MUTO_CHECK_HIP(ctx, copy_sync(ctx, f.chunks.p, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice));
     // brick-major segment arrays in final order; tails as fixed point with
-    // the finest power-of-two step that keeps twice the grid diagonal (the
-    // longest two-leg path inside the grid) below 2^32
+    // the finest power-of-two step that keeps the longest path below 2^32
     {
-      const double ex = pl->g.d[0] * pl->g.n[0], ey = pl->g.d[1] * pl->g.n[1],
-                   ez = pl->g.d[2] * pl->g.n[2];
-      const double bound = 2.0 * sqrt(ex * ex + ey * ey + ez * ez) + 1.0;
+      DBuf<unsigned long long> mp;
+      MUTO_CHECK_HIP(ctx, mp.alloc(1));
+      MUTO_CHECK_HIP(ctx, hipMemsetAsync(mp.p, 0, sizeof(unsigned long long), s));
+      k_max_path<<<grid_for(nt, 256), 256, 0, s>>>(nt, pl->row_ptr.p, pl->len.p, pl->tail.p, mp.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+      unsigned long long bitsv = 0;
+      MUTO_CHECK_HIP(ctx, copy_sync(ctx, &bitsv, mp.p, sizeof bitsv, hipMemcpyDeviceToHost));
+      double maxp;
+      memcpy(&maxp, &bitsv, sizeof maxp);
+      const double bound = maxp * 1.0001 + 1.0;
       int bits = (int)floor(log2(4294967295.0 / bound));
       if (bits > 40) bits = 40;
       f.tail_scale = ldexp(1.0, bits);
@@ -1027,16 +1059,18 @@ int fp32c_build(muto_em_plan* pl) {
     MUTO_CHECK_HIP(ctx, nseg_rank.alloc(nt + 1));
     MUTO_CHECK_HIP(ctx, tptr64.alloc(nt + 1));
     MUTO_CHECK_HIP(ctx, f.trk_d.alloc(4 * nt));
+    if (pl->n_scans > 1) MUTO_CHECK_HIP(ctx, f.trk_scan.alloc(nt));
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(nseg_rank.p, 0, (nt + 1) * sizeof(uint32_t), s));
     k_trk_d<<<grid_for(nt, 256), 256, 0, s>>>(nt, order.p, pl->trk4.p, pl->pr.p, nseg.p,
-                                              f.trk_d.p, nseg_rank.p);
+                                              f.trk_d.p, nseg_rank.p, pl->trk_scan.p,
+                                              pl->n_scans > 1 ? f.trk_scan.p : nullptr);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     MUTO_CHECK_HIP(ctx, exclusive_scan_u32_to_u64(nseg_rank.p, tptr64.p, nt + 1, s));
     std::vector<uint64_t> h64(nt + 1);
-    MUTO_CHECK_HIP(ctx, hipMemcpy(h64.data(), tptr64.p, (nt + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, h64.data(), tptr64.p, (nt + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
     std::vector<uint32_t> h32(h64.begin(), h64.end());
     MUTO_CHECK_HIP(ctx, f.tseg_ptr.alloc(nt + 1));
-    MUTO_CHECK_HIP(ctx, hipMemcpy(f.tseg_ptr.p, h32.data(), (nt + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, f.tseg_ptr.p, h32.data(), (nt + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
     MUTO_CHECK_HIP(ctx, f.tseg_pos.alloc(n_seg));
     if (n_seg) {
       k_tseg_fill<<<grid_for(n_seg, 256), 256, 0, s>>>(n_seg, sval.p, st_trk.p, sstart.p, rank.p,
@@ -1075,7 +1109,7 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
   const size_t lds_f = f.bs.vol() * sizeof(float), lds_b = 2 * f.bs.vol() * sizeof(float);
   const ErrM E = error_matrix(pl->cfg);
   const bool map = pl->is_map;
-  k_iter_begin<<<1, 64, 0, s>>>(pl->cost_acc.p, pl->dcount.p);
+  k_iter_begin<<<1, 256, 0, s>>>(pl->n_scans, pl->cost_acc.p, pl->dcount.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[0], s));
   if (f.n_tasks) {
@@ -1089,7 +1123,9 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
   k_track32<<<grid_for(pl->n_trk, 256), 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p,
                                                       reinterpret_cast<const float4*>(f.seg_part.p),
                                                       f.trk_d.p, E, f.tail_scale, f.lmax, f.ubound, 0,
-                                                      f.trk_g.p, f.cmax.p, pl->cost_acc.p, pl->dcount.p);
+                                                      f.trk_g.p, f.cmax.p,
+                                                      pl->n_scans > 1 ? f.trk_scan.p : nullptr,
+                                                      pl->cost_acc.p, pl->dcount.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[2], s));
   if (f.n_tasks) {
@@ -1109,7 +1145,7 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
     k_commit<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->tmp64.p, pl->lam64.p, pl->lam32.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
-  k_cost_push<<<1, 64, 0, s>>>(pl->cost_acc.p, f.hist.p, f.hist_n.p);
+  k_cost_push<<<1, 256, 0, s>>>(pl->n_scans, pl->cost_acc.p, f.hist.p, f.hist_n.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[4], s));
   return MUTO_OK;
@@ -1139,7 +1175,8 @@ static int fp32c_graph(muto_em_plan* pl) {
   return MUTO_OK;
 }
 
-// `iters` FP32C iterations; cost sums per iteration into sums[], *done the
+// `iters` FP32C iterations; cost sums per iteration (n_scans per iteration,
+// scan fastest) into sums[], *done the
 // iterations run.  Early stop (MutoMLSD.cpp:203-210) needs each cost before
 // the next iteration starts: then every iteration is synchronised,
 // otherwise all are enqueued back to back and the costs read once.
@@ -1150,12 +1187,13 @@ int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done) {
   hipStream_t s = ctx->stream;
   *done = 0;
   if (pl->is_map && !pl->tmp64.p) MUTO_CHECK_HIP(ctx, pl->tmp64.alloc(pl->nv));
-  if (f.hist_cap < (size_t)iters) {
+  const size_t ns = pl->n_scans;
+  if (f.hist_cap < (size_t)iters * ns) {
     if (f.graph) (void)hipGraphExecDestroy(f.graph);
     f.graph = nullptr;
     f.hist.release();
-    MUTO_CHECK_HIP(ctx, f.hist.alloc((size_t)iters));
-    f.hist_cap = (size_t)iters;
+    MUTO_CHECK_HIP(ctx, f.hist.alloc((size_t)iters * ns));
+    f.hist_cap = (size_t)iters * ns;
   }
   if (!f.hist_n.p) MUTO_CHECK_HIP(ctx, f.hist_n.alloc(1));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.hist_n.p, 0, sizeof(unsigned int), s));
@@ -1171,7 +1209,8 @@ int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done) {
     else if (int rc = fp32c_enqueue(pl, pl->timing ? pl->ev : nullptr))
       return rc;
     if (!each) continue;
-    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(&sums[it], f.hist.p + it, sizeof(double), hipMemcpyDeviceToHost, s));
+    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(&sums[it * ns], f.hist.p + it * ns, ns * sizeof(double),
+                                       hipMemcpyDeviceToHost, s));
     MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
     *done = it + 1;
     if (pl->timing) {
@@ -1180,12 +1219,12 @@ int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done) {
       for (int k = 0; k < 4; ++k) tsum[k] += t[k];
       tsum[4] += t[0] + t[1] + t[2] + t[3];
     }
-    const double c = sums[it] / (double)pl->n_trk;
+    const double c = sums[it * ns] / (double)pl->n_trk;  // early stop: single scan
     if (pl->cfg.early_stop && prev < c) break;
     prev = c;
   }
   if (!each && iters > 0) {
-    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(sums, f.hist.p, iters * sizeof(double), hipMemcpyDeviceToHost, s));
+    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(sums, f.hist.p, iters * ns * sizeof(double), hipMemcpyDeviceToHost, s));
     MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
     *done = iters;
   }
@@ -1224,7 +1263,9 @@ int fp32c_map_forward(muto_em_plan* pl, const double* lam, float* g_out,
   k_track32<<<grid_for(pl->n_trk, 256), 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p,
                                                       reinterpret_cast<const float4*>(f.seg_part.p),
                                                       f.trk_d.p, E, f.tail_scale, f.lmax, f.ubound, 1,
-                                                      f.trk_g.p, f.cmax.p, pl->cost_acc.p, pl->dcount.p);
+                                                      f.trk_g.p, f.cmax.p,
+                                                      pl->n_scans > 1 ? f.trk_scan.p : nullptr,
+                                                      pl->cost_acc.p, pl->dcount.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   MUTO_CHECK_HIP(ctx, hipMemcpyAsync(g_out, f.trk_g.p, 4 * pl->n_trk * sizeof(float), hipMemcpyDeviceToDevice, s));
   MUTO_CHECK_HIP(ctx, hipMemcpyAsync(cmax_out, f.cmax.p, sizeof(unsigned long long), hipMemcpyDeviceToDevice, s));
@@ -1253,6 +1294,15 @@ int fp32c_map_backward(muto_em_plan* pl, const float* g, const unsigned long lon
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->work.p, 0, 4 * sizeof(unsigned int), s));
+  return MUTO_OK;
+}
+
+// the float mirror of the plan's FP64 image
+int fp32c_mirror(muto_em_plan* pl) {
+  muto_ctx* ctx = pl->ctx;
+  if (!pl->lam32.p) MUTO_CHECK_HIP(ctx, pl->lam32.alloc(pl->nv));
+  k_mirror32<<<grid_for(pl->nv, 256), 256, 0, ctx->stream>>>(pl->nv, pl->lam64.p, pl->lam32.p);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
   return MUTO_OK;
 }
 

@@ -48,6 +48,7 @@ struct Fp32cData {
   DBuf<float> seg_part;     // [4*n_seg] forward partial sums (float4, brick-major)
   DBuf<float> trk_g;        // [4*n_trk] p^2 G' (g00, g01, g11, basis t_c bits)
   DBuf<unsigned long long> cmax;  // [1] bound on |c_ij| (FP64 bits), per iteration
+  DBuf<uint16_t> trk_scan;  // [n_trk] rank order: scan of each track (batches)
   double lmax = 0.0, ubound = 0.0;  // entry length bound; segment extent bound
   DBuf<Task> tasks;
   DBuf<Chunk> chunks;
@@ -82,6 +83,8 @@ struct muto_em_plan {
   muto::DBuf<double> len, tail;
   muto::DBuf<uint32_t> mj;      // entries per voxel
   muto::DBuf<double> geo;       // [9*n_trk] pin, pout, PoCA point (path recompute)
+  uint32_t n_scans = 1;         // > 1: a batch of scans laid side by side in one grid
+  muto::DBuf<uint16_t> trk_scan;  // [n_trk] scan of each track (batches only)
   bool has_lists = true;        // row_ptr/vox/len/tail hold the paths
 
   // voxel-major slots for median / trimmed mean

@@ -71,6 +71,16 @@ struct DBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// A copy ordered after the work already on the context's stream, waited
+// for.  (The stream is non-blocking, so a plain hipMemcpy on the null
+// stream would not wait for the stream's kernels.)
+inline hipError_t copy_sync(muto_ctx* ctx, void* dst, const void* src, size_t bytes,
+                            hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(ctx->stream);
+}
+
 inline unsigned grid_for(size_t n, unsigned block) {
   size_t g = (n + block - 1) / block;
   if (g == 0) g = 1;

@@ -490,3 +490,43 @@ def test_cli_poca_then_mlsd_matches_library(ctx, small_set, tmp_path):
     cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, num_iteration=5, initialize_method=0)
     ref, _, _ = ctx.mlsd_reconstruct(cfg, rc, init=pimg)
     np.testing.assert_allclose(mimg, ref.astype(np.float32), rtol=1e-5, atol=0)
+
+
+def test_batch_of_scans_matches_each_scan_alone(ctx):
+    """Six scans of different grids, voxel sizes and track counts in one
+    batch (one launch per pass, one graph per iteration): each scan's images
+    and costs equal those of its plan run alone in FP32C up to the order of
+    FP64 atomics and the fixed-point scale of the backprojection sums, which
+    the batch sets from the largest |c_ij| of all scans: costs rtol 1e-7,
+    images within 1e-5 of their largest value after 5 iterations (FP32C's
+    own distance to FP64 is larger)."""
+    specs = W.scan_batch(n_scans=6, seed=5, n_scale=0.05)
+    plans, alone = [], []
+    for g, gcfg, nt in specs:
+        rays, _ = M.generate(gcfg, nt)
+        cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100)
+        a = ctx.em_plan(cfg, rays=rays)
+        b = ctx.em_plan(cfg, rays=rays)
+        img = np.full(g.n_vox, 1e-7)
+        a.set_image(img)
+        b.set_image(img)
+        plans.append(a)
+        alone.append(b)
+    batch = ctx.batch(plans)
+    cost = batch.run(5)
+    ms, c = batch.stats()
+    assert ms > 0 and c["nnz"] == sum(p.counters()["nnz"] for p in plans)
+    for s, (a, b) in enumerate(zip(plans, alone)):
+        cb = b.run(5, M.MUTO_FP32C)
+        np.testing.assert_allclose(cost[:, s], cb, rtol=1e-7)
+        ia, ib = a.image(), b.image()
+        m = ib > 0
+        assert np.abs(ia - ib).max() <= 1e-5 * np.abs(ib).max(), (s, np.abs(ia - ib).max())
+        assert np.all(ia[~m] == 0)
+    # a second run continues from the images written back
+    cost2 = batch.run(2)
+    for s, b in enumerate(alone):
+        np.testing.assert_allclose(cost2[:, s], b.run(2, M.MUTO_FP32C), rtol=1e-7)
+    batch.close()
+    for p in plans + alone:
+        p.close()
