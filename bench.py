@@ -67,16 +67,121 @@ def cpu_baseline(g, rays, cfg_kw, n_sample, iters):
                       f"loop over {threads} threads"}
 
 
+def bench_scans(args):
+    """64 independent scans (grids 32^3-128^3, 10-30 mm voxels, 1e4-1e6
+    tracks each) run as one batch: FP32C mean update, one launch per pass
+    and one graph per iteration for all scans.  Every scan finishes with
+    the batch; the scans' latencies run one at a time are reported beside."""
+    import muontomography_amd as M
+    from muontomography_amd import workloads as W
+    from muontomography_amd._abi import mlsd_cfg
+    ctx = M.Context(0)
+    t0 = time.perf_counter()
+    specs = W.scan_batch(n_scans=args.n_scans, seed=77)
+    plans = []
+    for g, gcfg, nt in specs:
+        rays, _ = M.generate(gcfg, nt)
+        cfg = mlsd_cfg(g, update_with_mean=1, mean_percentile=100, error_xy=0.5,
+                       error_dz_outer=500.0, error_dz_inner=500.0, initialize_method=2)
+        p = ctx.em_plan(cfg, rays=rays)
+        p.set_image(np.full(g.n_vox, 1e-7))
+        plans.append(p)
+    nnz = [p.counters()["nnz"] for p in plans]
+    log(f"{len(plans)} scans, {sum(nnz)} entries, set-up {time.perf_counter() - t0:.1f} s")
+    batch = ctx.batch(plans)
+    batch.run(args.warmup)
+    for p in plans:
+        p.set_image(np.full(p.cfg.grid.n_vox, 1e-7))
+    t0 = time.perf_counter()
+    batch.run(args.steps)
+    wall = time.perf_counter() - t0
+    alone = []
+    for p in plans:
+        p.set_image(np.full(p.cfg.grid.n_vox, 1e-7))
+        p.run(1, M.MUTO_FP32C)  # brick build, graph capture
+        p.set_image(np.full(p.cfg.grid.n_vox, 1e-7))
+        t1 = time.perf_counter()
+        p.run(args.steps, M.MUTO_FP32C)
+        alone.append(time.perf_counter() - t1)
+    alone = np.array(alone) * 1e3
+    out = {"metric": "many-scan batch: track-voxel updates/s (FP32C mean update)",
+           "value": sum(nnz) * args.steps / wall, "unit": "track-voxel updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+           "higher_is_better": True, "dtype": "f32 entry terms, f64/int64 sums, f64 2x2 algebra",
+           "data": "synthetic (seeded generator, one block layout per scan)",
+           "config": {"workload": f"{len(plans)} scans, grids 32^3-128^3 of 10-30 mm voxels, "
+                                  "1e4-1e6 tracks each, 30 iterations", "entries": int(sum(nnz))},
+           "batch_latency_ms": wall * 1e3,
+           "alone_latency_ms_p50": float(np.percentile(alone, 50)),
+           "alone_latency_ms_p99": float(np.percentile(alone, 99)),
+           "alone_total_ms": float(alone.sum()),
+           "alone_value": sum(nnz) * args.steps / (alone.sum() * 1e-3)}
+    batch.close()
+    for p in plans:
+        p.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
+def bench_map_grad(args):
+    """MAP objective + gradient on 128^3 x 15 mm with 1e6 muons: FP32C
+    (brick stream), FP64 (stored lists) and FP64 with the paths walked again
+    (no lists); gradient error of FP32C against FP64."""
+    import muontomography_amd as M
+    from muontomography_amd import workloads as W
+    from muontomography_amd._abi import mlsd_cfg
+    ctx = M.Context(0)
+    g, rays, _ = W.muons("map128", n_muons=args.n_muons)
+    cfg = mlsd_cfg(g, update_with_mean=1, mean_percentile=100, error_xy=0.5, error_dz_outer=500.0,
+                   error_dz_inner=500.0, beta=1.0, p=1.2, delta=0.1, all_neighbours=1)
+    plan = ctx.em_plan(cfg, rays=rays)
+    lean = ctx.em_plan(cfg, rays=rays, list_budget=0)
+    lam = 1e-7 * (0.5 + np.random.default_rng(1).random(g.n_vox))
+    nnz = plan.counters()["nnz"]
+    res = {}
+    for name, pl, prec, rc in (("fp32c", plan, M.MUTO_FP32C, False), ("fp64", plan, M.MUTO_FP64, False),
+                               ("fp64_recompute", lean, M.MUTO_FP64, True)):
+        sv = pl.map_forward(lam, prec, recompute=rc)
+        gr = sv.backward(1.0)
+        sv.close()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sv = pl.map_forward(lam, prec, recompute=rc)
+            gr = sv.backward(1.0)
+            sv.close()
+        t = (time.perf_counter() - t0) / args.steps
+        res[name] = (t, gr, sv.phi)
+        log(f"{name}: {t * 1e3:.2f} ms per value+gradient")
+    g64 = res["fp64"][1]
+    out = {"metric": "MAP objective value+gradient evaluations/s",
+           "value": 1.0 / res["fp32c"][0], "unit": "evaluations/s", "n_gpus": 1,
+           "steps": args.steps, "higher_is_better": True,
+           "dtype": "fp32c (f32 entry terms); fp64 beside", "data": "synthetic",
+           "config": {"workload": "MAP gradient, 128^3 grid of 15 mm voxels, 1e6 muons",
+                      "entries": int(nnz)},
+           "ms": {k: v[0] * 1e3 for k, v in res.items()},
+           "entries_per_s": {k: 2 * nnz / v[0] for k, v in res.items()},
+           "grad_err_fp32c_vs_fp64_norm_inf": float(np.abs(res["fp32c"][1] - g64).max() / np.abs(g64).max()),
+           "grad_err_recompute_vs_fp64_norm_inf": float(np.abs(res["fp64_recompute"][1] - g64).max() /
+                                                         np.abs(g64).max())}
+    plan.close()
+    lean.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=45)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="em256")
+    ap.add_argument("--workload", default="em256",
+                    help="em256 (the headline); scans64 (many-scan batch); map128 (MAP gradient)")
     ap.add_argument("--n-muons", type=int, default=None)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=5)
+    ap.add_argument("--n-scans", type=int, default=64, help="scans64: number of scans")
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -97,6 +202,10 @@ def main():
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is designed, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "scans64":
+        return bench_scans(args)
+    if args.workload == "map128":
+        return bench_map_grad(args)
 
     import muontomography_amd as M
     from muontomography_amd import workloads as W
