@@ -181,6 +181,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=5)
+    ap.add_argument("--tune-iters", type=int, default=3,
+                    help="iterations timed per brick shape by the autotuner (0: 16^3, no tuning)")
     ap.add_argument("--n-scans", type=int, default=64, help="scans64: number of scans")
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--cpu-iters", type=int, default=3)
@@ -227,7 +229,13 @@ def main():
     log(f"plan: {c['n_scat_in_grid']} selected tracks, {nnz} entries "
         f"({nnz / max(c['n_scat_in_grid'], 1):.0f}/track) in {time.perf_counter() - t0:.1f} s")
 
-    # FP32C: warm-up (includes the one-time brick-stream build), then K steps
+    # FP32C: brick (LDS tile) shape chosen by the autotuner, warm-up (the
+    # stream build and graph capture), then K steps
+    plan.set_image(init)
+    t0 = time.perf_counter()
+    brick, tune = plan.autotune(iters=args.tune_iters) if args.tune_iters > 0 else ((16, 16, 16), {})
+    log(f"autotune {time.perf_counter() - t0:.1f} s: brick {brick}; " +
+        ", ".join(f"{k}: {v:.2f} ms" for k, v in tune.items()))
     plan.set_image(init)
     t0 = time.perf_counter()
     plan.run(args.warmup, M.MUTO_FP32C)
@@ -246,6 +254,7 @@ def main():
     plan.run(args.timing_steps, M.MUTO_FP32C)
     plan.set_option(M.MUTO_OPT_TIMING, 0)
     tk = plan.timing()
+    n_seg = plan.counters()["n_segments"]
     log(f"FP32C {args.steps} it: {ms_per_step:.3f} ms/it wall; kernels (events, "
         f"{args.timing_steps} it) {tk}")
 
@@ -314,7 +323,8 @@ def main():
                                "80-600-voxel paths, mean update, FP32C beside FP64",
                    "grid": [g.nx, g.ny, g.nz], "voxel_mm": g.dx, "muons": int(len(rays)),
                    "selected_tracks": c["n_scat_in_grid"], "entries": nnz,
-                   "segments": None, "parallelism": "1 GPU"},
+                   "segments": n_seg, "brick": list(brick),
+                   "parallelism": "1 GPU"},
         "kernel_ms": {k: round(v, 4) for k, v in tk.items()},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -267,7 +267,18 @@ int muto_batch_destroy(muto_batch* batch);
 #define MUTO_OPT_GRAPH 1
 #define MUTO_OPT_TIMING 2
 #define MUTO_OPT_ACC_MODE 3
+#define MUTO_OPT_BRICK 4  /* FP32C brick (LDS tile) shape bx | by << 8 | bz << 16,
+                             at most 16384 voxels; default 16 x 16 x 16 */
 int muto_em_plan_set_option(muto_em_plan* plan, int option, int value);
+
+/* FP32C autotuner: builds the brick stream for each candidate tile shape
+ * (shapes[k] = bx | by << 8 | bz << 16; NULL/0: a built-in list from 8x8x128
+ * to 32x32x8), times `iters` iterations of each from the plan's image
+ * (restored after every trial) and keeps the fastest.  ms[k] (may be NULL)
+ * receives each shape's ms per iteration, *best (may be NULL) the chosen
+ * shape.  No counterpart in the reference. */
+int muto_em_plan_autotune(muto_em_plan* plan, int iters, int n_shapes, const uint32_t* shapes,
+                          double* ms, uint32_t* best);
 
 int muto_em_plan_counters(const muto_em_plan* plan, muto_counters* out);
 /* Per-voxel entry counts M_j (host, nx*ny*nz uint32). */

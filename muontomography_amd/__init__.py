@@ -34,6 +34,11 @@ MUTO_MAP_RECOMPUTE_SIDDON = 1
 MUTO_OPT_GRAPH = 1
 MUTO_OPT_TIMING = 2
 MUTO_OPT_ACC_MODE = 3
+MUTO_OPT_BRICK = 4
+# the autotuner's built-in candidate list (muto_em_plan_autotune), in order
+DEFAULT_BRICKS = [bx | by << 8 | bz << 16 for bx, by, bz in
+                  ((16, 16, 16), (8, 8, 64), (8, 16, 32), (16, 8, 32), (16, 16, 32), (8, 8, 32),
+                   (32, 32, 8), (16, 16, 8), (8, 8, 128))]
 
 
 class MutoError(RuntimeError):
@@ -243,6 +248,22 @@ class EmPlan:
         sv = MapSaved(self, h, phi.value)
         self._saved.add(sv)
         return sv
+
+    def autotune(self, iters=3, shapes=None):
+        """FP32C brick (LDS tile) shape autotuner (muto_em_plan_autotune):
+        returns the chosen (bx, by, bz) and {shape: ms per iteration}.  The
+        plan's image is left as it was."""
+        arr = None if shapes is None else np.array([bx | by << 8 | bz << 16 for bx, by, bz in shapes],
+                                                   np.uint32)
+        n = 0 if arr is None else arr.size
+        ms = np.full(max(n, 16), np.nan)
+        best = C.c_uint32(0)
+        self.ctx._check(lib().muto_em_plan_autotune(self._h, int(iters), n, ptr(arr), ptr(ms),
+                                                    C.byref(best)), "muto_em_plan_autotune")
+        dec = lambda v: (int(v) & 255, (int(v) >> 8) & 255, (int(v) >> 16) & 255)
+        if arr is None:
+            arr = np.array(DEFAULT_BRICKS, np.uint32)
+        return dec(best.value), {dec(v): float(ms[k]) for k, v in enumerate(arr)}
 
     def set_option(self, option, value):
         """MUTO_OPT_GRAPH, MUTO_OPT_TIMING or MUTO_OPT_ACC_MODE (include/muto.h)."""

@@ -156,6 +156,7 @@ PROTOS = {
     "muto_em_plan_voxel_counts": (C.c_int, [P, P]),
     "muto_em_plan_timing": (C.c_int, [P, P]),
     "muto_em_plan_set_option": (C.c_int, [P, C.c_int, C.c_int]),
+    "muto_em_plan_autotune": (C.c_int, [P, C.c_int, C.c_int, P, P, C.POINTER(C.c_uint32)]),
     "muto_em_plan_create_ex": (C.c_int, [P, C.POINTER(MlsdCfg), C.c_size_t, P, C.c_uint64,
                                          C.POINTER(P)]),
     "muto_map_forward": (C.c_int, [P, P, C.c_int, C.c_int, DP, C.POINTER(P)]),

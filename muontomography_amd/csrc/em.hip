@@ -512,6 +512,7 @@ MapCfgD map_cfg(const muto_em_plan* pl) {
 // FP32C iteration (em32.hip)
 int fp32c_build(muto_em_plan* pl);
 int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done);
+int fp32c_autotune(muto_em_plan* pl, int iters, int n_cand, const uint32_t* cand, double* ms);
 int fp32c_release(muto_em_plan* pl);
 
 }  // namespace muto
@@ -970,6 +971,17 @@ extern "C" int muto_em_plan_set_option(muto_em_plan* pl, int option, int value) 
     case MUTO_OPT_TIMING:
       pl->timing = value != 0;
       return MUTO_OK;
+    case MUTO_OPT_BRICK: {
+      const uint32_t bx = value & 0xFF, by = (value >> 8) & 0xFF, bz = (value >> 16) & 0xFF;
+      if (bx == 0 || by == 0 || bz == 0 || bx * by * bz > 16384) break;
+      if (bx != pl->f32.bs.bx || by != pl->f32.bs.by || bz != pl->f32.bs.bz) {
+        pl->f32.reset();
+        pl->f32.bs.bx = bx;
+        pl->f32.bs.by = by;
+        pl->f32.bs.bz = bz;
+      }
+      return MUTO_OK;
+    }
     case MUTO_OPT_ACC_MODE:
       if (value != 0 && value != 1) break;
       if (pl->f32.acc_mode != value && pl->f32.graph) {
@@ -984,6 +996,33 @@ extern "C" int muto_em_plan_set_option(muto_em_plan* pl, int option, int value) 
   }
   pl->ctx->set_error("muto_em_plan_set_option: unknown option or value");
   return MUTO_ERR_ARG;
+}
+
+extern "C" int muto_em_plan_autotune(muto_em_plan* pl, int iters, int n_cand,
+                                     const uint32_t* shapes, double* ms, uint32_t* best) {
+  if (!pl) return MUTO_ERR_ARG;
+  muto_ctx* ctx = pl->ctx;
+  static const uint32_t kDefault[] = {
+      16u | 16u << 8 | 16u << 16, 8u | 8u << 8 | 64u << 16,  8u | 16u << 8 | 32u << 16,
+      16u | 8u << 8 | 32u << 16,  16u | 16u << 8 | 32u << 16, 8u | 8u << 8 | 32u << 16,
+      32u | 32u << 8 | 8u << 16,  16u | 16u << 8 | 8u << 16,  8u | 8u << 8 | 128u << 16};
+  if (!shapes || n_cand <= 0) {
+    shapes = kDefault;
+    n_cand = (int)(sizeof kDefault / sizeof kDefault[0]);
+  }
+  if (pl->cur_prec < 0) { ctx->set_error("muto_em_plan_autotune: no image set"); return MUTO_ERR_STATE; }
+  if (!pl->has_lists || pl->n_trk == 0) { ctx->set_error("muto_em_plan_autotune: nothing to tune"); return MUTO_ERR_STATE; }
+  if (!(pl->cfg.update_with_mean && pl->cfg.mean_percentile == 100)) {
+    ctx->set_error("muto_em_plan_autotune: FP32C runs the mean update");
+    return MUTO_ERR_ARG;
+  }
+  std::vector<double> t(n_cand);
+  pl->is_map = false;
+  if (int rc = fp32c_autotune(pl, iters, n_cand, shapes, t.data())) return rc;
+  pl->cur_prec = MUTO_FP64;
+  if (ms) std::copy(t.begin(), t.end(), ms);
+  if (best) *best = pl->f32.bs.bx | pl->f32.bs.by << 8 | pl->f32.bs.bz << 16;
+  return MUTO_OK;
 }
 
 extern "C" int muto_em_plan_counters(const muto_em_plan* pl, muto_counters* out) {

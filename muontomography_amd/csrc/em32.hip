@@ -56,6 +56,9 @@ struct ErrM {
 };
 ErrM error_matrix(const muto_mlsd_cfg& c);
 int map_step(muto_em_plan* pl, const double* last, const double* ml, double* out);
+int fp32c_mirror(muto_em_plan* pl);
+int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done);
+int fp32c_build(muto_em_plan* pl);
 
 namespace {
 
@@ -856,6 +859,16 @@ static hipError_t occupancy(int* f, int* b, size_t lds_f, size_t lds_b) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(b, k_bwd32<ACC>, BLOCK, lds_b);
 }
 
+// whether both passes' LDS (static + brick arrays) fit one workgroup
+static bool fp32c_fits(muto_ctx* ctx, const BrickShape& b) {
+  int max_lds = 0;
+  if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess)
+    return false;
+  const size_t fwd = 2 * 3 * BLOCK * 8 + 16 + (size_t)b.vol() * sizeof(float);
+  const size_t bwd = 2 * BLOCK * 16 + 16 + 2 * (size_t)b.vol() * sizeof(float);
+  return std::max(fwd, bwd) <= (size_t)max_lds;
+}
+
 int fp32c_build(muto_em_plan* pl) {
   Fp32cData& f = pl->f32;
   if (f.built) return MUTO_OK;
@@ -869,7 +882,11 @@ int fp32c_build(muto_em_plan* pl) {
   const BrickGeo G = brick_geo(pl);
   f.nb[0] = G.nb[0]; f.nb[1] = G.nb[1]; f.nb[2] = G.nb[2];
   const uint64_t nbk = (uint64_t)G.nb[0] * G.nb[1] * G.nb[2];
-  if (f.bs.vol() > 32768) { ctx->set_error("brick volume above 32768 voxels"); return MUTO_ERR_LIMIT; }
+  // the backprojection keeps 8 bytes per brick voxel in LDS
+  if (f.bs.vol() == 0 || !fp32c_fits(ctx, f.bs)) {
+    ctx->set_error("FP32C brick shape does not fit the LDS of one workgroup");
+    return MUTO_ERR_LIMIT;
+  }
   const size_t nt = pl->n_trk;
   DBuf<uint32_t> nseg, order, rank;
   DBuf<uint64_t> tkey, sstart;
@@ -1303,6 +1320,62 @@ int fp32c_mirror(muto_em_plan* pl) {
   if (!pl->lam32.p) MUTO_CHECK_HIP(ctx, pl->lam32.alloc(pl->nv));
   k_mirror32<<<grid_for(pl->nv, 256), 256, 0, ctx->stream>>>(pl->nv, pl->lam64.p, pl->lam32.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
+  return MUTO_OK;
+}
+
+// Brick-shape autotuner: for each candidate shape, rebuilds the stream and
+// times `iters` FP32C iterations (HIP events, after one untimed iteration)
+// from the plan's current image, which is restored after every trial; keeps
+// the fastest shape built.  ms[k] receives each candidate's ms per iteration
+// (shapes as bx | by << 8 | bz << 16 in shapes[k]).
+int fp32c_autotune(muto_em_plan* pl, int iters, int n_cand, const uint32_t* cand, double* ms) {
+  muto_ctx* ctx = pl->ctx;
+  hipStream_t s = ctx->stream;
+  if (iters < 1) iters = 1;
+  DBuf<double> save;
+  MUTO_CHECK_HIP(ctx, save.alloc(pl->nv));
+  MUTO_CHECK_HIP(ctx, hipMemcpyAsync(save.p, pl->lam64.p, pl->nv * sizeof(double), hipMemcpyDeviceToDevice, s));
+  const BrickShape keep = pl->f32.bs;
+  const int timing = pl->timing;
+  double best = INFINITY;
+  int best_k = -1;
+  std::vector<double> sums((size_t)(iters + 1) * pl->n_scans);
+  for (int k = 0; k < n_cand; ++k) {
+    BrickShape b;
+    b.bx = cand[k] & 0xFFu;
+    b.by = (cand[k] >> 8) & 0xFFu;
+    b.bz = (cand[k] >> 16) & 0xFFu;
+    ms[k] = INFINITY;
+    if (b.bx == 0 || b.by == 0 || b.bz == 0 || !fp32c_fits(ctx, b)) continue;
+    pl->f32.reset();
+    pl->f32.bs = b;
+    if (int rc = fp32c_build(pl)) return rc;
+    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(pl->lam64.p, save.p, pl->nv * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (int rc = fp32c_mirror(pl)) return rc;
+    int done = 0;
+    pl->timing = 0;
+    if (int rc = fp32c_run(pl, 1, sums.data(), &done)) return rc;  // warm (graph capture)
+    pl->timing = 1;
+    if (int rc = fp32c_run(pl, iters, sums.data(), &done)) return rc;
+    ms[k] = pl->t_ms[4];
+    if (ms[k] < best) {
+      best = ms[k];
+      best_k = k;
+    }
+  }
+  pl->timing = timing;
+  pl->f32.reset();
+  if (best_k >= 0) {
+    pl->f32.bs.bx = cand[best_k] & 0xFFu;
+    pl->f32.bs.by = (cand[best_k] >> 8) & 0xFFu;
+    pl->f32.bs.bz = (cand[best_k] >> 16) & 0xFFu;
+  } else {
+    pl->f32.bs = keep;
+  }
+  if (int rc = fp32c_build(pl)) return rc;
+  MUTO_CHECK_HIP(ctx, hipMemcpyAsync(pl->lam64.p, save.p, pl->nv * sizeof(double), hipMemcpyDeviceToDevice, s));
+  if (int rc = fp32c_mirror(pl)) return rc;
+  MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
   return MUTO_OK;
 }
 

@@ -61,6 +61,33 @@ struct Fp32cData {
   DBuf<double> hist;               // [hist_cap] cost sum per iteration of a run
   DBuf<unsigned int> hist_n;       // [1] entries in hist
   size_t hist_cap = 0;
+
+  // drops the built stream (the brick shape and accumulation mode stay)
+  void reset() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+    built = false;
+    n_seg = n_ent = n_ent_pad = 0;
+    n_tasks = n_chunks = 0;
+    seg_trk.release();
+    seg_tfx.release();
+    ent_vox.release();
+    ent_len.release();
+    gdesc.release();
+    gcarry.release();
+    tseg_ptr.release();
+    tseg_pos.release();
+    trk_d.release();
+    seg_part.release();
+    trk_g.release();
+    cmax.release();
+    trk_scan.release();
+    tasks.release();
+    chunks.release();
+    hist.release();
+    hist_n.release();
+    hist_cap = 0;
+  }
 };
 
 }  // namespace muto

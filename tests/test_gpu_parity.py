@@ -530,3 +530,32 @@ def test_batch_of_scans_matches_each_scan_alone(ctx):
     batch.close()
     for p in plans + alone:
         p.close()
+
+
+def test_autotune_keeps_image_and_picks_a_candidate(ctx, small_set):
+    """The brick-shape autotuner leaves the image as it was, picks one of
+    the candidates it timed, and the FP32C run with the chosen shape matches
+    the same shape set by the option (and stays within 1e-4 of FP64)."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100)
+    plan = ctx.em_plan(cfg, rays=rays)
+    img = np.full(g.n_vox, 1e-7)
+    plan.set_image(img)
+    shapes = [(16, 16, 16), (8, 8, 32), (8, 8, 64), (16, 16, 8)]
+    best, ms = plan.autotune(iters=2, shapes=shapes)
+    assert best in shapes and all(np.isfinite(v) and v > 0 for v in ms.values()), (best, ms)
+    np.testing.assert_array_equal(plan.image(), img)
+    c_t = plan.run(10, M.MUTO_FP32C)
+    a_t = plan.image()
+    other = ctx.em_plan(cfg, rays=rays)
+    other.set_option(M.MUTO_OPT_BRICK, best[0] | best[1] << 8 | best[2] << 16)
+    other.set_image(img)
+    c_o = other.run(10, M.MUTO_FP32C)
+    np.testing.assert_allclose(c_t, c_o, rtol=1e-9)
+    np.testing.assert_allclose(a_t, other.image(), rtol=1e-6, atol=1e-6 * np.abs(a_t).max())
+    other.set_image(img)
+    other.run(10, M.MUTO_FP64)
+    mx, _, _ = _rel_stats(a_t, other.image(), other.voxel_counts())
+    assert mx <= 1e-4, mx
+    plan.close()
+    other.close()
