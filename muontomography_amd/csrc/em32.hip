@@ -72,13 +72,31 @@ constexpr uint32_t MAX_TASK_SEG = 1u << 16;
 constexpr uint32_t ECAP = BLOCK * GE;    // entries per chunk, padding included
 // static LDS of both passes is a multiple of 16 bytes, so the dynamic region
 // after it is 16-byte aligned
-static_assert((2 * 3 * BLOCK * 8 + 16) % 16 == 0, "forward static LDS");
+static_assert((2 * 3 * BLOCK * 8 + 2 * BLOCK * 4 + 16) % 16 == 0, "forward static LDS");
 static_assert((2 * BLOCK * 16 + 16) % 16 == 0, "backward static LDS");
 
 struct BrickGeo {
   uint32_t n[3];   // grid
   uint32_t b[3];   // brick shape
   uint32_t nb[3];  // bricks per axis
+};
+
+// Packed track-voxel entry of the FP32C stream (4 bytes): the path length
+// as unsigned fixed point in the low ENT_LBITS bits (step lstep: twice the
+// voxel diagonal over the largest code, so a merged two-leg entry fits),
+// and above it the step from the segment's previous voxel as three 2-bit
+// two's-complement fields (dx, dy, dz in -1..1).  A segment's first voxel
+// is kept per segment.
+constexpr int ENT_LBITS = 26;
+constexpr uint32_t ENT_LMASK = (1u << ENT_LBITS) - 1u;
+struct EntCode {
+  float lstep;  // mm per length code
+  int bx, bxy;  // brick-local index strides
+  __host__ __device__ float len(uint32_t w) const { return (float)(w & ENT_LMASK) * lstep; }
+  __host__ __device__ int step(uint32_t w) const {
+    const int dx = (int)(w << 4) >> 30, dy = (int)(w << 2) >> 30, dz = (int)w >> 30;
+    return dx + bx * dy + bxy * dz;
+  }
 };
 
 __device__ __forceinline__ uint32_t brick_of(const BrickGeo& G, uint32_t j,
@@ -91,6 +109,17 @@ __device__ __forceinline__ uint32_t brick_of(const BrickGeo& G, uint32_t j,
   if (local)
     *local = (x - bx * G.b[0]) + G.b[0] * ((y - by * G.b[1]) + G.b[1] * (z - bz * G.b[2]));
   return bx + G.nb[0] * (by + G.nb[1] * bz);
+}
+
+// Whether voxel j follows voxel pj of the same track inside one segment:
+// same brick and a step of at most one voxel along each axis (the step is
+// what the packed entry stores).
+__device__ __forceinline__ bool seg_continues(const BrickGeo& G, uint32_t pj, uint32_t j) {
+  if (pj == 0xFFFFFFFFu) return false;
+  if (brick_of(G, pj, nullptr) != brick_of(G, j, nullptr)) return false;
+  const int px = (int)(pj % G.n[0]), py = (int)((pj / G.n[0]) % G.n[1]), pz = (int)(pj / G.n[0] / G.n[1]);
+  const int x = (int)(j % G.n[0]), y = (int)((j / G.n[0]) % G.n[1]), z = (int)(j / G.n[0] / G.n[1]);
+  return abs(x - px) <= 1 && abs(y - py) <= 1 && abs(z - pz) <= 1;
 }
 
 __device__ __forceinline__ uint32_t spread12(uint32_t v) {
@@ -122,11 +151,10 @@ k_seg_count(BrickGeo G, size_t n_trk, size_t nv,
   for (uint64_t k = b; k < e; ++k) {
     const uint32_t j = vox[k];
     if (j >= nv) { prev = 0xFFFFFFFFu; continue; }
-    const uint32_t br = brick_of(G, j, nullptr);
-    if (br != prev) ++cnt;
+    if (!seg_continues(G, prev, j)) ++cnt;
     if (!have) { jf = j; have = true; }
     jl = j;
-    prev = br;
+    prev = j;
   }
   nseg[t] = cnt;
   val[t] = (uint32_t)t;
@@ -178,7 +206,7 @@ k_seg_fill(BrickGeo G, size_t n_trk, size_t nv,
     const uint32_t j = vox[k];
     if (j >= nv) { prev = 0xFFFFFFFFu; continue; }
     const uint32_t br = brick_of(G, j, nullptr);
-    if (br != prev) {
+    if (!seg_continues(G, prev, j)) {
       cur = s++;
       skey[cur] = ((uint64_t)br << 32) | r;
       sval[cur] = (uint32_t)cur;
@@ -188,7 +216,7 @@ k_seg_fill(BrickGeo G, size_t n_trk, size_t nv,
     }
     st_len[cur] += 1;
     st_tend[cur] = tail[k];
-    prev = br;
+    prev = j;
   }
 }
 
@@ -237,8 +265,8 @@ __global__ void k_chunk_offsets(uint32_t n, const Chunk* __restrict__ chunks,
 // the length of its last segment after the group (summed in FP64).
 __global__ void __launch_bounds__(BLOCK)
 k_group_desc(const Chunk* __restrict__ chunks, const uint64_t* __restrict__ seg_off_p,
-             const float* __restrict__ ent_len, uint32_t* __restrict__ gdesc,
-             float* __restrict__ gcarry) {
+             const uint32_t* __restrict__ ent, const uint16_t* __restrict__ seg_v0, EntCode ec,
+             uint32_t* __restrict__ gdesc, float* __restrict__ gcarry, uint16_t* __restrict__ gv0) {
   __shared__ uint32_t so[BLOCK + 1];
   const Chunk ch = chunks[blockIdx.x];
   const uint32_t t = threadIdx.x;
@@ -258,10 +286,14 @@ k_group_desc(const Chunk* __restrict__ chunks, const uint64_t* __restrict__ seg_
     s = k;
   }
   double cr = 0.0;
-  for (uint32_t e = e0 + GE; e < so[s + 1]; ++e) cr += (double)ent_len[ch.base + e];
+  for (uint32_t e = e0 + GE; e < so[s + 1]; ++e) cr += (double)ec.len(ent[ch.base + e]);
+  // voxel of entry e0: its segment's first voxel plus the steps up to it
+  uint32_t v = seg_v0[ch.q0 + lo];
+  for (uint32_t e = so[lo] + 1; e <= e0; ++e) v += ec.step(ent[ch.base + e]);
   const uint64_t gi = ch.base / GE + t;
   gdesc[gi] = mask | (lo << 16);
   gcarry[gi] = (float)cr;
+  gv0[gi] = (uint16_t)v;
 }
 
 __global__ void __launch_bounds__(256)
@@ -269,8 +301,9 @@ k_ent_scatter(BrickGeo G, uint64_t n_seg, const uint32_t* __restrict__ sval,
               const uint64_t* __restrict__ st_start,
               const uint32_t* __restrict__ st_len,
               const uint64_t* __restrict__ seg_off,
-              const uint32_t* __restrict__ vox, const double* __restrict__ len,
-              uint16_t* __restrict__ ent_vox, float* __restrict__ ent_len) {
+              const uint32_t* __restrict__ vox, const double* __restrict__ len, double qscale,
+              uint32_t* __restrict__ ent, uint16_t* __restrict__ seg_v0,
+              unsigned long long* __restrict__ n_clamped) {
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_seg) return;
   const uint32_t s = sval[q];
@@ -278,12 +311,29 @@ k_ent_scatter(BrickGeo G, uint64_t n_seg, const uint32_t* __restrict__ sval,
   const uint32_t n = st_len[s];
   const uint64_t dst = seg_off[q];
   // entries with j >= nv end segments, so a segment is contiguous in the
-  // track-major list
+  // track-major list; its first voxel goes to seg_v0, every later entry
+  // stores its step from the one before
+  uint32_t pj = 0;
   for (uint32_t k = 0; k < n; ++k) {
-    uint32_t loc;
-    brick_of(G, vox[src + k], &loc);
-    ent_vox[dst + k] = (uint16_t)loc;
-    ent_len[dst + k] = (float)len[src + k];
+    const uint32_t j = vox[src + k];
+    uint32_t code = 0;
+    if (k == 0) {
+      uint32_t loc;
+      brick_of(G, j, &loc);
+      seg_v0[q] = (uint16_t)loc;
+    } else {
+      const int dx = (int)(j % G.n[0]) - (int)(pj % G.n[0]);
+      const int dy = (int)((j / G.n[0]) % G.n[1]) - (int)((pj / G.n[0]) % G.n[1]);
+      const int dz = (int)(j / G.n[0] / G.n[1]) - (int)(pj / G.n[0] / G.n[1]);
+      code = ((uint32_t)dx & 3u) | ((uint32_t)dy & 3u) << 2 | ((uint32_t)dz & 3u) << 4;
+    }
+    double lq = rint(len[src + k] * qscale);
+    if (!(lq >= 0.0 && lq <= (double)ENT_LMASK)) {  // outside the packed range: reported
+      atomicAdd(n_clamped, 1ull);
+      lq = lq > 0.0 ? (double)ENT_LMASK : 0.0;
+    }
+    ent[dst + k] = code << ENT_LBITS | (uint32_t)lq;
+    pj = j;
   }
 }
 
@@ -358,8 +408,10 @@ struct IterArgs {
   const float* gcarry;
   double inv_scale;
   const uint32_t* seg_trk;
-  const uint16_t* ent_vox;
-  const float* ent_len;
+  const uint16_t* seg_v0;
+  const uint32_t* ent;
+  const uint16_t* gv0;
+  EntCode ec;
 };
 
 __device__ __forceinline__ void brick_origin(const BrickGeo& G, uint32_t brick,
@@ -395,34 +447,37 @@ __device__ __forceinline__ void load_brick(const BrickGeo& G, uint32_t brick,
 // the chunk segment holding entry 0, and `carry` is the length of that
 // segment after the group (the tail of entry 15 inside its segment).
 struct Group {
-  uint32_t vw[GE / 2];  // voxel-in-brick indices, two u16 per word
-  float l[GE];          // lengths
-  uint32_t mask, s0;
+  uint32_t w[GE];  // packed entries
+  uint32_t mask, s0, v0;
   float carry;
   __device__ __forceinline__ void load(const IterArgs& a, uint64_t gi) {
-    const uint4* pv = reinterpret_cast<const uint4*>(a.ent_vox + gi * GE);
-    const float4* pl = reinterpret_cast<const float4*>(a.ent_len + gi * GE);
-#pragma unroll
-    for (int k = 0; k < GE / 8; ++k) {
-      const uint4 w = pv[k];
-      vw[4 * k] = w.x; vw[4 * k + 1] = w.y; vw[4 * k + 2] = w.z; vw[4 * k + 3] = w.w;
-    }
+    const uint4* pw = reinterpret_cast<const uint4*>(a.ent + gi * GE);
 #pragma unroll
     for (int k = 0; k < GE / 4; ++k) {
-      const float4 w = pl[k];
-      l[4 * k] = w.x; l[4 * k + 1] = w.y; l[4 * k + 2] = w.z; l[4 * k + 3] = w.w;
+      const uint4 v = pw[k];
+      w[4 * k] = v.x; w[4 * k + 1] = v.y; w[4 * k + 2] = v.z; w[4 * k + 3] = v.w;
     }
     const uint32_t d = a.gdesc[gi];
     mask = d & 0xFFFFu;
     s0 = d >> 16;
     carry = a.gcarry[gi];
-  }
-  __device__ __forceinline__ uint32_t vox(int i) const {
-    return (i & 1) ? (vw[i >> 1] >> 16) : (vw[i >> 1] & 0xFFFFu);
+    v0 = a.gv0[gi];
   }
   // chunk segment of entry i: s0 plus the segment starts in 1..i
   __device__ __forceinline__ uint32_t seg(int i) const {
     return s0 + (uint32_t)__popc(mask & ((2u << i) - 2u));
+  }
+  // voxel (inside the brick) of every entry: entry 0 from the descriptor,
+  // a segment start from the chunk's first-voxel table, else the previous
+  // voxel plus the stored step
+  template <class First>
+  __device__ __forceinline__ void voxels(const EntCode& ec, First first, uint32_t* v) const {
+    v[0] = v0;
+#pragma unroll
+    for (int i = 1; i < GE; ++i) {
+      const uint32_t f = first(seg(i));
+      v[i] = ((mask >> i) & 1u) ? f : v[i - 1] + (uint32_t)ec.step(w[i]);
+    }
   }
 };
 
@@ -445,6 +500,7 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
   extern __shared__ __align__(16) double lds_dyn[];
   float* lds_lam = reinterpret_cast<float*>(lds_dyn);
   __shared__ double sacc[2 * 3 * BLOCK];
+  __shared__ uint32_t sv0[2 * BLOCK];
   __shared__ uint32_t sh_[4];
   uint32_t& sh = sh_[0];
   const uint32_t t = threadIdx.x;
@@ -461,11 +517,22 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
     for (uint32_t c = tk.c0; c < tk.c1; ++c) {
       const Chunk ch = a.chunks[c];
       double* sa = sacc + buf * 3 * BLOCK;
+      uint32_t* sv = sv0 + buf * BLOCK;
       uint32_t tfx = 0;
-      if (t < ch.cnt) tfx = a.seg_tfx[ch.q0 + t];
-      if (t * GE < ch.ne) {
-        Group gr;
-        gr.load(a, (ch.base / GE) + t);
+      if (t < ch.cnt) {
+        tfx = a.seg_tfx[ch.q0 + t];
+        sv[t] = a.seg_v0[ch.q0 + t];
+      }
+      const bool act = t * GE < ch.ne;
+      Group gr;
+      if (act) gr.load(a, (ch.base / GE) + t);
+      __syncthreads();
+      if (act) {
+        uint32_t vx[GE];
+        gr.voxels(a.ec, [&](uint32_t q) { return sv[q]; }, vx);
+        float lam_e[GE];
+#pragma unroll
+        for (int i = 0; i < GE; ++i) lam_e[i] = lds_lam[vx[i]];
         uint32_t s = gr.seg(GE - 1);
         float T = gr.carry;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;                               // ACC 1
@@ -485,8 +552,8 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
         };
 #pragma unroll
         for (int i = GE - 1; i >= 0; --i) {
-          const float L = gr.l[i];
-          const float lj = lds_lam[gr.vox(i)];
+          const float L = a.ec.len(gr.w[i]);
+          const float lj = lam_e[i];
           const float LT = L * T;
           const float w01 = fmaf(0.5f * L, L, LT);
           const float w11 = fmaf(L * L, L * (1.0f / 3.0f), (L + T) * LT);
@@ -694,23 +761,26 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
         // A^T G' A with A = [[1, 0], [s, 1]], s = tau_q - t_c (exact)
         const double shf = (double)((int64_t)tfx - (int64_t)__float_as_uint(g.w)) * a.inv_scale;
         const double gy = fma(shf, (double)g.z, (double)g.y);
-        sgb[t] = make_float4((float)fma(shf, gy + (double)g.y, (double)g.x), (float)gy, g.z, 0.0f);
+        sgb[t] = make_float4((float)fma(shf, gy + (double)g.y, (double)g.x), (float)gy, g.z,
+                             __uint_as_float((uint32_t)a.seg_v0[q]));
       }
       const bool act = t * GE < ch.ne;
       Group gr;
       if (act) gr.load(a, (ch.base / GE) + t);
       __syncthreads();
       if (act) {
+        uint32_t vx[GE];
+        gr.voxels(a.ec, [&](uint32_t q) { return __float_as_uint(sgb[q].w); }, vx);
         float T = gr.carry;
 #pragma unroll
         for (int i = GE - 1; i >= 0; --i) {
           const float4 g = sgb[gr.seg(i)];
-          const float L = gr.l[i];
+          const float L = a.ec.len(gr.w[i]);
           const float LT = L * T;
           const float w01 = fmaf(0.5f * L, L, LT);
           const float w11 = fmaf(L * L, L * (1.0f / 3.0f), (L + T) * LT);
           const float cv = fmaf(L, g.x, fmaf(2.0f * w01, g.y, w11 * g.z));
-          const uint32_t v = gr.vox(i);
+          const uint32_t v = vx[i];
           if (ACC == 1) {
             // round(cv * fx) as int64: adding 1.5 * 2^52 leaves the integer
             // in the low mantissa bits (|cv * fx| < 2^51)
@@ -820,6 +890,14 @@ static BrickGeo brick_geo(const muto_em_plan* pl) {
   return G;
 }
 
+static EntCode entcode(const muto_em_plan* pl) {
+  EntCode ec;
+  ec.lstep = pl->f32.lstep;
+  ec.bx = (int)pl->f32.bs.bx;
+  ec.bxy = (int)(pl->f32.bs.bx * pl->f32.bs.by);
+  return ec;
+}
+
 static IterArgs iter_args(muto_em_plan* pl) {
   Fp32cData& f = pl->f32;
   IterArgs a;
@@ -834,8 +912,10 @@ static IterArgs iter_args(muto_em_plan* pl) {
   a.gcarry = f.gcarry.p;
   a.inv_scale = 1.0 / f.tail_scale;
   a.seg_trk = f.seg_trk.p;
-  a.ent_vox = f.ent_vox.p;
-  a.ent_len = f.ent_len.p;
+  a.seg_v0 = f.seg_v0.p;
+  a.ent = f.ent.p;
+  a.gv0 = f.gv0.p;
+  a.ec = entcode(pl);
   return a;
 }
 
@@ -1053,21 +1133,39 @@ int fp32c_build(muto_em_plan* pl) {
     }
     // entries with their padding (zero-length entries of voxel 0)
     const uint64_t n_grp = n_ent_pad / GE;
-    MUTO_CHECK_HIP(ctx, f.ent_vox.alloc(n_ent_pad + GE));
-    MUTO_CHECK_HIP(ctx, f.ent_len.alloc(n_ent_pad + GE));
+    MUTO_CHECK_HIP(ctx, f.ent.alloc(n_ent_pad + GE));
+    MUTO_CHECK_HIP(ctx, f.seg_v0.alloc(n_seg + 1));
     MUTO_CHECK_HIP(ctx, f.gdesc.alloc(n_grp + 1));
     MUTO_CHECK_HIP(ctx, f.gcarry.alloc(n_grp + 1));
-    MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.ent_vox.p, 0, (n_ent_pad + GE) * sizeof(uint16_t), s));
-    MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.ent_len.p, 0, (n_ent_pad + GE) * sizeof(float), s));
-    if (n_seg) {
-      k_ent_scatter<<<grid_for(n_seg, 256), 256, 0, s>>>(G, n_seg, sval.p, st_start.p, st_len.p,
-                                                         seg_off_p.p, pl->vox.p, pl->len.p,
-                                                         f.ent_vox.p, f.ent_len.p);
-      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    MUTO_CHECK_HIP(ctx, f.gv0.alloc(n_grp + 1));
+    MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.ent.p, 0, (n_ent_pad + GE) * sizeof(uint32_t), s));
+    {
+      // length codes: unsigned fixed point up to twice the voxel diagonal
+      // (an entry of a scattered track may hold both legs in one voxel)
+      const double dv = sqrt(pl->g.d[0] * pl->g.d[0] + pl->g.d[1] * pl->g.d[1] + pl->g.d[2] * pl->g.d[2]);
+      const double lmax_pack = 2.0 * dv;
+      f.lstep = (float)(lmax_pack / (double)ENT_LMASK);
+      const double qscale = 1.0 / (double)f.lstep;
+      DBuf<unsigned long long> ncl;
+      MUTO_CHECK_HIP(ctx, ncl.alloc(1));
+      MUTO_CHECK_HIP(ctx, hipMemsetAsync(ncl.p, 0, sizeof(unsigned long long), s));
+      if (n_seg) {
+        k_ent_scatter<<<grid_for(n_seg, 256), 256, 0, s>>>(G, n_seg, sval.p, st_start.p, st_len.p,
+                                                           seg_off_p.p, pl->vox.p, pl->len.p, qscale,
+                                                           f.ent.p, f.seg_v0.p, ncl.p);
+        MUTO_CHECK_HIP(ctx, hipGetLastError());
+      }
+      unsigned long long nc = 0;
+      MUTO_CHECK_HIP(ctx, copy_sync(ctx, &nc, ncl.p, sizeof nc, hipMemcpyDeviceToHost));
+      if (nc) {
+        ctx->set_error("FP32C: " + std::to_string(nc) +
+                       " path lengths outside 0..2 voxel diagonals (the packed length range)");
+        return MUTO_ERR_LIMIT;
+      }
     }
     if (f.n_chunks) {
-      k_group_desc<<<f.n_chunks, BLOCK, 0, s>>>(f.chunks.p, seg_off_p.p, f.ent_len.p, f.gdesc.p,
-                                                f.gcarry.p);
+      k_group_desc<<<f.n_chunks, BLOCK, 0, s>>>(f.chunks.p, seg_off_p.p, f.ent.p, f.seg_v0.p,
+                                                entcode(pl), f.gdesc.p, f.gcarry.p, f.gv0.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
     // per-rank segment lists and per-track data

@@ -38,8 +38,10 @@ struct Fp32cData {
   DBuf<uint32_t> seg_trk;   // [n_seg] track rank
   DBuf<uint32_t> seg_tfx;   // [n_seg] tail after the segment, fixed point
   double tail_scale = 1.0;  // fixed-point scale of seg_tfx (a power of two)
-  DBuf<uint16_t> ent_vox;   // [n_ent_pad] voxel index inside the brick
-  DBuf<float> ent_len;      // [n_ent_pad] path length
+  DBuf<uint32_t> ent;       // [n_ent_pad] packed entry: length code | voxel step << 26
+  DBuf<uint16_t> seg_v0;    // [n_seg] first voxel of the segment inside its brick
+  DBuf<uint16_t> gv0;       // [groups] voxel of the group's first entry
+  float lstep = 0.0f;       // mm per entry length code
   DBuf<uint32_t> gdesc;     // [groups] segment-start mask | first segment << 16
   DBuf<float> gcarry;       // [groups] length of the last segment after the group
   DBuf<uint32_t> tseg_ptr;  // [n_trk+1] per rank: segment list offsets
@@ -71,8 +73,9 @@ struct Fp32cData {
     n_tasks = n_chunks = 0;
     seg_trk.release();
     seg_tfx.release();
-    ent_vox.release();
-    ent_len.release();
+    ent.release();
+    seg_v0.release();
+    gv0.release();
     gdesc.release();
     gcarry.release();
     tseg_ptr.release();
