@@ -32,8 +32,9 @@
 //             flushed once per task;
 //   k_upd32   per voxel: lambda += lambda^2/(2 M_j) sum_i c_ij, which is
 //             half the mean of S_ij = 2 lambda_j + c_ij lambda_j^2.
-// Accumulation (acc_mode): 1 (default) FP64 -- registers in the forward
-// walk, one FP64 LDS atomic per entry in the backprojection; 0 FP32 with
+// Accumulation (acc_mode): 1 (default) FP32 sums per segment piece (at most
+// 16 entries) in the forward walk, combined per segment in FP64, and one
+// int64 fixed-point LDS atomic per entry in the backprojection; 0 FP32 with
 // two-sum compensation -- error-free transformation in the forward walk,
 // atomic two-sum (returned old value + error word) in the backprojection.
 // The persistent kernels take tasks from a queue sorted by size, largest
@@ -72,8 +73,12 @@ constexpr uint32_t MAX_TASK_SEG = 1u << 16;
 constexpr uint32_t ECAP = BLOCK * GE;    // entries per chunk, padding included
 // static LDS of both passes is a multiple of 16 bytes, so the dynamic region
 // after it is 16-byte aligned
-static_assert((2 * 3 * BLOCK * 8 + 2 * BLOCK * 4 + 16) % 16 == 0, "forward static LDS");
-static_assert((2 * BLOCK * 16 + 16) % 16 == 0, "backward static LDS");
+// (forward: piece tables, first voxels, step table, task word; backward:
+// two G' tables, step table, task word)
+constexpr size_t FWD_STATIC_LDS = 2 * BLOCK * 16 + BLOCK * 4 + 64 + 16;
+constexpr size_t BWD_STATIC_LDS = BLOCK * 16 + 64 + 16;
+static_assert(FWD_STATIC_LDS % 16 == 0, "forward static LDS");
+static_assert(BWD_STATIC_LDS % 16 == 0, "backward static LDS");
 
 struct BrickGeo {
   uint32_t n[3];   // grid
@@ -81,21 +86,38 @@ struct BrickGeo {
   uint32_t nb[3];  // bricks per axis
 };
 
-// Packed track-voxel entry of the FP32C stream (4 bytes): the path length
-// as unsigned fixed point in the low ENT_LBITS bits (step lstep: twice the
-// voxel diagonal over the largest code, so a merged two-leg entry fits),
-// and above it the step from the segment's previous voxel as three 2-bit
-// two's-complement fields (dx, dy, dz in -1..1).  A segment's first voxel
-// is kept per segment.
-constexpr int ENT_LBITS = 26;
-constexpr uint32_t ENT_LMASK = (1u << ENT_LBITS) - 1u;
+// Packed track-voxel entry of the FP32C stream (4 bytes).
+//   bits 28..0: the path length L in units of U (a power of two above every
+//               length: twice the voxel diagonal, so a merged two-leg entry
+//               fits) as the FP32 bit pattern of L/U minus that of 2^-63.
+//               L/U lies in [2^-63, 1), so the sign and the top exponent
+//               bit are constant and the 23 mantissa bits are kept: the
+//               length is L rounded to FP32, as an FP32 path list would
+//               hold it.  One AND-OR decodes it.  Shorter lengths, and the
+//               zero words that pad a chunk, read as 2^-63 U.
+//   bits 31..29: the step from the segment's previous voxel: 0 none, 1/2
+//               +-x, 3/4 +-y, 5/6 +-z.  Other steps start a new segment.
+// A segment's first voxel is kept per segment.
+constexpr uint32_t ENT_PMASK = 0x1FFFFFFFu;
+constexpr uint32_t ENT_BASE = 0x20000000u;  // FP32 bits of 2^-63
+__host__ __device__ __forceinline__ float ent_len(uint32_t w) {  // in units of U
+  return __uint_as_float((w & ENT_PMASK) | ENT_BASE);
+}
+__host__ __device__ __forceinline__ uint32_t ent_code(uint32_t w) { return w >> 29; }
 struct EntCode {
-  float lstep;  // mm per length code
+  double unit;  // U in mm
   int bx, bxy;  // brick-local index strides
-  __host__ __device__ float len(uint32_t w) const { return (float)(w & ENT_LMASK) * lstep; }
-  __host__ __device__ int step(uint32_t w) const {
-    const int dx = (int)(w << 4) >> 30, dy = (int)(w << 2) >> 30, dz = (int)w >> 30;
-    return dx + bx * dy + bxy * dz;
+  // brick-local index step of a step code
+  __host__ __device__ int step(uint32_t code) const {
+    switch (code) {
+      case 1: return 1;
+      case 2: return -1;
+      case 3: return bx;
+      case 4: return -bx;
+      case 5: return bxy;
+      case 6: return -bxy;
+      default: return 0;
+    }
   }
 };
 
@@ -114,12 +136,15 @@ __device__ __forceinline__ uint32_t brick_of(const BrickGeo& G, uint32_t j,
 // Whether voxel j follows voxel pj of the same track inside one segment:
 // same brick and a step of at most one voxel along each axis (the step is
 // what the packed entry stores).
+// a track's entry continues the segment of the entry before when both lie
+// in one brick and the step between them moves along at most one axis by
+// one voxel (the steps a packed entry can hold)
 __device__ __forceinline__ bool seg_continues(const BrickGeo& G, uint32_t pj, uint32_t j) {
   if (pj == 0xFFFFFFFFu) return false;
   if (brick_of(G, pj, nullptr) != brick_of(G, j, nullptr)) return false;
   const int px = (int)(pj % G.n[0]), py = (int)((pj / G.n[0]) % G.n[1]), pz = (int)(pj / G.n[0] / G.n[1]);
   const int x = (int)(j % G.n[0]), y = (int)((j / G.n[0]) % G.n[1]), z = (int)(j / G.n[0] / G.n[1]);
-  return abs(x - px) <= 1 && abs(y - py) <= 1 && abs(z - pz) <= 1;
+  return abs(x - px) + abs(y - py) + abs(z - pz) <= 1;
 }
 
 __device__ __forceinline__ uint32_t spread12(uint32_t v) {
@@ -285,11 +310,11 @@ k_group_desc(const Chunk* __restrict__ chunks, const uint64_t* __restrict__ seg_
     if (so[k] >= e0) mask |= 1u << (so[k] - e0);
     s = k;
   }
-  double cr = 0.0;
-  for (uint32_t e = e0 + GE; e < so[s + 1]; ++e) cr += (double)ec.len(ent[ch.base + e]);
+  double cr = 0.0;  // in units of U
+  for (uint32_t e = e0 + GE; e < so[s + 1]; ++e) cr += (double)ent_len(ent[ch.base + e]);
   // voxel of entry e0: its segment's first voxel plus the steps up to it
   uint32_t v = seg_v0[ch.q0 + lo];
-  for (uint32_t e = so[lo] + 1; e <= e0; ++e) v += ec.step(ent[ch.base + e]);
+  for (uint32_t e = so[lo] + 1; e <= e0; ++e) v += ec.step(ent_code(ent[ch.base + e]));
   const uint64_t gi = ch.base / GE + t;
   gdesc[gi] = mask | (lo << 16);
   gcarry[gi] = (float)cr;
@@ -301,7 +326,7 @@ k_ent_scatter(BrickGeo G, uint64_t n_seg, const uint32_t* __restrict__ sval,
               const uint64_t* __restrict__ st_start,
               const uint32_t* __restrict__ st_len,
               const uint64_t* __restrict__ seg_off,
-              const uint32_t* __restrict__ vox, const double* __restrict__ len, double qscale,
+              const uint32_t* __restrict__ vox, const double* __restrict__ len, double inv_unit,
               uint32_t* __restrict__ ent, uint16_t* __restrict__ seg_v0,
               unsigned long long* __restrict__ n_clamped) {
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -321,18 +346,20 @@ k_ent_scatter(BrickGeo G, uint64_t n_seg, const uint32_t* __restrict__ sval,
       uint32_t loc;
       brick_of(G, j, &loc);
       seg_v0[q] = (uint16_t)loc;
-    } else {
+    } else {  // a unit step along one axis (segments break on any other)
       const int dx = (int)(j % G.n[0]) - (int)(pj % G.n[0]);
       const int dy = (int)((j / G.n[0]) % G.n[1]) - (int)((pj / G.n[0]) % G.n[1]);
       const int dz = (int)(j / G.n[0] / G.n[1]) - (int)(pj / G.n[0] / G.n[1]);
-      code = ((uint32_t)dx & 3u) | ((uint32_t)dy & 3u) << 2 | ((uint32_t)dz & 3u) << 4;
+      code = dx > 0 ? 1u : dx < 0 ? 2u : dy > 0 ? 3u : dy < 0 ? 4u : dz > 0 ? 5u : dz < 0 ? 6u : 0u;
     }
-    double lq = rint(len[src + k] * qscale);
-    if (!(lq >= 0.0 && lq <= (double)ENT_LMASK)) {  // outside the packed range: reported
+    float lu = (float)(len[src + k] * inv_unit);  // L / U rounded to FP32
+    if (!(lu >= 0.0f && lu < 1.0f)) {  // outside the packed range: reported
       atomicAdd(n_clamped, 1ull);
-      lq = lq > 0.0 ? (double)ENT_LMASK : 0.0;
+      lu = lu > 0.0f ? 0.99999994f : 0.0f;
     }
-    ent[dst + k] = code << ENT_LBITS | (uint32_t)lq;
+    const uint32_t bits = __float_as_uint(lu);
+    const uint32_t pay = bits >= ENT_BASE ? bits - ENT_BASE : 0u;
+    ent[dst + k] = code << 29 | pay;
     pj = j;
   }
 }
@@ -441,11 +468,20 @@ __device__ __forceinline__ void load_brick(const BrickGeo& G, uint32_t brick,
   }
 }
 
+// per bit: m ? a : b (v_bfi_b32).  Written as an instruction so the
+// compiler cannot turn the select of an LDS value into a branch around the
+// LDS read, which would put the read's latency into a dependency chain.
+__device__ __forceinline__ uint32_t bit_select(uint32_t m, uint32_t a, uint32_t b) {
+  uint32_t r;
+  asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
+  return r;
+}
+
 // One thread's group of 16 consecutive entries of a chunk (chunks start on
 // 16-entry boundaries, so the loads are 16-byte vectors), with the group's
 // descriptor: bit i of `mask` is set when entry i starts a segment, `s0` is
-// the chunk segment holding entry 0, and `carry` is the length of that
-// segment after the group (the tail of entry 15 inside its segment).
+// the chunk segment holding entry 0, `v0` the voxel of entry 0 and `carry`
+// the length (units of U) of entry 15's segment after the group.
 struct Group {
   uint32_t w[GE];  // packed entries
   uint32_t mask, s0, v0;
@@ -463,23 +499,50 @@ struct Group {
     carry = a.gcarry[gi];
     v0 = a.gv0[gi];
   }
+  __device__ __forceinline__ bool start(int i) const { return (mask >> i) & 1u; }
   // chunk segment of entry i: s0 plus the segment starts in 1..i
   __device__ __forceinline__ uint32_t seg(int i) const {
     return s0 + (uint32_t)__popc(mask & ((2u << i) - 2u));
   }
-  // voxel (inside the brick) of every entry: entry 0 from the descriptor,
-  // a segment start from the chunk's first-voxel table, else the previous
-  // voxel plus the stored step
-  template <class First>
-  __device__ __forceinline__ void voxels(const EntCode& ec, First first, uint32_t* v) const {
-    v[0] = v0;
+  // voxel (inside the brick) of every entry, handed to sink(i, v): entry 0
+  // from the descriptor, a segment start from the chunk's first-voxel
+  // table, else the previous voxel plus the step of the entry's code (LDS
+  // table of 8 steps).  Both reads are made for every entry and the choice
+  // is a bitwise select, so the reads of a half group issue together and
+  // the chain is three VALU ops per entry.  Halves are kept apart
+  // (scheduling barrier) so that fewer reads are in flight per thread.
+  template <class First, class Sink>
+  __device__ __forceinline__ void voxels(const int* stab, First first, Sink sink) const {
+    uint32_t v = v0;
+    sink(0, v);
 #pragma unroll
-    for (int i = 1; i < GE; ++i) {
-      const uint32_t f = first(seg(i));
-      v[i] = ((mask >> i) & 1u) ? f : v[i - 1] + (uint32_t)ec.step(w[i]);
+    for (int h = 0; h < GE; h += GE / 2) {
+      uint32_t f[GE / 2];
+      int d[GE / 2];
+#pragma unroll
+      for (int k = 0; k < GE / 2; ++k) {
+        const int i = h + k;
+        if (i == 0) continue;
+        f[k] = first(seg(i));
+        d[k] = stab[ent_code(w[i])];
+      }
+#pragma unroll
+      for (int k = 0; k < GE / 2; ++k) {
+        const int i = h + k;
+        if (i == 0) continue;
+        const uint32_t m = (uint32_t)((int)(mask << (31 - i)) >> 31);  // all ones at a start
+        v = bit_select(m, f[k], v + (uint32_t)d[k]);
+        sink(i, v);
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 };
+
+// the step table of the brick shape (code 7 unused)
+__device__ __forceinline__ void fill_step_table(const IterArgs& a, int* stab) {
+  if (threadIdx.x < 8) stab[threadIdx.x] = a.ec.step(threadIdx.x);
+}
 
 // Forward: per segment, sum_j lambda_j W_ij (three components) with
 // lambda staged in LDS.  Entry-parallel: thread t takes the chunk's 16-entry
@@ -487,11 +550,22 @@ struct Group {
 // grows by each length from the group's carry and restarts at segment
 // starts.  Tails are measured from the segment's own end (at most a brick
 // diagonal), so the FP32 terms carry no large offsets; the track kernel
-// moves the partials to the track's basis in FP64.  A thread adds each
-// segment piece's sums into LDS (FP64 atomics; a segment is split over at
-// most a few threads) and one thread per segment writes the partial, with
-// the segment's end tail (fixed-point bits) as its fourth word.  The LDS
-// sums alternate between two buffers, so one barrier per chunk suffices.
+// moves the partials to the track's basis in FP64.  Lengths and tails are
+// in units of U (a power of two); the partial is scaled back exactly.
+// A group splits into pieces at segment starts.  The piece that starts at
+// entry i >= 1 (or at entry 0 when that starts a segment) is a segment's
+// first piece: the thread stores its sums and start entry in the segment's
+// LDS slot.  The group's head piece (entries 0.. up to its first start in
+// 1..15) goes to the group's slot.  A segment is then its first piece plus
+// the head pieces of the later groups it covers, which its own thread adds
+// and writes as the partial, with the segment's end tail (fixed-point bits)
+// as its fourth word.  Plain LDS stores, no atomics: the pass is
+// deterministic.  Two barriers per chunk.
+// ACC = 1: FP64 piece sums of the FP32 entry terms (the products with the
+//          tail weights in FP64);
+// ACC = 0: FP32 two-sum (compensated) piece sums.
+// A piece is stored rounded to FP32 (one rounding per piece; the partial is
+// rounded once more), which keeps the LDS tables small.
 template <int ACC>
 __global__ void __launch_bounds__(BLOCK)
 k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
@@ -499,13 +573,15 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
   // as 16-byte aligned doubles, so the brick array starts 16-byte aligned
   extern __shared__ __align__(16) double lds_dyn[];
   float* lds_lam = reinterpret_cast<float*>(lds_dyn);
-  __shared__ double sacc[2 * 3 * BLOCK];
-  __shared__ uint32_t sv0[2 * BLOCK];
+  __shared__ float4 spc[BLOCK];  // per segment: first piece (x, y, z), start entry bits (w)
+  __shared__ float4 hpc[BLOCK];  // per group: head piece
+  __shared__ uint32_t sv0[BLOCK];
+  __shared__ int stab[16];
   __shared__ uint32_t sh_[4];
   uint32_t& sh = sh_[0];
   const uint32_t t = threadIdx.x;
-  for (uint32_t i = t; i < 2 * 3 * BLOCK; i += BLOCK) sacc[i] = 0.0;
-  uint32_t buf = 0;
+  fill_step_table(a, stab);
+  const double U = a.ec.unit, U2 = U * U, U3 = U2 * U;
   for (;;) {
     if (t == 0) sh = atomicAdd(&a.work[0], 1u);
     __syncthreads();
@@ -516,74 +592,78 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
     __syncthreads();
     for (uint32_t c = tk.c0; c < tk.c1; ++c) {
       const Chunk ch = a.chunks[c];
-      double* sa = sacc + buf * 3 * BLOCK;
-      uint32_t* sv = sv0 + buf * BLOCK;
       uint32_t tfx = 0;
       if (t < ch.cnt) {
         tfx = a.seg_tfx[ch.q0 + t];
-        sv[t] = a.seg_v0[ch.q0 + t];
+        sv0[t] = a.seg_v0[ch.q0 + t];
       }
       const bool act = t * GE < ch.ne;
       Group gr;
       if (act) gr.load(a, (ch.base / GE) + t);
       __syncthreads();
       if (act) {
-        uint32_t vx[GE];
-        gr.voxels(a.ec, [&](uint32_t q) { return sv[q]; }, vx);
         float lam_e[GE];
-#pragma unroll
-        for (int i = 0; i < GE; ++i) lam_e[i] = lds_lam[vx[i]];
-        uint32_t s = gr.seg(GE - 1);
+        gr.voxels(stab, [&](uint32_t q) { return sv0[q]; },
+                  [&](int i, uint32_t v) { lam_e[i] = lds_lam[v]; });
+        // copies the compiler cannot see through: the flush branch
+        // recomputes segment indices and start entries from them instead of
+        // keeping 32 values live from the voxel chain through the walk
+        uint32_t e0 = t * GE, mk = gr.mask, sk = gr.s0;
+        asm volatile("" : "+v"(e0), "+v"(mk), "+v"(sk));
         float T = gr.carry;
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0;                               // ACC 1
-        float h0 = 0.f, l0 = 0.f, h1 = 0.f, l1 = 0.f, h2 = 0.f, l2 = 0.f;  // ACC 0
-        auto flush = [&]() {
-          if (ACC == 1) {
-            atomicAdd(&sa[3 * s], a0);
-            atomicAdd(&sa[3 * s + 1], a1);
-            atomicAdd(&sa[3 * s + 2], a2);
-            a0 = a1 = a2 = 0.0;
-          } else {
-            atomicAdd(&sa[3 * s], (double)h0 + (double)l0);
-            atomicAdd(&sa[3 * s + 1], (double)h1 + (double)l1);
-            atomicAdd(&sa[3 * s + 2], (double)h2 + (double)l2);
-            h0 = l0 = h1 = l1 = h2 = l2 = 0.f;
-          }
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0;  // ACC 1
+        float c0 = 0.f, c1 = 0.f, c2 = 0.f;   // ACC 0 high words
+        float l0 = 0.f, l1 = 0.f, l2 = 0.f;   // ACC 0 low words
+        auto piece = [&](uint32_t pos) {
+          return ACC == 1 ? make_float4((float)d0, (float)d1, (float)d2, __uint_as_float(pos))
+                          : make_float4((float)((double)c0 + (double)l0), (float)((double)c1 + (double)l1),
+                                        (float)((double)c2 + (double)l2), __uint_as_float(pos));
         };
 #pragma unroll
         for (int i = GE - 1; i >= 0; --i) {
-          const float L = a.ec.len(gr.w[i]);
-          const float lj = lam_e[i];
-          const float LT = L * T;
-          const float w01 = fmaf(0.5f * L, L, LT);
-          const float w11 = fmaf(L * L, L * (1.0f / 3.0f), (L + T) * LT);
+          // W = [[L, L(L/2 + T)], [., L(L^2/3 + L T + T^2)]]
+          const float L = ent_len(gr.w[i]);
+          const float av = lam_e[i] * L;
+          const float h = fmaf(0.5f, L, T);
+          const float k = fmaf(L, fmaf(L, 1.0f / 3.0f, T), T * T);
           if (ACC == 1) {
-            a0 += (double)(lj * L);
-            a1 += (double)(lj * w01);
-            a2 += (double)(lj * w11);
+            const double ad = (double)av;
+            d0 += ad;
+            d1 = fma(ad, (double)h, d1);
+            d2 = fma(ad, (double)k, d2);
           } else {
-            two_sum_acc(h0, l0, lj * L);
-            two_sum_acc(h1, l1, lj * w01);
-            two_sum_acc(h2, l2, lj * w11);
+            two_sum_acc(c0, l0, av);
+            two_sum_acc(c1, l1, av * h);
+            two_sum_acc(c2, l2, av * k);
           }
-          const bool st = (gr.mask >> i) & 1u;
-          if (i > 0 && st) {  // entry i starts segment s: its piece is complete
-            flush();
-            --s;
+          if (i > 0 && gr.start(i)) {  // entry i starts segment seg(i)
+            spc[sk + (uint32_t)__popc(mk & ((2u << i) - 2u))] = piece(e0 + i);
+            d0 = d1 = d2 = 0.0;
+            c0 = c1 = c2 = 0.f;
+            l0 = l1 = l2 = 0.f;
           }
-          T = st ? 0.0f : T + L;
+          T = gr.start(i) ? 0.0f : T + L;
         }
-        flush();  // the piece holding entry 0 (segment s0)
+        const float4 head = piece(e0);
+        hpc[t] = head;
+        if (gr.mask & 1u) spc[gr.s0] = head;
       }
       __syncthreads();
       if (t < ch.cnt) {
-        part[ch.q0 + t] = make_float4((float)sa[3 * t], (float)sa[3 * t + 1],
-                                      (float)sa[3 * t + 2], __uint_as_float(tfx));
-        sa[3 * t] = 0.0;
-        sa[3 * t + 1] = 0.0;
-        sa[3 * t + 2] = 0.0;
+        const float4 p = spc[t];
+        const uint32_t A = __float_as_uint(p.w);
+        const uint32_t B = t + 1 < ch.cnt ? __float_as_uint(spc[t + 1].w) : ch.ne;
+        double x = p.x, y = p.y, z = p.z;
+        const uint32_t g_last = min((B - 1) / GE, (ch.ne - 1) / GE);  // (bounded by the chunk)
+        for (uint32_t g = A / GE + 1; g <= g_last; ++g) {
+          const float4 hp = hpc[g];
+          x += hp.x;
+          y += hp.y;
+          z += hp.z;
+        }
+        part[ch.q0 + t] = make_float4((float)(x * U), (float)(y * U2), (float)(z * U3),
+                                      __uint_as_float(tfx));
       }
-      buf ^= 1u;
     }
   }
 }
@@ -716,8 +796,8 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
 // Backprojection: c_ij = <W_ij, G_i> accumulated per voxel in LDS, then
 // flushed once per task (stored when the task owns the brick, else added).
 // One thread per segment first moves G' from the track's basis to the
-// segment's local one (FP64) into an LDS table (two alternating tables, one
-// barrier per chunk); then entry-parallel like the forward pass.
+// segment's local one (FP64) into an LDS table; then entry-parallel like the
+// forward pass.  Two barriers per chunk (one table keeps the LDS small).
 // ACC = 1: one FP64 LDS atomic per entry;
 // ACC = 0: FP32 atomic two-sum -- the returned old value o and the stored
 //          o + c give the exact rounding error, added into a low word.
@@ -729,10 +809,13 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
   // as 16-byte aligned doubles: ds_add_f64 needs 8-byte aligned addresses
   extern __shared__ __align__(16) double lds_dyn[];
   float* lds_acc = reinterpret_cast<float*>(lds_dyn);
-  __shared__ float4 sg[2 * BLOCK];
+  __shared__ float4 sg[BLOCK];
+  __shared__ int stab[16];
   __shared__ uint32_t sh_[4];
   uint32_t& sh = sh_[0];
   const uint32_t t = threadIdx.x;
+  fill_step_table(a, stab);
+  const double U = a.ec.unit, U2 = U * U, U3 = U2 * U;
   const uint32_t vol = a.G.b[0] * a.G.b[1] * a.G.b[2];
   float* hi_s = lds_acc;
   float* lo_s = lds_acc + vol;
@@ -743,7 +826,6 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
   const double cb = __longlong_as_double((long long)cmax[0]);
   const double fx = cb > 0.0 ? ldexp(1.0, FX_BITS) / cb : 1.0;
   const double fx_inv = 1.0 / fx;
-  uint32_t buf = 0;
   for (;;) {
     if (t == 0) sh = atomicAdd(&a.work[1], 1u);
     for (uint32_t v = t; v < 2 * vol; v += blockDim.x) lds_acc[v] = 0.0f;
@@ -753,7 +835,7 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
     const Task tk = a.tasks[ti];
     for (uint32_t c = tk.c0; c < tk.c1; ++c) {
       const Chunk ch = a.chunks[c];
-      float4* sgb = sg + buf * BLOCK;
+      float4* sgb = sg;
       if (t < ch.cnt) {
         const uint64_t q = ch.q0 + t;
         const uint32_t tfx = a.seg_tfx[q];
@@ -761,8 +843,9 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
         // A^T G' A with A = [[1, 0], [s, 1]], s = tau_q - t_c (exact)
         const double shf = (double)((int64_t)tfx - (int64_t)__float_as_uint(g.w)) * a.inv_scale;
         const double gy = fma(shf, (double)g.z, (double)g.y);
-        sgb[t] = make_float4((float)fma(shf, gy + (double)g.y, (double)g.x), (float)gy, g.z,
-                             __uint_as_float((uint32_t)a.seg_v0[q]));
+        // in units of U for the entry terms: g00 U, g01 U^2, g11 U^3
+        sgb[t] = make_float4((float)(fma(shf, gy + (double)g.y, (double)g.x) * U), (float)(gy * U2),
+                             (float)((double)g.z * U3), __uint_as_float((uint32_t)a.seg_v0[q]));
       }
       const bool act = t * GE < ch.ne;
       Group gr;
@@ -770,16 +853,18 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
       __syncthreads();
       if (act) {
         uint32_t vx[GE];
-        gr.voxels(a.ec, [&](uint32_t q) { return __float_as_uint(sgb[q].w); }, vx);
+        gr.voxels(stab, [&](uint32_t q) { return __float_as_uint(sgb[q].w); },
+                  [&](int i, uint32_t v) { vx[i] = v; });
         float T = gr.carry;
+        uint32_t mk = gr.mask, sk = gr.s0;  // opaque copies, as in the forward walk
+        asm volatile("" : "+v"(mk), "+v"(sk));
+        float4 g = sgb[sk + (uint32_t)__popc(mk & 0xFFFEu)];
 #pragma unroll
         for (int i = GE - 1; i >= 0; --i) {
-          const float4 g = sgb[gr.seg(i)];
-          const float L = a.ec.len(gr.w[i]);
-          const float LT = L * T;
-          const float w01 = fmaf(0.5f * L, L, LT);
-          const float w11 = fmaf(L * L, L * (1.0f / 3.0f), (L + T) * LT);
-          const float cv = fmaf(L, g.x, fmaf(2.0f * w01, g.y, w11 * g.z));
+          // <W, G> = L (g00 + (L + 2T) g01 + (L (L/3 + T) + T^2) g11)
+          const float L = ent_len(gr.w[i]);
+          const float k = fmaf(L, fmaf(L, 1.0f / 3.0f, T), T * T);
+          const float cv = L * fmaf(k, g.z, fmaf(fmaf(2.0f, T, L), g.y, g.x));
           const uint32_t v = vx[i];
           if (ACC == 1) {
             // round(cv * fx) as int64: adding 1.5 * 2^52 leaves the integer
@@ -794,10 +879,12 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
             const float e = (old - (sum - bp)) + (cv - bp);
             if (e != 0.0f) atomicAdd(&lo_s[v], e);
           }
-          T = ((gr.mask >> i) & 1u) ? 0.0f : T + L;
+          // entry i - 1 lies in the segment before when entry i starts one
+          if (i > 0 && gr.start(i)) g = sgb[sk + (uint32_t)__popc(mk & ((2u << i) - 2u)) - 1u];
+          T = gr.start(i) ? 0.0f : T + L;
         }
       }
-      buf ^= 1u;
+      __syncthreads();  // the table is rewritten for the next chunk
     }
     __syncthreads();
     uint32_t o[3];
@@ -892,7 +979,7 @@ static BrickGeo brick_geo(const muto_em_plan* pl) {
 
 static EntCode entcode(const muto_em_plan* pl) {
   EntCode ec;
-  ec.lstep = pl->f32.lstep;
+  ec.unit = pl->f32.unit;
   ec.bx = (int)pl->f32.bs.bx;
   ec.bxy = (int)(pl->f32.bs.bx * pl->f32.bs.by);
   return ec;
@@ -944,8 +1031,8 @@ static bool fp32c_fits(muto_ctx* ctx, const BrickShape& b) {
   int max_lds = 0;
   if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess)
     return false;
-  const size_t fwd = 2 * 3 * BLOCK * 8 + 16 + (size_t)b.vol() * sizeof(float);
-  const size_t bwd = 2 * BLOCK * 16 + 16 + 2 * (size_t)b.vol() * sizeof(float);
+  const size_t fwd = FWD_STATIC_LDS + (size_t)b.vol() * sizeof(float);
+  const size_t bwd = BWD_STATIC_LDS + 2 * (size_t)b.vol() * sizeof(float);
   return std::max(fwd, bwd) <= (size_t)max_lds;
 }
 
@@ -1140,18 +1227,18 @@ int fp32c_build(muto_em_plan* pl) {
     MUTO_CHECK_HIP(ctx, f.gv0.alloc(n_grp + 1));
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.ent.p, 0, (n_ent_pad + GE) * sizeof(uint32_t), s));
     {
-      // length codes: unsigned fixed point up to twice the voxel diagonal
-      // (an entry of a scattered track may hold both legs in one voxel)
+      // lengths as FP32 in units of U, the power of two above twice the
+      // voxel diagonal (an entry of a scattered track may hold both legs in
+      // one voxel)
       const double dv = sqrt(pl->g.d[0] * pl->g.d[0] + pl->g.d[1] * pl->g.d[1] + pl->g.d[2] * pl->g.d[2]);
-      const double lmax_pack = 2.0 * dv;
-      f.lstep = (float)(lmax_pack / (double)ENT_LMASK);
-      const double qscale = 1.0 / (double)f.lstep;
+      f.unit = ldexp(1.0, (int)floor(log2(2.0 * dv)) + 1);
+      const double inv_unit = 1.0 / f.unit;
       DBuf<unsigned long long> ncl;
       MUTO_CHECK_HIP(ctx, ncl.alloc(1));
       MUTO_CHECK_HIP(ctx, hipMemsetAsync(ncl.p, 0, sizeof(unsigned long long), s));
       if (n_seg) {
         k_ent_scatter<<<grid_for(n_seg, 256), 256, 0, s>>>(G, n_seg, sval.p, st_start.p, st_len.p,
-                                                           seg_off_p.p, pl->vox.p, pl->len.p, qscale,
+                                                           seg_off_p.p, pl->vox.p, pl->len.p, inv_unit,
                                                            f.ent.p, f.seg_v0.p, ncl.p);
         MUTO_CHECK_HIP(ctx, hipGetLastError());
       }

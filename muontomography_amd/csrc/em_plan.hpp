@@ -41,7 +41,7 @@ struct Fp32cData {
   DBuf<uint32_t> ent;       // [n_ent_pad] packed entry: length code | voxel step << 26
   DBuf<uint16_t> seg_v0;    // [n_seg] first voxel of the segment inside its brick
   DBuf<uint16_t> gv0;       // [groups] voxel of the group's first entry
-  float lstep = 0.0f;       // mm per entry length code
+  double unit = 1.0;        // entry length unit U in mm (a power of two)
   DBuf<uint32_t> gdesc;     // [groups] segment-start mask | first segment << 16
   DBuf<float> gcarry;       // [groups] length of the last segment after the group
   DBuf<uint32_t> tseg_ptr;  // [n_trk+1] per rank: segment list offsets
