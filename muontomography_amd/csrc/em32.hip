@@ -477,6 +477,20 @@ __device__ __forceinline__ uint32_t bit_select(uint32_t m, uint32_t a, uint32_t 
   return r;
 }
 
+// L/U of a packed entry in one v_and_or_b32 (the mask from an SGPR, the
+// constant exponent bits from a VGPR: a gfx9 VOP3 instruction reads at
+// most one scalar operand)
+__device__ __forceinline__ float ent_len_v(uint32_t w, uint32_t base_v) {
+  uint32_t r;
+  asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(w), "s"(ENT_PMASK), "v"(base_v));
+  return __uint_as_float(r);
+}
+
+// all ones when bit i of m is set, else zero (one v_bfe_i32)
+__device__ __forceinline__ uint32_t bit_mask(uint32_t m, int i) {
+  return (uint32_t)((int)(m << (31 - i)) >> 31);
+}
+
 // One thread's group of 16 consecutive entries of a chunk (chunks start on
 // 16-entry boundaries, so the loads are 16-byte vectors), with the group's
 // descriptor: bit i of `mask` is set when entry i starts a segment, `s0` is
@@ -530,8 +544,7 @@ struct Group {
       for (int k = 0; k < GE / 2; ++k) {
         const int i = h + k;
         if (i == 0) continue;
-        const uint32_t m = (uint32_t)((int)(mask << (31 - i)) >> 31);  // all ones at a start
-        v = bit_select(m, f[k], v + (uint32_t)d[k]);
+        v = bit_select(bit_mask(mask, i), f[k], v + (uint32_t)d[k]);  // a start takes f
         sink(i, v);
       }
       __builtin_amdgcn_sched_barrier(0);
@@ -558,9 +571,11 @@ __device__ __forceinline__ void fill_step_table(const IterArgs& a, int* stab) {
 // LDS slot.  The group's head piece (entries 0.. up to its first start in
 // 1..15) goes to the group's slot.  A segment is then its first piece plus
 // the head pieces of the later groups it covers, which its own thread adds
-// and writes as the partial, with the segment's end tail (fixed-point bits)
-// as its fourth word.  Plain LDS stores, no atomics: the pass is
-// deterministic.  Two barriers per chunk.
+// in FP64 and writes as the partial, with the segment's end tail
+// (fixed-point bits) as its fourth word.  Plain LDS stores, no atomics: the
+// pass is deterministic.  Two barriers per chunk.  (Piece sums restart at
+// each start: running sums over the group would make a short piece the
+// difference of two larger FP32 sums.)
 // ACC = 1: FP64 piece sums of the FP32 entry terms (the products with the
 //          tail weights in FP64);
 // ACC = 0: FP32 two-sum (compensated) piece sums.
@@ -608,8 +623,8 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
         // copies the compiler cannot see through: the flush branch
         // recomputes segment indices and start entries from them instead of
         // keeping 32 values live from the voxel chain through the walk
-        uint32_t e0 = t * GE, mk = gr.mask, sk = gr.s0;
-        asm volatile("" : "+v"(e0), "+v"(mk), "+v"(sk));
+        uint32_t e0 = t * GE, mk = gr.mask, sk = gr.s0, base_v = ENT_BASE;
+        asm volatile("" : "+v"(e0), "+v"(mk), "+v"(sk), "+v"(base_v));
         float T = gr.carry;
         double d0 = 0.0, d1 = 0.0, d2 = 0.0;  // ACC 1
         float c0 = 0.f, c1 = 0.f, c2 = 0.f;   // ACC 0 high words
@@ -622,7 +637,8 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
 #pragma unroll
         for (int i = GE - 1; i >= 0; --i) {
           // W = [[L, L(L/2 + T)], [., L(L^2/3 + L T + T^2)]]
-          const float L = ent_len(gr.w[i]);
+          const float L = ent_len_v(gr.w[i], base_v);
+          const uint32_t st = bit_mask(mk, i);  // entry i starts a segment
           const float av = lam_e[i] * L;
           const float h = fmaf(0.5f, L, T);
           const float k = fmaf(L, fmaf(L, 1.0f / 3.0f, T), T * T);
@@ -636,13 +652,13 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
             two_sum_acc(c1, l1, av * h);
             two_sum_acc(c2, l2, av * k);
           }
-          if (i > 0 && gr.start(i)) {  // entry i starts segment seg(i)
+          if (i > 0 && st != 0u) {  // entry i starts segment seg(i)
             spc[sk + (uint32_t)__popc(mk & ((2u << i) - 2u))] = piece(e0 + i);
             d0 = d1 = d2 = 0.0;
             c0 = c1 = c2 = 0.f;
             l0 = l1 = l2 = 0.f;
           }
-          T = gr.start(i) ? 0.0f : T + L;
+          T = __uint_as_float(bit_select(st, 0u, __float_as_uint(T + L)));
         }
         const float4 head = piece(e0);
         hpc[t] = head;
@@ -856,13 +872,14 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
         gr.voxels(stab, [&](uint32_t q) { return __float_as_uint(sgb[q].w); },
                   [&](int i, uint32_t v) { vx[i] = v; });
         float T = gr.carry;
-        uint32_t mk = gr.mask, sk = gr.s0;  // opaque copies, as in the forward walk
-        asm volatile("" : "+v"(mk), "+v"(sk));
+        uint32_t mk = gr.mask, sk = gr.s0, base_v = ENT_BASE;  // opaque copies, as in the forward walk
+        asm volatile("" : "+v"(mk), "+v"(sk), "+v"(base_v));
         float4 g = sgb[sk + (uint32_t)__popc(mk & 0xFFFEu)];
 #pragma unroll
         for (int i = GE - 1; i >= 0; --i) {
           // <W, G> = L (g00 + (L + 2T) g01 + (L (L/3 + T) + T^2) g11)
-          const float L = ent_len(gr.w[i]);
+          const float L = ent_len_v(gr.w[i], base_v);
+          const uint32_t st = bit_mask(mk, i);  // entry i starts a segment
           const float k = fmaf(L, fmaf(L, 1.0f / 3.0f, T), T * T);
           const float cv = L * fmaf(k, g.z, fmaf(fmaf(2.0f, T, L), g.y, g.x));
           const uint32_t v = vx[i];
@@ -880,8 +897,8 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
             if (e != 0.0f) atomicAdd(&lo_s[v], e);
           }
           // entry i - 1 lies in the segment before when entry i starts one
-          if (i > 0 && gr.start(i)) g = sgb[sk + (uint32_t)__popc(mk & ((2u << i) - 2u)) - 1u];
-          T = gr.start(i) ? 0.0f : T + L;
+          if (i > 0 && st != 0u) g = sgb[sk + (uint32_t)__popc(mk & ((2u << i) - 2u)) - 1u];
+          T = __uint_as_float(bit_select(st, 0u, __float_as_uint(T + L)));
         }
       }
       __syncthreads();  // the table is rewritten for the next chunk
