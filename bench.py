@@ -183,8 +183,8 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=5)
     ap.add_argument("--tune-iters", type=int, default=3,
                     help="iterations timed per brick shape by the autotuner (0: 16^3, no tuning)")
-    ap.add_argument("--brick", default=None, metavar="BX,BY,BZ",
-                    help="em256: use this brick shape, no tuning (profiling runs)")
+    ap.add_argument("--brick", default=None, metavar="BX,BY,BZ[,BLOCK]",
+                    help="em256: use this brick shape (and block size), no tuning (profiling runs)")
     ap.add_argument("--n-scans", type=int, default=64, help="scans64: number of scans")
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--cpu-iters", type=int, default=3)
@@ -236,13 +236,13 @@ def main():
     plan.set_image(init)
     t0 = time.perf_counter()
     if args.brick:
-        brick = tuple(int(v) for v in args.brick.split(","))
-        plan.set_option(M.MUTO_OPT_BRICK, brick[0] | brick[1] << 8 | brick[2] << 16)
+        brick = M.brick_shape(M.brick_code(tuple(int(v) for v in args.brick.split(","))))
+        plan.set_option(M.MUTO_OPT_BRICK, M.brick_code(brick))
         tune = {}
     elif args.tune_iters > 0:
         brick, tune = plan.autotune(iters=args.tune_iters)
     else:
-        brick, tune = (16, 16, 16), {}
+        brick, tune = (16, 16, 16, 256), {}
     log(f"autotune {time.perf_counter() - t0:.1f} s: brick {brick}; " +
         ", ".join(f"{k}: {v:.2f} ms" for k, v in tune.items()))
     plan.set_image(init)

@@ -267,13 +267,15 @@ int muto_batch_destroy(muto_batch* batch);
 #define MUTO_OPT_GRAPH 1
 #define MUTO_OPT_TIMING 2
 #define MUTO_OPT_ACC_MODE 3
-#define MUTO_OPT_BRICK 4  /* FP32C brick (LDS tile) shape bx | by << 8 | bz << 16,
-                             at most 16384 voxels; default 16 x 16 x 16 */
+#define MUTO_OPT_BRICK 4  /* FP32C brick (LDS tile) shape and block size:
+                             bx | by << 8 | bz << 16 | log2(block / 256) << 24,
+                             at most 16384 voxels, block 256, 512 or 1024
+                             threads; default 16 x 16 x 16, 256 */
 int muto_em_plan_set_option(muto_em_plan* plan, int option, int value);
 
 /* FP32C autotuner: builds the brick stream for each candidate tile shape
- * (shapes[k] = bx | by << 8 | bz << 16; NULL/0: a built-in list from 8x8x128
- * to 32x32x8), times `iters` iterations of each from the plan's image
+ * (shapes[k] coded as for MUTO_OPT_BRICK; NULL/0: a built-in list of 4096-
+ * to 16384-voxel bricks), times `iters` iterations of each from the plan's image
  * (restored after every trial) and keeps the fastest.  ms[k] (may be NULL)
  * receives each shape's ms per iteration, *best (may be NULL) the chosen
  * shape.  No counterpart in the reference. */

@@ -36,9 +36,26 @@ MUTO_OPT_TIMING = 2
 MUTO_OPT_ACC_MODE = 3
 MUTO_OPT_BRICK = 4
 # the autotuner's built-in candidate list (muto_em_plan_autotune), in order
-DEFAULT_BRICKS = [bx | by << 8 | bz << 16 for bx, by, bz in
-                  ((16, 16, 16), (8, 8, 64), (8, 16, 32), (16, 8, 32), (16, 16, 32), (8, 8, 32),
-                   (32, 32, 8), (16, 16, 8), (8, 8, 128))]
+def brick_code(shape):
+    """MUTO_OPT_BRICK / tuner code of (bx, by, bz) or (bx, by, bz, block):
+    bx | by << 8 | bz << 16 | log2(block / 256) << 24 (block 256, 512 or 1024)."""
+    bx, by, bz = shape[:3]
+    block = shape[3] if len(shape) > 3 else 256
+    lb = {256: 0, 512: 1, 1024: 2}[int(block)]
+    return int(bx) | int(by) << 8 | int(bz) << 16 | lb << 24
+
+
+def brick_shape(code):
+    """(bx, by, bz, block) of a brick code."""
+    c = int(code)
+    return (c & 255, (c >> 8) & 255, (c >> 16) & 255, 256 << ((c >> 24) & 255))
+
+
+# the tuner's built-in candidates (muto_em_plan_autotune with no list)
+DEFAULT_BRICKS = [brick_code(s) for s in
+                  ((16, 16, 16), (16, 8, 32), (8, 16, 32), (8, 8, 64), (16, 16, 32, 512),
+                   (32, 16, 16, 512), (16, 32, 16, 512), (16, 16, 64, 1024), (32, 16, 32, 1024),
+                   (16, 16, 32))]
 
 
 class MutoError(RuntimeError):
@@ -250,20 +267,20 @@ class EmPlan:
         return sv
 
     def autotune(self, iters=3, shapes=None):
-        """FP32C brick (LDS tile) shape autotuner (muto_em_plan_autotune):
-        returns the chosen (bx, by, bz) and {shape: ms per iteration}.  The
-        plan's image is left as it was."""
-        arr = None if shapes is None else np.array([bx | by << 8 | bz << 16 for bx, by, bz in shapes],
-                                                   np.uint32)
+        """FP32C brick (LDS tile) shape autotuner (muto_em_plan_autotune).
+        `shapes`: (bx, by, bz) or (bx, by, bz, block) tuples, or None for the
+        built-in list.  Returns the chosen (bx, by, bz, block) and
+        {(bx, by, bz, block): ms per iteration}.  The plan's image is left as
+        it was."""
+        arr = None if shapes is None else np.array([brick_code(sh) for sh in shapes], np.uint32)
         n = 0 if arr is None else arr.size
         ms = np.full(max(n, 16), np.nan)
         best = C.c_uint32(0)
         self.ctx._check(lib().muto_em_plan_autotune(self._h, int(iters), n, ptr(arr), ptr(ms),
                                                     C.byref(best)), "muto_em_plan_autotune")
-        dec = lambda v: (int(v) & 255, (int(v) >> 8) & 255, (int(v) >> 16) & 255)
         if arr is None:
             arr = np.array(DEFAULT_BRICKS, np.uint32)
-        return dec(best.value), {dec(v): float(ms[k]) for k, v in enumerate(arr)}
+        return brick_shape(best.value), {brick_shape(v): float(ms[k]) for k, v in enumerate(arr)}
 
     def set_option(self, option, value):
         """MUTO_OPT_GRAPH, MUTO_OPT_TIMING or MUTO_OPT_ACC_MODE (include/muto.h)."""

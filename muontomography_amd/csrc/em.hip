@@ -972,13 +972,11 @@ extern "C" int muto_em_plan_set_option(muto_em_plan* pl, int option, int value) 
       pl->timing = value != 0;
       return MUTO_OK;
     case MUTO_OPT_BRICK: {
-      const uint32_t bx = value & 0xFF, by = (value >> 8) & 0xFF, bz = (value >> 16) & 0xFF;
-      if (bx == 0 || by == 0 || bz == 0 || bx * by * bz > 16384) break;
-      if (bx != pl->f32.bs.bx || by != pl->f32.bs.by || bz != pl->f32.bs.bz) {
+      const BrickShape b = BrickShape::from_code((uint32_t)value);
+      if (b.bx == 0 || b.by == 0 || b.bz == 0 || b.vol() > 16384 || b.block == 0) break;
+      if (b != pl->f32.bs) {
         pl->f32.reset();
-        pl->f32.bs.bx = bx;
-        pl->f32.bs.by = by;
-        pl->f32.bs.bz = bz;
+        pl->f32.bs = b;
       }
       return MUTO_OK;
     }
@@ -1002,10 +1000,14 @@ extern "C" int muto_em_plan_autotune(muto_em_plan* pl, int iters, int n_cand,
                                      const uint32_t* shapes, double* ms, uint32_t* best) {
   if (!pl) return MUTO_ERR_ARG;
   muto_ctx* ctx = pl->ctx;
+  // 256-thread blocks for bricks of up to 4096 voxels, 512 for 8192, 1024
+  // for 16384 (the larger LDS tile with the same waves per CU)
   static const uint32_t kDefault[] = {
-      16u | 16u << 8 | 16u << 16, 8u | 8u << 8 | 64u << 16,  8u | 16u << 8 | 32u << 16,
-      16u | 8u << 8 | 32u << 16,  16u | 16u << 8 | 32u << 16, 8u | 8u << 8 | 32u << 16,
-      32u | 32u << 8 | 8u << 16,  16u | 16u << 8 | 8u << 16,  8u | 8u << 8 | 128u << 16};
+      16u | 16u << 8 | 16u << 16,           16u | 8u << 8 | 32u << 16,
+      8u | 16u << 8 | 32u << 16,            8u | 8u << 8 | 64u << 16,
+      16u | 16u << 8 | 32u << 16 | 1u << 24, 32u | 16u << 8 | 16u << 16 | 1u << 24,
+      16u | 32u << 8 | 16u << 16 | 1u << 24, 16u | 16u << 8 | 64u << 16 | 2u << 24,
+      32u | 16u << 8 | 32u << 16 | 2u << 24, 16u | 16u << 8 | 32u << 16};
   if (!shapes || n_cand <= 0) {
     shapes = kDefault;
     n_cand = (int)(sizeof kDefault / sizeof kDefault[0]);
@@ -1021,7 +1023,7 @@ extern "C" int muto_em_plan_autotune(muto_em_plan* pl, int iters, int n_cand,
   if (int rc = fp32c_autotune(pl, iters, n_cand, shapes, t.data())) return rc;
   pl->cur_prec = MUTO_FP64;
   if (ms) std::copy(t.begin(), t.end(), ms);
-  if (best) *best = pl->f32.bs.bx | pl->f32.bs.by << 8 | pl->f32.bs.bz << 16;
+  if (best) *best = pl->f32.bs.code();
   return MUTO_OK;
 }
 

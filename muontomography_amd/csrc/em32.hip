@@ -45,6 +45,7 @@
 
 #include <algorithm>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "em_plan.hpp"
@@ -63,22 +64,25 @@ int fp32c_build(muto_em_plan* pl);
 
 namespace {
 
-constexpr uint32_t BLOCK = 256;          // threads per block (4 wavefronts)
+// threads per block of the persistent passes: 256, 512 or 1024 (the brick
+// shape's `block`); a chunk holds up to one segment per thread and GE
+// entries per thread
 constexpr int GE = 16;                   // entries per thread and chunk
 constexpr uint32_t DIR_BINS = 16;        // track-order direction bins per unit slope
 // backprojection LDS sums (ACC 1) are int64 fixed point: an entry is at
 // most 2^FX_BITS and a task holds at most MAX_TASK_SEG segments
 constexpr int FX_BITS = 46;
 constexpr uint32_t MAX_TASK_SEG = 1u << 16;
-constexpr uint32_t ECAP = BLOCK * GE;    // entries per chunk, padding included
 // static LDS of both passes is a multiple of 16 bytes, so the dynamic region
 // after it is 16-byte aligned
 // (forward: piece tables, first voxels, step table, task word; backward:
-// two G' tables, step table, task word)
-constexpr size_t FWD_STATIC_LDS = 2 * BLOCK * 16 + BLOCK * 4 + 64 + 16;
-constexpr size_t BWD_STATIC_LDS = BLOCK * 16 + 64 + 16;
-static_assert(FWD_STATIC_LDS % 16 == 0, "forward static LDS");
-static_assert(BWD_STATIC_LDS % 16 == 0, "backward static LDS");
+// G' table, step table, task word)
+constexpr size_t fwd_static_lds(uint32_t bs) { return 2 * (size_t)bs * 16 + (size_t)bs * 4 + 64 + 16; }
+constexpr size_t bwd_static_lds(uint32_t bs) { return (size_t)bs * 16 + 64 + 16; }
+static_assert(fwd_static_lds(256) % 16 == 0 && fwd_static_lds(512) % 16 == 0 &&
+              fwd_static_lds(1024) % 16 == 0, "forward static LDS");
+static_assert(bwd_static_lds(256) % 16 == 0 && bwd_static_lds(512) % 16 == 0 &&
+              bwd_static_lds(1024) % 16 == 0, "backward static LDS");
 
 struct BrickGeo {
   uint32_t n[3];   // grid
@@ -288,11 +292,12 @@ __global__ void k_chunk_offsets(uint32_t n, const Chunk* __restrict__ chunks,
 // Descriptor of every 16-entry group (one block per chunk, one thread per
 // group): segment-start mask, the chunk segment holding its first entry and
 // the length of its last segment after the group (summed in FP64).
-__global__ void __launch_bounds__(BLOCK)
+template <uint32_t BS>
+__global__ void __launch_bounds__(BS)
 k_group_desc(const Chunk* __restrict__ chunks, const uint64_t* __restrict__ seg_off_p,
              const uint32_t* __restrict__ ent, const uint16_t* __restrict__ seg_v0, EntCode ec,
              uint32_t* __restrict__ gdesc, float* __restrict__ gcarry, uint16_t* __restrict__ gv0) {
-  __shared__ uint32_t so[BLOCK + 1];
+  __shared__ uint32_t so[BS + 1];
   const Chunk ch = chunks[blockIdx.x];
   const uint32_t t = threadIdx.x;
   if (t < ch.cnt) so[t] = (uint32_t)(seg_off_p[ch.q0 + t] - ch.base);
@@ -581,16 +586,16 @@ __device__ __forceinline__ void fill_step_table(const IterArgs& a, int* stab) {
 // ACC = 0: FP32 two-sum (compensated) piece sums.
 // A piece is stored rounded to FP32 (one rounding per piece; the partial is
 // rounded once more), which keeps the LDS tables small.
-template <int ACC>
-__global__ void __launch_bounds__(BLOCK)
+template <int ACC, uint32_t BS>
+__global__ void __launch_bounds__(BS)
 k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
   // static LDS is a multiple of 16 bytes and the dynamic region is declared
   // as 16-byte aligned doubles, so the brick array starts 16-byte aligned
   extern __shared__ __align__(16) double lds_dyn[];
   float* lds_lam = reinterpret_cast<float*>(lds_dyn);
-  __shared__ float4 spc[BLOCK];  // per segment: first piece (x, y, z), start entry bits (w)
-  __shared__ float4 hpc[BLOCK];  // per group: head piece
-  __shared__ uint32_t sv0[BLOCK];
+  __shared__ float4 spc[BS];  // per segment: first piece (x, y, z), start entry bits (w)
+  __shared__ float4 hpc[BS];  // per group: head piece
+  __shared__ uint32_t sv0[BS];
   __shared__ int stab[16];
   __shared__ uint32_t sh_[4];
   uint32_t& sh = sh_[0];
@@ -817,15 +822,15 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
 // ACC = 1: one FP64 LDS atomic per entry;
 // ACC = 0: FP32 atomic two-sum -- the returned old value o and the stored
 //          o + c give the exact rounding error, added into a low word.
-template <int ACC>
-__global__ void __launch_bounds__(BLOCK)
+template <int ACC, uint32_t BS>
+__global__ void __launch_bounds__(BS)
 k_bwd32(IterArgs a, const float* __restrict__ trk_g,
         const unsigned long long* __restrict__ cmax, double* __restrict__ acc) {
   // static LDS is a multiple of 16 bytes and the dynamic region is declared
   // as 16-byte aligned doubles: ds_add_f64 needs 8-byte aligned addresses
   extern __shared__ __align__(16) double lds_dyn[];
   float* lds_acc = reinterpret_cast<float*>(lds_dyn);
-  __shared__ float4 sg[BLOCK];
+  __shared__ float4 sg[BS];
   __shared__ int stab[16];
   __shared__ uint32_t sh_[4];
   uint32_t& sh = sh_[0];
@@ -1036,20 +1041,62 @@ static int bits_for(uint64_t v) {
   return b;
 }
 
-template <int ACC>
-static hipError_t occupancy(int* f, int* b, size_t lds_f, size_t lds_b) {
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(f, k_fwd32<ACC>, BLOCK, lds_f);
-  if (e != hipSuccess) return e;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(b, k_bwd32<ACC>, BLOCK, lds_b);
+// calls f with the block size as a compile-time constant
+template <class F>
+static hipError_t by_block(uint32_t block, F&& f) {
+  switch (block) {
+    case 512: return f(std::integral_constant<uint32_t, 512>{});
+    case 1024: return f(std::integral_constant<uint32_t, 1024>{});
+    default: return f(std::integral_constant<uint32_t, 256>{});
+  }
 }
 
-// whether both passes' LDS (static + brick arrays) fit one workgroup
+static hipError_t occupancy(int acc, uint32_t block, int* f, int* b, size_t lds_f, size_t lds_b) {
+  return by_block(block, [&](auto B) {
+    constexpr uint32_t BS = decltype(B)::value;
+    hipError_t e = acc == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(f, k_fwd32<1, BS>, BS, lds_f)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(f, k_fwd32<0, BS>, BS, lds_f);
+    if (e != hipSuccess) return e;
+    return acc == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(b, k_bwd32<1, BS>, BS, lds_b)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(b, k_bwd32<0, BS>, BS, lds_b);
+  });
+}
+
+static hipError_t launch_fwd(const Fp32cData& f, const IterArgs& a, const float* lam, float4* part,
+                             hipStream_t s) {
+  const size_t lds = f.bs.vol() * sizeof(float);
+  return by_block(f.bs.block, [&](auto B) {
+    constexpr uint32_t BS = decltype(B)::value;
+    if (f.acc_mode == 1)
+      k_fwd32<1, BS><<<f.grid_fwd, BS, lds, s>>>(a, lam, part);
+    else
+      k_fwd32<0, BS><<<f.grid_fwd, BS, lds, s>>>(a, lam, part);
+    return hipGetLastError();
+  });
+}
+
+static hipError_t launch_bwd(const Fp32cData& f, const IterArgs& a, const float* g,
+                             const unsigned long long* cmax, double* acc, hipStream_t s) {
+  const size_t lds = 2 * f.bs.vol() * sizeof(float);
+  return by_block(f.bs.block, [&](auto B) {
+    constexpr uint32_t BS = decltype(B)::value;
+    if (f.acc_mode == 1)
+      k_bwd32<1, BS><<<f.grid_bwd, BS, lds, s>>>(a, g, cmax, acc);
+    else
+      k_bwd32<0, BS><<<f.grid_bwd, BS, lds, s>>>(a, g, cmax, acc);
+    return hipGetLastError();
+  });
+}
+
+// whether both passes' LDS (static + brick arrays) fit one workgroup, and
+// the block size is one the passes are built for
 static bool fp32c_fits(muto_ctx* ctx, const BrickShape& b) {
+  if (b.block != 256 && b.block != 512 && b.block != 1024) return false;
   int max_lds = 0;
   if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess)
     return false;
-  const size_t fwd = FWD_STATIC_LDS + (size_t)b.vol() * sizeof(float);
-  const size_t bwd = BWD_STATIC_LDS + 2 * (size_t)b.vol() * sizeof(float);
+  const size_t fwd = fwd_static_lds(b.block) + (size_t)b.vol() * sizeof(float);
+  const size_t bwd = bwd_static_lds(b.block) + 2 * (size_t)b.vol() * sizeof(float);
   return std::max(fwd, bwd) <= (size_t)max_lds;
 }
 
@@ -1094,10 +1141,7 @@ int fp32c_build(muto_em_plan* pl) {
   // occupancy-derived persistent grids
   const size_t lds_f = f.bs.vol() * sizeof(float), lds_b = 2 * f.bs.vol() * sizeof(float);
   int occ_f = 1, occ_b = 1;
-  if (f.acc_mode == 1)
-    MUTO_CHECK_HIP(ctx, occupancy<1>(&occ_f, &occ_b, lds_f, lds_b));
-  else
-    MUTO_CHECK_HIP(ctx, occupancy<0>(&occ_f, &occ_b, lds_f, lds_b));
+  MUTO_CHECK_HIP(ctx, occupancy(f.acc_mode, f.bs.block, &occ_f, &occ_b, lds_f, lds_b));
   f.grid_fwd = std::max(1, occ_f) * ctx->n_cu;
   f.grid_bwd = std::max(1, occ_b) * ctx->n_cu;
   {
@@ -1134,13 +1178,14 @@ int fp32c_build(muto_em_plan* pl) {
     uint64_t n_ent = 0;
     uint32_t max_len = 0;
     for (uint32_t v : hl) { n_ent += v; max_len = std::max(max_len, v); }
+    const uint32_t BSZ = f.bs.block, ECAP = BSZ * GE;  // chunk capacity
     if (max_len > ECAP) {
       ctx->set_error("a brick segment is longer than the staging capacity");
       return MUTO_ERR_LIMIT;
     }
     // tasks: bricks cut into runs of about `target` entries (at least 16 per
     // brick voxel so the flush is amortised; small enough for the queue to
-    // balance); tasks into chunks of <= BLOCK segments and <= ECAP entries.
+    // balance); tasks into chunks of <= block segments and <= ECAP entries.
     // Every chunk starts on a 16-entry boundary: its entries are padded to
     // whole groups with zero-length entries.
     const uint64_t workers = (uint64_t)std::max(f.grid_fwd, f.grid_bwd);
@@ -1164,11 +1209,11 @@ int fp32c_build(muto_em_plan* pl) {
         tk.c0 = (uint32_t)chunks.size();
         uint64_t te = 0;
         const uint64_t tq0 = q;
-        while (q < s1 && (te < per || bt.size() + 1 == k) && q - tq0 + BLOCK <= MAX_TASK_SEG) {
+        while (q < s1 && (te < per || bt.size() + 1 == k) && q - tq0 + BSZ <= MAX_TASK_SEG) {
           Chunk ch{};
           ch.q0 = q;
           uint32_t ce = 0, cn = 0;
-          while (q < s1 && cn < BLOCK && ce + hl[q] <= ECAP) { ce += hl[q]; ++cn; ++q; }
+          while (q < s1 && cn < BSZ && ce + hl[q] <= ECAP) { ce += hl[q]; ++cn; ++q; }
           ch.cnt = cn;
           ch.ne = ce;
           ch.base = n_ent_pad;
@@ -1268,9 +1313,12 @@ int fp32c_build(muto_em_plan* pl) {
       }
     }
     if (f.n_chunks) {
-      k_group_desc<<<f.n_chunks, BLOCK, 0, s>>>(f.chunks.p, seg_off_p.p, f.ent.p, f.seg_v0.p,
-                                                entcode(pl), f.gdesc.p, f.gcarry.p, f.gv0.p);
-      MUTO_CHECK_HIP(ctx, hipGetLastError());
+      MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
+        constexpr uint32_t BS = decltype(B)::value;
+        k_group_desc<BS><<<f.n_chunks, BS, 0, s>>>(f.chunks.p, seg_off_p.p, f.ent.p, f.seg_v0.p,
+                                                   entcode(pl), f.gdesc.p, f.gcarry.p, f.gv0.p);
+        return hipGetLastError();
+      }));
     }
     // per-rank segment lists and per-track data
     DBuf<uint32_t> nseg_rank;
@@ -1325,19 +1373,13 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
   muto_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream;
   const IterArgs a = iter_args(pl);
-  const size_t lds_f = f.bs.vol() * sizeof(float), lds_b = 2 * f.bs.vol() * sizeof(float);
   const ErrM E = error_matrix(pl->cfg);
   const bool map = pl->is_map;
   k_iter_begin<<<1, 256, 0, s>>>(pl->n_scans, pl->cost_acc.p, pl->dcount.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[0], s));
-  if (f.n_tasks) {
-    if (f.acc_mode == 1)
-      k_fwd32<1><<<f.grid_fwd, BLOCK, lds_f, s>>>(a, pl->lam32.p, reinterpret_cast<float4*>(f.seg_part.p));
-    else
-      k_fwd32<0><<<f.grid_fwd, BLOCK, lds_f, s>>>(a, pl->lam32.p, reinterpret_cast<float4*>(f.seg_part.p));
-    MUTO_CHECK_HIP(ctx, hipGetLastError());
-  }
+  if (f.n_tasks)
+    MUTO_CHECK_HIP(ctx, launch_fwd(f, a, pl->lam32.p, reinterpret_cast<float4*>(f.seg_part.p), s));
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[1], s));
   k_track32<<<grid_for(pl->n_trk, 256), 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p,
                                                       reinterpret_cast<const float4*>(f.seg_part.p),
@@ -1347,13 +1389,7 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
                                                       pl->cost_acc.p, pl->dcount.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[2], s));
-  if (f.n_tasks) {
-    if (f.acc_mode == 1)
-      k_bwd32<1><<<f.grid_bwd, BLOCK, lds_b, s>>>(a, f.trk_g.p, f.cmax.p, pl->acc64.p);
-    else
-      k_bwd32<0><<<f.grid_bwd, BLOCK, lds_b, s>>>(a, f.trk_g.p, f.cmax.p, pl->acc64.p);
-    MUTO_CHECK_HIP(ctx, hipGetLastError());
-  }
+  if (f.n_tasks) MUTO_CHECK_HIP(ctx, launch_bwd(f, a, f.trk_g.p, f.cmax.p, pl->acc64.p, s));
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[3], s));
   k_upd32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->acc64.p, pl->lam64.p,
                                                 pl->lam32.p, map ? pl->ml64.p : nullptr,
@@ -1470,15 +1506,9 @@ int fp32c_map_forward(muto_em_plan* pl, const double* lam, float* g_out,
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.cmax.p, 0, sizeof(unsigned long long), s));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->cost_acc.p, 0, sizeof(double), s));
   const IterArgs a = iter_args(pl);
-  const size_t lds_f = f.bs.vol() * sizeof(float);
   const ErrM E = error_matrix(pl->cfg);
-  if (f.n_tasks) {
-    if (f.acc_mode == 1)
-      k_fwd32<1><<<f.grid_fwd, BLOCK, lds_f, s>>>(a, pl->lam32.p, reinterpret_cast<float4*>(f.seg_part.p));
-    else
-      k_fwd32<0><<<f.grid_fwd, BLOCK, lds_f, s>>>(a, pl->lam32.p, reinterpret_cast<float4*>(f.seg_part.p));
-    MUTO_CHECK_HIP(ctx, hipGetLastError());
-  }
+  if (f.n_tasks)
+    MUTO_CHECK_HIP(ctx, launch_fwd(f, a, pl->lam32.p, reinterpret_cast<float4*>(f.seg_part.p), s));
   k_track32<<<grid_for(pl->n_trk, 256), 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p,
                                                       reinterpret_cast<const float4*>(f.seg_part.p),
                                                       f.trk_d.p, E, f.tail_scale, f.lmax, f.ubound, 1,
@@ -1502,16 +1532,9 @@ int fp32c_map_backward(muto_em_plan* pl, const float* g, const unsigned long lon
   muto_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream;
   const IterArgs a = iter_args(pl);
-  const size_t lds_b = 2 * f.bs.vol() * sizeof(float);
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(acc, 0, pl->nv * sizeof(double), s));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->work.p, 0, 4 * sizeof(unsigned int), s));
-  if (f.n_tasks) {
-    if (f.acc_mode == 1)
-      k_bwd32<1><<<f.grid_bwd, BLOCK, lds_b, s>>>(a, g, cmax, acc);
-    else
-      k_bwd32<0><<<f.grid_bwd, BLOCK, lds_b, s>>>(a, g, cmax, acc);
-    MUTO_CHECK_HIP(ctx, hipGetLastError());
-  }
+  if (f.n_tasks) MUTO_CHECK_HIP(ctx, launch_bwd(f, a, g, cmax, acc, s));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->work.p, 0, 4 * sizeof(unsigned int), s));
   return MUTO_OK;
 }
@@ -1543,10 +1566,7 @@ int fp32c_autotune(muto_em_plan* pl, int iters, int n_cand, const uint32_t* cand
   int best_k = -1;
   std::vector<double> sums((size_t)(iters + 1) * pl->n_scans);
   for (int k = 0; k < n_cand; ++k) {
-    BrickShape b;
-    b.bx = cand[k] & 0xFFu;
-    b.by = (cand[k] >> 8) & 0xFFu;
-    b.bz = (cand[k] >> 16) & 0xFFu;
+    const BrickShape b = BrickShape::from_code(cand[k]);
     ms[k] = INFINITY;
     if (b.bx == 0 || b.by == 0 || b.bz == 0 || !fp32c_fits(ctx, b)) continue;
     pl->f32.reset();
@@ -1568,9 +1588,7 @@ int fp32c_autotune(muto_em_plan* pl, int iters, int n_cand, const uint32_t* cand
   pl->timing = timing;
   pl->f32.reset();
   if (best_k >= 0) {
-    pl->f32.bs.bx = cand[best_k] & 0xFFu;
-    pl->f32.bs.by = (cand[best_k] >> 8) & 0xFFu;
-    pl->f32.bs.bz = (cand[best_k] >> 16) & 0xFFu;
+    pl->f32.bs = BrickShape::from_code(cand[best_k]);
   } else {
     pl->f32.bs = keep;
   }

@@ -6,12 +6,30 @@
 namespace muto {
 
 // Brick (LDS tile) shape of the FP32C path.
+// Brick (LDS tile) shape and the block size of the passes that process it.
+// Option / tuner code: bx | by << 8 | bz << 16 | log2(block / 256) << 24.
 struct BrickShape {
   uint32_t bx = 16, by = 16, bz = 16;
+  uint32_t block = 256;
   uint32_t vol() const { return bx * by * bz; }
+  uint32_t code() const {
+    return bx | by << 8 | bz << 16 | (block == 1024 ? 2u : block == 512 ? 1u : 0u) << 24;
+  }
+  static BrickShape from_code(uint32_t c) {
+    BrickShape b;
+    b.bx = c & 0xFFu;
+    b.by = (c >> 8) & 0xFFu;
+    b.bz = (c >> 16) & 0xFFu;
+    const uint32_t lb = (c >> 24) & 0xFFu;
+    b.block = lb > 2 ? 0u : 256u << lb;  // 0: not a block size the passes take
+    return b;
+  }
+  bool operator!=(const BrickShape& o) const {
+    return bx != o.bx || by != o.by || bz != o.bz || block != o.block;
+  }
 };
 
-// Chunk: up to 256 consecutive brick-major segments [q0, q0 + cnt) whose
+// Chunk: up to `block` consecutive brick-major segments [q0, q0 + cnt) whose
 // ne entries start at entry `base` (a multiple of 16; the entries are
 // padded to whole 16-entry groups); one block processes it as one unit.
 struct Chunk {
