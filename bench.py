@@ -186,8 +186,8 @@ def main():
     ap.add_argument("--brick", default=None, metavar="BX,BY,BZ[,BLOCK]",
                     help="em256: use this brick shape (and block size), no tuning (profiling runs)")
     ap.add_argument("--n-scans", type=int, default=64, help="scans64: number of scans")
-    ap.add_argument("--cpu-sample", type=int, default=100_000)
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=300_000)
+    ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM "
                          "counter calibration for rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE)")

@@ -19,11 +19,10 @@ pytestmark = pytest.mark.gpu
 
 
 def _segments_siddon_test(n=10000):
-    """Segments drawn like the reference's test/SiddonTest.cpp:8-34
-    (uniform in [-500, 500)^3, seeded) -- numpy's generator, same shape."""
-    rng = np.random.default_rng(0)
-    pts = rng.uniform(-500.0, 500.0, size=(n, 6))
-    return pts[:, :3].copy(), pts[:, 3:].copy()
+    """The segments of the reference's test/SiddonTest.cpp:8-34 (libstdc++
+    default_random_engine{0}, uniform over [-500, 500)), restated exactly."""
+    import siddon_inputs
+    return siddon_inputs.siddon_test_segments(n)
 
 
 def _adversarial_segments(g):
@@ -85,6 +84,24 @@ def test_siddon_bitexact(ctx, grid):
     assert _check_siddon(ctx, grid, p1, p2) > 0 or grid.nx == 7
     a1, a2 = _adversarial_segments(grid)
     _check_siddon(ctx, grid, a1, a2)
+
+
+def test_siddon_reference_test_golden(ctx):
+    """GPU lists for SiddonTest.cpp's 10000 segments on its 100^3 x 2 mm grid
+    equal the committed fixture (digests of counts, indices, length bits)."""
+    import hashlib
+    import json
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                           "siddon_test.json")) as f:
+        fx = json.load(f)
+    p1, p2 = _segments_siddon_test(fx["n_segments"])
+    gd = fx["grid"]
+    row, x, y, z, ln = ctx.siddon_csr(make_grid(gd["n"], gd["d"], gd["lo"]), p1, p2)
+    h = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    got = {"row_ptr_u64": h(np.asarray(row).astype("<u8")), "x_u32": h(np.asarray(x).astype("<u4")),
+           "y_u32": h(np.asarray(y).astype("<u4")), "z_u32": h(np.asarray(z).astype("<u4")),
+           "length_f64_bits": h(np.asarray(ln).astype("<f8"))}
+    assert got == fx["digests"]
 
 
 def test_siddon_empty_and_outside(ctx):

@@ -163,6 +163,79 @@ def test_siddon_corner_tie_breaking():
     np.testing.assert_allclose(l, np.sqrt(3.0), rtol=1e-12)
 
 
+def _golden_siddon_test():
+    import json
+    import os
+    import siddon_inputs
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "siddon_test.json")) as f:
+        fx = json.load(f)
+    p1, p2 = siddon_inputs.siddon_test_segments(fx["n_segments"])
+    gd = fx["grid"]
+    return fx, make_grid(gd["n"], gd["d"], gd["lo"]), p1, p2
+
+
+def test_siddon_reference_test_inputs_restated_exactly():
+    """The numpy restatement of SiddonTest.cpp's draws gives the bytes that
+    the libstdc++ calls give (hash in the fixture; the C++ program too when
+    g++ is here)."""
+    import hashlib
+    import os
+    import shutil
+    import subprocess
+    import tempfile
+    fx, g, p1, p2 = _golden_siddon_test()
+    raw = np.concatenate([p1, p2], 1).astype("<f8").tobytes()
+    assert hashlib.sha256(raw).hexdigest() == fx["inputs_sha256"]
+    if shutil.which("g++") is None:
+        return
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "siddon_test_inputs.cpp")
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "gen")
+        subprocess.run(["g++", "-O2", "-o", exe, src], check=True)
+        out = subprocess.run([exe, str(fx["n_segments"])], check=True, capture_output=True).stdout
+    assert out == raw
+
+
+def test_siddon_reference_test_golden_lists():
+    """Oracle lists for SiddonTest.cpp's 10000 segments on its 100^3 x 2 mm
+    grid equal the committed fixture (digests of counts, indices and length
+    bits), and the fixture's full lists and a subset of segments agree with
+    the exact-rational brute force."""
+    import hashlib
+    fx, g, p1, p2 = _golden_siddon_test()
+    row, x, y, z, ln = oracle.siddon_csr(g, p1, p2)
+    h = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    got = {"row_ptr_u64": h(row.astype("<u8")), "x_u32": h(x.astype("<u4")),
+           "y_u32": h(y.astype("<u4")), "z_u32": h(z.astype("<u4")),
+           "length_f64_bits": h(ln.astype("<f8"))}
+    assert got == fx["digests"]
+    assert int(row[-1]) == fx["entries"]
+    for s in fx["first_nonempty"]:
+        a = np.array([float.fromhex(v) for v in s["p1"]])
+        b = np.array([float.fromhex(v) for v in s["p2"]])
+        vox, wid, gap = _brute_siddon(g, a, b)
+        assert gap > 1e-9
+        assert [tuple(v) for v in s["voxels"]] == vox
+        L = float(np.linalg.norm(b - a))
+        np.testing.assert_allclose([float.fromhex(v) for v in s["lengths"]],
+                                   [float(w) * L for w in wid], rtol=1e-9, atol=1e-12)
+    # a seeded subset of the non-empty segments against the brute force
+    nz = np.nonzero(np.diff(row.astype(np.int64)))[0]
+    rng = np.random.default_rng(5)
+    checked = 0
+    for i in rng.choice(nz, 40, replace=False):
+        vox, wid, gap = _brute_siddon(g, p1[i], p2[i])
+        if gap < 1e-9:
+            continue
+        a, b = int(row[i]), int(row[i + 1])
+        assert list(zip(x[a:b].tolist(), y[a:b].tolist(), z[a:b].tolist())) == vox
+        L = float(np.linalg.norm(p2[i] - p1[i]))
+        np.testing.assert_allclose(ln[a:b], [float(w) * L for w in wid], rtol=1e-9, atol=1e-12)
+        checked += 1
+    assert checked >= 30
+
+
 # ---------------------------------------------------------------- PoCA
 def _ray(pin, din, pout, dout, e=3.0):
     r = np.zeros(1, M.RAY_DTYPE)
