@@ -123,6 +123,65 @@ def bench_scans(args):
     print(json.dumps(out))
 
 
+def bench_siddon(args):
+    """Plain PoCA + Siddon at 200^3 x 10 mm with 1e7 muons: PoCA points and
+    the Siddon walk of both legs (pin -> PoCA point -> pout) of every muon,
+    kernel times from HIP events (host copies excluded), beside the CPU
+    oracle on a sample."""
+    import muontomography_amd as M
+    import oracle
+    from muontomography_amd import workloads as W
+    from muontomography_amd._abi import poca_cfg
+    ctx = M.Context(0)
+    g, rays, _ = W.muons("siddon200", n_muons=args.n_muons)
+    ctx.poca_points(g, rays[:1000])
+    t = {}
+    P, v, a, c = ctx.poca_points(g, rays)
+    t["poca_points"] = ctx.kernel_ms()
+    entries = 0
+    t["siddon_walk"] = 0.0
+    for p1, p2 in ((rays["pin"], P), (P, rays["pout"])):
+        cnt, _ = ctx.siddon_digest(g, p1, p2)
+        t["siddon_walk"] += ctx.kernel_ms()
+        entries += int(cnt.sum(dtype=np.uint64))
+    cfg = poca_cfg(g, length_weighted=1, straight_line=0)
+    ctx.poca_reconstruct(cfg, rays)
+    t["poca_reconstruct"] = ctx.kernel_ms()
+    # list output (12 B written per entry as u32 index + f64 length, plus
+    # the y, z indices of the CSR API) on a sample that fits host memory
+    ns = 200_000
+    row, *_ = ctx.siddon_csr(g, rays["pin"][:ns], P[:ns])
+    t_fill = ctx.kernel_ms()
+    fill_entries = int(row[-1])
+    n_seg = 2 * len(rays)
+    walk_rate = entries / (t["siddon_walk"] * 1e-3)
+    cpu = None
+    if not args.no_cpu:
+        nc = min(2_000_000, len(rays))
+        t0 = time.perf_counter()
+        cc, _ = oracle.siddon_digest(g, rays["pin"][:nc], P[:nc])
+        cc2, _ = oracle.siddon_digest(g, P[:nc], rays["pout"][:nc])
+        tc = time.perf_counter() - t0
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": float(cc.sum(dtype=np.uint64) + cc2.sum(dtype=np.uint64)) / tc,
+               "unit": "Siddon entries/s", "cores": threads, "kind": "port",
+               "sample": f"both legs of the first {nc} muons, {tc:.2f} s, OpenMP over segments"}
+    out = {"metric": "Siddon voxel-walk entries/s (both PoCA legs), FP64 bit-exact",
+           "value": walk_rate, "unit": "Siddon entries/s", "n_gpus": 1, "steps": 1,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic (seeded generator)",
+           "config": {"workload": "plain PoCA + Siddon, 200^3 grid of 10 mm voxels, 1e7 muons",
+                      "muons": int(len(rays)), "segments": n_seg, "entries": entries,
+                      "poca_in_grid": c["n_scat_in_grid"]},
+           "kernel_ms": t,
+           "segments_per_s": n_seg / (t["siddon_walk"] * 1e-3),
+           "poca_points_per_s": len(rays) / (t["poca_points"] * 1e-3),
+           "csr_fill_sample": {"segments": ns, "entries": fill_entries, "ms": t_fill,
+                               "write_GBps": fill_entries * 20 / (t_fill * 1e-3) / 1e9},
+           "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(out))
+
+
 def bench_map_grad(args):
     """MAP objective + gradient on 128^3 x 15 mm with 1e6 muons: FP32C
     (brick stream), FP64 (stored lists) and FP64 with the paths walked again
@@ -176,7 +235,8 @@ def main():
     ap.add_argument("--steps", type=int, default=45)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="em256",
-                    help="em256 (the headline); scans64 (many-scan batch); map128 (MAP gradient)")
+                    help="em256 (the headline); scans64 (many-scan batch); map128 (MAP gradient); "
+                         "siddon200 (plain PoCA + Siddon)")
     ap.add_argument("--n-muons", type=int, default=None)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -210,6 +270,8 @@ def main():
         return bench_scans(args)
     if args.workload == "map128":
         return bench_map_grad(args)
+    if args.workload == "siddon200":
+        return bench_siddon(args)
 
     import muontomography_amd as M
     from muontomography_amd import workloads as W
@@ -285,6 +347,16 @@ def main():
         w64 = time.perf_counter() - t0
         img64 = plan.image()
         t64 = plan.timing()
+        # the same iterations with the wavefront-per-track FP64 kernel
+        plan.set_option(M.MUTO_OPT_FP64_KERNEL, 1)
+        plan.set_image(init)
+        plan.run(1, M.MUTO_FP64)
+        plan.set_image(init)
+        t0 = time.perf_counter()
+        plan.run(args.warmup + args.steps, M.MUTO_FP64)
+        w64w = time.perf_counter() - t0
+        img64w = plan.image()
+        plan.set_option(M.MUTO_OPT_FP64_KERNEL, 0)
         # FP32C image after the same number of iterations
         plan.set_image(init)
         plan.run(args.warmup + args.steps, M.MUTO_FP32C)
@@ -303,9 +375,16 @@ def main():
             "cost_rel_diff_last": float(abs(cost32[-1] - cost64[args.warmup + args.steps - 1]) /
                                         abs(cost64[args.warmup + args.steps - 1])),
             "fp64_kernel_ms": {k: round(v, 4) for k, v in t64.items()},
+            "fp64_wave_ms_per_step": w64w * 1e3 / (args.warmup + args.steps),
+            "fp64_wave_value": nnz * (args.warmup + args.steps) / w64w,
+            "fp32c_speedup_vs_fp64_wave": (w64w / (args.warmup + args.steps)) / (wall / args.steps),
+            "fp64_wave_vs_fp64_err_max_rel": float((np.abs(img64w[m] - img64[m]) / img64[m]).max())
+                                             if m.any() else None,
         }
-        log(f"FP64 {args.warmup + args.steps} it: {extra['fp64_ms_per_step']:.2f} ms/it; "
-            f"speed-up {extra['fp32c_speedup_vs_fp64']:.2f}x; err max {extra['image_err_max_rel']}")
+        log(f"FP64 {args.warmup + args.steps} it: {extra['fp64_ms_per_step']:.2f} ms/it "
+            f"(wavefront kernel {extra['fp64_wave_ms_per_step']:.2f} ms/it); "
+            f"speed-up {extra['fp32c_speedup_vs_fp64']:.2f}x "
+            f"({extra['fp32c_speedup_vs_fp64_wave']:.2f}x); err max {extra['image_err_max_rel']}")
     plan.close()
     ctx.close()
 

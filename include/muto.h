@@ -118,6 +118,12 @@ int muto_ctx_create(int device, muto_ctx** out);
 int muto_ctx_destroy(muto_ctx* ctx);
 const char* muto_last_error(const muto_ctx* ctx);
 int muto_abi_version(void);
+/* GPU time (ms, HIP events on the context's stream) of the kernels of the
+ * last muto_siddon_csr (fill pass), muto_siddon_digest, muto_poca_points or
+ * muto_poca_reconstruct call on this context, without the host copies.  No
+ * counterpart in the reference, which times whole runs with a stopwatch
+ * (MuToTest.cpp:25,80-81). */
+int muto_ctx_kernel_ms(muto_ctx* ctx, double* ms);
 
 /* ------------------------------------------------------------------ */
 /* Siddon voxel tracing                                                */
@@ -132,6 +138,17 @@ int muto_abi_version(void);
 int muto_siddon_csr(muto_ctx* ctx, const muto_grid* grid, size_t n,
                     const double* p1, const double* p2, uint64_t* row_ptr,
                     uint32_t* vx, uint32_t* vy, uint32_t* vz, double* len);
+
+/* Per-segment entry count and digest of the lists muto_siddon_csr would
+ * return, without storing them: count[n], digest[n] (host).  The digest is
+ * FNV-1a over 64-bit words (offset 0xcbf29ce484222325, prime 0x100000001b3),
+ * three words per entry in list order: x | y << 32, z, and the bit pattern of
+ * the FP64 length.  No counterpart in the reference: it verifies
+ * MutoSiddon::getVoxelPath (src/MutoSiddon.cpp:9-141) on segment sets whose
+ * lists do not fit host memory. */
+int muto_siddon_digest(muto_ctx* ctx, const muto_grid* grid, size_t n,
+                       const double* p1, const double* p2, uint32_t* count,
+                       uint64_t* digest);
 
 /* ------------------------------------------------------------------ */
 /* PoCA                                                                */
@@ -263,10 +280,16 @@ int muto_batch_destroy(muto_batch* batch);
  *                     without the graph (default 0)
  *   MUTO_OPT_ACC_MODE FP32C accumulation: 1 FP64 registers + int64 fixed-point
  *                     LDS sums (default), 0 FP32 with two-sum compensation
+ *   MUTO_OPT_FP64_KERNEL FP64 iteration kernel: 0 one lane per track, the
+ *                     reference's operation order (default; per-entry values
+ *                     bit-identical to the CPU path); 1 one wavefront per
+ *                     track, coalesced row loads and a butterfly sum of the
+ *                     covariance (per-track values within FP64 rounding)
  * No counterpart in the reference (which has no device state). */
 #define MUTO_OPT_GRAPH 1
 #define MUTO_OPT_TIMING 2
 #define MUTO_OPT_ACC_MODE 3
+#define MUTO_OPT_FP64_KERNEL 5
 #define MUTO_OPT_BRICK 4  /* FP32C brick (LDS tile) shape and block size:
                              bx | by << 8 | bz << 16 | log2(block / 256) << 24,
                              at most 16384 voxels, block 256, 512 or 1024

@@ -35,6 +35,7 @@ MUTO_OPT_GRAPH = 1
 MUTO_OPT_TIMING = 2
 MUTO_OPT_ACC_MODE = 3
 MUTO_OPT_BRICK = 4
+MUTO_OPT_FP64_KERNEL = 5
 # the autotuner's built-in candidate list (muto_em_plan_autotune), in order
 def brick_code(shape):
     """MUTO_OPT_BRICK / tuner code of (bx, by, bz) or (bx, by, bz, block):
@@ -101,6 +102,12 @@ class Context:
             msg = lib().muto_last_error(self._h)
             raise MutoError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
 
+    def kernel_ms(self):
+        """GPU time of the kernels of the last Siddon / PoCA call (ms)."""
+        t = C.c_double()
+        self._check(lib().muto_ctx_kernel_ms(self._h, C.byref(t)), "muto_ctx_kernel_ms")
+        return t.value
+
     # -- Siddon ---------------------------------------------------------
     def siddon_csr(self, grid, p1, p2):
         """Batched Siddon paths; returns (row_ptr, x, y, z, length)."""
@@ -118,6 +125,18 @@ class Context:
         self._check(lib().muto_siddon_csr(self._h, C.byref(grid), n, ptr(p1), ptr(p2), ptr(row),
                                           ptr(vx), ptr(vy), ptr(vz), ptr(ln)), "muto_siddon_csr")
         return row, vx[:nnz], vy[:nnz], vz[:nnz], ln[:nnz]
+
+    def siddon_digest(self, grid, p1, p2):
+        """Per-segment entry counts (u32) and list digests (u64) without
+        storing the lists (muto_siddon_digest)."""
+        p1 = np.ascontiguousarray(p1, dtype=np.float64).reshape(-1, 3)
+        p2 = np.ascontiguousarray(p2, dtype=np.float64).reshape(-1, 3)
+        n = p1.shape[0]
+        cnt = np.empty(max(n, 1), np.uint32)
+        dig = np.empty(max(n, 1), np.uint64)
+        self._check(lib().muto_siddon_digest(self._h, C.byref(grid), n, ptr(p1), ptr(p2), ptr(cnt),
+                                             ptr(dig)), "muto_siddon_digest")
+        return cnt[:n], dig[:n]
 
     # -- PoCA -----------------------------------------------------------
     def poca_points(self, grid, rays, eps=1e-8, mode=0):

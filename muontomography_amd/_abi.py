@@ -135,7 +135,9 @@ PROTOS = {
     "muto_ctx_create": (C.c_int, [C.c_int, C.POINTER(P)]),
     "muto_ctx_destroy": (C.c_int, [P]),
     "muto_last_error": (C.c_char_p, [P]),
+    "muto_ctx_kernel_ms": (C.c_int, [P, C.POINTER(C.c_double)]),
     "muto_siddon_csr": (C.c_int, [P, C.POINTER(Grid), C.c_size_t, P, P, P, P, P, P, P]),
+    "muto_siddon_digest": (C.c_int, [P, C.POINTER(Grid), C.c_size_t, P, P, P, P]),
     "muto_poca_points": (C.c_int, [P, C.POINTER(Grid), C.c_size_t, P, C.c_double, C.c_int,
                                    P, P, P, C.POINTER(Counters)]),
     "muto_poca_reconstruct": (C.c_int, [P, C.POINTER(PocaCfg), C.c_size_t, P, P,

@@ -23,6 +23,12 @@ extern "C" int muto_ctx_create(int device, muto_ctx** out) {
     delete c;
     return MUTO_ERR_HIP;
   }
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MUTO_ERR_HIP;
+  }
   *out = c;
   return MUTO_OK;
 }
@@ -33,6 +39,8 @@ extern "C" int muto_ctx_destroy(muto_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
   }
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   delete ctx;
   return MUTO_OK;
 }
@@ -40,4 +48,18 @@ extern "C" int muto_ctx_destroy(muto_ctx* ctx) {
 extern "C" const char* muto_last_error(const muto_ctx* ctx) {
   if (!ctx) return "null context";
   return ctx->err.c_str();
+}
+
+extern "C" int muto_ctx_kernel_ms(muto_ctx* ctx, double* ms) {
+  if (!ctx || !ms) return MUTO_ERR_ARG;
+  *ms = 0.0;
+  if (!ctx->ev_valid) {
+    ctx->set_error("muto_ctx_kernel_ms: no timed call on this context");
+    return MUTO_ERR_ARG;
+  }
+  float t = 0.f;
+  MUTO_CHECK_HIP(ctx, hipEventSynchronize(ctx->ev1));
+  MUTO_CHECK_HIP(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+  *ms = t;
+  return MUTO_OK;
 }

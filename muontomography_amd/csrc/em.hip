@@ -303,6 +303,88 @@ k_em64(size_t n_trk, const uint64_t* __restrict__ row_ptr,
   wave_count(&cnt[C_SINGULAR], sing);
 }
 
+// FP64 iteration, one wavefront per track (MUTO_OPT_FP64_KERNEL 1): the
+// lanes read the track's row with coalesced loads, the covariance sums are
+// combined by a wavefront butterfly (a different association from the
+// reference's running sum, so per-track values differ by FP64 rounding), the
+// 2x2 algebra is done by every lane on the reduced sums, and the second sweep
+// over the row (now in cache) forms S_ij as k_em64 does.
+template <int MODE>
+__global__ void __launch_bounds__(256)
+k_em64w(size_t n_trk, const uint64_t* __restrict__ row_ptr,
+        const uint32_t* __restrict__ vox, const double* __restrict__ len,
+        const double* __restrict__ tail, const double* __restrict__ trk4,
+        const double* __restrict__ pr, const double* __restrict__ lam,
+        size_t nv, ErrM E, double* __restrict__ acc,
+        const uint64_t* __restrict__ slot, double* __restrict__ S,
+        double* __restrict__ cost_acc, unsigned long long* __restrict__ cnt) {
+  const unsigned lane = threadIdx.x & 63;
+  const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  double cost = 0.0;
+  bool sing = false;
+  if (i < n_trk) {  // uniform over the wavefront
+    const double p = pr[i];
+    const uint64_t b = row_ptr[i], e = row_ptr[i + 1];
+    double s00 = 0.0, s01 = 0.0, s11 = 0.0;
+    for (uint64_t k = b + lane; k < e; k += 64) {
+      Mat2 W;
+      weight(len[k], tail[k], W);
+      const uint32_t j = vox[k];
+      const double lj = j < nv ? lam[j] : 0.0;
+      s00 = s00 + lj * W.m00;
+      s01 = s01 + lj * W.m01;
+      s11 = s11 + lj * W.m11;
+    }
+    s00 = wave_sum(s00);
+    s01 = wave_sum(s01);
+    s11 = wave_sum(s11);
+    const double pp = p * p;
+    Mat2 cov;
+    cov.m00 = pp * s00 + E.e00;
+    cov.m01 = pp * s01 + E.e01;
+    cov.m10 = cov.m01;
+    cov.m11 = pp * s11 + E.e11;
+    const double Sx = trk4[4 * i], Sy = trk4[4 * i + 1];
+    const double Dx = trk4[4 * i + 2], Dy = trk4[4 * i + 3];
+    Mat2 inv;
+    double det;
+    const bool ok = inv_det(cov, inv, det);
+    if (!ok || det <= 0) {
+      sing = lane == 0;
+    } else if (lane == 0) {
+      cost = log(det) + ((Sx * Sx + Sy * Sy) * inv.m00 +
+                         (Dx * Dx + Dy * Dy) * inv.m11 +
+                         (Sx * Dx + Sy * Dy) * 2.0 * inv.m01) * 0.5;
+    }
+    for (uint64_t k = b + lane; k < e; k += 64) {
+      const uint32_t j = vox[k];
+      if (j >= nv) continue;
+      Mat2 W;
+      weight(len[k], tail[k], W);
+      const double lj = lam[j];
+      const double pr2lj2 = p * p * lj * lj;
+      const double P00 = inv.m00 * W.m00 + inv.m01 * W.m10;
+      const double P01 = inv.m00 * W.m01 + inv.m01 * W.m11;
+      const double P10 = inv.m10 * W.m00 + inv.m11 * W.m10;
+      const double P11 = inv.m10 * W.m01 + inv.m11 * W.m11;
+      const double tr = P00 + P11;
+      const double C00 = P00 * inv.m00 + P01 * inv.m10;
+      const double C01 = P00 * inv.m01 + P01 * inv.m11;
+      const double C10 = P10 * inv.m00 + P11 * inv.m10;
+      const double C11 = P10 * inv.m01 + P11 * inv.m11;
+      const double xy = (Sx * Sx + Sy * Sy) * C00 + (Dx * Dx + Dy * Dy) * C11 +
+                        (Sx * Dx + Sy * Dy) * (C01 + C10);
+      const double sij = 2.0 * lj + (0.5 * xy - tr) * pr2lj2;
+      if (MODE == 0)
+        atomicAdd(&acc[j], sij);
+      else
+        S[slot[k]] = sij;
+    }
+  }
+  atomic_block_sum(cost_acc, cost);
+  wave_count(&cnt[C_SINGULAR], sing);
+}
+
 // mean of all S (updateImageVoxel, fPercentileMean == 100)
 __global__ void k_vox_mean64(size_t nv, const uint32_t* __restrict__ mj,
                              double* __restrict__ acc,
@@ -808,7 +890,19 @@ static int iterate64(muto_em_plan* pl, double* dst, bool time_it) {
     if (int rc = build_slots(pl)) return rc;
   const ErrM E = error_matrix(c);
   if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[0], s));
-  if (pl->n_trk) {
+  if (pl->n_trk && pl->fp64_kernel == 1) {
+    // 4 tracks per 256-thread block
+    const unsigned nblk = grid_for(pl->n_trk, 4);
+    if (mean_all)
+      k_em64w<0><<<nblk, 256, 0, s>>>(
+          pl->n_trk, pl->row_ptr.p, pl->vox.p, pl->len.p, pl->tail.p, pl->trk4.p, pl->pr.p,
+          pl->lam64.p, pl->nv, E, pl->acc64.p, nullptr, nullptr, pl->cost_acc.p, pl->dcount.p);
+    else
+      k_em64w<1><<<nblk, 256, 0, s>>>(
+          pl->n_trk, pl->row_ptr.p, pl->vox.p, pl->len.p, pl->tail.p, pl->trk4.p, pl->pr.p,
+          pl->lam64.p, pl->nv, E, nullptr, pl->slot.p, pl->S.p, pl->cost_acc.p, pl->dcount.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  } else if (pl->n_trk) {
     if (mean_all)
       k_em64<0><<<grid_for(pl->n_trk, 256), 256, 0, s>>>(
           pl->n_trk, pl->row_ptr.p, pl->vox.p, pl->len.p, pl->tail.p, pl->trk4.p, pl->pr.p,
@@ -980,6 +1074,10 @@ extern "C" int muto_em_plan_set_option(muto_em_plan* pl, int option, int value) 
       }
       return MUTO_OK;
     }
+    case MUTO_OPT_FP64_KERNEL:
+      if (value != 0 && value != 1) break;
+      pl->fp64_kernel = value;
+      return MUTO_OK;
     case MUTO_OPT_ACC_MODE:
       if (value != 0 && value != 1) break;
       if (pl->f32.acc_mode != value && pl->f32.graph) {

@@ -150,6 +150,7 @@ struct muto_em_plan {
   muto::DBuf<unsigned long long> dcount;  // device counters [8]
   muto::DBuf<unsigned int> work;   // work-queue counters [4]
   int cur_prec = -1;               // which image is current
+  int fp64_kernel = 0;             // FP64 iteration: 0 lane per track, 1 wavefront per track
   int use_graph = 1;               // FP32C: replay the captured iteration
   int timing = 0;                  // per-kernel events (no graph)
 

@@ -24,7 +24,16 @@ struct muto_ctx {
   int n_cu = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  // events around the kernels of the last one-shot call (Siddon, PoCA),
+  // read by muto_ctx_kernel_ms
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool ev_valid = false, ev_begun = false;
   void set_error(const std::string& m) { err = m; }
+  void mark_begin() {
+    ev_valid = false;
+    ev_begun = hipEventRecord(ev0, stream) == hipSuccess;
+  }
+  void mark_end() { ev_valid = ev_begun && hipEventRecord(ev1, stream) == hipSuccess; }
 };
 
 namespace muto {

@@ -159,9 +159,11 @@ extern "C" int muto_poca_points(muto_ctx* ctx, const muto_grid* grid,
   if (accept) MUTO_CHECK_HIP(ctx, da.alloc(n));
   MUTO_CHECK_HIP(ctx, hipMemcpyAsync(dr.p, rays, n * sizeof(muto_ray), hipMemcpyHostToDevice, s));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(dc.p, 0, sizeof(PocaCounters), s));
+  ctx->mark_begin();
   k_poca_points<<<grid_for(n, 256), 256, 0, s>>>(to_dev(*grid), dr.p, n, eps, mode,
                                                  dp.p, dv.p, da.p, dc.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
+  ctx->mark_end();
   if (p_scat) MUTO_CHECK_HIP(ctx, hipMemcpyAsync(p_scat, dp.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
   if (vox) MUTO_CHECK_HIP(ctx, hipMemcpyAsync(vox, dv.p, 3 * n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   if (accept) MUTO_CHECK_HIP(ctx, hipMemcpyAsync(accept, da.p, n, hipMemcpyDeviceToHost, s));
@@ -207,10 +209,12 @@ extern "C" int muto_poca_reconstruct(muto_ctx* ctx, const muto_poca_cfg* cfg,
   c.has_energy = cfg->has_energy;
   c.angmag = cfg->angle_unit_mag;
   c.eps = cfg->eps;
+  ctx->mark_begin();
   k_poca_image<<<grid_for(ntot, 256), 256, 0, s>>>(c, dr.p, ntot, di.p, dw.p, dc.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   k_poca_normalize<<<grid_for(nv, 256), 256, 0, s>>>(di.p, dw.p, nv, cfg->eps);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
+  ctx->mark_end();
   MUTO_CHECK_HIP(ctx, hipMemcpyAsync(img, di.p, nv * sizeof(double), hipMemcpyDeviceToHost, s));
   PocaCounters hc;
   MUTO_CHECK_HIP(ctx, hipMemcpyAsync(&hc, dc.p, sizeof hc, hipMemcpyDeviceToHost, s));

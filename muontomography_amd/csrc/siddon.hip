@@ -41,6 +41,26 @@ k_siddon_fill(GridD g, const double* __restrict__ p1,
   });
 }
 
+// Per-segment count and digest of the list: FNV-1a over 64-bit words, three
+// per entry (x | y << 32, z, length bits), in list order.
+__global__ void __launch_bounds__(256)
+k_siddon_digest(GridD g, const double* __restrict__ p1,
+                const double* __restrict__ p2, size_t n,
+                uint32_t* __restrict__ count, uint64_t* __restrict__ digest) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a[3] = {p1[3 * i], p1[3 * i + 1], p1[3 * i + 2]};
+  double b[3] = {p2[3 * i], p2[3 * i + 1], p2[3 * i + 2]};
+  uint64_t h = 0xcbf29ce484222325ull;
+  const uint64_t prime = 0x100000001b3ull;
+  count[i] = siddon_walk(g, a, b, [&](uint32_t x, uint32_t y, uint32_t z, double L) {
+    h = (h ^ ((uint64_t)x | ((uint64_t)y << 32))) * prime;
+    h = (h ^ (uint64_t)z) * prime;
+    h = (h ^ (uint64_t)__double_as_longlong(L)) * prime;
+  });
+  digest[i] = h;
+}
+
 }  // namespace
 
 hipError_t siddon_count(const GridD& g, const double* p1, const double* p2,
@@ -114,9 +134,11 @@ extern "C" int muto_siddon_csr(muto_ctx* ctx, const muto_grid* grid, size_t n,
   if (vy) MUTO_CHECK_HIP(ctx, dy.alloc(nnz));
   if (vz) MUTO_CHECK_HIP(ctx, dz.alloc(nnz));
   if (len) MUTO_CHECK_HIP(ctx, dl.alloc(nnz));
+  ctx->mark_begin();
   MUTO_CHECK_HIP(ctx, siddon_fill(g, dp1.p, dp2.p, n, drp.p, dx.p,
                                   vy ? dy.p : nullptr, vz ? dz.p : nullptr,
                                   len ? dl.p : nullptr, s));
+  ctx->mark_end();
   MUTO_CHECK_HIP(ctx, hipMemcpyAsync(vx, dx.p, nnz * sizeof(uint32_t),
                                      hipMemcpyDeviceToHost, s));
   if (vy)
@@ -128,6 +150,44 @@ extern "C" int muto_siddon_csr(muto_ctx* ctx, const muto_grid* grid, size_t n,
   if (len)
     MUTO_CHECK_HIP(ctx, hipMemcpyAsync(len, dl.p, nnz * sizeof(double),
                                        hipMemcpyDeviceToHost, s));
+  MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+  return MUTO_OK;
+}
+
+extern "C" int muto_siddon_digest(muto_ctx* ctx, const muto_grid* grid, size_t n,
+                                  const double* p1, const double* p2,
+                                  uint32_t* count, uint64_t* digest) {
+  if (!ctx) return MUTO_ERR_ARG;
+  if (!grid || !count || !digest || (n && (!p1 || !p2))) {
+    ctx->set_error("muto_siddon_digest: null argument");
+    return MUTO_ERR_ARG;
+  }
+  if (grid->nx == 0 || grid->ny == 0 || grid->nz == 0) {
+    ctx->set_error("muto_siddon_digest: empty grid");
+    return MUTO_ERR_ARG;
+  }
+  if (n == 0) return MUTO_OK;
+  const GridD g = to_dev(*grid);
+  hipStream_t s = ctx->stream;
+  DBuf<double> dp1, dp2;
+  DBuf<uint32_t> dc;
+  DBuf<uint64_t> dh;
+  MUTO_CHECK_HIP(ctx, dp1.alloc(3 * n));
+  MUTO_CHECK_HIP(ctx, dp2.alloc(3 * n));
+  MUTO_CHECK_HIP(ctx, dc.alloc(n));
+  MUTO_CHECK_HIP(ctx, dh.alloc(n));
+  MUTO_CHECK_HIP(ctx, hipMemcpyAsync(dp1.p, p1, 3 * n * sizeof(double),
+                                     hipMemcpyHostToDevice, s));
+  MUTO_CHECK_HIP(ctx, hipMemcpyAsync(dp2.p, p2, 3 * n * sizeof(double),
+                                     hipMemcpyHostToDevice, s));
+  ctx->mark_begin();
+  k_siddon_digest<<<grid_for(n, 256), 256, 0, s>>>(g, dp1.p, dp2.p, n, dc.p, dh.p);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  ctx->mark_end();
+  MUTO_CHECK_HIP(ctx, hipMemcpyAsync(count, dc.p, n * sizeof(uint32_t),
+                                     hipMemcpyDeviceToHost, s));
+  MUTO_CHECK_HIP(ctx, hipMemcpyAsync(digest, dh.p, n * sizeof(uint64_t),
+                                     hipMemcpyDeviceToHost, s));
   MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
   return MUTO_OK;
 }

@@ -40,6 +40,7 @@ def lib():
         L.orc_x86_u32.restype = C.c_uint32
         L.orc_x86_u32.argtypes = [C.c_double]
         L.orc_siddon_csr.argtypes = [C.POINTER(Grid), C.c_size_t, P, P, P, P, P, P, P]
+        L.orc_siddon_digest.argtypes = [C.POINTER(Grid), C.c_size_t, P, P, P, P]
         L.orc_poca_points.argtypes = [C.POINTER(Grid), C.c_size_t, P, C.c_double, C.c_int, P, P, P,
                                       C.POINTER(Counters)]
         L.orc_poca_reconstruct.argtypes = [C.POINTER(PocaCfg), C.c_size_t, P, P, C.POINTER(Counters)]
@@ -89,6 +90,17 @@ def siddon_csr(grid, p1, p2):
     assert lib().orc_siddon_csr(C.byref(grid), n, ptr(p1), ptr(p2), ptr(row), ptr(vx), ptr(vy),
                                 ptr(vz), ptr(ln)) == 0
     return row, vx[:nnz], vy[:nnz], vz[:nnz], ln[:nnz]
+
+
+def siddon_digest(grid, p1, p2):
+    """Per-segment counts and list digests (OpenMP over segments)."""
+    p1 = np.ascontiguousarray(p1, np.float64).reshape(-1, 3)
+    p2 = np.ascontiguousarray(p2, np.float64).reshape(-1, 3)
+    n = p1.shape[0]
+    cnt = np.empty(max(n, 1), np.uint32)
+    dig = np.empty(max(n, 1), np.uint64)
+    assert lib().orc_siddon_digest(C.byref(grid), n, ptr(p1), ptr(p2), ptr(cnt), ptr(dig)) == 0
+    return cnt[:n], dig[:n]
 
 
 def poca_points(grid, rays, eps=1e-8, mode=0):

@@ -50,9 +50,42 @@ uint32_t orc_x86_u32(double v) {
 /* ------------------------------------------------------------------ */
 /* Siddon: src/MutoSiddon.cpp:9-141                                    */
 /* ------------------------------------------------------------------ */
+/* Digest of a list: FNV-1a over 64-bit words, three words per entry
+ * (x | y << 32, z, the bit pattern of the length), in list order. */
+#define ORC_DIGEST_BASIS 0xcbf29ce484222325ULL
+#define ORC_DIGEST_PRIME 0x100000001b3ULL
+static inline uint64_t digest_word(uint64_t h, uint64_t w) {
+  return (h ^ w) * ORC_DIGEST_PRIME;
+}
+
+static size_t siddon_walk(const muto_grid* g, const double p1[3],
+                          const double p2[3], uint32_t* vx, uint32_t* vy,
+                          uint32_t* vz, double* vl, size_t cap,
+                          uint64_t* digest);
+
 size_t orc_siddon_path(const muto_grid* g, const double p1[3],
                        const double p2[3], uint32_t* vx, uint32_t* vy,
                        uint32_t* vz, double* vl, size_t cap) {
+  return siddon_walk(g, p1, p2, vx, vy, vz, vl, cap, NULL);
+}
+
+int orc_siddon_digest(const muto_grid* g, size_t n, const double* p1,
+                      const double* p2, uint32_t* count, uint64_t* digest) {
+  if (!g || !count || !digest || (n && (!p1 || !p2))) return MUTO_ERR_ARG;
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (size_t i = 0; i < n; ++i) {
+    uint64_t h = ORC_DIGEST_BASIS;
+    count[i] = (uint32_t)siddon_walk(g, p1 + 3 * i, p2 + 3 * i, NULL, NULL,
+                                     NULL, NULL, 0, &h);
+    digest[i] = h;
+  }
+  return MUTO_OK;
+}
+
+static size_t siddon_walk(const muto_grid* g, const double p1[3],
+                          const double p2[3], uint32_t* vx, uint32_t* vy,
+                          uint32_t* vz, double* vl, size_t cap,
+                          uint64_t* digest) {
   const double mn[3] = {g->x_min, g->y_min, g->z_min};
   const double d[3] = {g->dx, g->dy, g->dz};
   const uint32_t nn[3] = {g->nx, g->ny, g->nz};
@@ -122,11 +155,19 @@ size_t orc_siddon_path(const muto_grid* g, const double p1[3],
     }
     double dalpha = smin(alpha, amax) - alpha_last;
     if (eps < dalpha) {
+      const double L = dalpha * length;
       if (cnt < cap) {
         if (vx) vx[cnt] = cx;
         if (vy) vy[cnt] = cy;
         if (vz) vz[cnt] = cz;
-        if (vl) vl[cnt] = dalpha * length;
+        if (vl) vl[cnt] = L;
+      }
+      if (digest) {
+        uint64_t lb;
+        memcpy(&lb, &L, sizeof lb);
+        uint64_t h = digest_word(*digest, (uint64_t)cx | ((uint64_t)cy << 32));
+        h = digest_word(h, cz);
+        *digest = digest_word(h, lb);
       }
       ++cnt;
       alpha_last = alpha;

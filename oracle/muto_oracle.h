@@ -27,6 +27,10 @@ uint32_t orc_x86_u32(double v);
 size_t orc_siddon_path(const muto_grid* g, const double p1[3],
                        const double p2[3], uint32_t* vx, uint32_t* vy,
                        uint32_t* vz, double* vl, size_t cap);
+/* Per-segment count and digest (FNV-1a over the 64-bit words x | y << 32,
+ * z, length bits of each entry, in order) of the Siddon lists. */
+int orc_siddon_digest(const muto_grid* g, size_t n, const double* p1,
+                      const double* p2, uint32_t* count, uint64_t* digest);
 /* Batched, CSR output (two-call protocol like muto_siddon_csr). */
 int orc_siddon_csr(const muto_grid* g, size_t n, const double* p1,
                    const double* p2, uint64_t* row_ptr, uint32_t* vx,

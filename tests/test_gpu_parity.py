@@ -104,6 +104,52 @@ def test_siddon_reference_test_golden(ctx):
     assert got == fx["digests"]
 
 
+def test_poca_siddon_full_size_200(ctx):
+    """Plain PoCA + Siddon at full size: 200^3 grid of 10 mm voxels, 1e7
+    muons.  PoCA points, voxel casts and accept flags bit-exact for every
+    muon; the Siddon lists of both legs (pin -> PoCA point -> pout) of every
+    muon bit-exact by per-segment count and digest; full lists on a seeded
+    1e5-muon subset."""
+    g, rays, _ = W.muons("siddon200")
+    assert len(rays) == 10_000_000
+    Po, vo, ao, co = oracle.poca_points(g, rays)
+    Pg, vg, ag, cg = ctx.poca_points(g, rays)
+    assert np.array_equal(Po.view(np.uint64), Pg.view(np.uint64))
+    assert np.array_equal(vo, vg) and np.array_equal(ao, ag)
+    assert co["n_scat_in_grid"] == cg["n_scat_in_grid"] and co["n_not_parallel"] == cg["n_not_parallel"]
+    total = 0
+    for a, b in ((rays["pin"], Po), (Po, rays["pout"])):
+        cg_, dg = ctx.siddon_digest(g, a, b)
+        co_, do = oracle.siddon_digest(g, a, b)
+        assert np.array_equal(cg_, co_), "per-segment counts differ"
+        assert np.array_equal(dg, do), f"{int((dg != do).sum())} segment digests differ"
+        total += int(co_.sum(dtype=np.uint64))
+    assert total > 1_000_000_000
+    sub = np.sort(np.random.default_rng(3).choice(len(rays), 100_000, replace=False))
+    _check_siddon(ctx, g, rays["pin"][sub], Po[sub])
+    _check_siddon(ctx, g, Po[sub], rays["pout"][sub])
+
+
+def test_siddon_digest_matches_lists(ctx):
+    """The digest entry point hashes exactly the lists muto_siddon_csr returns."""
+    g = make_grid((40, 30, 20), (25.0, 10.0, 7.5), (-500.0, -150.0, -80.0))
+    p1, p2 = _segments_siddon_test(3000)
+    a1, a2 = _adversarial_segments(g)
+    p1, p2 = np.concatenate([p1, a1]), np.concatenate([p2, a2])
+    row, x, y, z, ln = ctx.siddon_csr(g, p1, p2)
+    cnt, dig = ctx.siddon_digest(g, p1, p2)
+    assert ctx.kernel_ms() > 0.0
+    assert np.array_equal(cnt.astype(np.uint64), np.diff(row))
+    mask = (1 << 64) - 1
+    lb = ln.view(np.uint64)
+    for i in range(0, len(p1), 37):
+        h = 0xcbf29ce484222325
+        for k in range(int(row[i]), int(row[i + 1])):
+            for w in (int(x[k]) | (int(y[k]) << 32), int(z[k]), int(lb[k])):
+                h = ((h ^ w) * 0x100000001b3) & mask
+        assert h == int(dig[i])
+
+
 def test_siddon_empty_and_outside(ctx):
     g = make_grid(10, 1.0, 0.0)
     p1 = np.array([[20.0, 20.0, 20.0], [-5.0, 5.0, 5.0], [5.0, 5.0, 5.0]])
@@ -210,6 +256,34 @@ def test_em_fp64_prepared(ctx, small_set, mode):
     else:
         assert np.array_equal(out.view(np.uint64), ref.view(np.uint64)), \
             f"max abs diff {np.max(np.abs(out - ref))}"
+
+
+@pytest.mark.parametrize("mode", ["median", "trimmed", "mean"])
+def test_em_fp64_wavefront_kernel(ctx, small_set, mode):
+    """The wavefront-per-track FP64 kernel (MUTO_OPT_FP64_KERNEL 1) sums each
+    covariance in a butterfly instead of the reference's running order, so
+    it agrees with the oracle to FP64 rounding: images within 1e-10 relative
+    after 4 iterations, costs within 1e-12."""
+    g, rays, _ = small_set
+    kw = {"median": dict(update_with_mean=0), "trimmed": dict(update_with_mean=1, mean_percentile=98),
+          "mean": dict(update_with_mean=1, mean_percentile=100)}[mode]
+    cfg = _em_cfg(g, **kw)
+    em = oracle.Em(cfg, rays)
+    plan = ctx.em_plan(cfg, prepared=em.prepared())
+    plan.set_option(M.MUTO_OPT_FP64_KERNEL, 1)
+    img = np.full(g.n_vox, 1e-7)
+    plan.set_image(img)
+    ref = img
+    costs = []
+    for _ in range(4):
+        ref, cost, _ns = em.iterate(ref)
+        costs.append(cost)
+    cg = plan.run(4)
+    out = plan.image()
+    plan.close()
+    np.testing.assert_allclose(cg, costs, rtol=1e-12)
+    np.testing.assert_allclose(out, ref, rtol=1e-10, atol=0)
+    assert np.array_equal(out == 0, ref == 0)
 
 
 def test_mlsd_one_shot(ctx, small_set):

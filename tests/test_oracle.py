@@ -236,6 +236,23 @@ def test_siddon_reference_test_golden_lists():
     assert checked >= 30
 
 
+def test_siddon_digest_is_fnv1a_of_the_lists():
+    """orc_siddon_digest = FNV-1a over (x | y << 32, z, length bits) of the
+    lists orc_siddon_csr returns, recomputed here in Python integers."""
+    fx, g, p1, p2 = _golden_siddon_test()
+    p1, p2 = p1[:2000], p2[:2000]
+    row, x, y, z, ln = oracle.siddon_csr(g, p1, p2)
+    cnt, dig = oracle.siddon_digest(g, p1, p2)
+    assert np.array_equal(cnt.astype(np.uint64), np.diff(row))
+    lb = ln.view(np.uint64)
+    for i in range(len(p1)):
+        h = 0xcbf29ce484222325
+        for k in range(int(row[i]), int(row[i + 1])):
+            for w in (int(x[k]) | (int(y[k]) << 32), int(z[k]), int(lb[k])):
+                h = ((h ^ w) * 0x100000001b3) & ((1 << 64) - 1)
+        assert h == int(dig[i])
+
+
 # ---------------------------------------------------------------- PoCA
 def _ray(pin, din, pout, dout, e=3.0):
     r = np.zeros(1, M.RAY_DTYPE)
