@@ -104,23 +104,31 @@ k_compact(GridD g, const muto_ray* __restrict__ rays, size_t n,
 }
 
 // Merged two-leg path of each selected track (MutoMLSD.cpp:382-415).
+// FILL = false: entry counts only.  FILL = true: voxel indices and lengths.
+// Each lane walks its own track; the entries it emits are staged in LDS
+// and written PATH_STAGE at a time, so the lines of a lane's list are
+// written in short bursts instead of one word per walk step (the lists of
+// the 64 lanes of a wavefront lie far apart).
+constexpr int PATH_STAGE = 16;
 template <bool FILL>
 __global__ void __launch_bounds__(256)
 k_paths(GridD g, const muto_ray* __restrict__ rays,
         const uint32_t* __restrict__ sel, const double* __restrict__ sel_p,
         size_t n_trk, int straight, uint64_t* __restrict__ counts,
         const uint64_t* __restrict__ row_ptr, uint32_t* __restrict__ vox,
-        double* __restrict__ len, double* __restrict__ tail,
-        uint32_t* __restrict__ mj, unsigned long long* __restrict__ cnt) {
+        double* __restrict__ len, unsigned long long* __restrict__ cnt) {
+  // entry-major staging (slot n of lane t at n * 256 + t: no bank conflicts
+  // when the lanes emit in step)
+  __shared__ uint32_t st_j[FILL ? PATH_STAGE * 256 : 1];
+  __shared__ double st_l[FILL ? PATH_STAGE * 256 : 1];
+  const uint32_t tid = threadIdx.x;
   size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool empty = false;
   uint32_t nout = 0;
   if (s < n_trk) {
     muto_ray r;
     load_ray(rays, sel[s], r);
     const double P[3] = {sel_p[3 * s], sel_p[3 * s + 1], sel_p[3 * s + 2]};
     const uint32_t nx = g.n[0], nxy = g.n[0] * g.n[1];
-    const size_t nv = (size_t)g.n[0] * g.n[1] * g.n[2];
     if (!FILL) {
       uint64_t c;
       if (straight) {
@@ -133,13 +141,23 @@ k_paths(GridD g, const muto_ray* __restrict__ rays,
       counts[s] = c;
     } else {
       const uint64_t b = row_ptr[s], e = row_ptr[s + 1];
-      uint64_t o = b;
+      uint64_t o = b;  // list position of the first staged entry
+      int n = 0;       // staged entries
+      auto flush = [&]() {
+        for (int k = 0; k < n; ++k) {
+          if (o + k < e) {
+            vox[o + k] = st_j[k * 256 + tid];
+            len[o + k] = st_l[k * 256 + tid];
+          }
+        }
+        o += n;
+        n = 0;
+      };
       auto put = [&](uint32_t x, uint32_t y, uint32_t z, double L) {
         if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) ++nout;
-        const uint32_t j = x + y * nx + z * nxy;  // xyz2index, u32 arithmetic
-        vox[o] = j;
-        len[o] = L;
-        ++o;
+        st_j[n * 256 + tid] = x + y * nx + z * nxy;  // xyz2index, u32 arithmetic
+        st_l[n * 256 + tid] = L;
+        if (++n == PATH_STAGE) flush();
       };
       if (straight) {
         siddon_walk(g, r.pin, r.pout, put);
@@ -149,31 +167,34 @@ k_paths(GridD g, const muto_ray* __restrict__ rays,
         siddon_walk(g, P, r.pout, [&](uint32_t x, uint32_t y, uint32_t z, double L) {
           if (first) {
             first = false;
-            if (ki) len[b + ki - 1] += L;  // in.back().length += out[0].length
+            // in.back().length += out[0].length: the in-leg's last entry is
+            // the last one staged, or already written when the stage was
+            // just flushed
+            if (ki) {
+              if (n > 0)
+                st_l[(n - 1) * 256 + tid] += L;
+              else if (o > b)
+                len[o - 1] += L;
+            }
             return;
           }
           put(x, y, z, L);
         });
       }
-      empty = (e == b);
-      if (e > b) {
-        tail[e - 1] = 0.0;
-        for (uint64_t k = e - 1; k > b; --k) tail[k - 1] = tail[k] + len[k];
-        for (uint64_t k = b; k < e; ++k)
-          if (vox[k] < nv) atomicAdd(&mj[vox[k]], 1u);
-      }
+      flush();
     }
   }
-  if (FILL) {
-    wave_count(&cnt[C_EMPTY], empty);
-    if (nout) atomicAdd(&cnt[C_OUT], (unsigned long long)nout);
-  }
+  if (FILL && nout) atomicAdd(&cnt[C_OUT], (unsigned long long)nout);
 }
 
+// Tails T_k = sum of the lengths after entry k (MutoMLSD.cpp:403-414, the
+// reference's backward running sum, same order), one lane per track, read
+// and written TAIL_BURST entries at a time so that a lane's loads and
+// stores of one burst share lines.
+constexpr int TAIL_BURST = 16;
 __global__ void __launch_bounds__(256)
 k_tails(size_t n_trk, const uint64_t* __restrict__ row_ptr,
-        const uint32_t* __restrict__ vox, const double* __restrict__ len,
-        double* __restrict__ tail, uint32_t* __restrict__ mj, size_t nv,
+        const double* __restrict__ len, double* __restrict__ tail,
         unsigned long long* __restrict__ cnt) {
   size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool empty = false;
@@ -181,13 +202,41 @@ k_tails(size_t n_trk, const uint64_t* __restrict__ row_ptr,
     const uint64_t b = row_ptr[s], e = row_ptr[s + 1];
     empty = (e == b);
     if (e > b) {
+      double run = 0.0;  // tail[k] while k walks down
       tail[e - 1] = 0.0;
-      for (uint64_t k = e - 1; k > b; --k) tail[k - 1] = tail[k] + len[k];
-      for (uint64_t k = b; k < e; ++k)
-        if (vox[k] < nv) atomicAdd(&mj[vox[k]], 1u);
+      uint64_t k = e - 1;  // entries k-1 .. b still to write
+      while (k - b >= (uint64_t)TAIL_BURST) {
+        double l[TAIL_BURST];
+#pragma unroll
+        for (int m = 0; m < TAIL_BURST; ++m) l[m] = len[k - m];
+        double t[TAIL_BURST];
+#pragma unroll
+        for (int m = 0; m < TAIL_BURST; ++m) {
+          run = run + l[m];  // tail[k-m-1] = tail[k-m] + len[k-m]
+          t[m] = run;
+        }
+#pragma unroll
+        for (int m = 0; m < TAIL_BURST; ++m) tail[k - m - 1] = t[m];
+        k -= TAIL_BURST;
+      }
+      for (; k > b; --k) {
+        run = run + len[k];
+        tail[k - 1] = run;
+      }
     }
   }
   wave_count(&cnt[C_EMPTY], empty);
+}
+
+// Entries per voxel, one thread per entry (entries outside the grid are
+// not counted).
+__global__ void __launch_bounds__(256)
+k_mj_count(uint64_t nnz, const uint32_t* __restrict__ vox, size_t nv,
+           uint32_t* __restrict__ mj) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const uint32_t j = vox[k];
+  if (j < nv) atomicAdd(&mj[j], 1u);
 }
 
 // Per-voxel entry counts and the empty / out-of-grid counters from a walk
@@ -714,7 +763,7 @@ extern "C" int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
     MUTO_CHECK_HIP(ctx, counts.alloc(nsel + 1));
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(counts.p, 0, (nsel + 1) * sizeof(uint64_t), s));
     k_paths<false><<<grid_for(nsel, 256), 256, 0, s>>>(pl->g, dr.p, sel.p, sel_p.p, nsel,
-        cfg->straight_line, counts.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+        cfg->straight_line, counts.p, nullptr, nullptr, nullptr, nullptr);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     MUTO_CHECK_HIP(ctx, pl->row_ptr.alloc(nsel + 1));
     MUTO_CHECK_HIP(ctx, exclusive_scan_u64(counts.p, pl->row_ptr.p, nsel + 1, s));
@@ -732,9 +781,15 @@ extern "C" int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
     MUTO_CHECK_HIP(ctx, pl->len.alloc(pl->nnz));
     MUTO_CHECK_HIP(ctx, pl->tail.alloc(pl->nnz));
     k_paths<true><<<grid_for(nsel, 256), 256, 0, s>>>(pl->g, dr.p, sel.p, sel_p.p, nsel,
-        cfg->straight_line, nullptr, pl->row_ptr.p, pl->vox.p, pl->len.p, pl->tail.p, pl->mj.p,
-        pl->dcount.p);
+        cfg->straight_line, nullptr, pl->row_ptr.p, pl->vox.p, pl->len.p, pl->dcount.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
+    k_tails<<<grid_for(nsel, 256), 256, 0, s>>>(nsel, pl->row_ptr.p, pl->len.p, pl->tail.p,
+                                                pl->dcount.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+    if (pl->nnz) {
+      k_mj_count<<<grid_for(pl->nnz, 256), 256, 0, s>>>(pl->nnz, pl->vox.p, pl->nv, pl->mj.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    }
     MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
     }
   }
@@ -791,9 +846,13 @@ extern "C" int muto_em_plan_create_prepared(
     MUTO_CHECK_HIP(ctx, copy_sync(ctx, pl->len.p, len, nnz * sizeof(double), hipMemcpyHostToDevice));
   }
   if (n_trk) {
-    k_tails<<<grid_for(n_trk, 256), 256, 0, s>>>(n_trk, pl->row_ptr.p, pl->vox.p, pl->len.p,
-                                                 pl->tail.p, pl->mj.p, pl->nv, pl->dcount.p);
+    k_tails<<<grid_for(n_trk, 256), 256, 0, s>>>(n_trk, pl->row_ptr.p, pl->len.p, pl->tail.p,
+                                                 pl->dcount.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
+    if (nnz) {
+      k_mj_count<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, pl->vox.p, pl->nv, pl->mj.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    }
   }
   MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
   if (int rc2 = muto_read_counters(pl)) { delete pl; return rc2; }

@@ -69,8 +69,9 @@ namespace {
 // entries per thread
 constexpr int GE = 16;                   // entries per thread and chunk
 constexpr uint32_t DIR_BINS = 16;        // track-order direction bins per unit slope
-// backprojection LDS sums (ACC 1) are int64 fixed point: an entry is at
-// most 2^FX_BITS and a task holds at most MAX_TASK_SEG segments
+// backprojection sums (ACC 1) are int64 fixed point: an entry is at most
+// 2^fx_bits, fx_bits = min(FX_BITS, 62 - bits(max M_j)) per plan; a task
+// holds at most MAX_TASK_SEG segments
 constexpr int FX_BITS = 46;
 constexpr uint32_t MAX_TASK_SEG = 1u << 16;
 // static LDS of both passes is a multiple of 16 bytes, so the dynamic region
@@ -84,11 +85,48 @@ static_assert(fwd_static_lds(256) % 16 == 0 && fwd_static_lds(512) % 16 == 0 &&
 static_assert(bwd_static_lds(256) % 16 == 0 && bwd_static_lds(512) % 16 == 0 &&
               bwd_static_lds(1024) % 16 == 0, "backward static LDS");
 
+// Unsigned division by a divisor fixed for a launch: with l = ceil(log2 d)
+// and m = floor(2^32 (2^l - d) / d) + 1, n / d = (umulhi(n, m) + n) >> l for
+// every 32-bit n (the sum taken in 64 bits).  Four VALU operations instead
+// of the ~30 of a general 32-bit division.
+struct FastDiv {
+  uint32_t d = 1, m = 1, l = 0;
+  static FastDiv make(uint32_t d) {
+    FastDiv f;
+    f.d = d;
+    f.l = 0;
+    while (f.l < 32 && (1ull << f.l) < d) ++f.l;
+    f.m = (uint32_t)(((((1ull << f.l) - d) << 32) / d) + 1);
+    return f;
+  }
+  __device__ __forceinline__ uint32_t div(uint32_t n) const {
+    const uint32_t t = __umulhi(n, m);
+    return (uint32_t)(((uint64_t)t + n) >> l);
+  }
+};
+
 struct BrickGeo {
   uint32_t n[3];   // grid
   uint32_t b[3];   // brick shape
   uint32_t nb[3];  // bricks per axis
+  FastDiv dn0, dn1, db0, db1, db2;  // by n[0], n[1], b[0], b[1], b[2]
 };
+
+// Coordinates of voxel j (x fastest), its brick and its index in the brick.
+struct Vox {
+  uint32_t x, y, z, brick, local;
+};
+__device__ __forceinline__ Vox locate(const BrickGeo& G, uint32_t j) {
+  Vox v;
+  const uint32_t t = G.dn0.div(j);
+  v.x = j - t * G.n[0];
+  v.z = G.dn1.div(t);
+  v.y = t - v.z * G.n[1];
+  const uint32_t bx = G.db0.div(v.x), by = G.db1.div(v.y), bz = G.db2.div(v.z);
+  v.local = (v.x - bx * G.b[0]) + G.b[0] * ((v.y - by * G.b[1]) + G.b[1] * (v.z - bz * G.b[2]));
+  v.brick = bx + G.nb[0] * (by + G.nb[1] * bz);
+  return v;
+}
 
 // Packed track-voxel entry of the FP32C stream (4 bytes).
 //   bits 28..0: the path length L in units of U (a power of two above every
@@ -125,30 +163,12 @@ struct EntCode {
   }
 };
 
-__device__ __forceinline__ uint32_t brick_of(const BrickGeo& G, uint32_t j,
-                                             uint32_t* local) {
-  const uint32_t x = j % G.n[0];
-  const uint32_t t = j / G.n[0];
-  const uint32_t y = t % G.n[1];
-  const uint32_t z = t / G.n[1];
-  const uint32_t bx = x / G.b[0], by = y / G.b[1], bz = z / G.b[2];
-  if (local)
-    *local = (x - bx * G.b[0]) + G.b[0] * ((y - by * G.b[1]) + G.b[1] * (z - bz * G.b[2]));
-  return bx + G.nb[0] * (by + G.nb[1] * bz);
-}
-
-// Whether voxel j follows voxel pj of the same track inside one segment:
-// same brick and a step of at most one voxel along each axis (the step is
-// what the packed entry stores).
-// a track's entry continues the segment of the entry before when both lie
+// A track's entry continues the segment of the entry before when both lie
 // in one brick and the step between them moves along at most one axis by
-// one voxel (the steps a packed entry can hold)
-__device__ __forceinline__ bool seg_continues(const BrickGeo& G, uint32_t pj, uint32_t j) {
-  if (pj == 0xFFFFFFFFu) return false;
-  if (brick_of(G, pj, nullptr) != brick_of(G, j, nullptr)) return false;
-  const int px = (int)(pj % G.n[0]), py = (int)((pj / G.n[0]) % G.n[1]), pz = (int)(pj / G.n[0] / G.n[1]);
-  const int x = (int)(j % G.n[0]), y = (int)((j / G.n[0]) % G.n[1]), z = (int)(j / G.n[0] / G.n[1]);
-  return abs(x - px) + abs(y - py) + abs(z - pz) <= 1;
+// one voxel (the steps a packed entry can hold).
+__device__ __forceinline__ bool seg_continues(const Vox& p, const Vox& c) {
+  if (p.brick != c.brick) return false;
+  return abs((int)c.x - (int)p.x) + abs((int)c.y - (int)p.y) + abs((int)c.z - (int)p.z) <= 1;
 }
 
 __device__ __forceinline__ uint32_t spread12(uint32_t v) {
@@ -175,23 +195,24 @@ k_seg_count(BrickGeo G, size_t n_trk, size_t nv,
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_trk) return;
   const uint64_t b = row_ptr[t], e = row_ptr[t + 1];
-  uint32_t cnt = 0, prev = 0xFFFFFFFFu, jf = 0, jl = 0;
-  bool have = false;
+  uint32_t cnt = 0;
+  bool have = false, pvalid = false;
+  Vox pv{}, vf{}, vl{};
   for (uint64_t k = b; k < e; ++k) {
     const uint32_t j = vox[k];
-    if (j >= nv) { prev = 0xFFFFFFFFu; continue; }
-    if (!seg_continues(G, prev, j)) ++cnt;
-    if (!have) { jf = j; have = true; }
-    jl = j;
-    prev = j;
+    if (j >= nv) { pvalid = false; continue; }
+    const Vox cv = locate(G, j);
+    if (!(pvalid && seg_continues(pv, cv))) ++cnt;
+    if (!have) { vf = cv; have = true; }
+    vl = cv;
+    pv = cv;
+    pvalid = true;
   }
   nseg[t] = cnt;
   val[t] = (uint32_t)t;
   if (!have) { key[t] = ~0ull; return; }
-  const float x0 = (float)(jf % G.n[0]), y0 = (float)((jf / G.n[0]) % G.n[1]);
-  const float z0 = (float)(jf / G.n[0] / G.n[1]);
-  const float x1 = (float)(jl % G.n[0]), y1 = (float)((jl / G.n[0]) % G.n[1]);
-  const float z1 = (float)(jl / G.n[0] / G.n[1]);
+  const float x0 = (float)vf.x, y0 = (float)vf.y, z0 = (float)vf.z;
+  const float x1 = (float)vl.x, y1 = (float)vl.y, z1 = (float)vl.z;
   const float dz = fmaxf(fabsf(z1 - z0), 1.0f);
   const float sx = fminf(fmaxf((x1 - x0) / dz, -2.0f), 2.0f);
   const float sy = fminf(fmaxf((y1 - y0) / dz, -2.0f), 2.0f);
@@ -228,25 +249,38 @@ k_seg_fill(BrickGeo G, size_t n_trk, size_t nv,
   if (t >= n_trk) return;
   const uint64_t b = row_ptr[t], e = row_ptr[t + 1];
   uint64_t s = sstart[t];
-  uint32_t prev = 0xFFFFFFFFu;
   const uint32_t r = rank[t];
-  uint64_t cur = ~0ull;
+  uint64_t cur = ~0ull, klast = 0;
+  uint32_t clen = 0;
+  bool pvalid = false;
+  Vox pv{};
+  // a segment's length and the tail after its last entry are written once,
+  // when the segment closes
+  auto close = [&]() {
+    if (cur != ~0ull) {
+      st_len[cur] = clen;
+      st_tend[cur] = tail[klast];
+    }
+  };
   for (uint64_t k = b; k < e; ++k) {
     const uint32_t j = vox[k];
-    if (j >= nv) { prev = 0xFFFFFFFFu; continue; }
-    const uint32_t br = brick_of(G, j, nullptr);
-    if (!seg_continues(G, prev, j)) {
+    if (j >= nv) { pvalid = false; continue; }
+    const Vox cv = locate(G, j);
+    if (!(pvalid && seg_continues(pv, cv))) {
+      close();
       cur = s++;
-      skey[cur] = ((uint64_t)br << 32) | r;
+      skey[cur] = ((uint64_t)cv.brick << 32) | r;
       sval[cur] = (uint32_t)cur;
       st_start[cur] = k;
-      st_len[cur] = 0;
       st_trk[cur] = (uint32_t)t;
+      clen = 0;
     }
-    st_len[cur] += 1;
-    st_tend[cur] = tail[k];
-    prev = j;
+    ++clen;
+    klast = k;
+    pv = cv;
+    pvalid = true;
   }
+  close();
 }
 
 // lengths in brick-major order and the first segment of each brick
@@ -326,47 +360,63 @@ k_group_desc(const Chunk* __restrict__ chunks, const uint64_t* __restrict__ seg_
   gv0[gi] = (uint16_t)v;
 }
 
-__global__ void __launch_bounds__(256)
-k_ent_scatter(BrickGeo G, uint64_t n_seg, const uint32_t* __restrict__ sval,
-              const uint64_t* __restrict__ st_start,
-              const uint32_t* __restrict__ st_len,
-              const uint64_t* __restrict__ seg_off,
-              const uint32_t* __restrict__ vox, const double* __restrict__ len, double inv_unit,
-              uint32_t* __restrict__ ent, uint16_t* __restrict__ seg_v0,
-              unsigned long long* __restrict__ n_clamped) {
-  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n_seg) return;
-  const uint32_t s = sval[q];
-  const uint64_t src = st_start[s];
-  const uint32_t n = st_len[s];
-  const uint64_t dst = seg_off[q];
-  // entries with j >= nv end segments, so a segment is contiguous in the
-  // track-major list; its first voxel goes to seg_v0, every later entry
-  // stores its step from the one before
-  uint32_t pj = 0;
-  for (uint32_t k = 0; k < n; ++k) {
-    const uint32_t j = vox[src + k];
+// Packed entries of one chunk per block, one thread per entry (strided by
+// the block size, so the loads of a wavefront cover a few whole segments and
+// the stores are contiguous): the chunk's segments are listed in LDS with
+// their source position in the track-major lists, and each entry finds its
+// segment through an entry -> segment table.
+template <uint32_t BS>
+__global__ void __launch_bounds__(BS)
+k_ent_fill(BrickGeo G, const Chunk* __restrict__ chunks, const uint32_t* __restrict__ sval,
+           const uint64_t* __restrict__ st_start, const uint64_t* __restrict__ seg_off_p,
+           const uint32_t* __restrict__ vox, const double* __restrict__ len, double inv_unit,
+           uint32_t* __restrict__ ent, uint16_t* __restrict__ seg_v0,
+           unsigned long long* __restrict__ n_clamped) {
+  __shared__ uint64_t s_src[BS];
+  __shared__ uint32_t s_off[BS];
+  __shared__ uint16_t s_id[GE * BS];
+  const Chunk ch = chunks[blockIdx.x];
+  const uint32_t t = threadIdx.x;
+  if (t < ch.cnt) {
+    const uint64_t q = ch.q0 + t;
+    const uint32_t off = (uint32_t)(seg_off_p[q] - ch.base);
+    const uint32_t end = t + 1 < ch.cnt ? (uint32_t)(seg_off_p[q + 1] - ch.base) : ch.ne;
+    s_src[t] = st_start[sval[q]];
+    s_off[t] = off;
+    for (uint32_t e = off; e < end; ++e) s_id[e] = (uint16_t)t;
+  }
+  __syncthreads();
+  const int64_t nx = G.n[0], nxy = (int64_t)G.n[0] * G.n[1];
+  uint32_t clamped = 0;
+  for (uint32_t e = t; e < ch.ne; e += BS) {
+    const uint32_t i = s_id[e];
+    const uint32_t k = e - s_off[i];
+    const uint64_t src = s_src[i] + k;
+    const uint32_t j = vox[src];
     uint32_t code = 0;
     if (k == 0) {
-      uint32_t loc;
-      brick_of(G, j, &loc);
-      seg_v0[q] = (uint16_t)loc;
-    } else {  // a unit step along one axis (segments break on any other)
-      const int dx = (int)(j % G.n[0]) - (int)(pj % G.n[0]);
-      const int dy = (int)((j / G.n[0]) % G.n[1]) - (int)((pj / G.n[0]) % G.n[1]);
-      const int dz = (int)(j / G.n[0] / G.n[1]) - (int)(pj / G.n[0] / G.n[1]);
-      code = dx > 0 ? 1u : dx < 0 ? 2u : dy > 0 ? 3u : dy < 0 ? 4u : dz > 0 ? 5u : dz < 0 ? 6u : 0u;
+      // a segment's first voxel goes to seg_v0; every later entry stores
+      // its step from the one before
+      seg_v0[ch.q0 + i] = (uint16_t)locate(G, j).local;
+    } else {
+      // inside a segment the step moves along at most one axis by one voxel
+      // inside one brick, so the index difference names it: +-nx*ny is z,
+      // else +-nx is y, else +-1 is x (an axis of one voxel never steps, so
+      // the order settles nx = 1 or ny = 1)
+      const int64_t dd = (int64_t)j - (int64_t)vox[src - 1];
+      code = dd == nxy ? 5u : dd == -nxy ? 6u : dd == nx ? 3u : dd == -nx ? 4u
+           : dd == 1 ? 1u : dd == -1 ? 2u : 0u;
     }
-    float lu = (float)(len[src + k] * inv_unit);  // L / U rounded to FP32
+    float lu = (float)(len[src] * inv_unit);  // L / U rounded to FP32
     if (!(lu >= 0.0f && lu < 1.0f)) {  // outside the packed range: reported
-      atomicAdd(n_clamped, 1ull);
+      ++clamped;
       lu = lu > 0.0f ? 0.99999994f : 0.0f;
     }
     const uint32_t bits = __float_as_uint(lu);
     const uint32_t pay = bits >= ENT_BASE ? bits - ENT_BASE : 0u;
-    ent[dst + k] = code << 29 | pay;
-    pj = j;
+    ent[ch.base + e] = code << 29 | pay;
   }
+  if (clamped) atomicAdd(n_clamped, (unsigned long long)clamped);
 }
 
 // Brick-major position of each track's segments, listed in rank order.
@@ -428,12 +478,22 @@ __device__ __forceinline__ void two_sum_acc(float& hi, float& lo, float x) {
   lo += e;
 }
 
+// Fixed-point scale of the backprojection sums: 2^fx_bits / (the bound on
+// |c_ij| the track kernel found this pass), so every scaled entry is below
+// 2^fx_bits.  fx_bits is set per plan so that M_j entries of one voxel stay
+// inside int64.
+__device__ __forceinline__ double fx_scale(const unsigned long long* cmax, int fx_bits) {
+  const double cb = __longlong_as_double((long long)cmax[0]);
+  return cb > 0.0 ? ldexp(1.0, fx_bits) / cb : 1.0;
+}
+
 struct IterArgs {
   BrickGeo G;
   const Task* tasks;
   const Chunk* chunks;
   uint32_t n_tasks;
   unsigned int* work;
+  int fx_bits;
   uint64_t n_seg;
   const uint32_t* seg_tfx;
   const uint32_t* gdesc;
@@ -463,8 +523,8 @@ __device__ __forceinline__ void load_brick(const BrickGeo& G, uint32_t brick,
   brick_origin(G, brick, o);
   const uint32_t vol = G.b[0] * G.b[1] * G.b[2];
   for (uint32_t v = threadIdx.x; v < vol; v += blockDim.x) {
-    const uint32_t lx = v % G.b[0], r = v / G.b[0];
-    const uint32_t ly = r % G.b[1], lz = r / G.b[1];
+    const uint32_t r = G.db0.div(v), lx = v - r * G.b[0];
+    const uint32_t lz = G.db1.div(r), ly = r - lz * G.b[1];
     const uint32_t x = o[0] + lx, y = o[1] + ly, z = o[2] + lz;
     float val = 0.0f;
     if (x < G.n[0] && y < G.n[1] && z < G.n[2])
@@ -841,12 +901,10 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
   float* hi_s = lds_acc;
   float* lo_s = lds_acc + vol;
   long long* d_s = reinterpret_cast<long long*>(lds_dyn);
-  // fixed-point scale of the LDS sums: every |c_ij| is below the bound the
-  // track kernel found, so a scaled entry is below 2^FX_BITS and a task's
-  // sum per voxel (at most one entry per segment) stays inside int64
-  const double cb = __longlong_as_double((long long)cmax[0]);
-  const double fx = cb > 0.0 ? ldexp(1.0, FX_BITS) / cb : 1.0;
-  const double fx_inv = 1.0 / fx;
+  // fixed-point scale of the sums: every |c_ij| is below the bound the
+  // track kernel found, so a scaled entry is at most 2^fx_bits, and the
+  // M_j entries of a voxel sum inside int64
+  const double fx = fx_scale(cmax, a.fx_bits);
   for (;;) {
     if (t == 0) sh = atomicAdd(&a.work[1], 1u);
     for (uint32_t v = t; v < 2 * vol; v += blockDim.x) lds_acc[v] = 0.0f;
@@ -911,17 +969,31 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
     __syncthreads();
     uint32_t o[3];
     brick_origin(a.G, tk.brick, o);
+    // Flush.  ACC 1: the sums stay int64 fixed point in the accumulator
+    // words; a task that covers its brick stores them, tasks sharing a brick
+    // add them (exact, so the result does not depend on the order the tasks
+    // ran in) and the voxel update turns them into FP64.  ACC 0: FP64 sums,
+    // stored or added.
+    const bool shared = tk.ntask > 1;
+    long long* acc_fx = reinterpret_cast<long long*>(acc);
     for (uint32_t v = t; v < vol; v += blockDim.x) {
-      const uint32_t lx = v % a.G.b[0], rr = v / a.G.b[0];
-      const uint32_t ly = rr % a.G.b[1], lz = rr / a.G.b[1];
+      const uint32_t rr = a.G.db0.div(v), lx = v - rr * a.G.b[0];
+      const uint32_t lz = a.G.db1.div(rr), ly = rr - lz * a.G.b[1];
       const uint32_t x = o[0] + lx, y = o[1] + ly, z = o[2] + lz;
       if (x >= a.G.n[0] || y >= a.G.n[1] || z >= a.G.n[2]) continue;
       const size_t j = (size_t)x + (size_t)a.G.n[0] * ((size_t)y + (size_t)a.G.n[1] * z);
-      const double val = ACC == 1 ? (double)d_s[v] * fx_inv : (double)hi_s[v] + (double)lo_s[v];
-      if (tk.owner)
-        acc[j] = val;
-      else if (val != 0.0)
-        atomicAdd(&acc[j], val);
+      if (ACC == 1) {
+        if (!shared)
+          acc_fx[j] = d_s[v];
+        else if (d_s[v] != 0)
+          atomicAdd(reinterpret_cast<unsigned long long*>(&acc_fx[j]), (unsigned long long)d_s[v]);
+      } else {
+        const double val = (double)hi_s[v] + (double)lo_s[v];
+        if (!shared)
+          acc[j] = val;
+        else if (val != 0.0)
+          atomicAdd(&acc[j], val);
+      }
     }
     __syncthreads();
   }
@@ -936,13 +1008,16 @@ __global__ void __launch_bounds__(256)
 k_upd32(size_t nv, const uint32_t* __restrict__ mj, double* __restrict__ acc,
         double* __restrict__ lam, float* __restrict__ lam32,
         double* __restrict__ ml, unsigned int* __restrict__ work,
-        unsigned long long* __restrict__ cmax) {
+        const unsigned long long* __restrict__ cmax, int fixed, int fx_bits) {
   size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j == 0) { work[0] = 0; work[1] = 0; cmax[0] = 0ull; }
+  if (j == 0) { work[0] = 0; work[1] = 0; }
   if (j >= nv) return;
   const uint32_t m = mj[j];
   const double l = lam[j];
-  const double out = m == 0 ? 0.0 : l + l * l / (2.0 * (double)m) * acc[j];
+  // the backprojection's sums: int64 fixed point (ACC 1) or FP64
+  const double c = fixed ? (double)reinterpret_cast<const long long*>(acc)[j] * (1.0 / fx_scale(cmax, fx_bits))
+                         : acc[j];
+  const double out = m == 0 ? 0.0 : l + l * l / (2.0 * (double)m) * c;
   acc[j] = 0.0;
   if (ml) {
     ml[j] = out;
@@ -952,11 +1027,32 @@ k_upd32(size_t nv, const uint32_t* __restrict__ mj, double* __restrict__ acc,
   }
 }
 
+// int64 fixed-point backprojection sums to FP64, in place
+__global__ void k_fx_to_f64(size_t nv, double* __restrict__ acc,
+                            const unsigned long long* __restrict__ cmax, int fx_bits) {
+  size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nv) return;
+  const long long q = reinterpret_cast<const long long*>(acc)[j];
+  acc[j] = (double)q * (1.0 / fx_scale(cmax, fx_bits));
+}
+
 // clears the iteration's cost sum and per-iteration counters
 __global__ void k_iter_begin(uint32_t n_scans, double* __restrict__ cost_acc,
-                             unsigned long long* __restrict__ cnt) {
+                             unsigned long long* __restrict__ cnt,
+                             unsigned long long* __restrict__ cmax) {
   for (uint32_t k = threadIdx.x; k < n_scans; k += blockDim.x) cost_acc[k] = 0.0;
   if (threadIdx.x < 3) cnt[threadIdx.x] = 0ull;
+  if (threadIdx.x == 0) cmax[0] = 0ull;  // the track kernel sets this pass's |c| bound
+}
+
+// largest M_j
+__global__ void __launch_bounds__(256)
+k_max_u32(size_t n, const uint32_t* __restrict__ a, unsigned int* __restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t v = i < n ? a[i] : 0u;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  if ((threadIdx.x & 63) == 0 && v) atomicMax(out, v);
 }
 
 // appends the iteration's cost sums (one per scan) to the history
@@ -996,6 +1092,11 @@ static BrickGeo brick_geo(const muto_em_plan* pl) {
   G.b[1] = pl->f32.bs.by;
   G.b[2] = pl->f32.bs.bz;
   for (int k = 0; k < 3; ++k) G.nb[k] = (G.n[k] + G.b[k] - 1) / G.b[k];
+  G.dn0 = FastDiv::make(G.n[0]);
+  G.dn1 = FastDiv::make(G.n[1]);
+  G.db0 = FastDiv::make(G.b[0]);
+  G.db1 = FastDiv::make(G.b[1]);
+  G.db2 = FastDiv::make(G.b[2]);
   return G;
 }
 
@@ -1015,6 +1116,7 @@ static IterArgs iter_args(muto_em_plan* pl) {
   a.chunks = f.chunks.p;
   a.n_tasks = f.n_tasks;
   a.work = pl->work.p;
+  a.fx_bits = f.fx_bits;
   a.n_seg = f.n_seg;
   a.seg_tfx = f.seg_tfx.p;
   a.gdesc = f.gdesc.p;
@@ -1226,7 +1328,7 @@ int fp32c_build(muto_em_plan* pl) {
         tent.push_back(te);
       }
       for (Task& tk : bt) {
-        tk.owner = bt.size() == 1 ? 1u : 0u;
+        tk.ntask = (uint32_t)bt.size();
         tasks.push_back(tk);
       }
     }
@@ -1298,11 +1400,14 @@ int fp32c_build(muto_em_plan* pl) {
       DBuf<unsigned long long> ncl;
       MUTO_CHECK_HIP(ctx, ncl.alloc(1));
       MUTO_CHECK_HIP(ctx, hipMemsetAsync(ncl.p, 0, sizeof(unsigned long long), s));
-      if (n_seg) {
-        k_ent_scatter<<<grid_for(n_seg, 256), 256, 0, s>>>(G, n_seg, sval.p, st_start.p, st_len.p,
-                                                           seg_off_p.p, pl->vox.p, pl->len.p, inv_unit,
-                                                           f.ent.p, f.seg_v0.p, ncl.p);
-        MUTO_CHECK_HIP(ctx, hipGetLastError());
+      if (f.n_chunks) {
+        MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
+          constexpr uint32_t BS = decltype(B)::value;
+          k_ent_fill<BS><<<f.n_chunks, BS, 0, s>>>(G, f.chunks.p, sval.p, st_start.p, seg_off_p.p,
+                                                   pl->vox.p, pl->len.p, inv_unit, f.ent.p,
+                                                   f.seg_v0.p, ncl.p);
+          return hipGetLastError();
+        }));
       }
       unsigned long long nc = 0;
       MUTO_CHECK_HIP(ctx, copy_sync(ctx, &nc, ncl.p, sizeof nc, hipMemcpyDeviceToHost));
@@ -1351,6 +1456,18 @@ int fp32c_build(muto_em_plan* pl) {
   MUTO_CHECK_HIP(ctx, f.cmax.alloc(1));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.cmax.p, 0, sizeof(unsigned long long), s));
   {
+    // fixed-point bits of the backprojection sums: a voxel's M_j entries,
+    // each at most 2^fx_bits, must sum inside int64
+    DBuf<unsigned int> mx;
+    MUTO_CHECK_HIP(ctx, mx.alloc(1));
+    MUTO_CHECK_HIP(ctx, hipMemsetAsync(mx.p, 0, sizeof(unsigned int), s));
+    k_max_u32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, mx.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+    unsigned int max_mj = 0;
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, &max_mj, mx.p, sizeof max_mj, hipMemcpyDeviceToHost));
+    f.fx_bits = std::min(FX_BITS, 62 - bits_for(std::max<uint64_t>(max_mj, 1)));
+  }
+  {
     const double dv = sqrt(pl->g.d[0] * pl->g.d[0] + pl->g.d[1] * pl->g.d[1] + pl->g.d[2] * pl->g.d[2]);
     const double bx = f.bs.bx * pl->g.d[0], by = f.bs.by * pl->g.d[1], bz = f.bs.bz * pl->g.d[2];
     f.lmax = 2.0 * dv;  // a merged entry holds both legs
@@ -1375,7 +1492,7 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
   const IterArgs a = iter_args(pl);
   const ErrM E = error_matrix(pl->cfg);
   const bool map = pl->is_map;
-  k_iter_begin<<<1, 256, 0, s>>>(pl->n_scans, pl->cost_acc.p, pl->dcount.p);
+  k_iter_begin<<<1, 256, 0, s>>>(pl->n_scans, pl->cost_acc.p, pl->dcount.p, f.cmax.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[0], s));
   if (f.n_tasks)
@@ -1393,7 +1510,8 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[3], s));
   k_upd32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->acc64.p, pl->lam64.p,
                                                 pl->lam32.p, map ? pl->ml64.p : nullptr,
-                                                pl->work.p, f.cmax.p);
+                                                pl->work.p, f.cmax.p, f.acc_mode == 1 ? 1 : 0,
+                                                f.fx_bits);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (map) {
     if (int rc = map_step(pl, pl->lam64.p, pl->ml64.p, pl->tmp64.p)) return rc;
@@ -1535,6 +1653,10 @@ int fp32c_map_backward(muto_em_plan* pl, const float* g, const unsigned long lon
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(acc, 0, pl->nv * sizeof(double), s));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->work.p, 0, 4 * sizeof(unsigned int), s));
   if (f.n_tasks) MUTO_CHECK_HIP(ctx, launch_bwd(f, a, g, cmax, acc, s));
+  if (f.acc_mode == 1) {
+    k_fx_to_f64<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, acc, cmax, f.fx_bits);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  }
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->work.p, 0, 4 * sizeof(unsigned int), s));
   return MUTO_OK;
 }

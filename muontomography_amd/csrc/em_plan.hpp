@@ -40,11 +40,12 @@ struct Chunk {
 };
 
 // A unit of work of the persistent FP32C kernels: chunks [c0, c1) that all
-// lie in brick `brick`.  `owner` is set when the task covers the whole
-// brick (its accumulators are stored, not added).
+// lie in brick `brick`, which `ntask` tasks share (1: the task covers the
+// whole brick and stores its sums; else the tasks add their fixed-point sums
+// and the last one to finish converts them).
 struct Task {
   uint32_t brick;
-  uint32_t owner;
+  uint32_t ntask;
   uint32_t c0, c1;
 };
 
@@ -71,6 +72,7 @@ struct Fp32cData {
   DBuf<uint16_t> trk_scan;  // [n_trk] rank order: scan of each track (batches)
   double lmax = 0.0, ubound = 0.0;  // entry length bound; segment extent bound
   DBuf<Task> tasks;
+  int fx_bits = 46;  // backprojection sums: int64 fixed point, |c| bound scaled to 2^fx_bits
   DBuf<Chunk> chunks;
   uint32_t n_tasks = 0, n_chunks = 0;
   int grid_fwd = 0, grid_bwd = 0, block = 256;

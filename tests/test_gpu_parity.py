@@ -326,8 +326,8 @@ def test_fp32c_matches_fp64(ctx, small_set):
 def test_fp32c_graph_replays_match_eager(ctx, small_set):
     """The captured FP32C iteration graph, replayed one iteration at a time
     with host reads in between and then several iterations back to back,
-    gives the images and costs of the eager launches (up to the order of
-    floating-point atomics: rtol 1e-9)."""
+    gives the images of the eager launches bit for bit and their costs up
+    to the order of the FP64 cost sum (rtol 1e-9)."""
     g, rays, _ = small_set
     cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100)
     img = np.full(g.n_vox, 1e-7)
@@ -345,8 +345,31 @@ def test_fp32c_graph_replays_match_eager(ctx, small_set):
         out[graph] = steps
         plan.close()
     for (a_e, c_e), (a_g, c_g) in zip(out[0], out[1]):
-        np.testing.assert_allclose(a_g, a_e, rtol=1e-9, atol=1e-30)
+        # images: bit-identical (fixed-point sums); costs: FP64 atomic order
+        assert np.array_equal(a_g.view(np.uint64), a_e.view(np.uint64))
         np.testing.assert_allclose(c_g, c_e, rtol=1e-9)
+
+
+@pytest.mark.parametrize("brick", [(16, 16, 16, 256), (8, 8, 8, 256)])
+def test_fp32c_image_is_deterministic(ctx, small_set, brick):
+    """FP32C images are the same bits run to run, also where several tasks
+    share a brick (their int64 sums are combined exactly by the last one)."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100)
+    img = np.full(g.n_vox, 1e-7)
+    outs = []
+    for _ in range(2):
+        plan = ctx.em_plan(cfg, rays=rays)
+        plan.set_option(M.MUTO_OPT_BRICK, M.brick_code(brick))
+        plan.set_image(img)
+        plan.run(6, M.MUTO_FP32C)
+        outs.append(plan.image())
+        plan.set_image(img)
+        plan.run(6, M.MUTO_FP32C)
+        outs.append(plan.image())
+        plan.close()
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.uint64), outs[0].view(np.uint64))
 
 
 def test_fp32c_precision_em256(ctx):
