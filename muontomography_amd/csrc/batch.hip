@@ -174,6 +174,7 @@ extern "C" int muto_batch_create(muto_ctx* ctx, size_t n, muto_em_plan* const* p
     ctx->set_error("muto_batch_create: out of device memory");
     return fail(MUTO_ERR_NOMEM);
   }
+  // the combined grid's M_j is counted from its FP32C stream when that is built
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(g->mj.p, 0, g->nv * sizeof(uint32_t), st));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(g->dcount.p, 0, 8 * sizeof(unsigned long long), st));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(g->row_ptr.p, 0, sizeof(uint64_t), st));
@@ -197,9 +198,6 @@ extern "C" int muto_batch_create(muto_ctx* ctx, size_t n, muto_em_plan* const* p
                                                         (uint32_t)X, (uint32_t)Y, g->vox.p + eoff);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
-    k_image_block<true><<<grid_for(p->nv, 256), 256, 0, st>>>(p->nv, sg.nx, sg.ny, b->xoff[s], (uint32_t)X,
-                                                              (uint32_t)Y, nullptr, nullptr, p->mj.p, g->mj.p);
-    MUTO_CHECK_HIP(ctx, hipGetLastError());
     eoff += p->nnz;
   }
   MUTO_CHECK_HIP(ctx, hipStreamSynchronize(st));

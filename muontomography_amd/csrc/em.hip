@@ -775,6 +775,7 @@ extern "C" int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
       k_mj_walk<<<grid_for(nsel, 256), 256, 0, s>>>(pl->g, pl->geo.p, nsel, cfg->straight_line,
                                                     pl->mj.p, pl->nv, pl->dcount.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
+      pl->mj_ready = true;
       MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
     } else {
     MUTO_CHECK_HIP(ctx, pl->vox.alloc(pl->nnz));
@@ -786,10 +787,6 @@ extern "C" int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
     k_tails<<<grid_for(nsel, 256), 256, 0, s>>>(nsel, pl->row_ptr.p, pl->len.p, pl->tail.p,
                                                 pl->dcount.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
-    if (pl->nnz) {
-      k_mj_count<<<grid_for(pl->nnz, 256), 256, 0, s>>>(pl->nnz, pl->vox.p, pl->nv, pl->mj.p);
-      MUTO_CHECK_HIP(ctx, hipGetLastError());
-    }
     MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
     }
   }
@@ -849,10 +846,6 @@ extern "C" int muto_em_plan_create_prepared(
     k_tails<<<grid_for(n_trk, 256), 256, 0, s>>>(n_trk, pl->row_ptr.p, pl->len.p, pl->tail.p,
                                                  pl->dcount.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
-    if (nnz) {
-      k_mj_count<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, pl->vox.p, pl->nv, pl->mj.p);
-      MUTO_CHECK_HIP(ctx, hipGetLastError());
-    }
   }
   MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
   if (int rc2 = muto_read_counters(pl)) { delete pl; return rc2; }
@@ -904,8 +897,22 @@ extern "C" int muto_em_plan_get_image(muto_em_plan* pl, double* img) {
   return MUTO_OK;
 }
 
+namespace muto {
+int ensure_mj(muto_em_plan* pl) {
+  if (pl->mj_ready) return MUTO_OK;
+  muto_ctx* ctx = pl->ctx;
+  if (pl->nnz) {
+    k_mj_count<<<grid_for(pl->nnz, 256), 256, 0, ctx->stream>>>(pl->nnz, pl->vox.p, pl->nv, pl->mj.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  }
+  pl->mj_ready = true;
+  return MUTO_OK;
+}
+}  // namespace muto
+
 static int build_slots(muto_em_plan* pl) {
   if (pl->slots_built) return MUTO_OK;
+  if (int rc = ensure_mj(pl)) return rc;
   muto_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream;
   MUTO_CHECK_HIP(ctx, pl->col_ptr.alloc(pl->nv + 1));
@@ -1051,7 +1058,8 @@ static int plan_run(muto_em_plan* pl, int iters, int precision, double* cost) {
   }
   if (int rc = sync_image(pl, precision)) return rc;
   if (precision == MUTO_FP32C)
-    if (int rc = fp32c_build(pl)) return rc;
+    if (int rc = fp32c_build(pl)) return rc;  // counts mj from the brick stream
+  if (int rc = ensure_mj(pl)) return rc;
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->dcount.p, 0, 8 * sizeof(unsigned long long), s));
   std::vector<double> costs(iters, 0.0);
   double cost_prev = INFINITY;
@@ -1193,6 +1201,7 @@ extern "C" int muto_em_plan_counters(const muto_em_plan* pl, muto_counters* out)
 
 extern "C" int muto_em_plan_voxel_counts(muto_em_plan* pl, uint32_t* mj) {
   if (!pl || !mj) return MUTO_ERR_ARG;
+  if (int rc = ensure_mj(pl)) return rc;
   MUTO_CHECK_HIP(pl->ctx, copy_sync(pl->ctx, mj, pl->mj.p, pl->nv * sizeof(uint32_t), hipMemcpyDeviceToHost));
   return MUTO_OK;
 }

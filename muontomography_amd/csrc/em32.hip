@@ -1045,14 +1045,68 @@ __global__ void k_iter_begin(uint32_t n_scans, double* __restrict__ cost_acc,
   if (threadIdx.x == 0) cmax[0] = 0ull;  // the track kernel sets this pass's |c| bound
 }
 
-// largest M_j
+// largest M_j (a grid-stride loop per thread, one atomic per block)
 __global__ void __launch_bounds__(256)
 k_max_u32(size_t n, const uint32_t* __restrict__ a, unsigned int* __restrict__ out) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t v = i < n ? a[i] : 0u;
+  __shared__ uint32_t wm[4];
+  uint32_t v = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    v = max(v, a[i]);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
-  if ((threadIdx.x & 63) == 0 && v) atomicMax(out, v);
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = max(max(wm[0], wm[1]), max(wm[2], wm[3]));
+    if (v) atomicMax(out, v);
+  }
+}
+
+// Entries per voxel from the brick stream: one block per task counts its
+// entries per brick voxel in LDS (the voxel chain of the passes, padding
+// entries skipped), then stores the counts (a task that covers its brick)
+// or adds them (tasks sharing a brick; mj starts at zero).
+template <uint32_t BS>
+__global__ void __launch_bounds__(BS)
+k_count32(IterArgs a, uint32_t* __restrict__ mj) {
+  extern __shared__ __align__(16) double lds_dyn[];
+  uint32_t* cnt_s = reinterpret_cast<uint32_t*>(lds_dyn);
+  __shared__ uint32_t sv0[BS];
+  __shared__ int stab[16];
+  const uint32_t t = threadIdx.x;
+  fill_step_table(a, stab);
+  const uint32_t vol = a.G.b[0] * a.G.b[1] * a.G.b[2];
+  const Task tk = a.tasks[blockIdx.x];
+  for (uint32_t v = t; v < vol; v += BS) cnt_s[v] = 0u;
+  __syncthreads();
+  for (uint32_t c = tk.c0; c < tk.c1; ++c) {
+    const Chunk ch = a.chunks[c];
+    if (t < ch.cnt) sv0[t] = a.seg_v0[ch.q0 + t];
+    const bool act = t * GE < ch.ne;
+    Group gr;
+    if (act) gr.load(a, (ch.base / GE) + t);
+    __syncthreads();
+    if (act) {
+      const uint32_t e0 = t * GE;
+      gr.voxels(stab, [&](uint32_t q) { return sv0[q]; }, [&](int i, uint32_t v) {
+        if (e0 + (uint32_t)i < ch.ne) atomicAdd(&cnt_s[v], 1u);
+      });
+    }
+    __syncthreads();
+  }
+  uint32_t o[3];
+  brick_origin(a.G, tk.brick, o);
+  for (uint32_t v = t; v < vol; v += BS) {
+    const uint32_t rr = a.G.db0.div(v), lx = v - rr * a.G.b[0];
+    const uint32_t lz = a.G.db1.div(rr), ly = rr - lz * a.G.b[1];
+    const uint32_t x = o[0] + lx, y = o[1] + ly, z = o[2] + lz;
+    if (x >= a.G.n[0] || y >= a.G.n[1] || z >= a.G.n[2]) continue;
+    const size_t j = (size_t)x + (size_t)a.G.n[0] * ((size_t)y + (size_t)a.G.n[1] * z);
+    if (tk.ntask == 1)
+      mj[j] = cnt_s[v];
+    else if (cnt_s[v])
+      atomicAdd(&mj[j], cnt_s[v]);
+  }
 }
 
 // appends the iteration's cost sums (one per scan) to the history
@@ -1458,10 +1512,23 @@ int fp32c_build(muto_em_plan* pl) {
   {
     // fixed-point bits of the backprojection sums: a voxel's M_j entries,
     // each at most 2^fx_bits, must sum inside int64
+    if (!pl->mj_ready) {
+      // M_j from the stream (mj holds zeros until first counted)
+      if (f.n_tasks) {
+        const IterArgs a = iter_args(pl);
+        const size_t lds = (size_t)f.bs.vol() * sizeof(uint32_t);
+        MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
+          constexpr uint32_t BS = decltype(B)::value;
+          k_count32<BS><<<f.n_tasks, BS, lds, s>>>(a, pl->mj.p);
+          return hipGetLastError();
+        }));
+      }
+      pl->mj_ready = true;
+    }
     DBuf<unsigned int> mx;
     MUTO_CHECK_HIP(ctx, mx.alloc(1));
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(mx.p, 0, sizeof(unsigned int), s));
-    k_max_u32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, mx.p);
+    k_max_u32<<<1024, 256, 0, s>>>(pl->nv, pl->mj.p, mx.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     unsigned int max_mj = 0;
     MUTO_CHECK_HIP(ctx, copy_sync(ctx, &max_mj, mx.p, sizeof max_mj, hipMemcpyDeviceToHost));

@@ -136,6 +136,9 @@ struct muto_em_plan {
   uint32_t n_scans = 1;         // > 1: a batch of scans laid side by side in one grid
   muto::DBuf<uint16_t> trk_scan;  // [n_trk] scan of each track (batches only)
   bool has_lists = true;        // row_ptr/vox/len/tail hold the paths
+  // mj counted: by the FP32C stream build when that comes first (LDS counts
+  // per brick), else by one atomic per entry (ensure_mj)
+  bool mj_ready = false;
 
   // voxel-major slots for median / trimmed mean
   bool slots_built = false;
@@ -161,3 +164,8 @@ struct muto_em_plan {
   hipEvent_t ev[8] = {};
   double t_ms[5] = {0, 0, 0, 0, 0};
 };
+
+namespace muto {
+// counts entries per voxel into plan->mj unless already done
+int ensure_mj(muto_em_plan* pl);
+}  // namespace muto
