@@ -1,11 +1,15 @@
 """Benchmark: batched EM on the 256^3 x 10 mm grid with 1e7 muons (paths of
-80-600 voxels), FP32C (FP32 sums with two-sum compensation, FP64 2x2
-algebra) timed, the plain FP64 GPU path run beside it for the same number
-of iterations, image error of FP32C against FP64 reported.
+80-600 voxels).  FP32C (FP32 entry terms, FP64 and int64 fixed-point sums,
+FP64 2x2 algebra) is timed; the FP64 GPU paths (reference order, and one
+wavefront per track) run beside it for the same number of iterations, and
+the image error of FP32C against FP64 is reported.
 
-A step is one EM iteration over every track (forward covariance sums,
-per-track 2x2 algebra, backprojection, voxel update: the mean update of
-src/MutoMLSD.cpp:285-308).  value = track-voxel updates per second.
+A step is one EM iteration over every track: forward covariance sums,
+per-track 2x2 algebra, backprojection and the mean voxel update
+(src/MutoMLSD.cpp:179-213, 442-517).  value = track-voxel updates per second.
+
+Other workloads (--workload): scans64 (many-scan batch), map128 (MAP
+objective and gradient), siddon200 (plain PoCA + Siddon).
 
 Prints one JSON line.  One GPU only: --gpus above 1 exits.
 """

@@ -22,7 +22,7 @@
 // chunk's group t (16-byte vector loads) and walks it backwards, the tail of
 // each entry inside its segment following from the group's carry; every
 // thread does the same work, whatever the segment lengths.
-// Per iteration (src/MutoMLSD.cpp:285-308, mean update):
+// Per iteration (src/MutoMLSD.cpp:179-213, 442-517, mean update):
 //   k_fwd32   per segment: partial covariance sums sum lambda_j W_ij with
 //             lambda staged in LDS, W from (L, tail) in registers, in the
 //             segment's local tail basis;

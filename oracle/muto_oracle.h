@@ -74,7 +74,7 @@ int orc_em_from_prepared(const muto_grid* g, size_t n_trk, const double* trk4,
                          const double* p, const uint64_t* row_ptr,
                          const uint32_t* vox, const double* len, orc_em* em);
 void orc_em_free(orc_em* em);
-/* One ML/EM iteration (MutoMLSD.cpp:285-308): img_in -> img_out.
+/* One ML/EM iteration (MutoMLSD.cpp:179-213, 442-517): img_in -> img_out.
  * Returns the mean cost.  n_singular counts warnings. */
 double orc_em_iterate(const muto_mlsd_cfg* cfg, orc_em* em,
                       const double* img_in, double* img_out,
