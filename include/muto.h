@@ -169,6 +169,24 @@ int muto_poca_reconstruct(muto_ctx* ctx, const muto_poca_cfg* cfg, size_t n,
                           const muto_ray* rays, double* img,
                           muto_counters* counters);
 
+/* MutoPoCut::reconstruct (src/MutoPoCut.cpp:19-121): the PoCA weights of
+ * the two-leg path (cfg->straight_line is ignored: PoCut forces it off,
+ * :5,14), and per voxel the sum of the lowest (u32)(M * cut_percentile) of
+ * its M scattering values (sorted ascending, summed in order) over the
+ * weight.  A threshold above M is clamped to M (the reference reads past
+ * the list). */
+int muto_pocut_reconstruct(muto_ctx* ctx, const muto_poca_cfg* cfg,
+                           double cut_percentile, size_t n,
+                           const muto_ray* rays, double* img,
+                           muto_counters* counters);
+
+/* MutoSLR::reconstruct (src/MutoSLR.cpp:47-103): each track's theta^2 / 2
+ * spread along the straight chord between the grid's top and bottom planes
+ * (cfg->straight_line is not used). */
+int muto_slr_reconstruct(muto_ctx* ctx, const muto_poca_cfg* cfg, size_t n,
+                         const muto_ray* rays, double* img,
+                         muto_counters* counters);
+
 /* ------------------------------------------------------------------ */
 /* MLSD / MAP one-shot                                                 */
 /* ------------------------------------------------------------------ */

@@ -158,6 +158,25 @@ class Context:
                                                 ptr(img), C.byref(c)), "muto_poca_reconstruct")
         return img, c.as_dict()
 
+    def pocut_reconstruct(self, cfg, rays, cut_percentile=1.0):
+        """MutoPoCut: PoCA with the per-voxel cut of the largest angles."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        img = np.empty(cfg.grid.n_vox, np.float64)
+        c = Counters()
+        self._check(lib().muto_pocut_reconstruct(self._h, C.byref(cfg), float(cut_percentile),
+                                                 rays.shape[0], ptr(rays), ptr(img), C.byref(c)),
+                    "muto_pocut_reconstruct")
+        return img, c.as_dict()
+
+    def slr_reconstruct(self, cfg, rays):
+        """MutoSLR: straight-line reconstruction."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        img = np.empty(cfg.grid.n_vox, np.float64)
+        c = Counters()
+        self._check(lib().muto_slr_reconstruct(self._h, C.byref(cfg), rays.shape[0], ptr(rays),
+                                               ptr(img), C.byref(c)), "muto_slr_reconstruct")
+        return img, c.as_dict()
+
     # -- MLSD / MAP -----------------------------------------------------
     def _recon(self, fn, cfg, rays, init):
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)

@@ -142,6 +142,10 @@ PROTOS = {
                                    P, P, P, C.POINTER(Counters)]),
     "muto_poca_reconstruct": (C.c_int, [P, C.POINTER(PocaCfg), C.c_size_t, P, P,
                                         C.POINTER(Counters)]),
+    "muto_pocut_reconstruct": (C.c_int, [P, C.POINTER(PocaCfg), C.c_double, C.c_size_t, P, P,
+                                         C.POINTER(Counters)]),
+    "muto_slr_reconstruct": (C.c_int, [P, C.POINTER(PocaCfg), C.c_size_t, P, P,
+                                       C.POINTER(Counters)]),
     "muto_mlsd_reconstruct": (C.c_int, [P, C.POINTER(MlsdCfg), C.c_size_t, P, P, P, P,
                                         C.POINTER(Counters)]),
     "muto_map_reconstruct": (C.c_int, [P, C.POINTER(MlsdCfg), C.c_size_t, P, P, P, P,

@@ -1,4 +1,5 @@
-// The reference's reconstruction classes (MutoPoCA, MutoMLSD, MutoMAP) and
+// The reference's reconstruction classes (MutoPoCA, MutoPoCut, MutoSLR,
+// MutoMLSD, MutoMAP) and
 // file I/O (MutoFile) over the C ABI of libmuto: JSON configuration on the
 // host, every reconstruction on the GPU.
 #include <cstdint>
@@ -13,6 +14,8 @@
 #include "MutoMAP.h"
 #include "MutoMLSD.h"
 #include "MutoPoCA.h"
+#include "MutoPoCut.h"
+#include "MutoSLR.h"
 #include "muto.h"
 
 static_assert(sizeof(VoxelGrid) == sizeof(muto_grid), "VoxelGrid and muto_grid layouts differ");
@@ -120,6 +123,90 @@ void MutoPoCA::logResults() {  // src/MutoPoCA.cpp:176-183
   std::cout << "number of rays processed: " << fNTotal << std::endl;
   std::cout << "number of unparalleled rays: " << fNNotParallel << std::endl;
   std::cout << "number of interactions in voxels: " << fNScatInGrid << std::endl;
+}
+
+// ---------------------------------------------------------------- PoCut
+MutoPoCut::MutoPoCut() : MutoPoCA(), fPercentile(1.0) { fStraightLine = false; }
+
+MutoPoCut::MutoPoCut(json config) : MutoPoCA(config) {
+  fStraightLine = false;  // src/MutoPoCut.cpp:13
+  fPercentile = config.value("cut_percentile", 1.0);
+}
+
+MutoPoCut::~MutoPoCut() {}
+
+Image MutoPoCut::reconstruct(const MutoMuonData& rays) {
+  std::cout << "==================================================" << std::endl;
+  std::cout << "|| PoCA with threshold for scattering angles    ||" << std::endl;
+  std::cout << "==================================================" << std::endl;
+  open_ctx(fCtx, fDevice);
+  muto_poca_cfg c{};
+  c.grid = to_grid(fGrid);
+  c.length_weighted = fLengthWeighted;
+  c.straight_line = 0;
+  c.has_energy = fHasEnergy;
+  c.log_results = 0;
+  c.total_num_rays = fNTotal;
+  c.angle_unit_mag = fAngMag;
+  c.eps = fEPS;
+  Image img(fGrid.nx, fGrid.ny, fGrid.nz);
+  muto_counters cnt{};
+  check(fCtx, muto_pocut_reconstruct(fCtx, &c, fPercentile, rays.size(), as_rays(rays), img.data(), &cnt),
+        "MutoPoCut::reconstruct");
+  fNTotal = (int)cnt.n_total;
+  fNNotParallel = (int)cnt.n_not_parallel;
+  fNScatInGrid = (int)cnt.n_scat_in_grid;
+  if (fLogPrint) logResults();
+  return img;
+}
+
+// ---------------------------------------------------------------- SLR
+MutoSLR::MutoSLR()
+    : fLengthWeighted(false), fHasEnergy(false), fLogPrint(false), fNTotal(0), fAngMag(1.0),
+      fEPS(1.0e-8), fDevice(0), fCtx(nullptr) {
+  default_grid(fGrid);
+}
+
+MutoSLR::MutoSLR(json config) : MutoSLR() {  // src/MutoSLR.cpp:21-43
+  read_grid(config, fGrid);
+  fLengthWeighted = config.value("length_weighted", false);
+  fHasEnergy = config.value("has_energy", false);
+  fLogPrint = config.value("log_results", false);
+  fNTotal = config.value("total_num_rays", 0);
+  fAngMag = config.value("angle_unit_mag", 1.0);
+  fEPS = config.value("eps", 1.0e-8);
+  fDevice = config.value("device", 0);
+}
+
+MutoSLR::~MutoSLR() {
+  if (fCtx) muto_ctx_destroy(fCtx);
+}
+
+Image MutoSLR::reconstruct(const MutoMuonData& rays) {
+  open_ctx(fCtx, fDevice);
+  muto_poca_cfg c{};
+  c.grid = to_grid(fGrid);
+  c.length_weighted = fLengthWeighted;
+  c.straight_line = 1;
+  c.has_energy = fHasEnergy;
+  c.log_results = 0;
+  c.total_num_rays = fNTotal;
+  c.angle_unit_mag = fAngMag;
+  c.eps = fEPS;
+  Image img(fGrid.nx, fGrid.ny, fGrid.nz);
+  muto_counters cnt{};
+  check(fCtx, muto_slr_reconstruct(fCtx, &c, rays.size(), as_rays(rays), img.data(), &cnt),
+        "MutoSLR::reconstruct");
+  fNTotal = (int)cnt.n_total;
+  if (fLogPrint) logResults();
+  return img;
+}
+
+void MutoSLR::logResults() {  // src/MutoSLR.cpp:111-116
+  std::cout << "==================================================" << std::endl;
+  std::cout << "|| SLR reconstruction results                  ||" << std::endl;
+  std::cout << "==================================================" << std::endl;
+  std::cout << "number of rays processed: " << fNTotal << std::endl;
 }
 
 // ---------------------------------------------------------------- MLSD

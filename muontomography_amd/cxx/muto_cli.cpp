@@ -1,7 +1,8 @@
 // Command-line driver in the manner of the reference's MuToTest.cpp:15-83:
 // muto_cli <config.json> reads the muon records, runs the method named by
-// the configuration's "poca", "mlsd" or "map" object on the GPU and writes
-// the image file.
+// the configuration's "poca", "mlsd", "map", "slr" or "pocut" object on the
+// GPU and writes the image file.  ("slr_iter" is not run: its in-place,
+// order-dependent sweep is not part of this library.)
 #include <chrono>
 #include <iostream>
 #include <memory>
@@ -11,6 +12,8 @@
 #include "MutoMAP.h"
 #include "MutoMLSD.h"
 #include "MutoPoCA.h"
+#include "MutoPoCut.h"
+#include "MutoSLR.h"
 
 int main(int argc, char** argv) {
   MutoConfig config;
@@ -48,13 +51,27 @@ int main(int argc, char** argv) {
       auto* r = new MutoMAP(config.get("map"));
       rec.reset(r);
       header.nIter = config.get("map")["num_iteration"];
+    } else if (config.exist("slr")) {
+      method = "slr";
+      auto* r = new MutoSLR(config.get("slr"));
+      rec.reset(r);
+      header.grid = r->getCurrentGrid();
+      header.nIter = 1;
+    } else if (config.exist("pocut")) {
+      method = "pocut";
+      auto* r = new MutoPoCut(config.get("pocut"));
+      rec.reset(r);
+      header.grid = r->getCurrentGrid();
+      header.nIter = 1;
     } else {
-      std::cerr << "The configuration names no method this library runs (poca, mlsd, map)." << std::endl;
+      std::cerr << "The configuration names no method this library runs "
+                   "(poca, mlsd, map, slr, pocut)." << std::endl;
       return 2;
     }
     Image img = rec->reconstruct(data);
     // MLSD/MAP may adopt the initial image's grid
-    if (method != "poca") header.grid = static_cast<MutoMLSD*>(rec.get())->getCurrentGrid();
+    if (method == "mlsd" || method == "map")
+      header.grid = static_cast<MutoMLSD*>(rec.get())->getCurrentGrid();
     std::string output = config.get("output_file");
     header.fPrecision = 4;
     header.method = method;

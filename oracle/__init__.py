@@ -44,6 +44,9 @@ def lib():
         L.orc_poca_points.argtypes = [C.POINTER(Grid), C.c_size_t, P, C.c_double, C.c_int, P, P, P,
                                       C.POINTER(Counters)]
         L.orc_poca_reconstruct.argtypes = [C.POINTER(PocaCfg), C.c_size_t, P, P, C.POINTER(Counters)]
+        L.orc_pocut_reconstruct.argtypes = [C.POINTER(PocaCfg), C.c_double, C.c_size_t, P, P,
+                                            C.POINTER(Counters)]
+        L.orc_slr_reconstruct.argtypes = [C.POINTER(PocaCfg), C.c_size_t, P, P, C.POINTER(Counters)]
         L.orc_em_prepare.argtypes = [C.POINTER(MlsdCfg), C.c_size_t, P, C.POINTER(OrcEm),
                                      C.POINTER(Counters)]
         L.orc_em_from_prepared.argtypes = [C.POINTER(Grid), C.c_size_t, P, P, P, P, P,
@@ -121,6 +124,24 @@ def poca_reconstruct(cfg, rays):
     c = Counters()
     assert lib().orc_poca_reconstruct(C.byref(cfg), rays.shape[0], ptr(rays), ptr(img),
                                       C.byref(c)) == 0
+    return img, c.as_dict()
+
+
+def pocut_reconstruct(cfg, rays, cut_percentile=1.0):
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    img = np.empty(cfg.grid.n_vox)
+    c = Counters()
+    assert lib().orc_pocut_reconstruct(C.byref(cfg), float(cut_percentile), rays.shape[0], ptr(rays),
+                                       ptr(img), C.byref(c)) == 0
+    return img, c.as_dict()
+
+
+def slr_reconstruct(cfg, rays):
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    img = np.empty(cfg.grid.n_vox)
+    c = Counters()
+    assert lib().orc_slr_reconstruct(C.byref(cfg), rays.shape[0], ptr(rays), ptr(img),
+                                     C.byref(c)) == 0
     return img, c.as_dict()
 
 

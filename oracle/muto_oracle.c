@@ -383,6 +383,162 @@ int orc_poca_reconstruct(const muto_poca_cfg* cfg, size_t n,
 }
 
 /* ------------------------------------------------------------------ */
+/* PoCut: src/MutoPoCut.cpp:19-121                                     */
+/* ------------------------------------------------------------------ */
+static int cmp_double(const void* a, const void* b) {
+  const double x = *(const double*)a, y = *(const double*)b;
+  return (x < y) ? -1 : (y < x) ? 1 : 0;
+}
+
+int orc_pocut_reconstruct(const muto_poca_cfg* cfg, double cut_percentile,
+                          size_t n, const muto_ray* rays, double* img,
+                          muto_counters* c) {
+  const muto_grid* g = &cfg->grid;
+  const size_t nv = n_vox_of(g);
+  if (!img || (n && !rays)) return MUTO_ERR_ARG;
+  memset(img, 0, nv * sizeof(double));
+  if (c) memset(c, 0, sizeof *c);
+  if (n == 0) return MUTO_OK;
+  size_t ntot = cfg->total_num_rays > 0 ? (size_t)cfg->total_num_rays : n;
+  if (ntot > n) ntot = n;
+  double* wimg = (double*)calloc(nv, sizeof(double));
+  /* per-ray (voxel, value) records; the per-voxel lists of :30-31 are
+   * these grouped by voxel (order inside a voxel is irrelevant: sorted) */
+  uint64_t* rv = (uint64_t*)malloc(ntot * sizeof(uint64_t));
+  double* ra = (double*)malloc(ntot * sizeof(double));
+  if (!wimg || !rv || !ra) { free(wimg); free(rv); free(ra); return MUTO_ERR_NOMEM; }
+  const double zmin_det = rays[0].pout[2], zmax_det = rays[0].pin[2]; /* :38-39 */
+  int64_t npar = 0, nin = 0, nout = 0;
+  path_buf pa = {0}, pbb = {0};
+  for (size_t i = 0; i < ntot; ++i) { /* :44-101 */
+    const muto_ray* r = &rays[i];
+    double P[3];
+    npar += orc_poca_point(r, cfg->eps, P);
+    const double a2 = orc_scat_angle_sq(r->din, r->dout, cfg->angle_unit_mag);
+    const uint32_t xs = orc_x86_u32((P[0] - g->x_min) / g->dx);
+    const uint32_t ys = orc_x86_u32((P[1] - g->y_min) / g->dy);
+    const uint32_t zs = orc_x86_u32((P[2] - g->z_min) / g->dz);
+    rv[i] = UINT64_MAX;
+    if (g->nx <= xs || g->ny <= ys || g->nz <= zs || P[2] < zmin_det ||
+        zmax_det < P[2])
+      continue;
+    ++nin;
+    rv[i] = tidx(g, xs, ys, zs); /* :62-66 */
+    ra[i] = cfg->has_energy ? a2 * r->ein * r->ein : a2;
+    /* :69-98, two legs (fStraightLine is forced false, :5,14) */
+    const size_t ki = trace(g, r->pin, P, &pa);
+    const size_t ko = trace(g, P, r->pout, &pbb);
+    for (size_t m = 0; m < ki; ++m) {
+      if (!in_grid(g, pa.x[m], pa.y[m], pa.z[m])) { ++nout; continue; }
+      wimg[tidx(g, pa.x[m], pa.y[m], pa.z[m])] += cfg->length_weighted ? pa.l[m] : g->dz;
+    }
+    for (size_t m = 0; m < ko; ++m) {
+      if (!in_grid(g, pbb.x[m], pbb.y[m], pbb.z[m])) { ++nout; continue; }
+      const size_t t = tidx(g, pbb.x[m], pbb.y[m], pbb.z[m]);
+      if (cfg->length_weighted)
+        wimg[t] += pbb.l[m];
+      else if (pbb.x[m] != xs || pbb.y[m] != ys || pbb.z[m] != zs)
+        wimg[t] += g->dz;
+    }
+  }
+  /* group by voxel: counting sort */
+  size_t* off = (size_t*)calloc(nv + 1, sizeof(size_t));
+  double* vals = (double*)malloc((nin ? (size_t)nin : 1) * sizeof(double));
+  if (!off || !vals) { free(off); free(vals); free(wimg); free(rv); free(ra); return MUTO_ERR_NOMEM; }
+  for (size_t i = 0; i < ntot; ++i)
+    if (rv[i] != UINT64_MAX) ++off[rv[i] + 1];
+  for (size_t j = 0; j < nv; ++j) off[j + 1] += off[j];
+  {
+    size_t* fill = (size_t*)calloc(nv, sizeof(size_t));
+    for (size_t i = 0; i < ntot; ++i)
+      if (rv[i] != UINT64_MAX) vals[off[rv[i]] + fill[rv[i]]++] = ra[i];
+    free(fill);
+  }
+  for (size_t j = 0; j < nv; ++j) { /* :112-117 */
+    const size_t m = off[j + 1] - off[j];
+    double* a = vals + off[j];
+    qsort(a, m, sizeof(double), cmp_double);
+    /* threshold = (MTindex)(size * fPercentile), x86 cast; the reference
+     * reads past the list when it exceeds the size: clamped here */
+    uint64_t thr = orc_x86_u32((double)m * cut_percentile);
+    if (thr > m) thr = m;
+    double sum = 0.0;
+    for (uint64_t k = 0; k < thr; ++k) sum += a[k];
+    img[j] = wimg[j] < cfg->eps ? 0 : sum / wimg[j];
+  }
+  free(off); free(vals); free(wimg); free(rv); free(ra);
+  pb_free(&pa);
+  pb_free(&pbb);
+  if (c) {
+    c->n_total = (int64_t)ntot;
+    c->n_not_parallel = npar;
+    c->n_scat_in_grid = nin;
+    c->n_out_of_grid = nout;
+  }
+  return MUTO_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* SLR: src/MutoSLR.cpp:47-103                                         */
+/* ------------------------------------------------------------------ */
+int orc_slr_reconstruct(const muto_poca_cfg* cfg, size_t n,
+                        const muto_ray* rays, double* img, muto_counters* c) {
+  const muto_grid* g = &cfg->grid;
+  const size_t nv = n_vox_of(g);
+  if (!img || (n && !rays)) return MUTO_ERR_ARG;
+  memset(img, 0, nv * sizeof(double));
+  if (c) memset(c, 0, sizeof *c);
+  if (n == 0) return MUTO_OK;
+  size_t ntot = cfg->total_num_rays > 0 ? (size_t)cfg->total_num_rays : n;
+  if (ntot > n) ntot = n;
+  double* wimg = (double*)calloc(nv, sizeof(double));
+  if (!wimg) return MUTO_ERR_NOMEM;
+  const double zlo = g->z_min; /* :54-55 */
+  const double zhi = g->z_min + g->dz * (double)g->nz;
+  int64_t nout = 0;
+  path_buf pa = {0};
+  for (size_t i = 0; i < ntot; ++i) { /* :58-96 */
+    const muto_ray* r = &rays[i];
+    const double axz = (atan2(r->din[0], r->din[2]) - atan2(r->dout[0], r->dout[2])) * cfg->angle_unit_mag;
+    const double ayz = (atan2(r->din[1], r->din[2]) - atan2(r->dout[1], r->dout[2])) * cfg->angle_unit_mag;
+    const double tin = (zhi - r->pin[2]) / r->din[2];
+    const double tout = (zlo - r->pout[2]) / r->dout[2];
+    double pi[3], po[3], d[3];
+    for (int k = 0; k < 3; ++k) {
+      pi[k] = r->pin[k] + tin * r->din[k];
+      po[k] = r->pout[k] + tout * r->dout[k];
+      d[k] = pi[k] - po[k];
+    }
+    const size_t k = trace(g, pi, po, &pa);
+    const double ltot = sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+    const double a2h = (axz * axz + ayz * ayz) * 0.5;
+    for (size_t m = 0; m < k; ++m) {
+      if (!in_grid(g, pa.x[m], pa.y[m], pa.z[m])) { ++nout; continue; }
+      const size_t t = tidx(g, pa.x[m], pa.y[m], pa.z[m]);
+      double v;
+      if (cfg->length_weighted) {
+        wimg[t] += pa.l[m] / g->dz;
+        v = a2h * (pa.l[m] / ltot) * (pa.l[m] / ltot);
+      } else {
+        wimg[t] += 1.0;
+        v = a2h / (double)(k * k);
+      }
+      if (cfg->has_energy) v *= r->ein * r->ein;
+      img[t] += v;
+    }
+  }
+  for (size_t j = 0; j < nv; ++j) /* :98-100 */
+    img[j] = wimg[j] < cfg->eps ? 0 : img[j] / wimg[j];
+  free(wimg);
+  pb_free(&pa);
+  if (c) {
+    c->n_total = (int64_t)ntot;
+    c->n_out_of_grid = nout;
+  }
+  return MUTO_OK;
+}
+
+/* ------------------------------------------------------------------ */
 /* MLSD preparation: src/MutoMLSD.cpp:268-436                          */
 /* ------------------------------------------------------------------ */
 static void scattering_angle(const double* d1, const double* d2, double* sx,
@@ -630,11 +786,6 @@ static int inverse_det(const mat2* m, mat2* inv, double* det) {
   inv->m01 = -m->m01 * invdet;
   inv->m11 = m->m00 * invdet;
   return 1;
-}
-
-static int cmp_double(const void* a, const void* b) {
-  double x = *(const double*)a, y = *(const double*)b;
-  return (x < y) ? -1 : (y < x) ? 1 : 0;
 }
 
 /* updateImageVoxel :493-517 (values in S[0..mj), may be reordered) */

@@ -45,6 +45,15 @@ int orc_poca_points(const muto_grid* g, size_t n, const muto_ray* rays,
                     uint8_t* accept, muto_counters* c);
 int orc_poca_reconstruct(const muto_poca_cfg* cfg, size_t n,
                          const muto_ray* rays, double* img, muto_counters* c);
+/* MutoPoCut::reconstruct (src/MutoPoCut.cpp:19-121): two-leg weights as
+ * PoCA, per voxel the lowest (u32)(M * cut_percentile) scattering values
+ * (sorted, summed in order) over the weight. */
+int orc_pocut_reconstruct(const muto_poca_cfg* cfg, double cut_percentile,
+                          size_t n, const muto_ray* rays, double* img,
+                          muto_counters* c);
+/* MutoSLR::reconstruct (src/MutoSLR.cpp:47-103). */
+int orc_slr_reconstruct(const muto_poca_cfg* cfg, size_t n,
+                        const muto_ray* rays, double* img, muto_counters* c);
 
 /* MLSD preparation (gatherInformation + getMuonPathAndTail).  Outputs are
  * malloc'ed; free with orc_em_free. */

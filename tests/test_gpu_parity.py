@@ -204,6 +204,48 @@ def test_poca_image(ctx, small_set, lw, sl):
     np.testing.assert_allclose(ig, io, rtol=1e-9, atol=0)
 
 
+@pytest.mark.parametrize("lw,pct", [(1, 1.0), (1, 0.9), (0, 0.5), (1, 0.0), (0, 1.7)])
+def test_pocut_image(ctx, small_set, lw, pct):
+    """PoCut against the oracle (src/MutoPoCut.cpp:19-121): same tolerance
+    as the PoCA image (atomic order of the weights, atan2 ulps)."""
+    g, rays, _ = small_set
+    cfg = poca_cfg(g, length_weighted=lw, straight_line=1)  # PoCut forces two legs
+    io, co = oracle.pocut_reconstruct(cfg, rays, pct)
+    ig, cg = ctx.pocut_reconstruct(cfg, rays, pct)
+    for k in ("n_scat_in_grid", "n_not_parallel", "n_out_of_grid"):
+        assert co[k] == cg[k], k
+    assert np.array_equal(io == 0, ig == 0)
+    np.testing.assert_allclose(ig, io, rtol=1e-9, atol=0)
+    assert ctx.kernel_ms() > 0.0
+
+
+@pytest.mark.parametrize("n", [4, 1])
+def test_pocut_long_voxel_lists(ctx, small_set, n):
+    """Coarse grids put many scattering points in a voxel: the lists go
+    through the block sort in LDS (4^3: ~1e2 per voxel) and the global
+    scratch sort (one voxel: ~8e3 points)."""
+    g0, rays, _ = small_set
+    g = make_grid(n, g0.dx * g0.nx / n, g0.x_min)
+    cfg = poca_cfg(g, length_weighted=1)
+    io, co = oracle.pocut_reconstruct(cfg, rays, 0.8)
+    ig, cg = ctx.pocut_reconstruct(cfg, rays, 0.8)
+    assert co["n_scat_in_grid"] == cg["n_scat_in_grid"]
+    assert co["n_scat_in_grid"] / n ** 3 > (4096 if n == 1 else 16)
+    np.testing.assert_allclose(ig, io, rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("lw,he", [(1, 0), (0, 0), (1, 1)])
+def test_slr_image(ctx, small_set, lw, he):
+    """SLR against the oracle (src/MutoSLR.cpp:47-103)."""
+    g, rays, _ = small_set
+    cfg = poca_cfg(g, length_weighted=lw, has_energy=he)
+    io, co = oracle.slr_reconstruct(cfg, rays)
+    ig, cg = ctx.slr_reconstruct(cfg, rays)
+    assert co["n_out_of_grid"] == cg["n_out_of_grid"] and co["n_total"] == cg["n_total"]
+    assert np.array_equal(io == 0, ig == 0)
+    np.testing.assert_allclose(ig, io, rtol=1e-9, atol=0)
+
+
 def _em_cfg(g, **kw):
     base = dict(error_xy=0.5, error_dz_outer=500.0, error_dz_inner=500.0,
                 initialize_method=2, initialize_value=1e-7, num_iteration=5)
@@ -623,6 +665,20 @@ def test_cli_poca_then_mlsd_matches_library(ctx, small_set, tmp_path):
     cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, num_iteration=5, initialize_method=0)
     ref, _, _ = ctx.mlsd_reconstruct(cfg, rc, init=pimg)
     np.testing.assert_allclose(mimg, ref.astype(np.float32), rtol=1e-5, atol=0)
+    # the other Siddon consumers the reference's driver dispatches
+    for method, extra, fn in (("pocut", dict(cut_percentile=0.8),
+                               lambda c: ctx.pocut_reconstruct(c, rc, 0.8)),
+                              ("slr", {}, lambda c: ctx.slr_reconstruct(c, rc))):
+        out = tmp_path / f"{method}.bin"
+        cfgx = dict(common, output_file=str(out), **{method: dict(grid=grid, length_weighted=True, **extra)})
+        (tmp_path / f"{method}.json").write_text(json.dumps(cfgx))
+        r = subprocess.run([cli, str(tmp_path / f"{method}.json")], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        ximg, xh = read_image(out)
+        assert xh["method"] == method and xh["n_iter"] == 1
+        ref, _ = fn(poca_cfg(g, length_weighted=1))
+        np.testing.assert_allclose(ximg, ref.astype(np.float32), rtol=1e-6, atol=0)
 
 
 def test_batch_of_scans_matches_each_scan_alone(ctx):

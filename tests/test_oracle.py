@@ -314,6 +314,71 @@ def test_poca_image_single_track_closed_form():
     np.testing.assert_allclose(img[j], a2 / w, rtol=1e-12)
 
 
+def _kinked_rays(g, P, slopes, zc_top=30.0):
+    """Vertical incoming lines through the points P[k], kinked there with
+    x-slope slopes[k]: the PoCA point is P[k] and theta^2 = atan(s)^2 / 2."""
+    rs = []
+    for Pk, sk in zip(P, slopes):
+        Pk = np.asarray(Pk, float)
+        din = np.array([0.0, 0.0, 1.0])
+        dout = np.array([sk, 0.0, 1.0]) / np.sqrt(1 + sk * sk)
+        pin = np.array([Pk[0], Pk[1], zc_top])
+        pout = Pk - (zc_top + Pk[2]) / dout[2] * dout  # reaches z = -zc_top
+        rs.append(_ray(pin, din, pout, dout))
+    return np.concatenate(rs)
+
+
+def test_pocut_keeps_the_lowest_fraction():
+    """Four tracks scattering in one voxel: PoCut / PoCA there is (sum of the
+    lowest (u32)(4 p) values) / (sum of all), whatever the weights; p = 1
+    gives the PoCA image, p = 0 an empty image."""
+    g = make_grid(6, 10.0, -30.0)
+    P = [(-5.0, 5.0, 4.0), (-4.0, 6.0, 5.0), (-6.0, 4.5, 6.0), (-5.5, 5.5, 3.0)]
+    s_ = [0.03, -0.01, 0.02, 0.05]
+    rays = _kinked_rays(g, P, s_)
+    a2 = np.sort(np.arctan(np.array(s_)) ** 2 / 2.0)
+    cfg = poca_cfg(g, length_weighted=1, straight_line=0)
+    ref, c0 = oracle.poca_reconstruct(cfg, rays)
+    j = 2 + 6 * (3 + 6 * 3)
+    assert c0["n_scat_in_grid"] == 4 and ref[j] > 0
+    full, c1 = oracle.pocut_reconstruct(cfg, rays, 1.0)
+    np.testing.assert_allclose(full, ref, rtol=1e-13, atol=0)
+    assert c1["n_scat_in_grid"] == 4 and c1["n_not_parallel"] == c0["n_not_parallel"]
+    for pct, m in ((0.5, 2), (0.6, 2), (0.25, 1), (0.74, 2), (0.75, 3)):
+        cut, _ = oracle.pocut_reconstruct(cfg, rays, pct)
+        np.testing.assert_allclose(cut[j] / ref[j], a2[:m].sum() / a2.sum(), rtol=1e-9)
+        assert np.count_nonzero(cut) == 1
+    zero, _ = oracle.pocut_reconstruct(cfg, rays, 0.0)
+    assert not zero.any()
+
+
+def test_slr_single_and_two_tracks_closed_form():
+    """SLR spreads theta^2/2 along the chord between the grid's top and
+    bottom planes: per voxel (L/Ltot)^2 (length-weighted, weight L/dz) or
+    1/k^2 (weight 1); two tracks through a voxel average their values."""
+    g = make_grid(6, 10.0, -30.0)
+    rays = _kinked_rays(g, [(-5.0, 5.0, 4.0), (-5.0, 5.0, -12.0)], [0.03, -0.02])
+    r = rays[0:1]
+    th2 = np.arctan(0.03) ** 2 / 2.0
+    pi = r["pin"][0] + (30.0 - r["pin"][0][2]) / r["din"][0][2] * r["din"][0]
+    po = r["pout"][0] + (-30.0 - r["pout"][0][2]) / r["dout"][0][2] * r["dout"][0]
+    row, x, y, z, ln = oracle.siddon_csr(g, pi[None], po[None])
+    ltot = np.linalg.norm(pi - po)
+    j = x.astype(int) + 6 * (y.astype(int) + 6 * z.astype(int))
+    img, c = oracle.slr_reconstruct(poca_cfg(g, length_weighted=1), r)
+    np.testing.assert_allclose(img[j], th2 * (ln / ltot) ** 2 / (ln / 10.0), rtol=1e-12)
+    assert np.count_nonzero(img) == len(j) and c["n_total"] == 1
+    img, _ = oracle.slr_reconstruct(poca_cfg(g, length_weighted=0), r)
+    np.testing.assert_allclose(img[j], th2 / len(j) ** 2, rtol=1e-12)
+    # both tracks: vertical incoming lines share the column above the kinks
+    both, _ = oracle.slr_reconstruct(poca_cfg(g, length_weighted=0), rays)
+    one, _ = oracle.slr_reconstruct(poca_cfg(g, length_weighted=0), rays[1:2])
+    img0, _ = oracle.slr_reconstruct(poca_cfg(g, length_weighted=0), rays[0:1])
+    common = (img0 > 0) & (one > 0)
+    assert common.any()
+    np.testing.assert_allclose(both[common], (img0[common] + one[common]) / 2.0, rtol=1e-12)
+
+
 # ---------------------------------------------------------------- EM algebra
 def _single_voxel_problem(K=7, seed=0):
     """1x1x1 grid, K tracks with one entry each: every S_ij in closed form."""
