@@ -151,6 +151,11 @@ def bench_siddon(args):
     cfg = poca_cfg(g, length_weighted=1, straight_line=0)
     ctx.poca_reconstruct(cfg, rays)
     t["poca_reconstruct"] = ctx.kernel_ms()
+    # the reference's other Siddon consumers on the same muons
+    ctx.pocut_reconstruct(cfg, rays, 0.9)
+    t["pocut_reconstruct"] = ctx.kernel_ms()
+    ctx.slr_reconstruct(cfg, rays)
+    t["slr_reconstruct"] = ctx.kernel_ms()
     # list output (12 B written per entry as u32 index + f64 length, plus
     # the y, z indices of the CSR API) on a sample that fits host memory
     ns = 200_000

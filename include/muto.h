@@ -119,8 +119,10 @@ int muto_ctx_destroy(muto_ctx* ctx);
 const char* muto_last_error(const muto_ctx* ctx);
 int muto_abi_version(void);
 /* GPU time (ms, HIP events on the context's stream) of the kernels of the
- * last muto_siddon_csr (fill pass), muto_siddon_digest, muto_poca_points or
- * muto_poca_reconstruct call on this context, without the host copies.  No
+ * last muto_siddon_csr (fill pass), muto_siddon_digest, muto_poca_points,
+ * muto_poca_reconstruct, muto_pocut_reconstruct or muto_slr_reconstruct call
+ * on this context, without the host copies (PoCut: with its host-side list
+ * of long voxel lists).  No
  * counterpart in the reference, which times whole runs with a stopwatch
  * (MuToTest.cpp:25,80-81). */
 int muto_ctx_kernel_ms(muto_ctx* ctx, double* ms);
