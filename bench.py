@@ -156,6 +156,14 @@ def bench_siddon(args):
     t["pocut_reconstruct"] = ctx.kernel_ms()
     ctx.slr_reconstruct(cfg, rays)
     t["slr_reconstruct"] = ctx.kernel_ms()
+    # muon records to rays (MutoFile::getMuonData's per-record work):
+    # float32 records of 6 planes, hits smeared by 0.5 mm
+    zi, zo = float(rays["pin"][0, 2]), float(rays["pout"][0, 2])
+    zl = [zi + 1000.0, zi + 500.0, zi, zo, zo - 500.0, zo - 1000.0]
+    rec = W.records_from_rays(rays, zl, sigma=0.5, seed=1)
+    ctx.rays_from_records(rec[:1000], zl, 3)
+    ctx.rays_from_records(rec, zl, 3)
+    t["rays_from_records"] = ctx.kernel_ms()
     # list output (12 B written per entry as u32 index + f64 length, plus
     # the y, z indices of the CSR API) on a sample that fits host memory
     ns = 200_000
@@ -165,7 +173,13 @@ def bench_siddon(args):
     n_seg = 2 * len(rays)
     walk_rate = entries / (t["siddon_walk"] * 1e-3)
     cpu = None
+    cpu_fit = None
     if not args.no_cpu:
+        nf = min(1_000_000, len(rays))
+        t0 = time.perf_counter()
+        oracle.rays_from_records(rec[:nf], zl, 3)
+        cpu_fit = {"records_per_s": nf / (time.perf_counter() - t0), "cores": 1,
+                   "sample": f"first {nf} records, one thread"}
         nc = min(2_000_000, len(rays))
         t0 = time.perf_counter()
         cc, _ = oracle.siddon_digest(g, rays["pin"][:nc], P[:nc])
@@ -184,6 +198,8 @@ def bench_siddon(args):
            "kernel_ms": t,
            "segments_per_s": n_seg / (t["siddon_walk"] * 1e-3),
            "poca_points_per_s": len(rays) / (t["poca_points"] * 1e-3),
+           "records_per_s": len(rays) / (t["rays_from_records"] * 1e-3),
+           "cpu_records": cpu_fit,
            "csr_fill_sample": {"segments": ns, "entries": fill_entries, "ms": t_fill,
                                "write_GBps": fill_entries * 20 / (t_fill * 1e-3) / 1e9},
            "cpu_baseline": cpu}

@@ -190,6 +190,24 @@ int muto_slr_reconstruct(muto_ctx* ctx, const muto_poca_cfg* cfg, size_t n,
                          muto_counters* counters);
 
 /* ------------------------------------------------------------------ */
+/* Muon records                                                        */
+/* ------------------------------------------------------------------ */
+/* MutoFile::getMuonData's per-record work (src/MutoFile.cpp:19-82,
+ * 248-304) for n_rec records already read from files: each record is
+ * e_in, (x, y) of the hit on each of n_layers tracker planes at heights
+ * z[n_layers], e_out, as float (precision 4) or double (8), host memory.
+ * Records holding a NaN are dropped (:72); each side (planes 0..n_middle-1
+ * above, the rest below) is fitted by the principal axis of its centred
+ * hits, sign d.z > 0, with pin at z[n_middle-1] and pout at z[n_middle]
+ * (:270-276, 292-298).  rays (host, room for n_rec) receives the kept
+ * records in order, *n_rays their number.  2 <= n_middle,
+ * n_layers - n_middle >= 2, n_layers <= MUTO_MAX_LAYERS. */
+#define MUTO_MAX_LAYERS 16
+int muto_rays_from_records(muto_ctx* ctx, size_t n_rec, int n_layers, int n_middle,
+                           const double* z, const void* records, int precision,
+                           muto_ray* rays, size_t* n_rays);
+
+/* ------------------------------------------------------------------ */
 /* MLSD / MAP one-shot                                                 */
 /* ------------------------------------------------------------------ */
 /* MutoMLSD::reconstruct (MutoMLSD.cpp:91-220).  init_img: the initial

@@ -25,6 +25,12 @@ class MutoFile {
  public:
   static MutoMuonData getMuonData(std::vector<std::string> flist, std::vector<MTfloat> zlayers,
                                   int nMiddle, int fPrecision = 4);
+  /* getMuonData with the NaN filter and the line fits on the GPU (HIP
+   * device `device`, through muto_rays_from_records); the files are read on
+   * the host.  Same records kept, same rays.  Throws std::runtime_error
+   * when the device cannot be reached. */
+  static MutoMuonData getMuonDataGPU(std::vector<std::string> flist, std::vector<MTfloat> zlayers,
+                                     int nMiddle, int fPrecision = 4, int device = 0);
   static bool saveImage(const Image& img, const ImageHeader& header, std::string filename);
   static bool saveImagePure(const Image& img, std::string filename, MTindex fPrecision = 4);
   static std::pair<Image, ImageHeader> loadImage(std::string filename);

@@ -158,6 +158,22 @@ class Context:
                                                 ptr(img), C.byref(c)), "muto_poca_reconstruct")
         return img, c.as_dict()
 
+    def rays_from_records(self, records, zlayers, n_middle):
+        """Tracker records (n, 2 L + 2) float32 or float64 to rays: NaN
+        records dropped, each side fitted on the GPU (MutoFile::getMuonData)."""
+        rec = np.ascontiguousarray(records)
+        if rec.dtype not in (np.float32, np.float64):
+            raise TypeError("records must be float32 or float64")
+        z = np.ascontiguousarray(zlayers, np.float64)
+        n = rec.shape[0]
+        assert rec.shape[1] == 2 * z.shape[0] + 2
+        out = np.zeros(max(n, 1), RAY_DTYPE)
+        k = C.c_size_t()
+        self._check(lib().muto_rays_from_records(self._h, n, z.shape[0], int(n_middle), ptr(z), ptr(rec),
+                                                 rec.dtype.itemsize, ptr(out), C.byref(k)),
+                    "muto_rays_from_records")
+        return out[:k.value]
+
     def pocut_reconstruct(self, cfg, rays, cut_percentile=1.0):
         """MutoPoCut: PoCA with the per-voxel cut of the largest angles."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)

@@ -142,6 +142,8 @@ PROTOS = {
                                    P, P, P, C.POINTER(Counters)]),
     "muto_poca_reconstruct": (C.c_int, [P, C.POINTER(PocaCfg), C.c_size_t, P, P,
                                         C.POINTER(Counters)]),
+    "muto_rays_from_records": (C.c_int, [P, C.c_size_t, C.c_int, C.c_int, P, P, C.c_int, P,
+                                         C.POINTER(C.c_size_t)]),
     "muto_pocut_reconstruct": (C.c_int, [P, C.POINTER(PocaCfg), C.c_double, C.c_size_t, P, P,
                                          C.POINTER(Counters)]),
     "muto_slr_reconstruct": (C.c_int, [P, C.POINTER(PocaCfg), C.c_size_t, P, P,

@@ -13,7 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmuto.so")
 
-HIP_SOURCES = ["ctx.hip", "scan.hip", "siddon.hip", "poca.hip", "em.hip", "em32.hip", "map_op.hip", "batch.hip"]
+HIP_SOURCES = ["ctx.hip", "scan.hip", "siddon.hip", "poca.hip", "em.hip", "em32.hip", "map_op.hip", "batch.hip",
+               "records.hip"]
 CXX_SOURCES = ["gen.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

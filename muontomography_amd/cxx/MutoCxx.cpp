@@ -640,3 +640,48 @@ MutoMuonData MutoFile::getMuonData(std::vector<std::string> flist, std::vector<M
   }
   return data;
 }
+
+// getMuonData with the per-record work on the GPU
+MutoMuonData MutoFile::getMuonDataGPU(std::vector<std::string> flist, std::vector<MTfloat> zlayers,
+                                      int nMiddle, int fPrecision, int device) {
+  MutoMuonData data;
+  if (fPrecision != 4 && fPrecision != 8) {
+    std::cerr << "Floating point precision not supported. " << std::endl;
+    return data;
+  }
+  const int L = (int)zlayers.size();
+  if (nMiddle < 2 || L - nMiddle < 2) {
+    std::cerr << "Error: at least two layers of detectors for both up and down area. " << std::endl;
+    return data;
+  }
+  muto_ctx* ctx = nullptr;
+  open_ctx(ctx, device);
+  const size_t each = 2 * (size_t)L + 2;
+  try {
+    for (const std::string& name : flist) {
+      std::ifstream f(name.c_str(), std::ios::in | std::ios::binary);
+      if (f.fail()) {
+        std::cout << "Reading muon data error for : " << name << std::endl;
+        continue;
+      }
+      f.seekg(0, std::ios::end);
+      const size_t bytes = (size_t)f.tellg();
+      f.seekg(0, std::ios::beg);
+      const size_t n = bytes / each / (size_t)fPrecision;
+      std::vector<char> raw(n * each * (size_t)fPrecision);
+      f.read(raw.data(), (std::streamsize)raw.size());
+      const size_t base = data.size();
+      data.resize(base + n);
+      size_t kept = 0;
+      check(ctx, muto_rays_from_records(ctx, n, L, nMiddle, zlayers.data(), raw.data(), fPrecision,
+                                        reinterpret_cast<muto_ray*>(data.data() + base), &kept),
+            "MutoFile::getMuonDataGPU");
+      data.resize(base + kept);
+    }
+  } catch (...) {
+    muto_ctx_destroy(ctx);
+    throw;
+  }
+  muto_ctx_destroy(ctx);
+  return data;
+}

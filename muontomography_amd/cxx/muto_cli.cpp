@@ -1,7 +1,7 @@
 // Command-line driver in the manner of the reference's MuToTest.cpp:15-83:
-// muto_cli <config.json> reads the muon records, runs the method named by
-// the configuration's "poca", "mlsd", "map", "slr" or "pocut" object on the
-// GPU and writes the image file.  ("slr_iter" is not run: its in-place,
+// muto_cli <config.json> reads the muon records (line fits on the GPU),
+// runs the method named by the configuration's "poca", "mlsd", "map", "slr"
+// or "pocut" object on the GPU and writes the image file.  ("slr_iter" is not run: its in-place,
 // order-dependent sweep is not part of this library.)
 #include <chrono>
 #include <iostream>
@@ -27,7 +27,7 @@ int main(int argc, char** argv) {
     std::vector<std::string> files = config.get("data_files");
     std::vector<MTfloat> zlayers = config.get("detector_height");
     int n_middle = config.get("detector_middle");
-    MutoMuonData data = MutoFile::getMuonData(files, zlayers, n_middle);
+    MutoMuonData data = MutoFile::getMuonDataGPU(files, zlayers, n_middle);
     if (data.empty()) {
       std::cerr << "No muon data read." << std::endl;
       return 1;

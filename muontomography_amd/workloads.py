@@ -78,3 +78,26 @@ def scan_batch(n_scans=64, seed=77, n_scale=1.0):
         blocks = gen_layout(g, 1000 + k, 4, 3 * d, 12 * d, [X0_IRON, X0_LEAD, X0_WATER])
         out.append((g, gen_config(g, 1000 + k, blocks), max(nt, 100)))
     return out
+
+
+def records_from_rays(rays, zlayers, n_middle=3, sigma=0.0, seed=0, dtype=np.float32):
+    """Tracker records (e_in, (x, y) per plane, e_out) of the rays' in and out
+    lines evaluated at the plane heights, hits smeared by sigma mm: the input
+    format of MutoFile::getMuonData (src/MutoFile.cpp:19-82)."""
+    n = rays.shape[0]
+    zl = np.asarray(zlayers, np.float64)
+    rec = np.empty((n, 2 * zl.shape[0] + 2), dtype)
+    rec[:, 0] = rays["ein"]
+    rec[:, -1] = rays["eout"]
+    rng = np.random.default_rng(seed)
+    for i, z in enumerate(zl):
+        p, d = (rays["pin"], rays["din"]) if i < n_middle else (rays["pout"], rays["dout"])
+        t = (z - p[:, 2]) / d[:, 2]
+        x = p[:, 0] + t * d[:, 0]
+        y = p[:, 1] + t * d[:, 1]
+        if sigma > 0:
+            x = x + rng.normal(0.0, sigma, n)
+            y = y + rng.normal(0.0, sigma, n)
+        rec[:, 1 + 2 * i] = x
+        rec[:, 2 + 2 * i] = y
+    return rec

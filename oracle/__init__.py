@@ -46,6 +46,9 @@ def lib():
         L.orc_poca_reconstruct.argtypes = [C.POINTER(PocaCfg), C.c_size_t, P, P, C.POINTER(Counters)]
         L.orc_pocut_reconstruct.argtypes = [C.POINTER(PocaCfg), C.c_double, C.c_size_t, P, P,
                                             C.POINTER(Counters)]
+        L.orc_rays_from_records.argtypes = [C.c_size_t, C.c_int, C.c_int, P, P, C.c_int, P,
+                                            C.POINTER(C.c_size_t)]
+        L.orc_fit_line.argtypes = [P, P, P, C.c_int, C.c_double, P, P]
         L.orc_slr_reconstruct.argtypes = [C.POINTER(PocaCfg), C.c_size_t, P, P, C.POINTER(Counters)]
         L.orc_em_prepare.argtypes = [C.POINTER(MlsdCfg), C.c_size_t, P, C.POINTER(OrcEm),
                                      C.POINTER(Counters)]
@@ -143,6 +146,25 @@ def slr_reconstruct(cfg, rays):
     assert lib().orc_slr_reconstruct(C.byref(cfg), rays.shape[0], ptr(rays), ptr(img),
                                      C.byref(c)) == 0
     return img, c.as_dict()
+
+
+def rays_from_records(records, zlayers, n_middle):
+    rec = np.ascontiguousarray(records)
+    z = np.ascontiguousarray(zlayers, np.float64)
+    n = rec.shape[0]
+    out = np.zeros(max(n, 1), RAY_DTYPE)
+    k = C.c_size_t()
+    assert lib().orc_rays_from_records(n, z.shape[0], int(n_middle), ptr(z), ptr(rec), rec.dtype.itemsize,
+                                       ptr(out), C.byref(k)) == 0
+    return out[:k.value]
+
+
+def fit_line(x, y, z, zref):
+    x, y, z = (np.ascontiguousarray(v, np.float64) for v in (x, y, z))
+    pt = np.empty(3)
+    d = np.empty(3)
+    lib().orc_fit_line(ptr(x), ptr(y), ptr(z), x.shape[0], float(zref), ptr(pt), ptr(d))
+    return pt, d
 
 
 class Em:

@@ -539,6 +539,99 @@ int orc_slr_reconstruct(const muto_poca_cfg* cfg, size_t n,
 }
 
 /* ------------------------------------------------------------------ */
+/* Muon records: src/MutoFile.cpp:19-82, 248-304                       */
+/* ------------------------------------------------------------------ */
+/* Line through n points: centroid and the principal axis of the centred
+ * points (the first right singular vector of the centred data that
+ * src/MutoFile.cpp:270-271 takes from Eigen's JacobiSVD), found by cyclic
+ * Jacobi rotations of the 3x3 scatter matrix; the sign is fixed to d.z > 0
+ * (Eigen's is arbitrary).  The point is the line at height zref
+ * (:272-273). */
+void orc_fit_line(const double* x, const double* y, const double* z, int n,
+                  double zref, double point[3], double dir[3]) {
+  double m[3] = {0.0, 0.0, 0.0};
+  for (int i = 0; i < n; ++i) { m[0] += x[i]; m[1] += y[i]; m[2] += z[i]; }
+  for (int k = 0; k < 3; ++k) m[k] /= n;
+  double a[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int i = 0; i < n; ++i) {
+    const double d[3] = {x[i] - m[0], y[i] - m[1], z[i] - m[2]};
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) a[r][c] += d[r] * d[c];
+  }
+  double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+    const double diag = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+    if (off <= 1e-30 * diag || off == 0.0) break;
+    for (int p = 0; p < 2; ++p) {
+      for (int q = p + 1; q < 3; ++q) {
+        if (a[p][q] == 0.0) continue;
+        const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < 3; ++k) {
+          const double akp = a[k][p], akq = a[k][q];
+          a[k][p] = c * akp - s * akq;
+          a[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double apk = a[p][k], aqk = a[q][k];
+          a[p][k] = c * apk - s * aqk;
+          a[q][k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double vkp = v[k][p], vkq = v[k][q];
+          v[k][p] = c * vkp - s * vkq;
+          v[k][q] = s * vkp + c * vkq;
+        }
+      }
+    }
+  }
+  int best = 0;
+  for (int k = 1; k < 3; ++k)
+    if (a[k][k] > a[best][best]) best = k;
+  double d[3] = {v[0][best], v[1][best], v[2][best]};
+  const double nrm = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  for (int k = 0; k < 3; ++k) d[k] /= nrm;
+  if (d[2] < 0.0)
+    for (int k = 0; k < 3; ++k) d[k] = -d[k];
+  const double t = (zref - m[2]) / d[2];
+  dir[0] = d[0]; dir[1] = d[1]; dir[2] = d[2];
+  point[0] = t * d[0] + m[0];
+  point[1] = t * d[1] + m[1];
+  point[2] = zref;
+}
+
+int orc_rays_from_records(size_t n_rec, int n_layers, int n_middle,
+                          const double* z, const void* records, int precision,
+                          muto_ray* rays, size_t* n_rays) {
+  if (!z || !n_rays || (n_rec && (!records || !rays))) return MUTO_ERR_ARG;
+  if (precision != 4 && precision != 8) return MUTO_ERR_ARG;
+  if (n_middle < 2 || n_layers - n_middle < 2 || n_layers > MUTO_MAX_LAYERS) return MUTO_ERR_ARG;
+  const size_t each = 2 * (size_t)n_layers + 2; /* :51 */
+  double rec[2 * MUTO_MAX_LAYERS + 2], xs[MUTO_MAX_LAYERS], ys[MUTO_MAX_LAYERS];
+  size_t out = 0;
+  for (size_t r = 0; r < n_rec; ++r) {
+    int ok = 1;
+    for (size_t k = 0; k < each; ++k) {
+      rec[k] = precision == 8 ? ((const double*)records)[r * each + k]
+                              : (double)((const float*)records)[r * each + k];
+      ok = ok && !isnan(rec[k]); /* :72 */
+    }
+    if (!ok) continue;
+    muto_ray* ray = &rays[out++];
+    ray->ein = rec[0];
+    for (int i = 0; i < n_layers; ++i) { xs[i] = rec[1 + 2 * i]; ys[i] = rec[2 + 2 * i]; }
+    orc_fit_line(xs, ys, z, n_middle, z[n_middle - 1], ray->pin, ray->din);
+    orc_fit_line(xs + n_middle, ys + n_middle, z + n_middle, n_layers - n_middle,
+                 z[n_middle], ray->pout, ray->dout);
+    ray->eout = rec[2 * n_layers + 1];
+  }
+  *n_rays = out;
+  return MUTO_OK;
+}
+
+/* ------------------------------------------------------------------ */
 /* MLSD preparation: src/MutoMLSD.cpp:268-436                          */
 /* ------------------------------------------------------------------ */
 static void scattering_angle(const double* d1, const double* d2, double* sx,

@@ -51,6 +51,14 @@ int orc_poca_reconstruct(const muto_poca_cfg* cfg, size_t n,
 int orc_pocut_reconstruct(const muto_poca_cfg* cfg, double cut_percentile,
                           size_t n, const muto_ray* rays, double* img,
                           muto_counters* c);
+/* Records of n_layers (x, y) hits between e_in and e_out, precision 4
+ * (float) or 8 (double), to rays: NaN records dropped, each side fitted by
+ * its principal axis (src/MutoFile.cpp:19-82, 248-304). */
+void orc_fit_line(const double* x, const double* y, const double* z, int n,
+                  double zref, double point[3], double dir[3]);
+int orc_rays_from_records(size_t n_rec, int n_layers, int n_middle,
+                          const double* z, const void* records, int precision,
+                          muto_ray* rays, size_t* n_rays);
 /* MutoSLR::reconstruct (src/MutoSLR.cpp:47-103). */
 int orc_slr_reconstruct(const muto_poca_cfg* cfg, size_t n,
                         const muto_ray* rays, double* img, muto_counters* c);

@@ -616,6 +616,24 @@ def _records_from_rays(rays, zl):
     return rec
 
 
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_rays_from_records_bitexact(ctx, small_set, dt):
+    """Records to rays on the GPU (NaN filter, per-side principal-axis fits)
+    equal the oracle's bit for bit, NaN records dropped in order."""
+    g, rays, _ = small_set
+    zi, zo = float(rays["pin"][0, 2]), float(rays["pout"][0, 2])
+    zl = [zi + 1000.0, zi + 500.0, zi, zo, zo - 500.0, zo - 1000.0]
+    rec = W.records_from_rays(rays, zl, sigma=0.5, seed=2, dtype=dt)
+    rng = np.random.default_rng(8)
+    bad = rng.choice(rec.shape[0], 300, replace=False)
+    rec[bad, rng.integers(0, rec.shape[1], 300)] = np.nan
+    rg = ctx.rays_from_records(rec, zl, 3)
+    ro = oracle.rays_from_records(rec, zl, 3)
+    assert rg.shape[0] == ro.shape[0] == rec.shape[0] - 300
+    assert np.array_equal(rg.view(np.uint8), ro.view(np.uint8))
+    assert ctx.kernel_ms() > 0.0
+
+
 def test_cli_poca_then_mlsd_matches_library(ctx, small_set, tmp_path):
     """The reference-style driver (muto_cli: JSON config, record files,
     image files) end to end: PoCA, then MLSD started from the PoCA image

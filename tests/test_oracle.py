@@ -379,6 +379,69 @@ def test_slr_single_and_two_tracks_closed_form():
     np.testing.assert_allclose(both[common], (img0[common] + one[common]) / 2.0, rtol=1e-12)
 
 
+# ---------------------------------------------------------------- records
+def test_fit_line_principal_axis_against_numpy_svd():
+    """The per-side line fit (src/MutoFile.cpp:248-304) is the first right
+    singular vector of the centred hits (numpy's SVD here), sign d.z > 0,
+    through the centroid; exactly collinear hits give the line itself."""
+    rng = np.random.default_rng(9)
+    for n in (2, 3, 4, 6):
+        for _ in range(50):
+            z = np.sort(rng.uniform(-1500, 1500, n))[::-1].copy()
+            d0 = np.array([rng.normal(0, 0.4), rng.normal(0, 0.4), 1.0])
+            p0 = rng.uniform(-500, 500, 3)
+            t = (z - p0[2]) / d0[2]
+            x = p0[0] + t * d0[0] + rng.normal(0, 0.5, n)
+            y = p0[1] + t * d0[1] + rng.normal(0, 0.5, n)
+            pt, d = oracle.fit_line(x, y, z, z[-1])
+            X = np.stack([x, y, z], 1)
+            m = X.mean(0)
+            v = np.linalg.svd(X - m)[2][0]
+            v = v if v[2] > 0 else -v
+            np.testing.assert_allclose(d, v, atol=1e-12)
+            assert d[2] > 0 and abs(np.linalg.norm(d) - 1) < 1e-15
+            # the point lies on the line through the centroid, at z[-1]
+            s_ = (z[-1] - m[2]) / d[2]
+            np.testing.assert_allclose(pt, m + s_ * d, rtol=0, atol=1e-9)
+    z = np.array([1000.0, 500.0, 0.0])
+    d0 = np.array([0.3, -0.2, 1.0]) / np.linalg.norm([0.3, -0.2, 1.0])
+    x = 10 + (z - 0) / d0[2] * d0[0]
+    y = -4 + (z - 0) / d0[2] * d0[1]
+    pt, d = oracle.fit_line(x, y, z, 0.0)
+    np.testing.assert_allclose(d, d0, atol=1e-14)
+    np.testing.assert_allclose(pt, [10.0, -4.0, 0.0], atol=1e-9)
+
+
+def test_records_to_rays_nan_filter_and_sides():
+    """Records with a NaN anywhere are dropped, the rest keep their order;
+    each side is fitted from its own planes; float32 and float64 records."""
+    rng = np.random.default_rng(4)
+    zl = np.array([1500.0, 1000.0, 500.0, -500.0, -1000.0, -1500.0])
+    n = 200
+    rec = np.empty((n, 14))
+    rec[:, 0] = rng.uniform(1, 10, n)
+    rec[:, -1] = rng.uniform(1, 10, n)
+    for i, z in enumerate(zl):
+        side = 0.01 if i < 3 else -0.02
+        rec[:, 1 + 2 * i] = 3.0 + side * z + rng.normal(0, 0.5, n)
+        rec[:, 2 + 2 * i] = -2.0 + rng.normal(0, 0.5, n)
+    bad = [3, 17, 150]
+    for b, k in zip(bad, (0, 5, 13)):
+        rec[b, k] = np.nan
+    for dt in (np.float64, np.float32):
+        r = oracle.rays_from_records(rec.astype(dt), zl, 3)
+        keep = np.setdiff1d(np.arange(n), bad)
+        assert r.shape[0] == keep.shape[0]
+        src = rec.astype(dt).astype(np.float64)[keep]
+        np.testing.assert_array_equal(r["ein"], src[:, 0])
+        np.testing.assert_array_equal(r["eout"], src[:, -1])
+        for j in (0, 57, len(keep) - 1):
+            pt, d = oracle.fit_line(src[j, 1:7:2], src[j, 2:7:2], zl[:3], zl[2])
+            assert np.array_equal(r["pin"][j], pt) and np.array_equal(r["din"][j], d)
+            pt, d = oracle.fit_line(src[j, 7:13:2], src[j, 8:13:2], zl[3:], zl[3])
+            assert np.array_equal(r["pout"][j], pt) and np.array_equal(r["dout"][j], d)
+
+
 # ---------------------------------------------------------------- EM algebra
 def _single_voxel_problem(K=7, seed=0):
     """1x1x1 grid, K tracks with one entry each: every S_ij in closed form."""
