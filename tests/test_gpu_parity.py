@@ -150,6 +150,35 @@ def test_siddon_digest_matches_lists(ctx):
         assert h == int(dig[i])
 
 
+def test_empty_and_fully_rejected_inputs(ctx, small_set):
+    """No rays, and rays that all miss: PoCA, PoCut, SLR and the record
+    reader return empty results as the oracle does; records that all hold a
+    NaN give no rays."""
+    g, rays, _ = small_set
+    cfg = poca_cfg(g, length_weighted=1)
+    none = rays[:0]
+    for fn, ofn in ((ctx.poca_reconstruct, oracle.poca_reconstruct),
+                    (lambda c, r: ctx.pocut_reconstruct(c, r, 0.5),
+                     lambda c, r: oracle.pocut_reconstruct(c, r, 0.5)),
+                    (ctx.slr_reconstruct, oracle.slr_reconstruct)):
+        img, c = fn(cfg, none)
+        assert not img.any() and c["n_total"] == 0
+        # every track far away from the grid
+        far = rays[:500].copy()
+        for k in ("pin", "pout"):
+            far[k][:, 0] += 1e6
+        img, c = fn(cfg, far)
+        io, co = ofn(cfg, far)
+        assert not img.any() and not io.any()
+        assert c["n_scat_in_grid"] == co["n_scat_in_grid"] == 0
+    zl = [1500.0, 1000.0, 500.0, -500.0, -1000.0, -1500.0]
+    rec = np.full((7, 14), np.nan)
+    assert ctx.rays_from_records(rec, zl, 3).shape[0] == 0
+    assert ctx.rays_from_records(rec[:0], zl, 3).shape[0] == 0
+    with pytest.raises(M.MutoError):
+        ctx.rays_from_records(np.zeros((3, 14)), zl, 1)  # one plane above
+
+
 def test_siddon_empty_and_outside(ctx):
     g = make_grid(10, 1.0, 0.0)
     p1 = np.array([[20.0, 20.0, 20.0], [-5.0, 5.0, 5.0], [5.0, 5.0, 5.0]])
@@ -337,6 +366,28 @@ def test_mlsd_one_shot(ctx, small_set):
     assert ng["iterations_done"] == 3
     # inputs differ from the oracle only by libm ulps in the angles
     np.testing.assert_allclose(ig, io, rtol=1e-9, atol=1e-22)
+    np.testing.assert_allclose(cg, co, rtol=1e-9)
+
+
+@pytest.mark.parametrize("mean", [0, 1])
+def test_mlsd_reference_config_poca_init_20_iterations(ctx, mean):
+    """The CPU reference configuration in full: 40^3 voxels of 25 mm, 1e5
+    muons, the PoCA image (length-weighted, two legs) handed over in FP64 as
+    the initial image, then 20 MLSD iterations, median (the reference's
+    default) and mean.  Inputs differ from the oracle's only by libm ulps in
+    the angles; after 20 iterations the images agree within 1e-8."""
+    g, rays, _ = W.muons("small")
+    assert len(rays) == 100_000
+    pc = poca_cfg(g, length_weighted=1, straight_line=0)
+    po, _ = oracle.poca_reconstruct(pc, rays)
+    pg, _ = ctx.poca_reconstruct(pc, rays)
+    np.testing.assert_allclose(pg, po, rtol=1e-9, atol=0)
+    cfg = _em_cfg(g, update_with_mean=mean, mean_percentile=100, num_iteration=20, initialize_method=0)
+    io, co, no = oracle.mlsd_reconstruct(cfg, rays, init=po)
+    ig, cg, ng = ctx.mlsd_reconstruct(cfg, rays, init=pg)
+    assert ng["iterations_done"] == no["iterations_done"] == 20
+    assert ng["n_scat_in_grid"] == no["n_scat_in_grid"] and ng["nnz"] == no["nnz"]
+    np.testing.assert_allclose(ig, io, rtol=1e-8, atol=1e-22)
     np.testing.assert_allclose(cg, co, rtol=1e-9)
 
 
