@@ -257,7 +257,8 @@ def bench_map_grad(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=45)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed iterations (default 45; scans64: 30, its configuration's count)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="em256",
                     help="em256 (the headline); scans64 (many-scan batch); map128 (MAP gradient); "
@@ -277,6 +278,8 @@ def main():
                     help="only run a plain device-to-device copy of MIB MiB (HBM "
                          "counter calibration for rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = {"scans64": 30, "map128": 20}.get(args.workload, 45)
     if args.copy_calib:
         import torch
         n = args.copy_calib * (1 << 20) // 4

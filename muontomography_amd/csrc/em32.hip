@@ -1013,6 +1013,17 @@ k_upd32(size_t nv, const uint32_t* __restrict__ mj, double* __restrict__ acc,
   if (j == 0) { work[0] = 0; work[1] = 0; }
   if (j >= nv) return;
   const uint32_t m = mj[j];
+  if (m == 0) {
+    // no track crosses the voxel: it becomes 0 (MutoMLSD.cpp:494-496) and
+    // its accumulator, which no entry adds to, stays 0.  After the first
+    // pass only the read of lambda remains (the padding of a batch's
+    // combined grid is mostly such voxels); the ML image is not read there.
+    if (!ml && lam[j] != 0.0) {
+      lam[j] = 0.0;
+      lam32[j] = 0.0f;
+    }
+    return;
+  }
   const double l = lam[j];
   // the backprojection's sums: int64 fixed point (ACC 1) or FP64
   const double c = fixed ? (double)reinterpret_cast<const long long*>(acc)[j] * (1.0 / fx_scale(cmax, fx_bits))
