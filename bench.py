@@ -92,6 +92,22 @@ def bench_scans(args):
         plans.append(p)
     nnz = [p.counters()["nnz"] for p in plans]
     log(f"{len(plans)} scans, {sum(nnz)} entries, set-up {time.perf_counter() - t0:.1f} s")
+    # brick shape of the batch (taken from the first plan): the fastest of a
+    # few candidates over 3 iterations, each batch built and timed in turn
+    tune = {}
+    for shape in ((16, 16, 16, 256), (16, 16, 32, 512), (16, 8, 32, 256)):
+        plans[0].set_option(M.MUTO_OPT_BRICK, M.brick_code(shape))
+        b = ctx.batch(plans)
+        b.run(1)
+        t1 = time.perf_counter()
+        b.run(3)
+        tune[shape] = (time.perf_counter() - t1) / 3 * 1e3
+        b.close()
+    brick = min(tune, key=tune.get)
+    plans[0].set_option(M.MUTO_OPT_BRICK, M.brick_code(brick))
+    log("batch brick shapes (ms per iteration): " + ", ".join(f"{k}: {v:.2f}" for k, v in tune.items()))
+    for p in plans:
+        p.set_image(np.full(p.cfg.grid.n_vox, 1e-7))
     batch = ctx.batch(plans)
     batch.run(args.warmup)
     for p in plans:
@@ -114,7 +130,9 @@ def bench_scans(args):
            "higher_is_better": True, "dtype": "f32 entry terms, f64/int64 sums, f64 2x2 algebra",
            "data": "synthetic (seeded generator, one block layout per scan)",
            "config": {"workload": f"{len(plans)} scans, grids 32^3-128^3 of 10-30 mm voxels, "
-                                  "1e4-1e6 tracks each, 30 iterations", "entries": int(sum(nnz))},
+                                  "1e4-1e6 tracks each, 30 iterations", "entries": int(sum(nnz)),
+                      "brick": list(brick)},
+           "brick_ms": {",".join(map(str, k)): v for k, v in tune.items()},
            "batch_latency_ms": wall * 1e3,
            "alone_latency_ms_p50": float(np.percentile(alone, 50)),
            "alone_latency_ms_p99": float(np.percentile(alone, 99)),

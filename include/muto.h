@@ -303,9 +303,10 @@ int muto_map_saved_destroy(muto_map_saved* saved);
  * one size-sorted task queue for the persistent FP32C kernels and one
  * captured graph per iteration.  muto_batch_run runs `iters` mean-update
  * iterations of every scan in lock step, from and back to the scans' plan
- * images; costs (may be NULL): iters x n values, scan fastest.  The plans
- * must outlive the batch.  No counterpart in the reference (one scan per
- * process). */
+ * images; costs (may be NULL): iters x n values, scan fastest.  The batch
+ * takes the brick shape and block size of plans[0] (MUTO_OPT_BRICK).  The
+ * plans must outlive the batch.  No counterpart in the reference (one scan
+ * per process). */
 typedef struct muto_batch muto_batch;
 int muto_batch_create(muto_ctx* ctx, size_t n, muto_em_plan* const* plans, muto_batch** out);
 int muto_batch_run(muto_batch* batch, int iters, double* costs);

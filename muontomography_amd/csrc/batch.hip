@@ -120,6 +120,7 @@ extern "C" int muto_batch_create(muto_ctx* ctx, size_t n, muto_em_plan* const* p
   muto_em_plan* g = new muto_em_plan();
   g->ctx = ctx;
   g->cfg = plans[0]->cfg;
+  g->f32.bs = plans[0]->f32.bs;  // the first plan's brick shape and block size
   const uint32_t bx = g->f32.bs.bx, by = g->f32.bs.by, bz = g->f32.bs.bz;
   uint64_t X = 0, Y = 0, Z = 0;
   double d[3] = {0, 0, 0};
