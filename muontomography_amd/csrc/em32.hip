@@ -187,27 +187,64 @@ __device__ __forceinline__ uint32_t spread12(uint32_t v) {
 // dy/|dz| over [-2, 2]) above the Morton code of where that chord crosses
 // the top plane of the grid; in a batch of scans the scan comes first, so
 // each scan's tracks form one run of ranks.
+// a voxel's coordinates and brick from another lane of the wavefront
+__device__ __forceinline__ Vox shfl_vox(const Vox& v, int src) {
+  Vox r;
+  r.x = __shfl(v.x, src, 64);
+  r.y = __shfl(v.y, src, 64);
+  r.z = __shfl(v.z, src, 64);
+  r.brick = __shfl(v.brick, src, 64);
+  r.local = 0;
+  return r;
+}
+__device__ __forceinline__ Vox shfl_up_vox(const Vox& v) {
+  Vox r;
+  r.x = __shfl_up(v.x, 1u, 64);
+  r.y = __shfl_up(v.y, 1u, 64);
+  r.z = __shfl_up(v.z, 1u, 64);
+  r.brick = __shfl_up(v.brick, 1u, 64);
+  r.local = 0;
+  return r;
+}
+
+// One wavefront per track: the lanes read 64 consecutive entries of the
+// track's list at a time; each entry's predecessor comes from the lane below
+// (lane 0: the last entry of the previous 64).
 __global__ void __launch_bounds__(256)
 k_seg_count(BrickGeo G, size_t n_trk, size_t nv,
             const uint64_t* __restrict__ row_ptr,
             const uint32_t* __restrict__ vox, const uint16_t* __restrict__ scan,
             uint32_t* __restrict__ nseg, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_trk) return;
+  const int lane = (int)(threadIdx.x & 63u);
+  const size_t t = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (t >= n_trk) return;  // uniform over the wavefront
   const uint64_t b = row_ptr[t], e = row_ptr[t + 1];
   uint32_t cnt = 0;
-  bool have = false, pvalid = false;
-  Vox pv{}, vf{}, vl{};
-  for (uint64_t k = b; k < e; ++k) {
-    const uint32_t j = vox[k];
-    if (j >= nv) { pvalid = false; continue; }
-    const Vox cv = locate(G, j);
-    if (!(pvalid && seg_continues(pv, cv))) ++cnt;
-    if (!have) { vf = cv; have = true; }
-    vl = cv;
-    pv = cv;
-    pvalid = true;
+  bool have = false, pcv = false;
+  Vox vf{}, vl{}, pc{};
+  for (uint64_t c0 = b; c0 < e; c0 += 64) {
+    const uint64_t k = c0 + (uint64_t)lane;
+    const bool in = k < e;
+    const uint32_t j = in ? vox[k] : 0xFFFFFFFFu;
+    const bool valid = in && j < nv;
+    const Vox cv = valid ? locate(G, j) : Vox{};
+    Vox pv = shfl_up_vox(cv);
+    bool pval = __shfl_up((int)valid, 1u, 64) != 0;
+    if (lane == 0) { pv = pc; pval = pcv; }
+    const bool start = valid && !(pval && seg_continues(pv, cv));
+    cnt += (uint32_t)__popcll(__ballot(start));
+    const unsigned long long vm = __ballot(valid);
+    if (vm) {
+      if (!have) {
+        vf = shfl_vox(cv, __ffsll((long long)vm) - 1);
+        have = true;
+      }
+      vl = shfl_vox(cv, 63 - __clzll((long long)vm));
+    }
+    pc = shfl_vox(cv, 63);
+    pcv = __shfl((int)valid, 63, 64) != 0;
   }
+  if (lane != 0) return;
   nseg[t] = cnt;
   val[t] = (uint32_t)t;
   if (!have) { key[t] = ~0ull; return; }
@@ -236,6 +273,10 @@ __global__ void k_invert(size_t n, const uint32_t* __restrict__ order,
 
 // Track-major segments: key (brick, rank), first entry, length, tail after
 // the last entry (FP64), track.
+// Segment records, one wavefront per track as in k_seg_count: a segment
+// start writes its key, first entry and track; the segment's last entry
+// (followed by a start, an entry outside the grid or the end of the list)
+// writes its length and the tail after it.
 __global__ void __launch_bounds__(256)
 k_seg_fill(BrickGeo G, size_t n_trk, size_t nv,
            const uint64_t* __restrict__ row_ptr,
@@ -245,42 +286,64 @@ k_seg_fill(BrickGeo G, size_t n_trk, size_t nv,
            uint32_t* __restrict__ sval, uint64_t* __restrict__ st_start,
            uint32_t* __restrict__ st_len, double* __restrict__ st_tend,
            uint32_t* __restrict__ st_trk) {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_trk) return;
+  const int lane = (int)(threadIdx.x & 63u);
+  const size_t t = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (t >= n_trk) return;  // uniform over the wavefront
   const uint64_t b = row_ptr[t], e = row_ptr[t + 1];
-  uint64_t s = sstart[t];
+  const uint64_t s0 = sstart[t];
   const uint32_t r = rank[t];
-  uint64_t cur = ~0ull, klast = 0;
-  uint32_t clen = 0;
-  bool pvalid = false;
-  Vox pv{};
-  // a segment's length and the tail after its last entry are written once,
-  // when the segment closes
-  auto close = [&]() {
-    if (cur != ~0ull) {
-      st_len[cur] = clen;
-      st_tend[cur] = tail[klast];
+  const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;  // lanes <= this one
+  uint64_t before = 0;      // segments started in earlier chunks
+  uint64_t last_start = b;  // list position of the latest start in earlier chunks
+  bool pcv = false;
+  Vox pc{};
+  for (uint64_t c0 = b; c0 < e; c0 += 64) {
+    const uint64_t k = c0 + (uint64_t)lane;
+    const bool in = k < e;
+    const uint32_t j = in ? vox[k] : 0xFFFFFFFFu;
+    const bool valid = in && j < nv;
+    const Vox cv = valid ? locate(G, j) : Vox{};
+    Vox pv = shfl_up_vox(cv);
+    bool pval = __shfl_up((int)valid, 1u, 64) != 0;
+    if (lane == 0) { pv = pc; pval = pcv; }
+    const bool start = valid && !(pval && seg_continues(pv, cv));
+    const unsigned long long sm = __ballot(start);
+    // whether entry k + 1 starts a segment or lies outside the grid / list
+    bool nvalid = __shfl_down((int)valid, 1u, 64) != 0;
+    bool nstart = __shfl_down((int)start, 1u, 64) != 0;
+    if (lane == 63) {
+      const bool in2 = k + 1 < e;
+      const uint32_t j2 = in2 ? vox[k + 1] : 0xFFFFFFFFu;
+      nvalid = in2 && j2 < nv;
+      nstart = nvalid && !(valid && seg_continues(cv, locate(G, j2)));
     }
-  };
-  for (uint64_t k = b; k < e; ++k) {
-    const uint32_t j = vox[k];
-    if (j >= nv) { pvalid = false; continue; }
-    const Vox cv = locate(G, j);
-    if (!(pvalid && seg_continues(pv, cv))) {
-      close();
-      cur = s++;
+    const bool last = valid && (!nvalid || nstart);
+    // position of the latest start at or before this entry: a max-scan of
+    // (offset + 1) over the lanes, 0 where none in this chunk
+    uint32_t sp = start ? (uint32_t)(k - b + 1) : 0u;
+#pragma unroll
+    for (unsigned o = 1; o < 64; o <<= 1) {
+      const uint32_t w = __shfl_up(sp, o, 64);
+      if ((unsigned)lane >= o) sp = max(sp, w);
+    }
+    const uint64_t spos = sp ? b + sp - 1 : last_start;
+    const uint64_t cur = s0 + before + (uint64_t)__popcll(sm & upto) - 1;  // this entry's segment
+    if (start) {
       skey[cur] = ((uint64_t)cv.brick << 32) | r;
       sval[cur] = (uint32_t)cur;
       st_start[cur] = k;
       st_trk[cur] = (uint32_t)t;
-      clen = 0;
     }
-    ++clen;
-    klast = k;
-    pv = cv;
-    pvalid = true;
+    if (last) {
+      st_len[cur] = (uint32_t)(k - spos + 1);
+      st_tend[cur] = tail[k];
+    }
+    before += (uint64_t)__popcll(sm);
+    const uint32_t sp63 = __shfl(sp, 63, 64);
+    if (sp63) last_start = b + sp63 - 1;
+    pc = shfl_vox(cv, 63);
+    pcv = __shfl((int)valid, 63, 64) != 0;
   }
-  close();
 }
 
 // lengths in brick-major order and the first segment of each brick
@@ -1294,8 +1357,9 @@ int fp32c_build(muto_em_plan* pl) {
   MUTO_CHECK_HIP(ctx, tkey.alloc(nt));
   MUTO_CHECK_HIP(ctx, sstart.alloc(nt + 1));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(nseg.p, 0, (nt + 1) * sizeof(uint32_t), s));
-  k_seg_count<<<grid_for(nt, 256), 256, 0, s>>>(G, nt, pl->nv, pl->row_ptr.p, pl->vox.p,
-                                                pl->trk_scan.p, nseg.p, tkey.p, order.p);
+  // one wavefront (64 threads) per track
+  k_seg_count<<<grid_for(nt, 4), 256, 0, s>>>(G, nt, pl->nv, pl->row_ptr.p, pl->vox.p,
+                                              pl->trk_scan.p, nseg.p, tkey.p, order.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   MUTO_CHECK_HIP(ctx, sort_pairs_u64_u32(tkey.p, order.p, nt, 0, 64, s));
   k_invert<<<grid_for(nt, 256), 256, 0, s>>>(nt, order.p, rank.p);
@@ -1321,9 +1385,9 @@ int fp32c_build(muto_em_plan* pl) {
     MUTO_CHECK_HIP(ctx, st_len.alloc(n_seg));
     MUTO_CHECK_HIP(ctx, st_tend.alloc(n_seg));
     MUTO_CHECK_HIP(ctx, st_trk.alloc(n_seg));
-    k_seg_fill<<<grid_for(nt, 256), 256, 0, s>>>(G, nt, pl->nv, pl->row_ptr.p, pl->vox.p,
-                                                 pl->tail.p, sstart.p, rank.p, skey.p, sval.p,
-                                                 st_start.p, st_len.p, st_tend.p, st_trk.p);
+    k_seg_fill<<<grid_for(nt, 4), 256, 0, s>>>(G, nt, pl->nv, pl->row_ptr.p, pl->vox.p,
+                                               pl->tail.p, sstart.p, rank.p, skey.p, sval.p,
+                                               st_start.p, st_len.p, st_tend.p, st_trk.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     MUTO_CHECK_HIP(ctx, sort_pairs_u64_u32(skey.p, sval.p, n_seg, 0, 32 + bits_for(nbk), s));
     // lengths (brick-major) and brick starts
