@@ -48,10 +48,11 @@ namespace {
 __global__ void __launch_bounds__(256)
 k_select(GridD g, const muto_ray* __restrict__ rays, size_t n, double eps,
          uint32_t* __restrict__ flag, double* __restrict__ pscat) {
+  __shared__ __align__(16) muto_ray sh[256];
+  stage_rays<256>(rays, n, sh);
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  muto_ray r;
-  load_ray(rays, i, r);
+  const muto_ray r = sh[threadIdx.x];
   double P[3];
   poca_point(r, eps, P);
   const bool a = mlsd_accept(g, P, eps, rays[0].pout[2], rays[0].pin[2]);
@@ -68,6 +69,8 @@ k_compact(GridD g, const muto_ray* __restrict__ rays, size_t n,
           int has_energy, uint32_t* __restrict__ sel,
           double* __restrict__ sel_p, double* __restrict__ trk4,
           double* __restrict__ pr, double* __restrict__ geo) {
+  __shared__ __align__(16) muto_ray sh[256];
+  stage_rays<256>(rays, n, sh);
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || !flag[i]) return;
   const size_t s = pos[i];
@@ -75,8 +78,7 @@ k_compact(GridD g, const muto_ray* __restrict__ rays, size_t n,
   sel_p[3 * s] = pscat[3 * i];
   sel_p[3 * s + 1] = pscat[3 * i + 1];
   sel_p[3 * s + 2] = pscat[3 * i + 2];
-  muto_ray r;
-  load_ray(rays, i, r);
+  const muto_ray r = sh[threadIdx.x];
   for (int k = 0; k < 3; ++k) {  // path endpoints for recomputing the walk
     geo[9 * s + k] = r.pin[k];
     geo[9 * s + 3 + k] = r.pout[k];
@@ -286,7 +288,7 @@ k_em64(size_t n_trk, const uint64_t* __restrict__ row_ptr,
        const double* __restrict__ pr, const double* __restrict__ lam,
        size_t nv, ErrM E, double* __restrict__ acc,
        const uint64_t* __restrict__ slot, double* __restrict__ S,
-       double* __restrict__ cost_acc, unsigned long long* __restrict__ cnt) {
+       double* __restrict__ cost_part, unsigned long long* __restrict__ cnt) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   double cost = 0.0;
   bool sing = false;
@@ -348,7 +350,9 @@ k_em64(size_t n_trk, const uint64_t* __restrict__ row_ptr,
         S[slot[k]] = sij;
     }
   }
-  atomic_block_sum(cost_acc, cost);
+  // per-block cost sums, added up in a fixed order by k_sum_parts_f64 (one
+  // atomic per track or wave on one address would serialise)
+  block_sum_to(cost_part, cost);
   wave_count(&cnt[C_SINGULAR], sing);
 }
 
@@ -366,7 +370,7 @@ k_em64w(size_t n_trk, const uint64_t* __restrict__ row_ptr,
         const double* __restrict__ pr, const double* __restrict__ lam,
         size_t nv, ErrM E, double* __restrict__ acc,
         const uint64_t* __restrict__ slot, double* __restrict__ S,
-        double* __restrict__ cost_acc, unsigned long long* __restrict__ cnt) {
+        double* __restrict__ cost_part, unsigned long long* __restrict__ cnt) {
   const unsigned lane = threadIdx.x & 63;
   const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   double cost = 0.0;
@@ -430,7 +434,9 @@ k_em64w(size_t n_trk, const uint64_t* __restrict__ row_ptr,
         S[slot[k]] = sij;
     }
   }
-  atomic_block_sum(cost_acc, cost);
+  // per-block cost sums, added up in a fixed order by k_sum_parts_f64 (one
+  // atomic per track or wave on one address would serialise)
+  block_sum_to(cost_part, cost);
   wave_count(&cnt[C_SINGULAR], sing);
 }
 
@@ -956,27 +962,32 @@ static int iterate64(muto_em_plan* pl, double* dst, bool time_it) {
     if (int rc = build_slots(pl)) return rc;
   const ErrM E = error_matrix(c);
   if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[0], s));
+  const unsigned nblk = pl->fp64_kernel == 1 ? grid_for(pl->n_trk, 4) : grid_for(pl->n_trk, 256);
+  if (pl->n_trk && pl->cost_part.n < nblk) MUTO_CHECK_HIP(ctx, pl->cost_part.alloc(nblk));
   if (pl->n_trk && pl->fp64_kernel == 1) {
     // 4 tracks per 256-thread block
-    const unsigned nblk = grid_for(pl->n_trk, 4);
     if (mean_all)
       k_em64w<0><<<nblk, 256, 0, s>>>(
           pl->n_trk, pl->row_ptr.p, pl->vox.p, pl->len.p, pl->tail.p, pl->trk4.p, pl->pr.p,
-          pl->lam64.p, pl->nv, E, pl->acc64.p, nullptr, nullptr, pl->cost_acc.p, pl->dcount.p);
+          pl->lam64.p, pl->nv, E, pl->acc64.p, nullptr, nullptr, pl->cost_part.p, pl->dcount.p);
     else
       k_em64w<1><<<nblk, 256, 0, s>>>(
           pl->n_trk, pl->row_ptr.p, pl->vox.p, pl->len.p, pl->tail.p, pl->trk4.p, pl->pr.p,
-          pl->lam64.p, pl->nv, E, nullptr, pl->slot.p, pl->S.p, pl->cost_acc.p, pl->dcount.p);
+          pl->lam64.p, pl->nv, E, nullptr, pl->slot.p, pl->S.p, pl->cost_part.p, pl->dcount.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   } else if (pl->n_trk) {
     if (mean_all)
       k_em64<0><<<grid_for(pl->n_trk, 256), 256, 0, s>>>(
           pl->n_trk, pl->row_ptr.p, pl->vox.p, pl->len.p, pl->tail.p, pl->trk4.p, pl->pr.p,
-          pl->lam64.p, pl->nv, E, pl->acc64.p, nullptr, nullptr, pl->cost_acc.p, pl->dcount.p);
+          pl->lam64.p, pl->nv, E, pl->acc64.p, nullptr, nullptr, pl->cost_part.p, pl->dcount.p);
     else
       k_em64<1><<<grid_for(pl->n_trk, 256), 256, 0, s>>>(
           pl->n_trk, pl->row_ptr.p, pl->vox.p, pl->len.p, pl->tail.p, pl->trk4.p, pl->pr.p,
-          pl->lam64.p, pl->nv, E, nullptr, pl->slot.p, pl->S.p, pl->cost_acc.p, pl->dcount.p);
+          pl->lam64.p, pl->nv, E, nullptr, pl->slot.p, pl->S.p, pl->cost_part.p, pl->dcount.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  }
+  if (pl->n_trk) {
+    k_sum_parts_f64<<<1, 256, 0, s>>>(pl->cost_part.p, nblk, pl->cost_acc.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
   if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[1], s));

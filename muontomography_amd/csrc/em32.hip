@@ -507,7 +507,14 @@ k_max_path(size_t n_trk, const uint64_t* __restrict__ row_ptr, const double* __r
   if (t < n_trk && row_ptr[t + 1] > row_ptr[t]) v = fmax(tail[row_ptr[t]] + len[row_ptr[t]], 0.0);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
+  // one atomic per block (same-address atomics serialise)
+  __shared__ double wm[4];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+    if (v > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
+  }
 }
 
 // Per-track data in rank order.
@@ -915,22 +922,50 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
       bnd = lmax * (fabs((double)g.x) + 2.0 * R * fabs((double)g.y) + R * R * fabs((double)g.z)) * 1.001;
     }
   }
+  // The |c| bound (a maximum) and the cost per scan, combined over the
+  // block first: one atomic per block on each address instead of one per
+  // wavefront (same-address atomics serialise).
+  __shared__ double red_m[16], red_c[16];
+  __shared__ uint32_t red_s[16];
+  const uint32_t w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   {
     double m = bnd;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(cmax, (unsigned long long)__double_as_longlong(m));
+    if ((threadIdx.x & 63) == 0) red_m[w] = m;
   }
-  // cost per scan: one atomic per wave when the wave's tracks share a scan
+  // cost per scan: summed per wavefront when its tracks share a scan, else
+  // added per track
   const uint32_t sc = (trk_scan && r < n_trk) ? trk_scan[r] : 0u;
   const uint32_t s0 = __shfl(sc, 0, 64);
-  if (__all(sc == s0)) {
+  const bool same = __all(sc == s0);
+  if (same) {
     double v = cost;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(&cost_acc[s0], v);
-  } else if (cost != 0.0) {
-    atomicAdd(&cost_acc[sc], cost);
+    if ((threadIdx.x & 63) == 0) { red_c[w] = v; red_s[w] = s0; }
+  } else {
+    if (cost != 0.0) atomicAdd(&cost_acc[sc], cost);
+    if ((threadIdx.x & 63) == 0) { red_c[w] = 0.0; red_s[w] = 0xFFFFFFFFu; }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (uint32_t k = 0; k < nw; ++k) m = fmax(m, red_m[k]);
+    if (m > 0.0) atomicMax(cmax, (unsigned long long)__double_as_longlong(m));
+    // waves of one scan in a row are added together
+    uint32_t cs = 0xFFFFFFFFu;
+    double cv = 0.0;
+    for (uint32_t k = 0; k < nw; ++k) {
+      if (red_s[k] == 0xFFFFFFFFu) continue;
+      if (red_s[k] != cs) {
+        if (cs != 0xFFFFFFFFu && cv != 0.0) atomicAdd(&cost_acc[cs], cv);
+        cs = red_s[k];
+        cv = 0.0;
+      }
+      cv += red_c[k];
+    }
+    if (cs != 0xFFFFFFFFu && cv != 0.0) atomicAdd(&cost_acc[cs], cv);
   }
   unsigned long long m = __ballot(sing);
   if (m && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)m) - 1))

@@ -72,6 +72,16 @@ __device__ __forceinline__ void block_sum_to(double* part, double v) {
   }
 }
 
+namespace {
+// out[0] = sum of part[0..n) in a fixed order (launch with one block of 256)
+__global__ void __launch_bounds__(256)
+k_sum_parts_f64(const double* __restrict__ part, size_t n, double* __restrict__ out) {
+  double v = 0.0;
+  for (size_t k = threadIdx.x; k < n; k += blockDim.x) v += part[k];
+  block_sum_to(out, v);
+}
+}  // namespace
+
 __device__ __forceinline__ void wave_count(unsigned long long* dst, bool v) {
   unsigned long long m = __ballot(v);
   if (m && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)m) - 1))

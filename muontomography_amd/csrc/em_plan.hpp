@@ -133,6 +133,7 @@ struct muto_em_plan {
   muto::DBuf<double> len, tail;
   muto::DBuf<uint32_t> mj;      // entries per voxel
   muto::DBuf<double> geo;       // [9*n_trk] pin, pout, PoCA point (path recompute)
+  muto::DBuf<double> cost_part; // per-block cost sums of the FP64 iteration
   uint32_t n_scans = 1;         // > 1: a batch of scans laid side by side in one grid
   muto::DBuf<uint16_t> trk_scan;  // [n_trk] scan of each track (batches only)
   bool has_lists = true;        // row_ptr/vox/len/tail hold the paths

@@ -167,4 +167,22 @@ __device__ __forceinline__ void load_ray(const muto_ray* __restrict__ rays,
   for (int k = 0; k < 14; ++k) d[k] = s[k];
 }
 
+// The block's rays [blockIdx.x * BS, +BS) (those below n) copied to LDS with
+// 16-byte loads that consecutive threads issue at consecutive addresses (a
+// ray is 112 bytes, seven such words); every thread of the block must call
+// it.  Each thread then reads its own ray from LDS instead of making 14
+// strided 8-byte loads.
+static_assert(sizeof(muto_ray) == 112, "a ray is seven 16-byte words");
+
+template <unsigned BS>
+__device__ __forceinline__ void stage_rays(const muto_ray* __restrict__ rays, size_t n,
+                                           muto_ray* sh) {
+  const size_t base = (size_t)blockIdx.x * BS;
+  const size_t cnt = base < n ? (n - base < BS ? n - base : BS) : 0;
+  const uint4* src = reinterpret_cast<const uint4*>(rays + base);
+  uint4* dst = reinterpret_cast<uint4*>(sh);
+  for (size_t u = threadIdx.x; u < cnt * 7; u += BS) dst[u] = src[u];
+  __syncthreads();
+}
+
 }  // namespace muto

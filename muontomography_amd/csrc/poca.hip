@@ -14,11 +14,28 @@ struct PocaCounters {
   unsigned long long not_parallel, in_grid, out_of_grid;
 };
 
-__device__ __forceinline__ void wave_count(unsigned long long* dst, bool v) {
-  // one atomic per wavefront
-  unsigned long long m = __ballot(v);
-  if (m && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)m) - 1))
-    atomicAdd(dst, (unsigned long long)__popcll(m));
+// Counts of a 256-thread block added to dst[0..2] with one atomic each
+// (every thread must call it).  Per-wavefront atomics on a single address
+// serialise behind each other: 1e7 rays make 3e5 of them.
+__device__ __forceinline__ void block_counts(unsigned long long* d0, bool c0,
+                                             unsigned long long* d1, bool c1,
+                                             unsigned long long* d2, uint32_t c2) {
+  __shared__ uint32_t ws[4][3];
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t a = (uint32_t)__popcll(__ballot(c0));
+  const uint32_t b = (uint32_t)__popcll(__ballot(c1));
+  uint32_t c = c2;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) { ws[w][0] = a; ws[w][1] = b; ws[w][2] = c; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t0 = 0, t1 = 0, t2 = 0;
+    for (int k = 0; k < 4; ++k) { t0 += ws[k][0]; t1 += ws[k][1]; t2 += ws[k][2]; }
+    if (d0 && t0) atomicAdd(d0, t0);
+    if (d1 && t1) atomicAdd(d1, t1);
+    if (d2 && t2) atomicAdd(d2, t2);
+  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -26,11 +43,12 @@ k_poca_points(GridD g, const muto_ray* __restrict__ rays, size_t n,
               double eps, int mode, double* __restrict__ pscat,
               uint32_t* __restrict__ vox, uint8_t* __restrict__ accept,
               PocaCounters* __restrict__ cnt) {
+  __shared__ __align__(16) muto_ray sh[256];
+  stage_rays<256>(rays, n, sh);
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool npar = false, acc = false;
   if (i < n) {
-    muto_ray r;
-    load_ray(rays, i, r);
+    const muto_ray r = sh[threadIdx.x];
     const double zmin_det = rays[0].pout[2], zmax_det = rays[0].pin[2];
     double P[3];
     npar = poca_point(r, eps, P);
@@ -48,8 +66,7 @@ k_poca_points(GridD g, const muto_ray* __restrict__ rays, size_t n,
     if (vox) { vox[3 * i] = xs; vox[3 * i + 1] = ys; vox[3 * i + 2] = zs; }
     if (accept) accept[i] = acc ? 1 : 0;
   }
-  wave_count(&cnt->not_parallel, npar);
-  wave_count(&cnt->in_grid, acc);
+  block_counts(&cnt->not_parallel, npar, &cnt->in_grid, acc, nullptr, 0u);
 }
 
 struct PocaCfgD {
@@ -64,13 +81,14 @@ k_poca_image(PocaCfgD c, const muto_ray* __restrict__ rays, size_t n,
              double* __restrict__ img, double* __restrict__ wimg,
              PocaCounters* __restrict__ cnt, uint32_t* __restrict__ cut_vox,
              double* __restrict__ cut_val) {
+  __shared__ __align__(16) muto_ray sh[256];
+  stage_rays<256>(rays, n, sh);
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool npar = false, acc = false;
   uint32_t nout = 0;
   const GridD& g = c.g;
   if (i < n) {
-    muto_ray r;
-    load_ray(rays, i, r);
+    const muto_ray r = sh[threadIdx.x];
     const double zmin_det = rays[0].pout[2], zmax_det = rays[0].pin[2];
     double P[3];
     npar = poca_point(r, c.eps, P);
@@ -116,9 +134,7 @@ k_poca_image(PocaCfgD c, const muto_ray* __restrict__ rays, size_t n,
       }
     }
   }
-  wave_count(&cnt->not_parallel, npar);
-  wave_count(&cnt->in_grid, acc);
-  if (__any(nout != 0) && nout) atomicAdd(&cnt->out_of_grid, (unsigned long long)nout);
+  block_counts(&cnt->not_parallel, npar, &cnt->in_grid, acc, &cnt->out_of_grid, nout);
 }
 
 // SLR (MutoSLR.cpp:47-103): theta^2 / 2 spread along the chord between
@@ -127,12 +143,13 @@ __global__ void __launch_bounds__(256)
 k_slr_image(PocaCfgD c, const muto_ray* __restrict__ rays, size_t n,
             double* __restrict__ img, double* __restrict__ wimg,
             PocaCounters* __restrict__ cnt) {
+  __shared__ __align__(16) muto_ray sh[256];
+  stage_rays<256>(rays, n, sh);
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t nout = 0;
   const GridD& g = c.g;
   if (i < n) {
-    muto_ray r;
-    load_ray(rays, i, r);
+    const muto_ray r = sh[threadIdx.x];
     const double axz = (atan2(r.din[0], r.din[2]) - atan2(r.dout[0], r.dout[2])) * c.angmag;
     const double ayz = (atan2(r.din[1], r.din[2]) - atan2(r.dout[1], r.dout[2])) * c.angmag;
     const double zlo = g.mn[2];
@@ -169,7 +186,7 @@ k_slr_image(PocaCfgD c, const muto_ray* __restrict__ rays, size_t n,
       atomicAdd(&img[t], v);
     });
   }
-  if (__any(nout != 0) && nout) atomicAdd(&cnt->out_of_grid, (unsigned long long)nout);
+  block_counts(nullptr, false, nullptr, false, &cnt->out_of_grid, nout);
 }
 
 // PoCut per-voxel lists: counts, slots, then the cut sums.
