@@ -12,12 +12,13 @@
 //   - each brick's segment list is cut into tasks (the persistent kernels'
 //     work units) and each task into chunks of at most 256 segments and
 //     ECAP entries;
-//   - entries are stored brick-major as (u16 voxel-in-brick, f32 length):
-//     6 bytes per track-voxel entry; each chunk starts on a 16-entry
-//     boundary and is padded to whole 16-entry groups with zero-length
-//     entries, and each group has a descriptor (segment-start mask, the
-//     segment holding its first entry, the length of its last segment after
-//     the group): 0.5 bytes per entry.
+//   - entries are stored brick-major as one 32-bit word each (the FP32
+//     length and a 3-bit step from the previous voxel of the segment; a
+//     segment's first voxel is kept per segment); each chunk starts on a
+//     16-entry boundary and is padded to whole 16-entry groups with zero
+//     words, and each group has a descriptor (segment-start mask, the
+//     segment holding its first entry, its first voxel, the length of its
+//     last segment after the group): about 4.6 bytes per entry in all.
 // The passes over entries are entry-parallel: thread t of a block takes the
 // chunk's group t (16-byte vector loads) and walks it backwards, the tail of
 // each entry inside its segment following from the group's carry; every
@@ -32,9 +33,10 @@
 //             flushed once per task;
 //   k_upd32   per voxel: lambda += lambda^2/(2 M_j) sum_i c_ij, which is
 //             half the mean of S_ij = 2 lambda_j + c_ij lambda_j^2.
-// Accumulation (acc_mode): 1 (default) FP32 sums per segment piece (at most
-// 16 entries) in the forward walk, combined per segment in FP64, and one
-// int64 fixed-point LDS atomic per entry in the backprojection; 0 FP32 with
+// Accumulation (acc_mode): 1 (default) FP64 sums of the FP32 entry terms
+// per segment piece (at most 16 entries) in the forward walk, each rounded
+// once to FP32 and combined per segment in FP64, and one int64 fixed-point
+// LDS atomic per entry in the backprojection; 0 FP32 with
 // two-sum compensation -- error-free transformation in the forward walk,
 // atomic two-sum (returned old value + error word) in the backprojection.
 // The persistent kernels take tasks from a queue sorted by size, largest
