@@ -398,8 +398,9 @@ def _rel_stats(a, ref, mj):
 
 
 def test_fp32c_matches_fp64(ctx, small_set):
-    """FP32C (FP32 sums with two-sum, FP64 2x2 algebra) stays within 1e-4
-    relative of FP64 after 20 iterations, mean update."""
+    """FP32C (FP32 entry terms, FP64 and int64 fixed-point sums, FP64 2x2
+    algebra) stays within 1e-4 relative of FP64 after 20 iterations, mean
+    update."""
     g, rays, _ = small_set
     cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100)
     p64 = ctx.em_plan(cfg, rays=rays)
@@ -532,6 +533,36 @@ def test_map_fp64(ctx, small_set, nb, obj, p):
     ig, cg, ng = ctx.map_reconstruct(cfg, rays)
     np.testing.assert_allclose(ig, io, rtol=1e-8, atol=1e-20)
     assert ng["n_cubic_warn"] == no["n_cubic_warn"]
+
+
+@pytest.mark.parametrize("nb,p", [(0, 2.0), (1, 1.2)])
+def test_map_fp32c_iterations_close_to_fp64(ctx, small_set, nb, p):
+    """MAP iterations (MutoMAP.cpp:39-205) in FP32C -- the mean ML step from
+    the brick stream, then the FP64 voxel step on its image -- against the
+    same iterations in FP64: images within 1e-4 relative after 8 iterations,
+    costs (of the ML step) within 1e-5."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, beta=1.0, p=p, delta=0.1,
+                  all_neighbours=nb, add_object_prior=0, num_iteration=8)
+    p64 = ctx.em_plan(cfg, rays=rays)
+    p32 = ctx.em_plan(cfg, rays=rays)
+    img = np.full(g.n_vox, 1e-7)
+    p64.set_image(img)
+    p32.set_image(img)
+    c64 = p64.run(8, M.MUTO_FP64, map_=True)
+    c32 = p32.run(8, M.MUTO_FP32C, map_=True)
+    a64, a32 = p64.image(), p32.image()
+    assert np.all(np.isfinite(a32)) and np.abs(a64).max() > 0
+    mx, rms, norm = _rel_stats(a32, a64, p64.voxel_counts())
+    assert mx <= 1e-4, (mx, rms, norm)
+    np.testing.assert_allclose(c32, c64, rtol=1e-5)
+    # the MAP step moved the image away from the plain ML result
+    p_ml = ctx.em_plan(cfg, rays=rays)
+    p_ml.set_image(img)
+    p_ml.run(8, M.MUTO_FP32C)
+    assert np.abs(p_ml.image() - a32).max() > 1e-3 * np.abs(a32).max()
+    for q in (p64, p32, p_ml):
+        q.close()
 
 
 def _lam_field(n, seed=7):
