@@ -565,6 +565,38 @@ def test_map_fp32c_iterations_close_to_fp64(ctx, small_set, nb, p):
         q.close()
 
 
+def test_early_stop_map_from_ml_image(ctx, small_set):
+    """early_stop (MutoMLSD.cpp:209-212, MutoMAP.cpp:194-197): MAP started
+    from a converged ML image raises the ML cost at its second iteration, so
+    the run stops there and keeps that iteration's image, as the oracle
+    does.  One-shot FP64 against the oracle; FP64 and FP32C plans stop at
+    the same iteration."""
+    g, rays, _ = small_set
+    base = dict(update_with_mean=1, mean_percentile=100)
+    plan = ctx.em_plan(_em_cfg(g, num_iteration=60, **base), rays=rays)
+    plan.set_image(np.full(g.n_vox, 1e-7))
+    plan.run(60, M.MUTO_FP64)
+    ml = plan.image()
+    plan.close()
+    cfg = _em_cfg(g, num_iteration=10, beta=1.0, p=2.0, delta=0.1, early_stop=1,
+                  initialize_method=0, **base)
+    io, co, no = oracle.mlsd_reconstruct(cfg, rays, init=ml, map_=True)
+    assert no["iterations_done"] == 2 and co[1] > co[0], co
+    ig, cg, ng = ctx.map_reconstruct(cfg, rays, init=ml)
+    assert ng["iterations_done"] == 2
+    np.testing.assert_allclose(cg, co, rtol=1e-9)
+    np.testing.assert_allclose(ig, io, rtol=1e-8, atol=1e-20)
+    for prec, tol in ((M.MUTO_FP64, 1e-8), (M.MUTO_FP32C, 1e-4)):
+        p = ctx.em_plan(cfg, rays=rays)
+        p.set_image(ml)
+        c = p.run(10, prec, map_=True)
+        assert len(c) == 2, (prec, c)
+        np.testing.assert_allclose(c, co, rtol=tol)
+        mx, rms, norm = _rel_stats(p.image(), io, p.voxel_counts())
+        assert mx <= tol, (prec, mx, rms, norm)
+        p.close()
+
+
 def _lam_field(n, seed=7):
     rng = np.random.default_rng(seed)
     return 1e-7 * (0.5 + rng.random(n))

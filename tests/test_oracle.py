@@ -662,3 +662,26 @@ def test_mlsd_reconstruct_recovers_dense_blocks():
     dense = truth > 1e-7
     assert img[dense].mean() > 3 * img[~dense & (img > 0)].mean()
     assert np.all(np.diff(cost) < 0)
+
+
+def test_early_stop_keeps_the_rising_iteration():
+    """early_stop (MutoMLSD.cpp:209-212, MutoMAP.cpp:194-197): the loop
+    breaks after the first iteration whose cost exceeds the one before, so
+    the result equals a plain run of that many iterations.  MAP started from
+    a converged ML image raises the ML cost at its second iteration."""
+    g, rays, _ = W.muons("small", n_muons=4000)
+    base = dict(error_xy=0.5, error_dz_outer=500.0, error_dz_inner=500.0, update_with_mean=1,
+                mean_percentile=100)
+    ml, cost, _ = oracle.mlsd_reconstruct(
+        mlsd_cfg(g, initialize_method=2, initialize_value=1e-7, num_iteration=40, **base), rays)
+    assert np.all(np.diff(cost) < 0)  # plain ML never stops early here
+    mp = dict(beta=1.0, p=2.0, delta=0.1, initialize_method=0, **base)
+    img_es, c_es, d_es = oracle.mlsd_reconstruct(
+        mlsd_cfg(g, num_iteration=10, early_stop=1, **mp), rays, init=ml, map_=True)
+    k = d_es["iterations_done"]
+    assert 2 <= k < 10 and c_es[k - 1] > c_es[k - 2] and np.all(np.diff(c_es[:k - 1]) <= 0)
+    img_k, c_k, d_k = oracle.mlsd_reconstruct(
+        mlsd_cfg(g, num_iteration=k, early_stop=0, **mp), rays, init=ml, map_=True)
+    assert d_k["iterations_done"] == k
+    np.testing.assert_array_equal(c_es, c_k)
+    np.testing.assert_array_equal(img_es, img_k)
