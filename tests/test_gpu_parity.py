@@ -329,6 +329,34 @@ def test_em_fp64_prepared(ctx, small_set, mode):
             f"max abs diff {np.max(np.abs(out - ref))}"
 
 
+@pytest.mark.parametrize("mode", ["median", "mean"])
+def test_em_singular_covariances(ctx, small_set, mode):
+    """Covariances that are not invertible (MutoMLSD.cpp:456-463): with no
+    measurement error and lambda = 0 over half the grid, the tracks that
+    cross only that half have Sigma = 0.  Each such track counts a warning,
+    adds no cost and leaves S_ij = 2 lambda_j (SigmaInv stays zero).  The
+    GPU path matches the oracle's image and costs as the regular case does,
+    and counts the same singular tracks."""
+    g, rays, _ = small_set
+    kw = dict(update_with_mean=0) if mode == "median" else dict(update_with_mean=1, mean_percentile=100)
+    cfg = _em_cfg(g, error_xy=0.0, **kw)  # error matrix E = 0
+    em = oracle.Em(cfg, rays)
+    img = np.where(np.arange(g.n_vox) % g.nx < g.nx // 2, 1e-7, 0.0)
+    ref, cost, ns = em.iterate(img)
+    assert 0 < ns < em.n_sel, (ns, em.n_sel)
+    plan = ctx.em_plan(cfg, prepared=em.prepared())
+    plan.set_image(img)
+    cg = plan.run(1)
+    out = plan.image()
+    assert plan.counters()["n_singular"] == ns
+    np.testing.assert_allclose(cg, [cost], rtol=1e-12)
+    if mode == "mean":
+        np.testing.assert_allclose(out, ref, rtol=1e-13, atol=0)
+    else:
+        assert np.array_equal(out.view(np.uint64), ref.view(np.uint64))
+    plan.close()
+
+
 @pytest.mark.parametrize("mode", ["median", "trimmed", "mean"])
 def test_em_fp64_wavefront_kernel(ctx, small_set, mode):
     """The wavefront-per-track FP64 kernel (MUTO_OPT_FP64_KERNEL 1) sums each

@@ -685,3 +685,28 @@ def test_early_stop_keeps_the_rising_iteration():
     assert d_k["iterations_done"] == k
     np.testing.assert_array_equal(c_es, c_k)
     np.testing.assert_array_equal(img_es, img_k)
+
+
+@pytest.mark.parametrize("f", [1.001, 0.999])
+def test_em_determinant_threshold(f):
+    """Eigen's computeInverseAndDetWithCheck with its default threshold
+    (|det| > 1e-12, the inverse left unwritten otherwise), as
+    MutoMLSD.cpp:456-463 uses it: one track, one entry, no measurement
+    error, so det Sigma = p^4 lambda^2 L^4 / 12 is set to 1e-12 * f.  Above
+    the threshold the cost and S follow the closed form; below it the track
+    counts a warning, adds no cost and leaves S = 2 lambda."""
+    g, prep = _single_voxel_problem(1, seed=3)
+    trk4, p, row, vox, ln = prep
+    lam = np.sqrt(12.0e-12 * f) / (p[0] ** 2 * ln[0] ** 2)
+    cfg = mlsd_cfg(g, update_with_mean=1, mean_percentile=100, error_xy=0.0,
+                   error_dz_outer=500.0, error_dz_inner=500.0)
+    em = oracle.Em(cfg, prepared=prep)
+    out, cost, ns = em.iterate(np.array([lam]))
+    if f > 1.0:
+        S, c = _closed_form_S(lam, prep, e=0.0)
+        assert ns == 0
+        np.testing.assert_allclose(cost, c[0], rtol=1e-9)
+        np.testing.assert_allclose(out[0], 0.5 * S[0], rtol=1e-9)
+    else:
+        assert ns == 1 and cost == 0.0
+        assert out[0] == lam  # half of S = 2 lambda
