@@ -625,6 +625,32 @@ def test_early_stop_map_from_ml_image(ctx, small_set):
         p.close()
 
 
+def test_initial_image_methods(ctx, small_set):
+    """initailizeImage (MutoMLSD.cpp:226-266), through the one-shot entry
+    point with no iteration: method 2 is the constant; method 1 draws
+    N(2, 0.5) * 1e-7 floored at 1e-10 (the reference seeds it from
+    std::random_device, here a fixed seed, so runs repeat); method 0 with no
+    image given falls back to method 1; method 0 with an image starts from
+    it."""
+    g, rays, _ = small_set
+    rays = rays[:2000]
+    out = {}
+    for m in (0, 1, 2):
+        cfg = _em_cfg(g, num_iteration=0, initialize_method=m, initialize_value=3e-7)
+        img, cost, c = ctx.mlsd_reconstruct(cfg, rays)
+        assert c["iterations_done"] == 0 and len(cost) == 0
+        out[m] = img
+    assert np.all(out[2] == 3e-7)
+    r = out[1]
+    assert r.min() >= 1e-10 and abs(r.mean() / 2e-7 - 1) < 0.01 and abs(r.std() / 0.5e-7 - 1) < 0.02
+    assert np.array_equal(out[0], out[1])
+    img2, _, _ = ctx.mlsd_reconstruct(_em_cfg(g, num_iteration=0, initialize_method=1), rays)
+    assert np.array_equal(img2, out[1])
+    given = np.linspace(1e-8, 1e-6, g.n_vox)
+    img3, _, _ = ctx.mlsd_reconstruct(_em_cfg(g, num_iteration=0, initialize_method=0), rays, init=given)
+    assert np.array_equal(img3, given)
+
+
 def _lam_field(n, seed=7):
     rng = np.random.default_rng(seed)
     return 1e-7 * (0.5 + rng.random(n))
