@@ -81,11 +81,21 @@ constexpr uint32_t MAX_TASK_SEG = 1u << 16;
 // (forward: piece tables, first voxels, step table, task word; backward:
 // G' table, step table, task word)
 constexpr size_t fwd_static_lds(uint32_t bs) { return 2 * (size_t)bs * 16 + (size_t)bs * 4 + 64 + 16; }
-constexpr size_t bwd_static_lds(uint32_t bs) { return (size_t)bs * 16 + 64 + 16; }
+// A chunk holds at most block - 2 segments.  The backprojection can take
+// two chunks per workgroup of twice the block size: its G' table for them
+// (2 (block - 2) entries of 16 bytes), step table and task word then come to
+// 16368 bytes at block 512, so two such workgroups of a 16 x 16 x 32 brick
+// (64 KiB of int64 sums each) fit the 160 KiB of LDS of one CU.
+constexpr uint32_t chunk_seg_cap(uint32_t bs) { return bs - 2; }
+constexpr size_t bwd_static_lds(uint32_t bs, uint32_t sub) {
+  return (size_t)sub * chunk_seg_cap(bs) * 16 + 32 + 16;
+}
 static_assert(fwd_static_lds(256) % 16 == 0 && fwd_static_lds(512) % 16 == 0 &&
               fwd_static_lds(1024) % 16 == 0, "forward static LDS");
-static_assert(bwd_static_lds(256) % 16 == 0 && bwd_static_lds(512) % 16 == 0 &&
-              bwd_static_lds(1024) % 16 == 0, "backward static LDS");
+static_assert(bwd_static_lds(256, 1) % 16 == 0 && bwd_static_lds(512, 1) % 16 == 0 &&
+              bwd_static_lds(1024, 1) % 16 == 0 && bwd_static_lds(256, 2) % 16 == 0 &&
+              bwd_static_lds(512, 2) % 16 == 0, "backward static LDS");
+static_assert(bwd_static_lds(512, 2) + 65536 <= 81920, "two backprojection workgroups per CU");
 
 // Unsigned division by a divisor fixed for a launch: with l = ceil(log2 d)
 // and m = floor(2^32 (2^l - d) / d) + 1, n / d = (umulhi(n, m) + n) >> l for
@@ -979,22 +989,29 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
 // One thread per segment first moves G' from the track's basis to the
 // segment's local one (FP64) into an LDS table; then entry-parallel like the
 // forward pass.  Two barriers per chunk (one table keeps the LDS small).
-// ACC = 1: one FP64 LDS atomic per entry;
+// SUB chunks of CW threads each per workgroup (CW the plan's block size):
+// with SUB = 2 each half of a workgroup of 2 CW threads takes one of two
+// consecutive chunks of the task, both adding into the brick's sums, so
+// that the brick array is held once for twice the waves.
+// ACC = 1: one int64 fixed-point LDS atomic per entry;
 // ACC = 0: FP32 atomic two-sum -- the returned old value o and the stored
 //          o + c give the exact rounding error, added into a low word.
-template <int ACC, uint32_t BS>
-__global__ void __launch_bounds__(BS)
+template <int ACC, uint32_t CW, uint32_t SUB>
+__global__ void __launch_bounds__(CW * SUB)
 k_bwd32(IterArgs a, const float* __restrict__ trk_g,
         const unsigned long long* __restrict__ cmax, double* __restrict__ acc) {
+  constexpr uint32_t TAB = chunk_seg_cap(CW);  // G' entries per chunk
   // static LDS is a multiple of 16 bytes and the dynamic region is declared
   // as 16-byte aligned doubles: ds_add_f64 needs 8-byte aligned addresses
   extern __shared__ __align__(16) double lds_dyn[];
   float* lds_acc = reinterpret_cast<float*>(lds_dyn);
-  __shared__ float4 sg[BS];
-  __shared__ int stab[16];
+  __shared__ float4 sg[SUB * TAB];
+  __shared__ int stab[8];
   __shared__ uint32_t sh_[4];
   uint32_t& sh = sh_[0];
   const uint32_t t = threadIdx.x;
+  const uint32_t sub = SUB == 1 ? 0u : t / CW;  // this thread's chunk of the SUB (whole wavefronts)
+  const uint32_t tl = SUB == 1 ? t : t % CW;    // and its thread in that chunk
   fill_step_table(a, stab);
   const double U = a.ec.unit, U2 = U * U, U3 = U2 * U;
   const uint32_t vol = a.G.b[0] * a.G.b[1] * a.G.b[2];
@@ -1012,23 +1029,24 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
     const uint32_t ti = sh;
     if (ti >= a.n_tasks) break;
     const Task tk = a.tasks[ti];
-    for (uint32_t c = tk.c0; c < tk.c1; ++c) {
-      const Chunk ch = a.chunks[c];
-      float4* sgb = sg;
-      if (t < ch.cnt) {
-        const uint64_t q = ch.q0 + t;
+    for (uint32_t c0 = tk.c0; c0 < tk.c1; c0 += SUB) {
+      Chunk ch{};  // no chunk (cnt = ne = 0) past the task's last
+      if (c0 + sub < tk.c1) ch = a.chunks[c0 + sub];
+      float4* sgb = sg + sub * TAB;
+      if (tl < ch.cnt) {
+        const uint64_t q = ch.q0 + tl;
         const uint32_t tfx = a.seg_tfx[q];
         const float4 g = *reinterpret_cast<const float4*>(trk_g + 4 * (size_t)a.seg_trk[q]);
         // A^T G' A with A = [[1, 0], [s, 1]], s = tau_q - t_c (exact)
         const double shf = (double)((int64_t)tfx - (int64_t)__float_as_uint(g.w)) * a.inv_scale;
         const double gy = fma(shf, (double)g.z, (double)g.y);
         // in units of U for the entry terms: g00 U, g01 U^2, g11 U^3
-        sgb[t] = make_float4((float)(fma(shf, gy + (double)g.y, (double)g.x) * U), (float)(gy * U2),
-                             (float)((double)g.z * U3), __uint_as_float((uint32_t)a.seg_v0[q]));
+        sgb[tl] = make_float4((float)(fma(shf, gy + (double)g.y, (double)g.x) * U), (float)(gy * U2),
+                              (float)((double)g.z * U3), __uint_as_float((uint32_t)a.seg_v0[q]));
       }
-      const bool act = t * GE < ch.ne;
+      const bool act = tl * GE < ch.ne;
       Group gr;
-      if (act) gr.load(a, (ch.base / GE) + t);
+      if (act) gr.load(a, (ch.base / GE) + tl);
       __syncthreads();
       if (act) {
         uint32_t vx[GE];
@@ -1318,14 +1336,31 @@ static hipError_t by_block(uint32_t block, F&& f) {
   }
 }
 
-static hipError_t occupancy(int acc, uint32_t block, int* f, int* b, size_t lds_f, size_t lds_b) {
+// Workgroups per CU of the forward pass (*f) and of the backprojection
+// (*b), and the backprojection's chunks per workgroup (*sub): two when that
+// keeps more waves resident (the brick array is held once per workgroup).
+static hipError_t occupancy(int acc, uint32_t block, int* f, int* b, int* sub, size_t lds_f,
+                            size_t lds_b) {
   return by_block(block, [&](auto B) {
     constexpr uint32_t BS = decltype(B)::value;
     hipError_t e = acc == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(f, k_fwd32<1, BS>, BS, lds_f)
                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(f, k_fwd32<0, BS>, BS, lds_f);
     if (e != hipSuccess) return e;
-    return acc == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(b, k_bwd32<1, BS>, BS, lds_b)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(b, k_bwd32<0, BS>, BS, lds_b);
+    e = acc == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(b, k_bwd32<1, BS, 1>, BS, lds_b)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(b, k_bwd32<0, BS, 1>, BS, lds_b);
+    if (e != hipSuccess) return e;
+    *sub = 1;
+    if constexpr (BS <= 512) {
+      int b2 = 0;
+      e = acc == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_bwd32<1, BS, 2>, 2 * BS, lds_b)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_bwd32<0, BS, 2>, 2 * BS, lds_b);
+      if (e != hipSuccess) return e;
+      if (2 * b2 > *b) {
+        *b = b2;
+        *sub = 2;
+      }
+    }
+    return hipSuccess;
   });
 }
 
@@ -1347,10 +1382,19 @@ static hipError_t launch_bwd(const Fp32cData& f, const IterArgs& a, const float*
   const size_t lds = 2 * f.bs.vol() * sizeof(float);
   return by_block(f.bs.block, [&](auto B) {
     constexpr uint32_t BS = decltype(B)::value;
+    if constexpr (BS <= 512) {
+      if (f.bwd_sub == 2) {
+        if (f.acc_mode == 1)
+          k_bwd32<1, BS, 2><<<f.grid_bwd, 2 * BS, lds, s>>>(a, g, cmax, acc);
+        else
+          k_bwd32<0, BS, 2><<<f.grid_bwd, 2 * BS, lds, s>>>(a, g, cmax, acc);
+        return hipGetLastError();
+      }
+    }
     if (f.acc_mode == 1)
-      k_bwd32<1, BS><<<f.grid_bwd, BS, lds, s>>>(a, g, cmax, acc);
+      k_bwd32<1, BS, 1><<<f.grid_bwd, BS, lds, s>>>(a, g, cmax, acc);
     else
-      k_bwd32<0, BS><<<f.grid_bwd, BS, lds, s>>>(a, g, cmax, acc);
+      k_bwd32<0, BS, 1><<<f.grid_bwd, BS, lds, s>>>(a, g, cmax, acc);
     return hipGetLastError();
   });
 }
@@ -1363,7 +1407,7 @@ static bool fp32c_fits(muto_ctx* ctx, const BrickShape& b) {
   if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess)
     return false;
   const size_t fwd = fwd_static_lds(b.block) + (size_t)b.vol() * sizeof(float);
-  const size_t bwd = bwd_static_lds(b.block) + 2 * (size_t)b.vol() * sizeof(float);
+  const size_t bwd = bwd_static_lds(b.block, 1) + 2 * (size_t)b.vol() * sizeof(float);
   return std::max(fwd, bwd) <= (size_t)max_lds;
 }
 
@@ -1408,8 +1452,9 @@ int fp32c_build(muto_em_plan* pl) {
   f.n_seg = n_seg;
   // occupancy-derived persistent grids
   const size_t lds_f = f.bs.vol() * sizeof(float), lds_b = 2 * f.bs.vol() * sizeof(float);
-  int occ_f = 1, occ_b = 1;
-  MUTO_CHECK_HIP(ctx, occupancy(f.acc_mode, f.bs.block, &occ_f, &occ_b, lds_f, lds_b));
+  int occ_f = 1, occ_b = 1, sub_b = 1;
+  MUTO_CHECK_HIP(ctx, occupancy(f.acc_mode, f.bs.block, &occ_f, &occ_b, &sub_b, lds_f, lds_b));
+  f.bwd_sub = sub_b;
   f.grid_fwd = std::max(1, occ_f) * ctx->n_cu;
   f.grid_bwd = std::max(1, occ_b) * ctx->n_cu;
   {
@@ -1453,7 +1498,8 @@ int fp32c_build(muto_em_plan* pl) {
     }
     // tasks: bricks cut into runs of about `target` entries (at least 16 per
     // brick voxel so the flush is amortised; small enough for the queue to
-    // balance); tasks into chunks of <= block segments and <= ECAP entries.
+    // balance); tasks into chunks of <= block - 2 segments and <= ECAP
+    // entries.
     // Every chunk starts on a 16-entry boundary: its entries are padded to
     // whole groups with zero-length entries.
     const uint64_t workers = (uint64_t)std::max(f.grid_fwd, f.grid_bwd);
@@ -1481,7 +1527,7 @@ int fp32c_build(muto_em_plan* pl) {
           Chunk ch{};
           ch.q0 = q;
           uint32_t ce = 0, cn = 0;
-          while (q < s1 && cn < BSZ && ce + hl[q] <= ECAP) { ce += hl[q]; ++cn; ++q; }
+          while (q < s1 && cn < chunk_seg_cap(BSZ) && ce + hl[q] <= ECAP) { ce += hl[q]; ++cn; ++q; }
           ch.cnt = cn;
           ch.ne = ce;
           ch.base = n_ent_pad;

@@ -29,7 +29,7 @@ struct BrickShape {
   }
 };
 
-// Chunk: up to `block` consecutive brick-major segments [q0, q0 + cnt) whose
+// Chunk: up to `block` - 2 consecutive brick-major segments [q0, q0 + cnt) whose
 // ne entries start at entry `base` (a multiple of 16; the entries are
 // padded to whole 16-entry groups); one block processes it as one unit.
 struct Chunk {
@@ -76,6 +76,7 @@ struct Fp32cData {
   DBuf<Chunk> chunks;
   uint32_t n_tasks = 0, n_chunks = 0;
   int grid_fwd = 0, grid_bwd = 0, block = 256;
+  int bwd_sub = 1;  // chunks per backprojection workgroup (1 or 2)
   int acc_mode = 1;  // accumulation: 1 FP64 / int64 fixed point, 0 FP32 two-sum
   bool acc_mode_set = false;  // set through the option call (else environment)
   hipGraphExec_t graph = nullptr;  // one iteration, captured
