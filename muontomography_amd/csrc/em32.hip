@@ -70,7 +70,7 @@ namespace {
 // shape's `block`); a chunk holds up to one segment per thread and GE
 // entries per thread
 constexpr int GE = 16;                   // entries per thread and chunk
-constexpr uint32_t DIR_BINS = 16;        // track-order direction bins per unit slope
+constexpr uint32_t DIR_BINS = 8;         // track-order direction bins per unit slope
 // backprojection sums (ACC 1) are int64 fixed point: an entry is at most
 // 2^fx_bits, fx_bits = min(FX_BITS, 62 - bits(max M_j)) per plan; a task
 // holds at most MAX_TASK_SEG segments
@@ -265,9 +265,10 @@ k_seg_count(BrickGeo G, size_t n_trk, size_t nv,
   const float dz = fmaxf(fabsf(z1 - z0), 1.0f);
   const float sx = fminf(fmaxf((x1 - x0) / dz, -2.0f), 2.0f);
   const float sy = fminf(fmaxf((y1 - y0) / dz, -2.0f), 2.0f);
-  // slope bins of 1/DIR_BINS: over the grid height the tracks of one bin
-  // spread by about a brick, so the tracks crossing a brick in one bin are
-  // a dense run of ranks
+  // slope bins of 1/DIR_BINS: over a 256-voxel grid height the tracks of
+  // one bin spread by about two bricks, so the tracks crossing a brick in
+  // one bin are a dense run of ranks, and the track kernel's gathers of
+  // neighbouring ranks share lines (1/8 measured 9% faster there than 1/16)
   const uint32_t dbin = (uint32_t)((sx + 2.0f) * DIR_BINS + 0.5f) * (4u * DIR_BINS + 1u) +
                         (uint32_t)((sy + 2.0f) * DIR_BINS + 0.5f);
   const float s = (z1 != z0) ? ((float)G.n[2] - z0) / (z1 - z0) : 0.0f;
