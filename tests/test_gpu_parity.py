@@ -469,6 +469,41 @@ def test_fp32c_two_sum_mode(ctx, small_set, shape):
     p32.close()
 
 
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_fp32c_one_voxel_axis(ctx, small_set, axis):
+    """Grids one voxel wide along an axis (40 x 40 voxels of 25 mm across
+    the others, one 1000 mm voxel along it), where the packed entries' step
+    codes must tell the remaining axes apart (a step of +-nx is a y step
+    when nx = 1): FP32C within 1e-4 of FP64 after 10 mean-update
+    iterations, with the FP64 plan's merged paths checked against the
+    oracle's first."""
+    g0, rays, _ = small_set
+    n = [40, 40, 40]
+    d = [25.0, 25.0, 25.0]
+    n[axis], d[axis] = 1, 1000.0
+    g = make_grid(tuple(n), tuple(d), -500.0)
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100)
+    em = oracle.Em(cfg, rays)
+    assert em.n_sel > 1000
+    p64 = ctx.em_plan(cfg, rays=rays)
+    row, vox, ln, tail, trk4, pr = p64.export()
+    np.testing.assert_array_equal(row, em.row_ptr)
+    np.testing.assert_array_equal(vox, em.vox)
+    p32 = ctx.em_plan(cfg, rays=rays)
+    p32.set_option(M.MUTO_OPT_BRICK, M.brick_code((16, 16, 32, 512)))
+    img = np.full(g.n_vox, 1e-7)
+    p64.set_image(img)
+    p32.set_image(img)
+    c64 = p64.run(10, M.MUTO_FP64)
+    c32 = p32.run(10, M.MUTO_FP32C)
+    mx, rms, norm = _rel_stats(p32.image(), p64.image(), p64.voxel_counts())
+    assert mx <= 1e-4, (mx, rms, norm)
+    np.testing.assert_allclose(c32, c64, rtol=1e-5)
+    assert p32.counters()["n_segments"] > 0
+    p64.close()
+    p32.close()
+
+
 def test_fp32c_graph_replays_match_eager(ctx, small_set):
     """The captured FP32C iteration graph, replayed one iteration at a time
     with host reads in between and then several iterations back to back,
