@@ -301,6 +301,50 @@ def test_em_plan_paths_match_oracle(ctx, small_set):
     assert c["n_scat_in_grid"] == em.n_sel and c["nnz"] == em.nnz
 
 
+@pytest.mark.parametrize("sl,he", [(1, 0), (0, 1), (1, 1)])
+def test_mlsd_straight_line_and_energy(ctx, small_set, sl, he):
+    """MutoMLSD's straight_line (the path is the one chord pin -> pout, not
+    the two legs through the PoCA point: MutoMLSD.cpp:389) and has_energy
+    (a momentum ratio per track, not one for all: MutoMLSD.cpp:189, 285,
+    320) options: the plan's paths, tails and p_r match the oracle bit for
+    bit (the angles and displacements, through libm, to 1e-13); on those
+    prepared inputs four FP64 median iterations match it bit for bit; and
+    10 FP32C mean-update iterations stay within 1e-4 of FP64."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=0, straight_line=sl, has_energy=he)
+    em = oracle.Em(cfg, rays)
+    plan = ctx.em_plan(cfg, rays=rays)
+    row, vox, ln, tl, t4, pr = plan.export()
+    assert np.array_equal(row, em.row_ptr) and np.array_equal(vox, em.vox)
+    assert np.array_equal(ln.view(np.uint64), em.len.view(np.uint64))
+    assert np.array_equal(tl.view(np.uint64), em.tail.view(np.uint64))
+    np.testing.assert_allclose(t4, em.trk4, rtol=1e-13, atol=1e-15)
+    np.testing.assert_array_equal(pr, em.p)
+    assert (len(np.unique(pr)) > 1) == bool(he)  # one ratio for all tracks without energies
+    plan.close()
+    plan = ctx.em_plan(cfg, prepared=em.prepared())
+    img = np.full(g.n_vox, 1e-7)
+    plan.set_image(img)
+    ref = img
+    for _ in range(4):
+        ref, _c, _ns = em.iterate(ref)
+    plan.run(4)
+    assert np.array_equal(plan.image().view(np.uint64), ref.view(np.uint64))
+    plan.close()
+    cfgm = _em_cfg(g, update_with_mean=1, mean_percentile=100, straight_line=sl, has_energy=he)
+    p64 = ctx.em_plan(cfgm, rays=rays)
+    p32 = ctx.em_plan(cfgm, rays=rays)
+    p64.set_image(img)
+    p32.set_image(img)
+    c64 = p64.run(10, M.MUTO_FP64)
+    c32 = p32.run(10, M.MUTO_FP32C)
+    mx, rms, norm = _rel_stats(p32.image(), p64.image(), p64.voxel_counts())
+    assert mx <= 1e-4, (mx, rms, norm)
+    np.testing.assert_allclose(c32, c64, rtol=1e-5)
+    p64.close()
+    p32.close()
+
+
 @pytest.mark.parametrize("mode", ["median", "trimmed", "mean"])
 def test_em_fp64_prepared(ctx, small_set, mode):
     """Same prepared inputs: per-track Sigma, det, inverse and S_ij are
