@@ -1118,6 +1118,18 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
   }
 }
 
+// Voxels no track crosses become 0 (src/MutoMLSD.cpp:494-496); done once
+// before a run's first mean-update pass, since no later pass touches them.
+__global__ void __launch_bounds__(256)
+k_zero_uncrossed(size_t nv, const uint32_t* __restrict__ mj, double* __restrict__ lam,
+                 float* __restrict__ lam32) {
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nv && mj[j] == 0 && lam[j] != 0.0) {
+    lam[j] = 0.0;
+    lam32[j] = 0.0f;
+  }
+}
+
 // lambda_j <- lambda_j + lambda_j^2 / (2 M_j) * sum_i c_ij from the FP64
 // image, which stays the iteration's state (the float mirror only feeds the
 // forward pass); clears the accumulator and the work counters for the next
@@ -1132,17 +1144,11 @@ k_upd32(size_t nv, const uint32_t* __restrict__ mj, double* __restrict__ acc,
   if (j == 0) { work[0] = 0; work[1] = 0; }
   if (j >= nv) return;
   const uint32_t m = mj[j];
-  if (m == 0) {
-    // no track crosses the voxel: it becomes 0 (MutoMLSD.cpp:494-496) and
-    // its accumulator, which no entry adds to, stays 0.  After the first
-    // pass only the read of lambda remains (the padding of a batch's
-    // combined grid is mostly such voxels); the ML image is not read there.
-    if (!ml && lam[j] != 0.0) {
-      lam[j] = 0.0;
-      lam32[j] = 0.0f;
-    }
-    return;
-  }
+  // no track crosses the voxel: k_zero_uncrossed made it 0 (MutoMLSD.cpp:
+  // 494-496) before the run's first pass, and its accumulator, which no
+  // entry adds to, stays 0 (the padding of a batch's combined grid is
+  // mostly such voxels); the ML image is not read there
+  if (m == 0) return;
   const double l = lam[j];
   // the backprojection's sums: int64 fixed point (ACC 1) or FP64
   const double c = fixed ? (double)reinterpret_cast<const long long*>(acc)[j] * (1.0 / fx_scale(cmax, fx_bits))
@@ -1796,6 +1802,10 @@ int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done) {
   }
   if (!f.hist_n.p) MUTO_CHECK_HIP(ctx, f.hist_n.alloc(1));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.hist_n.p, 0, sizeof(unsigned int), s));
+  if (!pl->is_map && iters > 0) {
+    k_zero_uncrossed<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->lam64.p, pl->lam32.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  }
   const bool graph = pl->use_graph && !pl->timing;
   if (graph)
     if (int rc = fp32c_graph(pl)) return rc;
