@@ -345,6 +345,46 @@ def test_mlsd_straight_line_and_energy(ctx, small_set, sl, he):
     p32.close()
 
 
+def test_mlsd_closed_form_tracks(ctx):
+    """The constructed tracks of tests/mlsd_cases.py through the GPU plan:
+    kinked tracks give the closed-form angles and displacements
+    (src/MutoMLSD.cpp:350-376), PoCA points on the selection window's
+    boundaries are accepted or rejected as src/MutoMLSD.cpp:298-301 says,
+    and a kink inside one voxel, or an empty in-leg, gives the closed-form
+    merged path (src/MutoMLSD.cpp:382-415).  The same inputs through the
+    oracle match bit for bit (paths) or to 1e-13 (angles, through libm)."""
+    import mlsd_cases as MC
+    g = make_grid(20, 10.0, -100.0)
+    for plane in (None, "xz", "yz"):
+        rays, want = MC.random_kinks(g, 40, seed=11, in_plane=plane)
+        cfg = _em_cfg(g)
+        plan = ctx.em_plan(cfg, rays=rays)
+        row, vox, ln, tl, t4, pr = plan.export()
+        assert t4.shape == (40, 4)
+        np.testing.assert_allclose(t4[:, :2], want[:, :2], rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(t4[:, 2:], want[:, 2:], rtol=1e-9, atol=1e-9)
+        em = oracle.Em(cfg, rays)
+        np.testing.assert_allclose(t4, em.trk4, rtol=1e-13, atol=1e-15)
+        assert np.array_equal(vox, em.vox) and np.array_equal(ln.view(np.uint64), em.len.view(np.uint64))
+        plan.close()
+    for gw, zdet in ((make_grid(10, 10.0, -50.0), (-1000.0, 1000.0)),
+                     (make_grid(10, 100.0, -500.0), (-200.0, 200.0))):
+        rays, want = MC.window_rays(gw, zdet)
+        _, _, acc, _ = ctx.poca_points(gw, rays, mode=1)
+        np.testing.assert_array_equal(acc, want)
+        plan = ctx.em_plan(_em_cfg(gw), rays=rays)
+        assert plan.counters()["n_scat_in_grid"] == int(want.sum())
+        plan.close()
+    for name, (rays, vox_w, ln_w, tail_w, _ni, _no) in MC.merge_cases().items():
+        plan = ctx.em_plan(_em_cfg(MC.MERGE_GRID), rays=rays)
+        row, vox, ln, tl, t4, pr = plan.export()
+        np.testing.assert_array_equal(vox, vox_w)
+        np.testing.assert_allclose(ln, ln_w, rtol=1e-12)
+        np.testing.assert_allclose(tl, tail_w, rtol=1e-12, atol=1e-12)
+        assert plan.counters()["n_empty_path"] == 0, name
+        plan.close()
+
+
 @pytest.mark.parametrize("mode", ["median", "trimmed", "mean"])
 def test_em_fp64_prepared(ctx, small_set, mode):
     """Same prepared inputs: per-track Sigma, det, inverse and S_ij are

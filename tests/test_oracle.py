@@ -710,3 +710,107 @@ def test_em_determinant_threshold(f):
     else:
         assert ns == 1 and cost == 0.0
         assert out[0] == lam  # half of S = 2 lambda
+
+
+# ------------------------------------------- MLSD selection, angles, paths
+# Closed forms for gatherInformation / getScatteringAngle / getDisplacement /
+# getMuonPathAndTail (src/MutoMLSD.cpp:268-415); the inputs are built in
+# tests/mlsd_cases.py and shared with the GPU parity tests.
+import mlsd_cases as MC  # noqa: E402
+
+
+def _prep(g, rays, **kw):
+    cfg = mlsd_cfg(g, error_xy=0.5, error_dz_outer=500.0, error_dz_inner=500.0, **kw)
+    return oracle.Em(cfg, rays)
+
+
+@pytest.mark.parametrize("plane", [None, "xz", "yz"])
+def test_mlsd_kinked_track_angles_and_displacements(plane):
+    """A track kinked at K: S = a_in - a_out and D = (z_min - K.z)
+    tan(a_out - a_in) |din| / din.z per plane (the reference's projection to
+    z_min, src/MutoMLSD.cpp:356-376, reduced by a trig identity).  A sign
+    flip, a dropped cos or Lxy factor, or x/y crossed fails this.  In-plane
+    tracks have S and D of the other plane exactly 0."""
+    g = make_grid(20, 10.0, -100.0)
+    rays, want = MC.random_kinks(g, 40, seed=11, in_plane=plane)
+    em = _prep(g, rays)
+    assert em.n_sel == len(rays)
+    t = em.trk4
+    np.testing.assert_allclose(t[:, :2], want[:, :2], rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(t[:, 2:], want[:, 2:], rtol=1e-9, atol=1e-9)
+    assert np.all(np.abs(want[:, 2:]) > 1e-3) or plane is not None
+    if plane == "xz":
+        assert np.all(t[:, 1] == 0.0) and np.all(t[:, 3] == 0.0)
+    if plane == "yz":
+        assert np.all(t[:, 0] == 0.0) and np.all(t[:, 2] == 0.0)
+
+
+def test_mlsd_straight_track_has_no_scattering():
+    """din == dout along one line: S = 0 exactly, D = 0 to rounding (both
+    projections to z_min are the same point)."""
+    g = make_grid(20, 10.0, -100.0)
+    rng = np.random.default_rng(5)
+    rays = []
+    for _ in range(30):
+        d = np.array([*rng.uniform(-0.5, 0.5, 2), 1.0])
+        K = rng.uniform(-60.0, 60.0, 3)
+        rays.append(MC.kinked(K, d, d))
+    em = _prep(g, np.concatenate(rays))
+    assert em.n_sel == 30
+    t = em.trk4
+    assert np.all(t[:, :2] == 0.0)
+    np.testing.assert_allclose(t[:, 2:], 0.0, atol=1e-9)
+
+
+@pytest.mark.parametrize("case", ["grid_binds", "detector_binds"])
+def test_mlsd_selection_window_boundaries(case):
+    """PoCA points exactly on each boundary of the eps-shrunk selection
+    window and one ulp outside it (src/MutoMLSD.cpp:298-301): every face is
+    inclusive except the detector top, zDetMax - eps <= P.z, which rejects.
+    With the grid inside the detector window the grid's z faces bind; with
+    the detector inside the grid, zDetMin + eps and zDetMax - eps bind.
+    Dropping an eps, or making a face strict or inclusive the other way,
+    fails one of these points.  The PoCA accept test of MutoPoCA.cpp:75-79
+    (no eps, an inclusive detector window) differs and is checked too."""
+    if case == "grid_binds":
+        g, zdet = make_grid(10, 10.0, -50.0), (-1000.0, 1000.0)
+    else:
+        g, zdet = make_grid(10, 100.0, -500.0), (-200.0, 200.0)
+    rays, want = MC.window_rays(g, zdet)
+    _, _, acc, _ = oracle.poca_points(g, rays, mode=1)
+    np.testing.assert_array_equal(acc, want)
+    em = _prep(g, rays)
+    np.testing.assert_array_equal(em.sel, np.nonzero(want)[0])
+    # PoCA's own test accepts the points the eps excludes, when they fall in
+    # a voxel (x86 cast < n) and in the inclusive detector window
+    _, v, acc0, _ = oracle.poca_points(g, rays, mode=0)
+    P = rays["pin"]
+    inside = np.all(v < np.array([g.nx, g.ny, g.nz]), axis=1)
+    want0 = inside & (P[:, 2] >= zdet[0]) & (P[:, 2] <= zdet[1])
+    np.testing.assert_array_equal(acc0, want0)
+    assert np.any(want0 & ~want)
+
+
+@pytest.mark.parametrize("name", ["inside", "empty_in"])
+def test_mlsd_two_leg_merge_closed_form(name):
+    """getMuonPathAndTail (src/MutoMLSD.cpp:382-415) on a track kinked
+    inside one voxel: the scattering voxel's entry is in.back() + out[0] and
+    out[0] is dropped; with an empty in-leg out[0] is dropped all the same.
+    Lengths, voxels and tails in closed form (tests/mlsd_cases.py)."""
+    rays, vox, ln, tail, n_in_want, n_out_want = MC.merge_cases()[name]
+    em = _prep(MC.MERGE_GRID, rays)
+    assert em.n_sel == 1 and em.counters["n_empty_path"] == 0
+    np.testing.assert_array_equal(em.vox, vox)
+    np.testing.assert_allclose(em.len, ln, rtol=1e-12)
+    np.testing.assert_allclose(em.tail, tail, rtol=1e-12, atol=1e-12)
+    # the legs as Siddon gives them (pinned above by the rational brute force)
+    P = em.pscat[0]
+    ri = oracle.siddon_csr(MC.MERGE_GRID, rays["pin"], P[None])
+    ro = oracle.siddon_csr(MC.MERGE_GRID, P[None], rays["pout"])
+    n_in, n_out = int(ri[0][-1]), int(ro[0][-1])
+    assert (n_in, n_out) == (n_in_want, n_out_want)
+    assert em.nnz == n_in + n_out - 1
+    # straight_line: one chord pin -> pout, no merge
+    em_s = _prep(MC.MERGE_GRID, rays, straight_line=1)
+    rs = oracle.siddon_csr(MC.MERGE_GRID, rays["pin"], rays["pout"])
+    np.testing.assert_array_equal(em_s.len, rs[4])
