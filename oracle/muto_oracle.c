@@ -14,7 +14,9 @@
  * Pinning (what checks this restatement, since the reference cannot be
  * compiled here -- Eigen is absent): tests/test_oracle.py checks Siddon
  * against exact rational brute force and closed forms, PoCA against
- * constructed skew/intersecting/parallel lines, the W/Sigma/S algebra
+ * constructed skew/intersecting/parallel lines, the MLSD selection window,
+ * angles, displacements and two-leg merge against kinked tracks in closed
+ * form (tests/mlsd_cases.py), the W/Sigma/S algebra
  * against the compact gradient identity and finite differences, the median
  * / mean / trimmed updates against numpy on small sets, the cubic solver
  * against numpy.roots, the MAP surrogate against its tangency to Phi, and
