@@ -44,9 +44,20 @@ extern "C" {
 #define MUTO_ERR_STATE (-5)   /* call out of order / plan not ready    */
 #define MUTO_ERR_LIMIT (-6)   /* a size exceeds what the kernels take  */
 
+/* Where a caller's buffer lives, for the entry points that take a `mem`
+ * argument or the MUTO_MAP_DEVICE_BUFFERS flag.  Device buffers must be on
+ * the context's GPU and fully written before the call (for example, the
+ * caller's stream synchronised); each call has finished reading and
+ * writing them when it returns. */
+#define MUTO_MEM_HOST 0    /* host memory, copied in or out              */
+#define MUTO_MEM_DEVICE 1  /* device memory, read or written in place    */
+
 /* arithmetic of the EM / MAP iterations */
 #define MUTO_FP64 0   /* reference order, FP64 throughout                    */
-#define MUTO_FP32C 1  /* FP64 Siddon + 2x2 algebra, FP32 sums with two-sum   */
+#define MUTO_FP32C 1  /* FP64 Siddon, 2x2 algebra and sums; FP32 entry terms;
+                         the sums are FP64 registers and int64 fixed-point
+                         LDS atomics by default, FP32 with two-sum
+                         compensation under MUTO_OPT_ACC_MODE 0          */
 
 /* Voxel grid: same field order, types and 64-byte layout as VoxelGrid
  * (include/MutoTypes.h:25-37).  Linear voxel index j = x + y*nx + z*nx*ny
@@ -250,6 +261,12 @@ int muto_em_plan_destroy(muto_em_plan* plan);
 /* Set the image (host FP64, nx*ny*nz) the next iteration starts from. */
 int muto_em_plan_set_image(muto_em_plan* plan, const double* img);
 int muto_em_plan_get_image(muto_em_plan* plan, double* img);
+/* The same, with img in host (MUTO_MEM_HOST) or device (MUTO_MEM_DEVICE)
+ * memory: the reference's image hand-over (MutoMLSD.cpp:226-245 reads it,
+ * MutoVReconstruction::reconstruct returns it, include/MutoVReconstruction.h:
+ * 11-16) without a trip through host memory. */
+int muto_em_plan_set_image_ex(muto_em_plan* plan, const double* img, int mem);
+int muto_em_plan_get_image_ex(muto_em_plan* plan, double* img, int mem);
 /* Run `iters` iterations in `precision` (MUTO_FP64 / MUTO_FP32C) with the
  * plan's update rule (median / mean / trimmed mean, MLSD or MAP).
  * cost (may be NULL) receives one value per iteration.  Stops early as the
@@ -281,7 +298,8 @@ int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg, size_t n,
  * never evaluates Phi or its gradient; in mean mode its MAP step
  * (MutoMAP.cpp:162-184) minimises a per-voxel surrogate tangent to Phi, so
  * this gradient is the reference's MAP gradient there.
- * muto_map_forward evaluates Phi at lambda (host, n_vox doubles) on the
+ * muto_map_forward evaluates Phi at lambda (n_vox doubles, host memory or,
+ * with MUTO_MAP_DEVICE_BUFFERS, device memory) on the
  * plan's tracks and returns a saved context; muto_map_backward writes
  * grad_lambda = grad_phi dPhi/dlambda (host, n_vox doubles) and may be
  * called several times.  precision: MUTO_FP64 (track-major, FP64) or
@@ -291,6 +309,10 @@ int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg, size_t n,
  * must be destroyed before its plan. */
 typedef struct muto_map_saved muto_map_saved;
 #define MUTO_MAP_RECOMPUTE_SIDDON 1
+/* flag of muto_map_forward: lambda, and grad_lambda of every
+ * muto_map_backward on the saved context, are device pointers (see
+ * MUTO_MEM_DEVICE); the gradient is accumulated in the caller's array */
+#define MUTO_MAP_DEVICE_BUFFERS 2
 int muto_map_forward(muto_em_plan* plan, const double* lambda, int precision, int flags,
                      double* phi, muto_map_saved** saved);
 int muto_map_backward(muto_map_saved* saved, double grad_phi, double* grad_lambda);

@@ -31,6 +31,42 @@ def lib():
 
 
 MUTO_MAP_RECOMPUTE_SIDDON = 1
+MUTO_MAP_DEVICE_BUFFERS = 2
+MUTO_MEM_HOST = 0
+MUTO_MEM_DEVICE = 1
+
+
+def _device_ptr(x, n=None, what="buffer"):
+    """Device address of a float64 device buffer -- a devmem.DeviceArray, or
+    a contiguous torch tensor on the GPU (after its stream has finished with
+    it) -- or None for anything else (host arrays go through numpy)."""
+    from .devmem import DeviceArray
+    if isinstance(x, DeviceArray):
+        if x.dtype != np.float64:
+            raise MutoError(f"{what}: device buffers must be float64")
+        if n is not None and x.n != n:
+            raise MutoError(f"{what}: {x.n} values, expected {n}")
+        return x.ptr
+    if not (hasattr(x, "data_ptr") and getattr(x, "is_cuda", False)):
+        return None
+    import torch
+    if x.dtype != torch.float64 or not x.is_contiguous():
+        raise MutoError(f"{what}: device buffers must be contiguous float64")
+    if n is not None and x.numel() != n:
+        raise MutoError(f"{what}: {x.numel()} values, expected {n}")
+    torch.cuda.current_stream(x.device).synchronize()
+    return C.c_void_p(x.data_ptr())
+
+
+def _device_like(x, n):
+    """A new float64 device buffer of n values of the same kind as x."""
+    from .devmem import DeviceArray
+    if isinstance(x, DeviceArray):
+        return DeviceArray(n)
+    import torch
+    return torch.empty(n, dtype=torch.float64, device=x.device)
+
+
 MUTO_OPT_GRAPH = 1
 MUTO_OPT_TIMING = 2
 MUTO_OPT_ACC_MODE = 3
@@ -267,11 +303,25 @@ class EmPlan:
             pass
 
     def set_image(self, img):
+        """img: numpy (host) or a float64 device buffer (devmem.DeviceArray
+        or a GPU torch tensor), copied in device memory."""
+        d = _device_ptr(img, self.cfg.grid.n_vox, "set_image")
+        if d is not None:
+            self.ctx._check(lib().muto_em_plan_set_image_ex(self._h, d, MUTO_MEM_DEVICE), "set_image_ex")
+            return
         img = np.ascontiguousarray(img, np.float64).ravel()
         assert img.size == self.cfg.grid.n_vox
         self.ctx._check(lib().muto_em_plan_set_image(self._h, ptr(img)), "set_image")
 
-    def image(self):
+    def image(self, out=None):
+        """The current image: a new numpy array, or written into `out` (a
+        float64 device buffer, filled in device memory)."""
+        if out is not None:
+            d = _device_ptr(out, self.cfg.grid.n_vox, "image")
+            if d is None:
+                raise MutoError("image(out=...): out must be a float64 device buffer")
+            self.ctx._check(lib().muto_em_plan_get_image_ex(self._h, d, MUTO_MEM_DEVICE), "get_image_ex")
+            return out
         img = np.empty(self.cfg.grid.n_vox, np.float64)
         self.ctx._check(lib().muto_em_plan_get_image(self._h, ptr(img)), "get_image")
         return img
@@ -309,14 +359,20 @@ class EmPlan:
     def map_forward(self, lam, precision=MUTO_FP64, recompute=False):
         """Phi(lam) of the MAP objective (muto_map_forward); returns a
         MapSaved whose backward(grad_phi) gives grad_phi * dPhi/dlam."""
-        lam = np.ascontiguousarray(lam, np.float64).ravel()
-        assert lam.size == self.cfg.grid.n_vox
+        flags = MUTO_MAP_RECOMPUTE_SIDDON if recompute else 0
+        d = _device_ptr(lam, self.cfg.grid.n_vox, "map_forward")
+        if d is not None:  # a device buffer: gradients in device memory too
+            flags |= MUTO_MAP_DEVICE_BUFFERS
+            lp = d
+        else:
+            lam = np.ascontiguousarray(lam, np.float64).ravel()
+            assert lam.size == self.cfg.grid.n_vox
+            lp = ptr(lam)
         phi = C.c_double()
         h = C.c_void_p()
-        self.ctx._check(lib().muto_map_forward(self._h, ptr(lam), int(precision),
-                                               MUTO_MAP_RECOMPUTE_SIDDON if recompute else 0,
+        self.ctx._check(lib().muto_map_forward(self._h, lp, int(precision), flags,
                                                C.byref(phi), C.byref(h)), "muto_map_forward")
-        sv = MapSaved(self, h, phi.value)
+        sv = MapSaved(self, h, phi.value, like=lam if d is not None else None)
         self._saved.add(sv)
         return sv
 
@@ -389,13 +445,26 @@ class MapSaved:
     """Saved context of one muto_map_forward call (autograd-Function style):
     phi, and backward(grad_phi) -> grad_phi * dPhi/dlambda (n_vox doubles)."""
 
-    def __init__(self, plan, h, phi):
+    def __init__(self, plan, h, phi, like=None):
         self.plan = plan
         self._h = h
         self.phi = phi
+        self.like = like  # the device lambda of a forward on device buffers, else None
 
-    def backward(self, grad_phi=1.0):
-        g = np.empty(self.plan.cfg.grid.n_vox, np.float64)
+    def backward(self, grad_phi=1.0, out=None):
+        """grad_phi * dPhi/dlambda: numpy, or (forward on device buffers) a
+        device buffer of the same kind (`out`, or a new one)."""
+        n = self.plan.cfg.grid.n_vox
+        if self.like is not None:
+            if out is None:
+                out = _device_like(self.like, n)
+            d = _device_ptr(out, n, "backward")
+            if d is None:
+                raise MutoError("backward(out=...): out must be a float64 device buffer")
+            self.plan.ctx._check(lib().muto_map_backward(self._h, float(grad_phi), d),
+                                 "muto_map_backward")
+            return out
+        g = np.empty(n, np.float64)
         self.plan.ctx._check(lib().muto_map_backward(self._h, float(grad_phi), ptr(g)),
                              "muto_map_backward")
         return g

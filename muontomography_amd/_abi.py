@@ -158,6 +158,8 @@ PROTOS = {
     "muto_em_plan_destroy": (C.c_int, [P]),
     "muto_em_plan_set_image": (C.c_int, [P, P]),
     "muto_em_plan_get_image": (C.c_int, [P, P]),
+    "muto_em_plan_set_image_ex": (C.c_int, [P, P, C.c_int]),
+    "muto_em_plan_get_image_ex": (C.c_int, [P, P, C.c_int]),
     "muto_em_plan_run": (C.c_int, [P, C.c_int, C.c_int, P]),
     "muto_map_plan_run": (C.c_int, [P, C.c_int, C.c_int, P]),
     "muto_em_plan_counters": (C.c_int, [P, C.POINTER(Counters)]),

@@ -510,23 +510,27 @@ k_vox_select(size_t nv, const uint32_t* __restrict__ mj,
   }
 }
 
-// voxels with M_j > SEL_CAP: one block sorts each in global scratch
+// voxels with M_j > SEL_CAP, listed in big[0..n_big): one block per listed
+// voxel sorts its values in its own part of the global scratch
+// (scratch + off[b], next power of two above M_j doubles)
 __global__ void __launch_bounds__(1024)
-k_vox_select_big(size_t nv, const uint32_t* __restrict__ mj,
-                 const uint64_t* __restrict__ col_ptr,
-                 const double* __restrict__ S, int use_mean, int pct,
-                 double* __restrict__ scratch, double* __restrict__ out) {
-  for (size_t j = 0; j < nv; ++j) {
+k_vox_select_big(uint32_t n_big, const uint32_t* __restrict__ big,
+                 const uint64_t* __restrict__ off, const uint32_t* __restrict__ mj,
+                 const uint64_t* __restrict__ col_ptr, const double* __restrict__ S,
+                 int use_mean, int pct, double* __restrict__ scratch,
+                 double* __restrict__ out) {
+  for (uint32_t b = blockIdx.x; b < n_big; b += gridDim.x) {
+    const uint32_t j = big[b];
     const uint32_t m = mj[j];
-    if (m <= SEL_CAP) continue;
     uint32_t n = 1;
     while (n < m) n <<= 1;
+    double* a = scratch + off[b];
     const double* src = S + col_ptr[j];
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-      scratch[i] = i < m ? src[i] : INFINITY;
+      a[i] = i < m ? src[i] : INFINITY;
     __syncthreads();
-    block_bitonic(scratch, n);
-    if (threadIdx.x == 0) out[j] = order_update(scratch, m, use_mean, pct);
+    block_bitonic(a, n);
+    if (threadIdx.x == 0) out[j] = order_update(a, m, use_mean, pct);
     __syncthreads();
   }
 }
@@ -870,12 +874,21 @@ extern "C" int muto_em_plan_destroy(muto_em_plan* pl) {
   return MUTO_OK;
 }
 
-extern "C" int muto_em_plan_set_image(muto_em_plan* pl, const double* img) {
+extern "C" int muto_em_plan_set_image_ex(muto_em_plan* pl, const double* img, int mem) {
   if (!pl || !img) return MUTO_ERR_ARG;
   muto_ctx* ctx = pl->ctx;
-  MUTO_CHECK_HIP(ctx, copy_sync(ctx, pl->lam64.p, img, pl->nv * sizeof(double), hipMemcpyHostToDevice));
+  if (mem != MUTO_MEM_HOST && mem != MUTO_MEM_DEVICE) {
+    ctx->set_error("muto_em_plan_set_image_ex: mem must be MUTO_MEM_HOST or MUTO_MEM_DEVICE");
+    return MUTO_ERR_ARG;
+  }
+  MUTO_CHECK_HIP(ctx, copy_sync(ctx, pl->lam64.p, img, pl->nv * sizeof(double),
+                                mem == MUTO_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
   pl->cur_prec = MUTO_FP64;
   return MUTO_OK;
+}
+
+extern "C" int muto_em_plan_set_image(muto_em_plan* pl, const double* img) {
+  return muto_em_plan_set_image_ex(pl, img, MUTO_MEM_HOST);
 }
 
 // lam64 is the image in both precisions (FP32C updates it in FP64 too);
@@ -895,12 +908,20 @@ static int sync_image(muto_em_plan* pl, int prec) {
   return MUTO_OK;
 }
 
-extern "C" int muto_em_plan_get_image(muto_em_plan* pl, double* img) {
+extern "C" int muto_em_plan_get_image_ex(muto_em_plan* pl, double* img, int mem) {
   if (!pl || !img) return MUTO_ERR_ARG;
   muto_ctx* ctx = pl->ctx;
-  MUTO_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  MUTO_CHECK_HIP(ctx, copy_sync(ctx, img, pl->lam64.p, pl->nv * sizeof(double), hipMemcpyDeviceToHost));
+  if (mem != MUTO_MEM_HOST && mem != MUTO_MEM_DEVICE) {
+    ctx->set_error("muto_em_plan_get_image_ex: mem must be MUTO_MEM_HOST or MUTO_MEM_DEVICE");
+    return MUTO_ERR_ARG;
+  }
+  MUTO_CHECK_HIP(ctx, copy_sync(ctx, img, pl->lam64.p, pl->nv * sizeof(double),
+                                mem == MUTO_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
   return MUTO_OK;
+}
+
+extern "C" int muto_em_plan_get_image(muto_em_plan* pl, double* img) {
+  return muto_em_plan_get_image_ex(pl, img, MUTO_MEM_HOST);
 }
 
 namespace muto {
@@ -924,6 +945,7 @@ static int build_slots(muto_em_plan* pl) {
   MUTO_CHECK_HIP(ctx, pl->col_ptr.alloc(pl->nv + 1));
   MUTO_CHECK_HIP(ctx, pl->slot.alloc(pl->nnz));
   MUTO_CHECK_HIP(ctx, pl->S.alloc(pl->nnz));
+  std::vector<uint32_t> h(pl->nv);
   {
     DBuf<uint32_t> mj1, fill;
     MUTO_CHECK_HIP(ctx, mj1.alloc(pl->nv + 1));
@@ -936,7 +958,6 @@ static int build_slots(muto_em_plan* pl) {
       k_slots<<<grid_for(pl->nnz, 256), 256, 0, s>>>(pl->nnz, pl->vox.p, pl->nv, pl->col_ptr.p, fill.p, pl->slot.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
-    std::vector<uint32_t> h(pl->nv);
     MUTO_CHECK_HIP(ctx, hipMemcpyAsync(h.data(), pl->mj.p, pl->nv * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
     uint64_t mx = 0;
@@ -944,9 +965,25 @@ static int build_slots(muto_em_plan* pl) {
     pl->max_mj = mx;
   }
   if (pl->max_mj > SEL_CAP) {
-    uint64_t n = 1;
-    while (n < pl->max_mj) n <<= 1;
-    MUTO_CHECK_HIP(ctx, pl->sel_scratch.alloc(n));
+    // the voxels too long for the LDS sort, each with its own scratch range
+    std::vector<uint32_t> ids;
+    std::vector<uint64_t> off;
+    uint64_t tot = 0;
+    for (size_t j = 0; j < pl->nv; ++j) {
+      const uint32_t m = h[j];
+      if (m <= SEL_CAP) continue;
+      uint64_t n = 1;
+      while (n < m) n <<= 1;
+      ids.push_back((uint32_t)j);
+      off.push_back(tot);
+      tot += n;
+    }
+    pl->n_big = (uint32_t)ids.size();
+    MUTO_CHECK_HIP(ctx, pl->big_vox.alloc(ids.size()));
+    MUTO_CHECK_HIP(ctx, pl->big_off.alloc(off.size()));
+    MUTO_CHECK_HIP(ctx, pl->sel_scratch.alloc(tot));
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, pl->big_vox.p, ids.data(), ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, pl->big_off.p, off.data(), off.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
   }
   pl->slots_built = true;
   return MUTO_OK;
@@ -998,10 +1035,12 @@ static int iterate64(muto_em_plan* pl, double* dst, bool time_it) {
     k_vox_select<<<nb, 256, 0, s>>>(pl->nv, pl->mj.p, pl->col_ptr.p, pl->S.p,
                                     c.update_with_mean, c.mean_percentile, dst);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
-    if (pl->max_mj > SEL_CAP)
-      k_vox_select_big<<<1, 1024, 0, s>>>(pl->nv, pl->mj.p, pl->col_ptr.p, pl->S.p,
-                                          c.update_with_mean, c.mean_percentile,
-                                          pl->sel_scratch.p, dst);
+    if (pl->n_big) {
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+      k_vox_select_big<<<std::min<uint32_t>(pl->n_big, 1024), 1024, 0, s>>>(
+          pl->n_big, pl->big_vox.p, pl->big_off.p, pl->mj.p, pl->col_ptr.p, pl->S.p,
+          c.update_with_mean, c.mean_percentile, pl->sel_scratch.p, dst);
+    }
   }
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (time_it) MUTO_CHECK_HIP(ctx, hipEventRecord(pl->ev[2], s));

@@ -148,6 +148,9 @@ struct muto_em_plan {
   muto::DBuf<uint64_t> slot;
   muto::DBuf<double> S;
   uint64_t max_mj = 0;
+  uint32_t n_big = 0;               // voxels with M_j above the LDS sort's capacity
+  muto::DBuf<uint32_t> big_vox;     // [n_big] their indices
+  muto::DBuf<uint64_t> big_off;     // [n_big] their offsets in sel_scratch
   muto::DBuf<double> sel_scratch;
 
   // images and accumulators

@@ -23,6 +23,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "em_dev.hpp"
@@ -50,75 +51,138 @@ struct muto_map_saved {
   DBuf<double> grad;               // [nv] device gradient
   DBuf<double> sum;                // [2] cost sum, prior sum
   DBuf<double> part;               // per-block partial sums
+  bool dev = false;                // MUTO_MAP_DEVICE_BUFFERS: caller buffers in device memory
 };
 
 namespace {
 
-// Per track: Sigma_i from lambda (stored lists in the reference's order, or
+// What the per-track passes read: the stored track-major lists (RC = 0) or
+// the track geometry whose path is walked again (RC = 1).
+struct MapTrackArgs {
+  size_t n_trk, nv;
+  const uint64_t* row_ptr;
+  const uint32_t* vox;
+  const double* len;
+  const double* tail;
+  GridD g;
+  const double* geo;
+  int straight;
+  const double* trk4;
+  const double* pr;
+  ErrM E;
+};
+
+MapTrackArgs track_args(const muto_em_plan* pl) {
+  MapTrackArgs a;
+  a.n_trk = pl->n_trk;
+  a.nv = pl->nv;
+  a.row_ptr = pl->row_ptr.p;
+  a.vox = pl->vox.p;
+  a.len = pl->len.p;
+  a.tail = pl->tail.p;
+  a.g = pl->g;
+  a.geo = pl->geo.p;
+  a.straight = pl->cfg.straight_line;
+  a.trk4 = pl->trk4.p;
+  a.pr = pl->pr.p;
+  a.E = error_matrix(pl->cfg);
+  return a;
+}
+
+// Track i: Sigma_i from lambda (stored lists in the reference's order, or
 // the path walked again with moments about its start moved to the tail
-// basis), Eigen's 2x2 det/inverse, cost_i, and G_i (zero where cost_i is 0).
+// basis), Eigen's 2x2 det/inverse, cost_i, and G_i (zero where cost_i is
+// 0); P = the path length (RC = 1).
+template <int RC>
+__device__ __forceinline__ void map_track(const MapTrackArgs& a, size_t i, const double* __restrict__ lam,
+                                          double& cost, bool& sing, double& G00, double& G01,
+                                          double& G11, double& P) {
+  Mat2 cov = {0.0, 0.0, 0.0, 0.0};
+  P = 0.0;
+  if (RC == 0) {
+    for (uint64_t k = a.row_ptr[i]; k < a.row_ptr[i + 1]; ++k) {
+      Mat2 W;
+      weight(a.len[k], a.tail[k], W);
+      const uint32_t j = a.vox[k];
+      const double lj = j < a.nv ? lam[j] : 0.0;
+      cov.m00 = cov.m00 + lj * W.m00;
+      cov.m01 = cov.m01 + lj * W.m01;
+      cov.m10 = cov.m10 + lj * W.m10;
+      cov.m11 = cov.m11 + lj * W.m11;
+    }
+  } else {
+    // moments in tau = path length from the start: int (1, tau)(1, tau)^T
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    P = walk_path(a.g, a.geo + 9 * i, a.straight, [&](uint32_t j, double L, double s, bool) {
+      const double lj = j < a.nv ? lam[j] : 0.0;
+      a0 += lj * L;
+      a1 += lj * (L * s + 0.5 * L * L);
+      a2 += lj * (L * s * s + L * L * s + L * L * L / 3.0);
+    });
+    // tail t = P - tau: u(t) = A u(tau), A = [[1, 0], [P, -1]]
+    cov.m00 = a0;
+    cov.m01 = cov.m10 = P * a0 - a1;
+    cov.m11 = P * (P * a0 - 2.0 * a1) + a2;
+  }
+  const double p = a.pr[i], pp = p * p;
+  cov.m00 = pp * cov.m00 + a.E.e00;
+  cov.m01 = pp * cov.m01 + a.E.e01;
+  cov.m10 = pp * cov.m10 + a.E.e01;
+  cov.m11 = pp * cov.m11 + a.E.e11;
+  const double Sx = a.trk4[4 * i], Sy = a.trk4[4 * i + 1];
+  const double Dx = a.trk4[4 * i + 2], Dy = a.trk4[4 * i + 3];
+  const double d00 = Sx * Sx + Sy * Sy, d11 = Dx * Dx + Dy * Dy, d01 = Sx * Dx + Sy * Dy;
+  Mat2 inv;
+  double det;
+  const bool ok = inv_det(cov, inv, det);
+  G00 = G01 = G11 = 0.0;
+  if (!ok || det <= 0) {
+    sing = true;
+  } else {
+    cost = log(det) + (d00 * inv.m00 + d11 * inv.m11 + d01 * 2.0 * inv.m01) * 0.5;
+    // G = p^2 (1/2 inv D inv - inv)
+    const double A00 = inv.m00 * d00 + inv.m01 * d01, A01 = inv.m00 * d01 + inv.m01 * d11;
+    const double A10 = inv.m10 * d00 + inv.m11 * d01, A11 = inv.m10 * d01 + inv.m11 * d11;
+    G00 = pp * (0.5 * (A00 * inv.m00 + A01 * inv.m10) - inv.m00);
+    G01 = pp * (0.5 * (A00 * inv.m01 + A01 * inv.m11) - inv.m01);
+    G11 = pp * (0.5 * (A10 * inv.m01 + A11 * inv.m11) - inv.m11);
+  }
+}
+
+// grad_j += scale <W_ij, G_i> over track i's path (FP64 atomics)
+template <int RC>
+__device__ __forceinline__ void map_scatter(const MapTrackArgs& a, size_t i, double G00, double G01,
+                                            double G11, double P, double scale,
+                                            double* __restrict__ grad) {
+  if (G00 == 0.0 && G01 == 0.0 && G11 == 0.0) return;
+  auto add = [&](uint32_t j, double L, double T) {
+    Mat2 W;
+    weight(L, T, W);
+    atomicAdd(&grad[j], scale * (W.m00 * G00 + 2.0 * W.m01 * G01 + W.m11 * G11));
+  };
+  if (RC == 0) {
+    for (uint64_t k = a.row_ptr[i]; k < a.row_ptr[i + 1]; ++k) {
+      const uint32_t j = a.vox[k];
+      if (j < a.nv) add(j, a.len[k], a.tail[k]);
+    }
+  } else {
+    walk_path(a.g, a.geo + 9 * i, a.straight, [&](uint32_t j, double L, double s, bool) {
+      if (j < a.nv) add(j, L, P - s - L);
+    });
+  }
+}
+
+// Forward: cost_i per block, G_i (and the path length) per track.
 template <int RC>
 __global__ void __launch_bounds__(256)
-k_map_fwd64(size_t n_trk, const uint64_t* __restrict__ row_ptr,
-            const uint32_t* __restrict__ vox, const double* __restrict__ len,
-            const double* __restrict__ tail, GridD g, const double* __restrict__ geo,
-            int straight, const double* __restrict__ trk4, const double* __restrict__ pr,
-            const double* __restrict__ lam, size_t nv, ErrM E, double* __restrict__ gsave,
+k_map_fwd64(MapTrackArgs a, const double* __restrict__ lam, double* __restrict__ gsave,
             double* __restrict__ cost_part, unsigned long long* __restrict__ cnt) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   double cost = 0.0;
   bool sing = false;
-  if (i < n_trk) {
-    Mat2 cov = {0.0, 0.0, 0.0, 0.0};
-    double P = 0.0;
-    if (RC == 0) {
-      for (uint64_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
-        Mat2 W;
-        weight(len[k], tail[k], W);
-        const uint32_t j = vox[k];
-        const double lj = j < nv ? lam[j] : 0.0;
-        cov.m00 = cov.m00 + lj * W.m00;
-        cov.m01 = cov.m01 + lj * W.m01;
-        cov.m10 = cov.m10 + lj * W.m10;
-        cov.m11 = cov.m11 + lj * W.m11;
-      }
-    } else {
-      // moments in tau = path length from the start: int (1, tau)(1, tau)^T
-      double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-      P = walk_path(g, geo + 9 * i, straight, [&](uint32_t j, double L, double s, bool) {
-        const double lj = j < nv ? lam[j] : 0.0;
-        a0 += lj * L;
-        a1 += lj * (L * s + 0.5 * L * L);
-        a2 += lj * (L * s * s + L * L * s + L * L * L / 3.0);
-      });
-      // tail t = P - tau: u(t) = A u(tau), A = [[1, 0], [P, -1]]
-      cov.m00 = a0;
-      cov.m01 = cov.m10 = P * a0 - a1;
-      cov.m11 = P * (P * a0 - 2.0 * a1) + a2;
-    }
-    const double p = pr[i], pp = p * p;
-    cov.m00 = pp * cov.m00 + E.e00;
-    cov.m01 = pp * cov.m01 + E.e01;
-    cov.m10 = pp * cov.m10 + E.e01;
-    cov.m11 = pp * cov.m11 + E.e11;
-    const double Sx = trk4[4 * i], Sy = trk4[4 * i + 1];
-    const double Dx = trk4[4 * i + 2], Dy = trk4[4 * i + 3];
-    const double d00 = Sx * Sx + Sy * Sy, d11 = Dx * Dx + Dy * Dy, d01 = Sx * Dx + Sy * Dy;
-    Mat2 inv;
-    double det;
-    const bool ok = inv_det(cov, inv, det);
-    double G00 = 0.0, G01 = 0.0, G11 = 0.0;
-    if (!ok || det <= 0) {
-      sing = true;
-    } else {
-      cost = log(det) + (d00 * inv.m00 + d11 * inv.m11 + d01 * 2.0 * inv.m01) * 0.5;
-      // G = p^2 (1/2 inv D inv - inv)
-      const double A00 = inv.m00 * d00 + inv.m01 * d01, A01 = inv.m00 * d01 + inv.m01 * d11;
-      const double A10 = inv.m10 * d00 + inv.m11 * d01, A11 = inv.m10 * d01 + inv.m11 * d11;
-      G00 = pp * (0.5 * (A00 * inv.m00 + A01 * inv.m10) - inv.m00);
-      G01 = pp * (0.5 * (A00 * inv.m01 + A01 * inv.m11) - inv.m01);
-      G11 = pp * (0.5 * (A10 * inv.m01 + A11 * inv.m11) - inv.m11);
-    }
+  if (i < a.n_trk) {
+    double G00, G01, G11, P;
+    map_track<RC>(a, i, lam, cost, sing, G00, G01, G11, P);
     gsave[4 * i] = G00;
     gsave[4 * i + 1] = G01;
     gsave[4 * i + 2] = G11;
@@ -128,34 +192,15 @@ k_map_fwd64(size_t n_trk, const uint64_t* __restrict__ row_ptr,
   wave_count(&cnt[C_SINGULAR], sing);
 }
 
-// Per track: grad_j += scale <W_ij, G_i> over its path (scale = -1/2 grad_phi).
+// Backward: grad_j += scale <W_ij, G_i> (scale = -1/2 grad_phi).
 template <int RC>
 __global__ void __launch_bounds__(256)
-k_map_bwd64(size_t n_trk, const uint64_t* __restrict__ row_ptr,
-            const uint32_t* __restrict__ vox, const double* __restrict__ len,
-            const double* __restrict__ tail, GridD g, const double* __restrict__ geo,
-            int straight, const double* __restrict__ gsave, size_t nv, double scale,
+k_map_bwd64(MapTrackArgs a, const double* __restrict__ gsave, double scale,
             double* __restrict__ grad) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_trk) return;
-  const double G00 = gsave[4 * i], G01 = gsave[4 * i + 1], G11 = gsave[4 * i + 2];
-  if (G00 == 0.0 && G01 == 0.0 && G11 == 0.0) return;
-  auto add = [&](uint32_t j, double L, double T) {
-    Mat2 W;
-    weight(L, T, W);
-    atomicAdd(&grad[j], scale * (W.m00 * G00 + 2.0 * W.m01 * G01 + W.m11 * G11));
-  };
-  if (RC == 0) {
-    for (uint64_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
-      const uint32_t j = vox[k];
-      if (j < nv) add(j, len[k], tail[k]);
-    }
-  } else {
-    const double P = gsave[4 * i + 3];
-    walk_path(g, geo + 9 * i, straight, [&](uint32_t j, double L, double s, bool) {
-      if (j < nv) add(j, L, P - s - L);
-    });
-  }
+  if (i >= a.n_trk) return;
+  map_scatter<RC>(a, i, gsave[4 * i], gsave[4 * i + 1], gsave[4 * i + 2], gsave[4 * i + 3], scale,
+                  grad);
 }
 
 __device__ __forceinline__ double psi(double t, double p, double dd) {
@@ -215,6 +260,12 @@ __global__ void k_scale(size_t n, double a, double* __restrict__ x) {
   if (j < n) x[j] *= a;
 }
 
+// x[0..n) = 0 by a kernel (not a memset node: callers may capture the call)
+__global__ void k_zero_f64(size_t n, double* __restrict__ x) {
+  size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) x[j] = 0.0;
+}
+
 }  // namespace
 
 extern "C" int muto_map_forward(muto_em_plan* pl, const double* lambda, int precision, int flags,
@@ -227,7 +278,12 @@ extern "C" int muto_map_forward(muto_em_plan* pl, const double* lambda, int prec
     ctx->set_error("muto_map_forward: precision must be MUTO_FP64 or MUTO_FP32C");
     return MUTO_ERR_ARG;
   }
+  if (flags & ~(MUTO_MAP_RECOMPUTE_SIDDON | MUTO_MAP_DEVICE_BUFFERS)) {
+    ctx->set_error("muto_map_forward: unknown flag");
+    return MUTO_ERR_ARG;
+  }
   const bool recompute = (flags & MUTO_MAP_RECOMPUTE_SIDDON) || !pl->has_lists;
+  const bool dev = (flags & MUTO_MAP_DEVICE_BUFFERS) != 0;
   if (recompute && precision != MUTO_FP64) {
     ctx->set_error("muto_map_forward: recomputed paths run in MUTO_FP64 (the FP32C brick stream is "
                    "built from stored lists)");
@@ -242,29 +298,32 @@ extern "C" int muto_map_forward(muto_em_plan* pl, const double* lambda, int prec
   sv->pl = pl;
   sv->precision = precision;
   sv->recompute = recompute;
+  sv->dev = dev;
   const size_t nv = pl->nv, nt = pl->n_trk;
   auto fail = [&](int rc) { delete sv; return rc; };
   const size_t n_part = std::max<size_t>(grid_for(nt, 256), grid_for(nv, 256));
   if (sv->lam.alloc(nv) != hipSuccess || sv->sum.alloc(2) != hipSuccess ||
-      sv->grad.alloc(nv) != hipSuccess || sv->part.alloc(n_part) != hipSuccess) {
+      (!dev && sv->grad.alloc(nv) != hipSuccess) || sv->part.alloc(n_part) != hipSuccess) {
     ctx->set_error("muto_map_forward: out of device memory");
     return fail(MUTO_ERR_NOMEM);
   }
-  MUTO_CHECK_HIP(ctx, hipMemcpyAsync(sv->lam.p, lambda, nv * sizeof(double), hipMemcpyHostToDevice, s));
-  MUTO_CHECK_HIP(ctx, hipMemsetAsync(sv->sum.p, 0, 2 * sizeof(double), s));
+  // the saved context keeps its own copy of lambda (the caller may change
+  // its buffer before the backward call)
+  if (hipMemcpyAsync(sv->lam.p, lambda, nv * sizeof(double),
+                     dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemsetAsync(sv->sum.p, 0, 2 * sizeof(double), s) != hipSuccess) {
+    ctx->set_error("muto_map_forward: copy of lambda failed");
+    return fail(MUTO_ERR_HIP);
+  }
   double cost_sum = 0.0;
   if (nt) {
     if (precision == MUTO_FP64) {
       if (sv->g64.alloc(4 * nt) != hipSuccess) { ctx->set_error("muto_map_forward: out of device memory"); return fail(MUTO_ERR_NOMEM); }
-      const ErrM E = error_matrix(pl->cfg);
+      const MapTrackArgs a = track_args(pl);
       if (recompute)
-        k_map_fwd64<1><<<grid_for(nt, 256), 256, 0, s>>>(nt, nullptr, nullptr, nullptr, nullptr, pl->g,
-            pl->geo.p, pl->cfg.straight_line, pl->trk4.p, pl->pr.p, sv->lam.p, nv, E, sv->g64.p,
-            sv->part.p, pl->dcount.p);
+        k_map_fwd64<1><<<grid_for(nt, 256), 256, 0, s>>>(a, sv->lam.p, sv->g64.p, sv->part.p, pl->dcount.p);
       else
-        k_map_fwd64<0><<<grid_for(nt, 256), 256, 0, s>>>(nt, pl->row_ptr.p, pl->vox.p, pl->len.p,
-            pl->tail.p, pl->g, nullptr, 0, pl->trk4.p, pl->pr.p, sv->lam.p, nv, E, sv->g64.p,
-            sv->part.p, pl->dcount.p);
+        k_map_fwd64<0><<<grid_for(nt, 256), 256, 0, s>>>(a, sv->lam.p, sv->g64.p, sv->part.p, pl->dcount.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
       k_sum_parts<<<1, 256, 0, s>>>(sv->part.p, grid_for(nt, 256), sv->sum.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
@@ -299,26 +358,29 @@ extern "C" int muto_map_backward(muto_map_saved* sv, double grad_phi, double* gr
   if (!grad_lambda) { ctx->set_error("muto_map_backward: null argument"); return MUTO_ERR_ARG; }
   hipStream_t s = ctx->stream;
   const size_t nv = pl->nv, nt = pl->n_trk;
-  MUTO_CHECK_HIP(ctx, hipMemsetAsync(sv->grad.p, 0, nv * sizeof(double), s));
+  // device buffers: the gradient accumulates in the caller's array
+  double* grad = sv->dev ? grad_lambda : sv->grad.p;
+  k_zero_f64<<<grid_for(nv, 256), 256, 0, s>>>(nv, grad);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (nt) {
     if (sv->precision == MUTO_FP64) {
+      const MapTrackArgs a = track_args(pl);
       if (sv->recompute)
-        k_map_bwd64<1><<<grid_for(nt, 256), 256, 0, s>>>(nt, nullptr, nullptr, nullptr, nullptr, pl->g,
-            pl->geo.p, pl->cfg.straight_line, sv->g64.p, nv, -0.5 * grad_phi, sv->grad.p);
+        k_map_bwd64<1><<<grid_for(nt, 256), 256, 0, s>>>(a, sv->g64.p, -0.5 * grad_phi, grad);
       else
-        k_map_bwd64<0><<<grid_for(nt, 256), 256, 0, s>>>(nt, pl->row_ptr.p, pl->vox.p, pl->len.p,
-            pl->tail.p, pl->g, nullptr, 0, sv->g64.p, nv, -0.5 * grad_phi, sv->grad.p);
+        k_map_bwd64<0><<<grid_for(nt, 256), 256, 0, s>>>(a, sv->g64.p, -0.5 * grad_phi, grad);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     } else {
-      if (int rc = fp32c_map_backward(pl, sv->g32.p, sv->cmax.p, sv->grad.p)) return rc;
-      k_scale<<<grid_for(nv, 256), 256, 0, s>>>(nv, -0.5 * grad_phi, sv->grad.p);
+      if (int rc = fp32c_map_backward(pl, sv->g32.p, sv->cmax.p, grad)) return rc;
+      k_scale<<<grid_for(nv, 256), 256, 0, s>>>(nv, -0.5 * grad_phi, grad);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
   }
   k_prior_grad<<<grid_for(nv, 256), 256, 0, s>>>(map_cfg(pl), nv, sv->lam.p, grad_phi * pl->cfg.beta,
-                                                 sv->grad.p);
+                                                 grad);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
-  MUTO_CHECK_HIP(ctx, hipMemcpyAsync(grad_lambda, sv->grad.p, nv * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (!sv->dev)
+    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(grad_lambda, grad, nv * sizeof(double), hipMemcpyDeviceToHost, s));
   MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
   return MUTO_OK;
 }

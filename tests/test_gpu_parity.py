@@ -413,6 +413,35 @@ def test_em_fp64_prepared(ctx, small_set, mode):
             f"max abs diff {np.max(np.abs(out - ref))}"
 
 
+@pytest.mark.parametrize("mode", ["median", "trimmed"])
+@pytest.mark.parametrize("n", [(2, 2, 2), (3, 3, 2)])
+def test_em_order_statistics_long_voxel_lists(ctx, small_set, mode, n):
+    """Coarse grids over the same 1 m cube: 2^3 and 3x3x2 voxels of 333-500
+    mm, so most voxels hold thousands of entries and several hold more than
+    the 4096 that one block sorts in LDS (those go to the listed-voxel sort
+    in global scratch, one block per voxel).  Median and trimmed mean
+    (src/MutoMLSD.cpp:493-517) bit-exact against the oracle on its prepared
+    inputs for 3 iterations (6e4 muons of the small set's generator)."""
+    g0, rays, _ = W.muons("small", n_muons=60_000)
+    ext = g0.dx * g0.nx
+    g = make_grid(n, tuple(ext / k for k in n), g0.x_min)
+    kw = dict(update_with_mean=0) if mode == "median" else dict(update_with_mean=1, mean_percentile=60)
+    cfg = _em_cfg(g, **kw)
+    em = oracle.Em(cfg, rays)
+    mj = em.mj()
+    assert (mj > 4096).sum() >= 2, mj
+    plan = ctx.em_plan(cfg, prepared=em.prepared())
+    np.testing.assert_array_equal(plan.voxel_counts(), mj)
+    img = np.full(g.n_vox, 1e-7)
+    plan.set_image(img)
+    ref = img
+    for _ in range(3):
+        ref, _c, _ns = em.iterate(ref)
+    plan.run(3)
+    out = plan.image()
+    assert np.array_equal(out.view(np.uint64), ref.view(np.uint64)), (out, ref, mj)
+
+
 @pytest.mark.parametrize("mode", ["median", "mean"])
 def test_em_singular_covariances(ctx, small_set, mode):
     """Covariances that are not invertible (MutoMLSD.cpp:456-463): with no
@@ -882,6 +911,51 @@ def test_map_objective_fp32c_close_to_fp64(ctx, small_set):
     assert np.abs(g32 - g64).max() <= 1e-4 * np.abs(g64).max()
     s64.close()
     s32.close()
+    plan.close()
+
+
+@pytest.mark.parametrize("prec,recompute", [(M.MUTO_FP64, False), (M.MUTO_FP64, True),
+                                            (M.MUTO_FP32C, False)])
+def test_device_buffers_match_host_buffers(ctx, small_set, prec, recompute):
+    """Device-pointer mode of the image hand-over (muto_em_plan_set_image_ex
+    / get_image_ex, MUTO_MEM_DEVICE) and of the MAP operator
+    (MUTO_MAP_DEVICE_BUFFERS): the same image bits as the host calls, Phi
+    bit-identical, and the gradient equal up to the order of its FP64
+    atomic additions (1e-13 of its largest component)."""
+    from muontomography_amd.devmem import DeviceArray
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, beta=1.0, p=1.2, delta=0.1,
+                  all_neighbours=1)
+    plan = ctx.em_plan(cfg, rays=rays)
+    lam = _lam_field(g.n_vox, 9)
+    tl = DeviceArray.from_numpy(lam)
+    plan.set_image(tl)
+    out = DeviceArray(g.n_vox)
+    out.fill_bytes(0xFF)
+    plan.image(out=out)
+    host = plan.image()
+    assert np.array_equal(out.numpy().view(np.uint64), lam.view(np.uint64))
+    assert np.array_equal(host.view(np.uint64), lam.view(np.uint64))
+    if not recompute:
+        plan.run(2, prec)
+        plan.image(out=out)
+        assert np.array_equal(out.numpy().view(np.uint64), plan.image().view(np.uint64))
+    sh = plan.map_forward(lam, prec, recompute=recompute)
+    sd = plan.map_forward(tl, prec, recompute=recompute)
+    assert sd.phi == sh.phi, (sd.phi, sh.phi)
+    gh = sh.backward(1.0)
+    gd = sd.backward(1.0)
+    assert isinstance(gd, DeviceArray)
+    gd = gd.numpy()
+    scale = np.abs(gh).max()
+    assert scale > 0
+    np.testing.assert_allclose(gd, gh, rtol=0, atol=1e-13 * scale)
+    # the caller's buffer is overwritten (not accumulated into) on a second call
+    o2 = DeviceArray.from_numpy(np.full(g.n_vox, 7.0))
+    sd.backward(-2.0, out=o2)
+    np.testing.assert_allclose(o2.numpy(), -2.0 * gh, rtol=0, atol=1e-12 * scale)
+    sh.close()
+    sd.close()
     plan.close()
 
 
