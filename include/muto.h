@@ -318,6 +318,22 @@ int muto_map_forward(muto_em_plan* plan, const double* lambda, int precision, in
 int muto_map_backward(muto_map_saved* saved, double grad_phi, double* grad_lambda);
 int muto_map_saved_destroy(muto_map_saved* saved);
 
+/* Phi and grad_lambda = dPhi/dlambda in one call that allocates nothing:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  FP64, from the plan's stored lists or, with
+ * MUTO_MAP_RECOMPUTE_SIDDON (implied for plans without lists), with the
+ * paths walked again; the same Phi (bit for bit) and gradient (up to the
+ * order of FP64 atomic additions) as muto_map_forward + muto_map_backward
+ * with grad_phi = 1.  lambda[n_vox], phi[1] and grad_lambda[n_vox] are
+ * device pointers on the context's GPU (MUTO_MEM_DEVICE rules).  Its only
+ * memory beyond those and the plan is a static per-GPU scratch of 1 MiB of
+ * partial sums, so plans of more than 2^24 tracks or 2^24 voxels are
+ * refused (MUTO_ERR_LIMIT).  Calls on one GPU share that scratch: calls
+ * that overlap on one GPU (several host threads) race. */
+int muto_map_value_grad_noalloc(muto_em_plan* plan, const double* lambda, int flags,
+                                double* phi, double* grad_lambda);
+
 /* ---- many independent scans (one launch per pass for all of them) ----
  * A batch combines plans (each with its own grid, voxel size and tracks;
  * sharing the EM configuration, mean update) into one problem: their grids

@@ -376,6 +376,20 @@ class EmPlan:
         self._saved.add(sv)
         return sv
 
+    def map_value_grad_noalloc(self, lam, grad, phi, recompute=False):
+        """Phi into phi[0] and dPhi/dlambda into grad, in one call that
+        allocates nothing (muto_map_value_grad_noalloc, FP64): lam and grad
+        (n_vox values) and phi (1 value) are float64 device buffers."""
+        n = self.cfg.grid.n_vox
+        dl = _device_ptr(lam, n, "map_value_grad_noalloc lam")
+        dg = _device_ptr(grad, n, "map_value_grad_noalloc grad")
+        dp = _device_ptr(phi, 1, "map_value_grad_noalloc phi")
+        if dl is None or dg is None or dp is None:
+            raise MutoError("map_value_grad_noalloc: lam, grad and phi must be device buffers")
+        self.ctx._check(lib().muto_map_value_grad_noalloc(
+            self._h, dl, MUTO_MAP_RECOMPUTE_SIDDON if recompute else 0, dp, dg),
+            "muto_map_value_grad_noalloc")
+
     def autotune(self, iters=3, shapes=None):
         """FP32C brick (LDS tile) shape autotuner (muto_em_plan_autotune).
         `shapes`: (bx, by, bz) or (bx, by, bz, block) tuples, or None for the

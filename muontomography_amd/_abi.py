@@ -172,6 +172,7 @@ PROTOS = {
     "muto_map_forward": (C.c_int, [P, P, C.c_int, C.c_int, DP, C.POINTER(P)]),
     "muto_map_backward": (C.c_int, [P, C.c_double, P]),
     "muto_map_saved_destroy": (C.c_int, [P]),
+    "muto_map_value_grad_noalloc": (C.c_int, [P, P, C.c_int, P, P]),
     "muto_batch_create": (C.c_int, [P, C.c_size_t, P, C.POINTER(P)]),
     "muto_batch_run": (C.c_int, [P, C.c_int, P]),
     "muto_batch_stats": (C.c_int, [P, DP, C.POINTER(Counters)]),

@@ -16,6 +16,9 @@ extern "C" int muto_ctx_create(int device, muto_ctx** out) {
   // the code objects are built for gfx950 only
   if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return MUTO_ERR_NODEV;
   if (hipSetDevice(device) != hipSuccess) return MUTO_ERR_HIP;
+  // the no-alloc entry point's scratch address on this device (its status is
+  // kept and reported by that entry point; nothing else depends on it)
+  (void)muto::noalloc_scratch_init(device);
   auto* c = new muto_ctx();
   c->device = device;
   c->n_cu = prop.multiProcessorCount;

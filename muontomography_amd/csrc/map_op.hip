@@ -23,6 +23,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -203,6 +204,25 @@ k_map_bwd64(MapTrackArgs a, const double* __restrict__ gsave, double scale,
                   grad);
 }
 
+// Forward and backward of one track in one pass, nothing stored per track
+// (muto_map_value_grad_noalloc): the same cost_i, G_i and scatter as the
+// two kernels above.
+template <int RC>
+__global__ void __launch_bounds__(256)
+k_map_fb64(MapTrackArgs a, const double* __restrict__ lam, double scale, double* __restrict__ grad,
+           double* __restrict__ cost_part, unsigned long long* __restrict__ cnt) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double cost = 0.0;
+  bool sing = false;
+  if (i < a.n_trk) {
+    double G00, G01, G11, P;
+    map_track<RC>(a, i, lam, cost, sing, G00, G01, G11, P);
+    map_scatter<RC>(a, i, G00, G01, G11, P, scale, grad);
+  }
+  block_sum_to(cost_part, cost);
+  wave_count(&cnt[C_SINGULAR], sing);
+}
+
 __device__ __forceinline__ double psi(double t, double p, double dd) {
   return pow(t * t + dd, p / 2.0);
 }
@@ -264,6 +284,11 @@ __global__ void k_scale(size_t n, double a, double* __restrict__ x) {
 __global__ void k_zero_f64(size_t n, double* __restrict__ x) {
   size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j < n) x[j] = 0.0;
+}
+
+// phi = 1/2 cost sum + beta prior sum, on the device
+__global__ void k_phi_combine(const double* __restrict__ sums, double beta, double* __restrict__ phi) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) phi[0] = 0.5 * sums[0] + beta * sums[1];
 }
 
 }  // namespace
@@ -389,5 +414,107 @@ extern "C" int muto_map_saved_destroy(muto_map_saved* sv) {
   if (!sv) return MUTO_OK;
   if (sv->pl && sv->pl->ctx) (void)hipStreamSynchronize(sv->pl->ctx->stream);
   delete sv;
+  return MUTO_OK;
+}
+
+// ------------------------------------------------------------------ //
+// Phi and its gradient without allocating (muto_map_value_grad_noalloc) //
+// ------------------------------------------------------------------ //
+// The entry point's only memory beyond its caller's buffers and the plan:
+// the per-block partial sums of the track costs and of the prior, and their
+// two totals.  One copy per GPU (each device holds its own, at its own
+// address), kept until the process ends; every call on a GPU reuses it, so
+// calls that overlap on one GPU race.
+constexpr size_t NOALLOC_PARTS = 65536;  // at most 2^24 tracks and 2^24 voxels
+constexpr size_t NOALLOC_DOUBLES = 2 * NOALLOC_PARTS + 32;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+hipError_t g_scratch_err[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+namespace muto {
+// Called by muto_ctx_create with `device` current: the device address of
+// this GPU's copy of `scratch` (host code must never pass the symbol itself
+// to a kernel or a copy).
+hipError_t noalloc_scratch_init(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = static_cast<double*>(ptr);
+  return hipSuccess;
+}
+}  // namespace muto
+
+extern "C" int muto_map_value_grad_noalloc(muto_em_plan* pl, const double* lambda, int flags,
+                                           double* phi, double* grad_lambda) {
+  if (!pl) return MUTO_ERR_ARG;
+  muto_ctx* ctx = pl->ctx;
+  if (!lambda || !phi || !grad_lambda) {
+    ctx->set_error("muto_map_value_grad_noalloc: null argument");
+    return MUTO_ERR_ARG;
+  }
+  if (flags & ~MUTO_MAP_RECOMPUTE_SIDDON) {
+    ctx->set_error("muto_map_value_grad_noalloc: unknown flag (the buffers are always device memory)");
+    return MUTO_ERR_ARG;
+  }
+  const size_t nv = pl->nv, nt = pl->n_trk;
+  if (grid_for(nt, 256) > NOALLOC_PARTS || grid_for(nv, 256) > NOALLOC_PARTS) {
+    ctx->set_error("muto_map_value_grad_noalloc: more than 2^24 tracks or 2^24 voxels (the size of "
+                   "its static scratch); use muto_map_forward / muto_map_backward");
+    return MUTO_ERR_LIMIT;
+  }
+  const bool recompute = (flags & MUTO_MAP_RECOMPUTE_SIDDON) || !pl->has_lists;
+  if (recompute && !pl->geo.p && nt) {
+    ctx->set_error("muto_map_value_grad_noalloc: the plan has no track geometry to walk (prepared plan)");
+    return MUTO_ERR_STATE;
+  }
+  double* scr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES) scr = g_scratch_ptr[ctx->device];
+  }
+  if (!scr) {
+    const hipError_t e = ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES
+                             ? g_scratch_err[ctx->device] : hipErrorInvalidDevice;
+    ctx->set_error(std::string("muto_map_value_grad_noalloc: no scratch address on this device: ") +
+                   hipGetErrorString(e));
+    return MUTO_ERR_HIP;
+  }
+  double* part_t = scr;                  // [NOALLOC_PARTS] track cost partials
+  double* part_p = scr + NOALLOC_PARTS;  // [NOALLOC_PARTS] prior partials
+  double* sums = scr + 2 * NOALLOC_PARTS;  // [2] cost sum, prior sum
+  hipStream_t s = ctx->stream;
+  k_zero_f64<<<grid_for(nv, 256), 256, 0, s>>>(nv, grad_lambda);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  k_zero_f64<<<1, 256, 0, s>>>(2, sums);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  if (nt) {
+    const MapTrackArgs a = track_args(pl);
+    if (recompute)
+      k_map_fb64<1><<<grid_for(nt, 256), 256, 0, s>>>(a, lambda, -0.5, grad_lambda, part_t, pl->dcount.p);
+    else
+      k_map_fb64<0><<<grid_for(nt, 256), 256, 0, s>>>(a, lambda, -0.5, grad_lambda, part_t, pl->dcount.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+    k_sum_parts<<<1, 256, 0, s>>>(part_t, grid_for(nt, 256), sums);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  }
+  const MapCfgD mc = map_cfg(pl);
+  k_prior_phi<<<grid_for(nv, 256), 256, 0, s>>>(mc, nv, lambda, part_p);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  k_sum_parts<<<1, 256, 0, s>>>(part_p, grid_for(nv, 256), sums + 1);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  k_prior_grad<<<grid_for(nv, 256), 256, 0, s>>>(mc, nv, lambda, pl->cfg.beta, grad_lambda);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  k_phi_combine<<<1, 64, 0, s>>>(sums, pl->cfg.beta, phi);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
   return MUTO_OK;
 }

@@ -97,6 +97,11 @@ inline unsigned grid_for(size_t n, unsigned block) {
   return (unsigned)g;
 }
 
+// Device address of this GPU's copy of the static scratch of
+// muto_map_value_grad_noalloc, looked up once per device with the device
+// current (map_op.hip); the status is kept for that entry point to report.
+hipError_t noalloc_scratch_init(int device);
+
 // Device-wide helpers (scan.hip)
 hipError_t exclusive_scan_u64(const uint64_t* in, uint64_t* out, size_t n,
                               hipStream_t s);
