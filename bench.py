@@ -45,6 +45,17 @@ def read_traffic(workload, kernel):
     return None
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), as lscpu reports it."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(g, rays, cfg_kw, n_sample, iters):
     """The CPU FP64 oracle (reference operation order; serial track loop,
     OpenMP voxel loop) on a seeded subset of the same muons at the full
@@ -64,11 +75,146 @@ def cpu_baseline(g, rays, cfg_kw, n_sample, iters):
     nnz = em.nnz
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": nnz * iters / t, "unit": "track-voxel updates/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"{n_sample} of the same muons (first {n_sample}), full 256^3 grid, "
                       f"{nnz} entries, {iters} FP64 mean-update iterations in {t:.2f} s "
                       f"(set-up {t_prep:.2f} s not counted); serial track loop, OpenMP voxel "
                       f"loop over {threads} threads"}
+
+
+N_COMPARE = 50  # iterations of the FP32C-vs-FP64 image comparison (the configuration's count)
+
+
+def _rel(a, ref, mj):
+    m = (mj > 0) & (ref > 0)
+    r = np.abs(a[m] - ref[m]) / ref[m]
+    return (float(r.max()) if r.size else None, float(np.sqrt(np.mean(r * r))) if r.size else None,
+            float(np.abs(a - ref).max() / np.abs(ref).max()))
+
+
+def bench_fp64(args, plan, rays, cfg_kw, init, nnz, wall32, cost32):
+    """The FP64 paths beside FP32C, each for N_COMPARE iterations from the
+    same image: the reference-order kernel (one lane per track), the
+    wavefront-per-track kernel, and the FP64 brick stream; the FP32C and
+    FP64-stream images against the reference-order one.  Then the
+    reference's default update rule, the median (src/MutoMLSD.cpp:505-515):
+    reference-order FP64 for 3 iterations beside the FP64 stream."""
+    import muontomography_amd as M
+    n = N_COMPARE
+    res = {}
+    mj = plan.voxel_counts()
+    # reference order
+    plan.set_image(init)
+    t0 = time.perf_counter()
+    cost64 = plan.run(n, M.MUTO_FP64)
+    w64 = (time.perf_counter() - t0) / n
+    img64 = plan.image()
+    t64 = plan.timing()
+    # wavefront per track
+    plan.set_option(M.MUTO_OPT_FP64_KERNEL, 1)
+    plan.set_image(init)
+    plan.run(1, M.MUTO_FP64)
+    plan.set_image(init)
+    t0 = time.perf_counter()
+    plan.run(n, M.MUTO_FP64)
+    w64w = (time.perf_counter() - t0) / n
+    img64w = plan.image()
+    # FP32C after the same iterations
+    plan.set_image(init)
+    cost32n = plan.run(n, M.MUTO_FP32C)
+    img32 = plan.image()
+    e32 = _rel(img32, img64, mj)
+    # FP64 brick stream: its own brick shape (FP64 bricks and tables in LDS),
+    # the fastest of a few over 2 iterations
+    plan.set_option(M.MUTO_OPT_FP64_KERNEL, 2)
+    tune = {}
+    for shape in ((16, 16, 16, 256), (16, 8, 32, 256), (8, 16, 32, 512), (16, 16, 32, 512)):
+        plan.set_option(M.MUTO_OPT_BRICK, M.brick_code(shape))
+        plan.set_image(init)
+        plan.run(1, M.MUTO_FP64)
+        plan.set_option(M.MUTO_OPT_TIMING, 1)
+        plan.run(2, M.MUTO_FP64)
+        plan.set_option(M.MUTO_OPT_TIMING, 0)
+        tune[shape] = plan.timing()["iteration"]
+    b64 = min(tune, key=tune.get)
+    plan.set_option(M.MUTO_OPT_BRICK, M.brick_code(b64))
+    plan.set_image(init)
+    plan.run(1, M.MUTO_FP64)
+    plan.set_image(init)
+    t0 = time.perf_counter()
+    cost64s = plan.run(n, M.MUTO_FP64)
+    w64s = (time.perf_counter() - t0) / n
+    img64s = plan.image()
+    plan.set_option(M.MUTO_OPT_TIMING, 1)
+    plan.set_image(init)
+    plan.run(args.timing_steps, M.MUTO_FP64)
+    plan.set_option(M.MUTO_OPT_TIMING, 0)
+    t64s = plan.timing()
+    e64s = _rel(img64s, img64, mj)
+    log(f"FP64 {n} it: reference order {w64 * 1e3:.2f} ms/it, wavefront {w64w * 1e3:.2f}, "
+        f"stream {w64s * 1e3:.2f} (brick {b64}; {tune}); FP32C err max {e32[0]}, "
+        f"FP64 stream err max {e64s[0]}")
+    # median, the reference's default rule
+    kw = dict(cfg_kw)
+    kw["update_with_mean"] = 0
+    med = plan.ctx.em_plan(M.mlsd_cfg(plan.cfg.grid, **kw), rays=rays)
+    nm = 3
+    med.set_image(init)
+    t0 = time.perf_counter()
+    med.run(nm, M.MUTO_FP64)
+    wm_ref = (time.perf_counter() - t0) / nm
+    imgm_ref = med.image()
+    med.set_option(M.MUTO_OPT_FP64_KERNEL, 2)
+    med.set_option(M.MUTO_OPT_BRICK, M.brick_code(b64))
+    med.set_image(init)
+    med.run(1, M.MUTO_FP64)  # stream, slots, graph
+    med.set_image(init)
+    med.run(nm, M.MUTO_FP64)
+    imgm = med.image()
+    em_ = _rel(imgm, imgm_ref, mj)
+    med.set_image(init)
+    t0 = time.perf_counter()
+    med.run(args.steps, M.MUTO_FP64)
+    wm = (time.perf_counter() - t0) / args.steps
+    med.set_option(M.MUTO_OPT_TIMING, 1)
+    med.set_image(init)
+    med.run(args.timing_steps, M.MUTO_FP64)
+    tm = med.timing()
+    med.close()
+    log(f"median: reference order {wm_ref * 1e3:.1f} ms/it, FP64 stream {wm * 1e3:.2f} ms/it "
+        f"(phases {tm}); err after {nm} it {em_}")
+    ms32 = wall32 / args.steps * 1e3
+    # the FP64 stream's iteration against its HBM roofline (12 B per update)
+    bw64s = BYTES_PER_UPDATE_FP64 * nnz / (t64s["iteration"] * 1e-3)
+    return {
+        "iterations_compared": n,
+        "fp64_ms_per_step": w64 * 1e3,
+        "fp64_value": nnz / w64,
+        "fp32c_speedup_vs_fp64": w64 * 1e3 / ms32,
+        "image_err_max_rel": e32[0], "image_err_rms_rel": e32[1], "image_err_norm_inf": e32[2],
+        "cost_rel_diff_last": float(abs(cost32n[-1] - cost64[-1]) / abs(cost64[-1])),
+        "fp64_kernel_ms": {k: round(v, 4) for k, v in t64.items()},
+        "fp64_wave_ms_per_step": w64w * 1e3,
+        "fp64_wave_value": nnz / w64w,
+        "fp32c_speedup_vs_fp64_wave": w64w * 1e3 / ms32,
+        "fp64_wave_vs_fp64_err_max_rel": _rel(img64w, img64, mj)[0],
+        "fp64_stream": {"ms_per_step": w64s * 1e3, "value": nnz / w64s, "brick": list(b64),
+                        "kernel_ms": {k: round(v, 4) for k, v in t64s.items()},
+                        "brick_ms": {",".join(map(str, k)): round(v, 3) for k, v in tune.items()},
+                        "speedup_vs_fp64": w64 / w64s,
+                        "err_vs_fp64_max_rel": e64s[0], "err_vs_fp64_rms_rel": e64s[1],
+                        "cost_rel_diff_last": float(abs(cost64s[-1] - cost64[-1]) / abs(cost64[-1])),
+                        "iteration_GBps": bw64s / 1e9, "iteration_frac": bw64s / HBM_PEAK},
+        "median": {"fp64_ms_per_step": wm_ref * 1e3, "fp64_stream_ms_per_step": wm * 1e3,
+                   "roofline": {"bound": "hbm", "kernel": "whole FP64 stream median iteration",
+                                "achieved": BYTES_PER_UPDATE_FP64 * nnz / wm / 1e9,
+                                "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                                "frac": BYTES_PER_UPDATE_FP64 * nnz / wm / HBM_PEAK},
+                   "fp64_stream_value": nnz / wm, "speedup": wm_ref / wm,
+                   "fp64_stream_kernel_ms": {k: round(v, 4) for k, v in tm.items()},
+                   "iterations_compared": nm, "err_max_rel": em_[0], "err_rms_rel": em_[1]},
+    }
+
 
 
 def bench_scans(args):
@@ -381,56 +527,22 @@ def main():
     t_k = tk[kern] * 1e-3
     achieved = BYTES_PER_UPDATE_FP32C * nnz / t_k
     traffic = read_traffic(args.workload, kname)
+    # the whole iteration against the same roofline, with the per-voxel and
+    # per-track terms of the algorithmic figure: B_iter = nnz 8 + n_vox (4 +
+    # 16 + 4) + n_trk 40 bytes (lambda read, accumulator, M_j; Sx, Sy, Dx,
+    # Dy, p_r), over the timed region's ms per iteration
+    b_iter = BYTES_PER_UPDATE_FP32C * nnz + 24.0 * g.n_vox + 40.0 * c["n_scat_in_grid"]
     roof = {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
             "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-            "iteration_frac": BYTES_PER_UPDATE_FP32C * nnz / (tk["iteration"] * 1e-3) / HBM_PEAK}
+            "timing": f"HIP events on the plan's stream, mean of {args.timing_steps} iterations "
+                      f"({kname} {tk[kern]:.4f} ms)",
+            "iteration_bytes": b_iter,
+            "iteration_achieved": b_iter / (ms_per_step * 1e-3) / 1e9,
+            "iteration_frac": b_iter / (ms_per_step * 1e-3) / HBM_PEAK}
 
     extra = {}
     if not args.no_fp64:
-        plan.set_image(init)
-        t0 = time.perf_counter()
-        cost64 = plan.run(args.warmup + args.steps, M.MUTO_FP64)
-        w64 = time.perf_counter() - t0
-        img64 = plan.image()
-        t64 = plan.timing()
-        # the same iterations with the wavefront-per-track FP64 kernel
-        plan.set_option(M.MUTO_OPT_FP64_KERNEL, 1)
-        plan.set_image(init)
-        plan.run(1, M.MUTO_FP64)
-        plan.set_image(init)
-        t0 = time.perf_counter()
-        plan.run(args.warmup + args.steps, M.MUTO_FP64)
-        w64w = time.perf_counter() - t0
-        img64w = plan.image()
-        plan.set_option(M.MUTO_OPT_FP64_KERNEL, 0)
-        # FP32C image after the same number of iterations
-        plan.set_image(init)
-        plan.run(args.warmup + args.steps, M.MUTO_FP32C)
-        img32b = plan.image()
-        mj = plan.voxel_counts()
-        m = (mj > 0) & (img64 > 0)
-        r = np.abs(img32b[m] - img64[m]) / img64[m]
-        extra = {
-            "fp64_ms_per_step": w64 * 1e3 / (args.warmup + args.steps),
-            "fp64_value": nnz * (args.warmup + args.steps) / w64,
-            "fp32c_speedup_vs_fp64": (w64 / (args.warmup + args.steps)) / (wall / args.steps),
-            "image_err_max_rel": float(r.max()) if r.size else None,
-            "image_err_rms_rel": float(np.sqrt(np.mean(r * r))) if r.size else None,
-            "image_err_norm_inf": float(np.abs(img32b - img64).max() / np.abs(img64).max()),
-            "iterations_compared": args.warmup + args.steps,
-            "cost_rel_diff_last": float(abs(cost32[-1] - cost64[args.warmup + args.steps - 1]) /
-                                        abs(cost64[args.warmup + args.steps - 1])),
-            "fp64_kernel_ms": {k: round(v, 4) for k, v in t64.items()},
-            "fp64_wave_ms_per_step": w64w * 1e3 / (args.warmup + args.steps),
-            "fp64_wave_value": nnz * (args.warmup + args.steps) / w64w,
-            "fp32c_speedup_vs_fp64_wave": (w64w / (args.warmup + args.steps)) / (wall / args.steps),
-            "fp64_wave_vs_fp64_err_max_rel": float((np.abs(img64w[m] - img64[m]) / img64[m]).max())
-                                             if m.any() else None,
-        }
-        log(f"FP64 {args.warmup + args.steps} it: {extra['fp64_ms_per_step']:.2f} ms/it "
-            f"(wavefront kernel {extra['fp64_wave_ms_per_step']:.2f} ms/it); "
-            f"speed-up {extra['fp32c_speedup_vs_fp64']:.2f}x "
-            f"({extra['fp32c_speedup_vs_fp64_wave']:.2f}x); err max {extra['image_err_max_rel']}")
+        extra = bench_fp64(args, plan, rays, cfg_kw, init, nnz, wall, cost32)
     plan.close()
     ctx.close()
 

@@ -361,7 +361,13 @@ int muto_batch_destroy(muto_batch* batch);
  *                     reference's operation order (default; per-entry values
  *                     bit-identical to the CPU path); 1 one wavefront per
  *                     track, coalesced row loads and a butterfly sum of the
- *                     covariance (per-track values within FP64 rounding)
+ *                     covariance (per-track values within FP64 rounding);
+ *                     2 the FP64 brick stream: the FP32C stream's layout with
+ *                     the exact FP64 lengths and tails, per-segment FP64
+ *                     partial sums, LDS-staged backprojection (mean) or
+ *                     per-entry c_ij in voxel-major slots and a per-voxel
+ *                     sort (median, trimmed mean); every W_ij is the
+ *                     reference's, only the association of the sums differs
  * No counterpart in the reference (which has no device state). */
 #define MUTO_OPT_GRAPH 1
 #define MUTO_OPT_TIMING 2

@@ -655,6 +655,8 @@ int fp32c_build(muto_em_plan* pl);
 int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done);
 int fp32c_autotune(muto_em_plan* pl, int iters, int n_cand, const uint32_t* cand, double* ms);
 int fp32c_release(muto_em_plan* pl);
+int fp64s_run(muto_em_plan* pl, int iters, double* sums, int* done);
+int fp64s_build(muto_em_plan* pl);
 
 }  // namespace muto
 
@@ -1109,6 +1111,8 @@ static int plan_run(muto_em_plan* pl, int iters, int precision, double* cost) {
   if (int rc = sync_image(pl, precision)) return rc;
   if (precision == MUTO_FP32C)
     if (int rc = fp32c_build(pl)) return rc;  // counts mj from the brick stream
+  if (precision == MUTO_FP64 && pl->fp64_kernel == 2)
+    if (int rc = fp64s_build(pl)) return rc;
   if (int rc = ensure_mj(pl)) return rc;
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->dcount.p, 0, 8 * sizeof(unsigned long long), s));
   std::vector<double> costs(iters, 0.0);
@@ -1116,9 +1120,11 @@ static int plan_run(muto_em_plan* pl, int iters, int precision, double* cost) {
   int done = 0;
   double tsum[5] = {0, 0, 0, 0, 0};
   const bool time_it = true;
-  if (precision == MUTO_FP32C) {
+  if (precision == MUTO_FP32C || pl->fp64_kernel == 2) {
     std::vector<double> sums(iters, 0.0);
-    if (int rc = fp32c_run(pl, iters, sums.data(), &done)) return rc;
+    if (int rc = precision == MUTO_FP32C ? fp32c_run(pl, iters, sums.data(), &done)
+                                         : fp64s_run(pl, iters, sums.data(), &done))
+      return rc;
     for (int it = 0; it < done; ++it) {
       const double cst = sums[it] / (double)pl->n_trk;  // MutoMLSD.cpp:202
       if (c.log_results) printf("Iteration #%d finished.\n\tcost function = %f\n", it, cst);
@@ -1192,7 +1198,7 @@ extern "C" int muto_em_plan_set_option(muto_em_plan* pl, int option, int value) 
       return MUTO_OK;
     }
     case MUTO_OPT_FP64_KERNEL:
-      if (value != 0 && value != 1) break;
+      if (value < 0 || value > 2) break;
       pl->fp64_kernel = value;
       return MUTO_OK;
     case MUTO_OPT_ACC_MODE:

@@ -50,15 +50,11 @@
 #include <type_traits>
 #include <vector>
 
+#include "em_dev.hpp"
 #include "em_plan.hpp"
 
 namespace muto {
 
-enum { C_SINGULAR = 0 };
-struct ErrM {
-  double e00, e01, e11;
-};
-ErrM error_matrix(const muto_mlsd_cfg& c);
 int map_step(muto_em_plan* pl, const double* last, const double* ml, double* out);
 int fp32c_mirror(muto_em_plan* pl);
 int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done);
@@ -379,12 +375,14 @@ k_seg_gather(uint64_t n_seg, const uint32_t* __restrict__ sval,
              const double* __restrict__ st_tend,
              const uint32_t* __restrict__ st_trk,
              const uint32_t* __restrict__ rank, double tail_scale,
-             uint32_t* __restrict__ seg_trk, uint32_t* __restrict__ seg_tfx) {
+             uint32_t* __restrict__ seg_trk, uint32_t* __restrict__ seg_tfx,
+             double* __restrict__ seg_tail64) {
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_seg) return;
   const uint32_t s = sval[q];
   seg_trk[q] = rank[st_trk[s]];
   seg_tfx[q] = (uint32_t)llrint(st_tend[s] * tail_scale);
+  if (seg_tail64) seg_tail64[q] = st_tend[s];  // the FP64 stream: the exact tail
 }
 
 // Entry offset of every segment: its chunk's base (16-entry aligned, set by
@@ -447,7 +445,8 @@ k_ent_fill(BrickGeo G, const Chunk* __restrict__ chunks, const uint32_t* __restr
            const uint64_t* __restrict__ st_start, const uint64_t* __restrict__ seg_off_p,
            const uint32_t* __restrict__ vox, const double* __restrict__ len, double inv_unit,
            uint32_t* __restrict__ ent, uint16_t* __restrict__ seg_v0,
-           unsigned long long* __restrict__ n_clamped) {
+           unsigned long long* __restrict__ n_clamped, const double* __restrict__ tail,
+           double* __restrict__ len64, double* __restrict__ gtail64) {
   __shared__ uint64_t s_src[BS];
   __shared__ uint32_t s_off[BS];
   __shared__ uint16_t s_id[GE * BS];
@@ -491,6 +490,13 @@ k_ent_fill(BrickGeo G, const Chunk* __restrict__ chunks, const uint32_t* __restr
     const uint32_t bits = __float_as_uint(lu);
     const uint32_t pay = bits >= ENT_BASE ? bits - ENT_BASE : 0u;
     ent[ch.base + e] = code << 29 | pay;
+    if (len64) {
+      // the FP64 stream: the exact length, and per 16-entry group the exact
+      // tail (the reference's tail array) of its entry 15, or of the chunk's
+      // last entry when entry 15 is padding (zero length, the same tail)
+      len64[ch.base + e] = len[src];
+      if ((e % GE) == GE - 1 || e + 1 == ch.ne) gtail64[(ch.base + e) / GE] = tail[src];
+    }
   }
   if (clamped) atomicAdd(n_clamped, (unsigned long long)clamped);
 }
@@ -1118,6 +1124,551 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
   }
 }
 
+// ---------------- FP64 stream iteration --------------------------------
+// The same brick stream with the reference's exact FP64 lengths (len64) and
+// tails: each group stores the tail of its entry 15 (gtail64) and each
+// segment the tail of its last entry (seg_tail64), both taken from the
+// tail array of src/MutoMLSD.cpp:403-414, and the walk runs that array's
+// own backward recurrence T_{k-1} = T_k + L_k inside a segment.  So every
+// W_ij (src/MutoMLSD.cpp:417-430) is bit-identical to the reference's; the
+// sums over a track differ from its running order only in association.
+struct Args64 {
+  const double* len64;
+  const double* gtail64;
+  const double* seg_tail64;
+  const uint32_t* slot;  // median / trimmed mean: slot of each entry
+  double* cval;          // ... and the per-entry c_ij, voxel-major
+};
+
+constexpr size_t fwd64_static_lds(uint32_t bs) {
+  return 2 * (size_t)bs * 32 + (size_t)bs * 4 + (size_t)bs * 8 + 64 + 16;
+}
+constexpr size_t bwd64_static_lds(uint32_t bs) { return (size_t)bs * 32 + (size_t)bs * 4 + 32 + 16; }
+
+// FP64 lambda of the brick into LDS (zero outside the grid)
+__device__ __forceinline__ void load_brick64(const BrickGeo& G, uint32_t brick,
+                                             const double* __restrict__ lam, double* lds) {
+  uint32_t o[3];
+  brick_origin(G, brick, o);
+  const uint32_t vol = G.b[0] * G.b[1] * G.b[2];
+  for (uint32_t v = threadIdx.x; v < vol; v += blockDim.x) {
+    const uint32_t r = G.db0.div(v), lx = v - r * G.b[0];
+    const uint32_t lz = G.db1.div(r), ly = r - lz * G.b[1];
+    const uint32_t x = o[0] + lx, y = o[1] + ly, z = o[2] + lz;
+    double val = 0.0;
+    if (x < G.n[0] && y < G.n[1] && z < G.n[2])
+      val = lam[(size_t)x + (size_t)G.n[0] * ((size_t)y + (size_t)G.n[1] * z)];
+    lds[v] = val;
+  }
+}
+
+// Forward, FP64: per segment, sum_j lambda_j W_ij over its entries (the
+// three components), in the track's own tail basis, so no basis shift is
+// needed.  Entry-parallel as k_fwd32: thread t walks the chunk's group t
+// backwards in two halves of 8 entries (lengths and lambdas loaded per
+// half), pieces split at segment starts and put together per segment by
+// the segment's own thread.
+template <uint32_t BS>
+__global__ void __launch_bounds__(BS)
+k_fwd64s(IterArgs a, Args64 b, const double* __restrict__ lam, double4* __restrict__ part) {
+  extern __shared__ __align__(16) double lds_dyn[];
+  double* lds_lam = lds_dyn;
+  __shared__ double4 spc[BS];  // per segment: first piece (x, y, z), start entry (w)
+  __shared__ double4 hpc[BS];  // per group: head piece
+  __shared__ uint32_t sv0[BS];
+  __shared__ double stl[BS];   // per segment: tail of its last entry
+  __shared__ int stab[16];
+  __shared__ uint32_t sh_[4];
+  uint32_t& sh = sh_[0];
+  const uint32_t t = threadIdx.x;
+  fill_step_table(a, stab);
+  for (;;) {
+    if (t == 0) sh = atomicAdd(&a.work[0], 1u);
+    __syncthreads();
+    const uint32_t ti = sh;
+    if (ti >= a.n_tasks) break;
+    const Task tk = a.tasks[ti];
+    load_brick64(a.G, tk.brick, lam, lds_lam);
+    __syncthreads();
+    for (uint32_t c = tk.c0; c < tk.c1; ++c) {
+      const Chunk ch = a.chunks[c];
+      if (t < ch.cnt) {
+        sv0[t] = a.seg_v0[ch.q0 + t];
+        stl[t] = b.seg_tail64[ch.q0 + t];
+      }
+      const bool act = t * GE < ch.ne;
+      const uint64_t gi = (ch.base / GE) + t;
+      Group gr;
+      if (act) gr.load(a, gi);
+      __syncthreads();
+      if (act) {
+        uint32_t vx[GE];
+        gr.voxels(stab, [&](uint32_t q) { return sv0[q]; }, [&](int i, uint32_t v) { vx[i] = v; });
+        uint32_t e0 = t * GE, mk = gr.mask, sk = gr.s0;
+        asm volatile("" : "+v"(e0), "+v"(mk), "+v"(sk));
+        double T = b.gtail64[gi];
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+        const double2* pl2 = reinterpret_cast<const double2*>(b.len64 + gi * GE);
+#pragma unroll
+        for (int h = GE / 2; h >= 0; h -= GE / 2) {
+          double L8[GE / 2], l8[GE / 2];
+#pragma unroll
+          for (int k = 0; k < GE / 4; ++k) {
+            const double2 v2 = pl2[h / 2 + k];
+            L8[2 * k] = v2.x;
+            L8[2 * k + 1] = v2.y;
+          }
+#pragma unroll
+          for (int k = 0; k < GE / 2; ++k) l8[k] = lds_lam[vx[h + k]];
+#pragma unroll
+          for (int k = GE / 2 - 1; k >= 0; --k) {
+            const int i = h + k;
+            const double L = L8[k], lj = l8[k];
+            // generateWeightMatrix (MutoMLSD.cpp:417-430), the reference's own expressions
+            const double LT = L * T;
+            const double w01 = LT + L * L * 0.5;
+            const double w11 = L * L * L / 3.0 + (L + T) * LT;
+            d0 = d0 + lj * L;
+            d1 = d1 + lj * w01;
+            d2 = d2 + lj * w11;
+            if (i > 0 && ((mk >> i) & 1u)) {  // entry i starts segment seg(i)
+              const uint32_t q = sk + (uint32_t)__popc(mk & ((2u << i) - 2u));
+              spc[q] = make_double4(d0, d1, d2, (double)(e0 + i));
+              d0 = d1 = d2 = 0.0;
+              T = stl[q - 1];  // entry i - 1 is the previous segment's last
+            } else {
+              T = T + L;
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        const double4 head = make_double4(d0, d1, d2, (double)e0);
+        hpc[t] = head;
+        if (gr.mask & 1u) spc[gr.s0] = head;
+      }
+      __syncthreads();
+      if (t < ch.cnt) {
+        const double4 p = spc[t];
+        const uint32_t A = (uint32_t)p.w;
+        const uint32_t B = t + 1 < ch.cnt ? (uint32_t)spc[t + 1].w : ch.ne;
+        double x = p.x, y = p.y, z = p.z;
+        const uint32_t g_last = min((B - 1) / GE, (ch.ne - 1) / GE);
+        for (uint32_t g = A / GE + 1; g <= g_last; ++g) {
+          const double4 hp = hpc[g];
+          x = x + hp.x;
+          y = y + hp.y;
+          z = z + hp.z;
+        }
+        part[ch.q0 + t] = make_double4(x, y, z, 0.0);
+      }
+    }
+  }
+}
+
+// Per track (rank order), FP64: Sigma from the segment partials, Eigen's
+// 2x2 det/inverse (threshold 1e-12), cost_i (src/MutoMLSD.cpp:442-468) and
+// p^2 G with G = 1/2 Sigma^-1 D Sigma^-1 - Sigma^-1, so that c_ij =
+// <W_ij, p^2 G> and S_ij = 2 lambda_j + c_ij lambda_j^2 (the identity
+// 1/2 xy - tr = <W_ij, G> of the reference's :471-483 expressions).  With
+// grad_mode, G = 0 where cost_i = 0 (the MAP objective's convention);
+// otherwise G follows the computed inverse as the reference's S_ij does.
+__global__ void __launch_bounds__(256)
+k_track64s(size_t n_trk, const uint32_t* __restrict__ tseg_ptr, const uint32_t* __restrict__ tseg_pos,
+           const double4* __restrict__ part, const double* __restrict__ trk_d, ErrM E, int grad_mode,
+           double4* __restrict__ trk_g, double* __restrict__ cost_part,
+           unsigned long long* __restrict__ cnt) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double cost = 0.0;
+  bool sing = false;
+  if (r < n_trk) {
+    double S00 = 0.0, S01 = 0.0, S11 = 0.0;
+    const uint32_t k0 = tseg_ptr[r], k1 = tseg_ptr[r + 1];
+    uint32_t k = k0;
+    for (; k + 4 <= k1; k += 4) {
+      const double4 pa = part[tseg_pos[k]], pb = part[tseg_pos[k + 1]];
+      const double4 pc = part[tseg_pos[k + 2]], pd = part[tseg_pos[k + 3]];
+      S00 = S00 + pa.x; S01 = S01 + pa.y; S11 = S11 + pa.z;
+      S00 = S00 + pb.x; S01 = S01 + pb.y; S11 = S11 + pb.z;
+      S00 = S00 + pc.x; S01 = S01 + pc.y; S11 = S11 + pc.z;
+      S00 = S00 + pd.x; S01 = S01 + pd.y; S11 = S11 + pd.z;
+    }
+    for (; k < k1; ++k) {
+      const double4 pa = part[tseg_pos[k]];
+      S00 = S00 + pa.x; S01 = S01 + pa.y; S11 = S11 + pa.z;
+    }
+    const double d00 = trk_d[4 * r], d01 = trk_d[4 * r + 1];
+    const double d11 = trk_d[4 * r + 2], p = trk_d[4 * r + 3];
+    const double pp = p * p;
+    Mat2 cov;
+    cov.m00 = pp * S00 + E.e00;
+    cov.m01 = pp * S01 + E.e01;
+    cov.m10 = pp * S01 + E.e01;
+    cov.m11 = pp * S11 + E.e11;
+    Mat2 inv;
+    double det;
+    const bool ok = inv_det(cov, inv, det);
+    if (!ok || det <= 0) {
+      sing = true;
+    } else {
+      cost = log(det) + (d00 * inv.m00 + d11 * inv.m11 + d01 * 2.0 * inv.m01) * 0.5;
+    }
+    const double A00 = inv.m00 * d00 + inv.m01 * d01, A01 = inv.m00 * d01 + inv.m01 * d11;
+    const double A10 = inv.m10 * d00 + inv.m11 * d01, A11 = inv.m10 * d01 + inv.m11 * d11;
+    double4 g;
+    g.x = pp * (0.5 * (A00 * inv.m00 + A01 * inv.m10) - inv.m00);
+    g.y = pp * (0.5 * (A00 * inv.m01 + A01 * inv.m11) - inv.m01);
+    g.z = pp * (0.5 * (A10 * inv.m01 + A11 * inv.m11) - inv.m11);
+    g.w = 0.0;
+    if (grad_mode && sing) g.x = g.y = g.z = 0.0;
+    trk_g[r] = g;
+  }
+  block_sum_to(cost_part, cost);
+  wave_count(&cnt[C_SINGULAR], sing);
+}
+
+// Backprojection, FP64: c_ij = <W_ij, p^2 G_i> per entry, from the same
+// exact lengths and tails as the forward walk.  MODE 0 (mean): added per
+// voxel in LDS (FP64 atomics), flushed once per task (stored when the task
+// covers its brick, else added); MODE 1 (median, trimmed mean): stored in
+// the entry's voxel-major slot.
+template <int MODE, uint32_t BS>
+__global__ void __launch_bounds__(BS)
+k_bwd64s(IterArgs a, Args64 b, const double4* __restrict__ trk_g, double* __restrict__ acc) {
+  extern __shared__ __align__(16) double lds_dyn[];
+  double* acc_s = lds_dyn;
+  __shared__ double4 sg[BS];  // per segment: p^2 G of its track (x, y, z), tail of its last entry (w)
+  __shared__ uint32_t sv0[BS];
+  __shared__ int stab[8];
+  __shared__ uint32_t sh_[4];
+  uint32_t& sh = sh_[0];
+  const uint32_t t = threadIdx.x;
+  fill_step_table(a, stab);
+  const uint32_t vol = a.G.b[0] * a.G.b[1] * a.G.b[2];
+  for (;;) {
+    if (t == 0) sh = atomicAdd(&a.work[1], 1u);
+    if (MODE == 0)
+      for (uint32_t v = t; v < vol; v += BS) acc_s[v] = 0.0;
+    __syncthreads();
+    const uint32_t ti = sh;
+    if (ti >= a.n_tasks) break;
+    const Task tk = a.tasks[ti];
+    for (uint32_t c = tk.c0; c < tk.c1; ++c) {
+      const Chunk ch = a.chunks[c];
+      if (t < ch.cnt) {
+        const uint64_t q = ch.q0 + t;
+        const double4 g = trk_g[a.seg_trk[q]];
+        sg[t] = make_double4(g.x, g.y, g.z, b.seg_tail64[q]);
+        sv0[t] = a.seg_v0[q];
+      }
+      const bool act = t * GE < ch.ne;
+      const uint64_t gi = (ch.base / GE) + t;
+      Group gr;
+      if (act) gr.load(a, gi);
+      __syncthreads();
+      if (act) {
+        uint32_t vx[GE];
+        gr.voxels(stab, [&](uint32_t q) { return sv0[q]; }, [&](int i, uint32_t v) { vx[i] = v; });
+        uint32_t mk = gr.mask, sk = gr.s0;
+        asm volatile("" : "+v"(mk), "+v"(sk));
+        double T = b.gtail64[gi];
+        double4 g = sg[sk + (uint32_t)__popc(mk & 0xFFFEu)];
+        const double2* pl2 = reinterpret_cast<const double2*>(b.len64 + gi * GE);
+        uint32_t sl[GE];
+        if (MODE == 1) {
+          const uint4* ps = reinterpret_cast<const uint4*>(b.slot + gi * GE);
+#pragma unroll
+          for (int k = 0; k < GE / 4; ++k) {
+            const uint4 v4 = ps[k];
+            sl[4 * k] = v4.x; sl[4 * k + 1] = v4.y; sl[4 * k + 2] = v4.z; sl[4 * k + 3] = v4.w;
+          }
+        }
+#pragma unroll
+        for (int h = GE / 2; h >= 0; h -= GE / 2) {
+          double L8[GE / 2];
+#pragma unroll
+          for (int k = 0; k < GE / 4; ++k) {
+            const double2 v2 = pl2[h / 2 + k];
+            L8[2 * k] = v2.x;
+            L8[2 * k + 1] = v2.y;
+          }
+#pragma unroll
+          for (int k = GE / 2 - 1; k >= 0; --k) {
+            const int i = h + k;
+            const double L = L8[k];
+            const double LT = L * T;
+            const double w01 = LT + L * L * 0.5;
+            const double w11 = L * L * L / 3.0 + (L + T) * LT;
+            const double cv = L * g.x + 2.0 * w01 * g.y + w11 * g.z;
+            if (MODE == 0) {
+              atomicAdd(&acc_s[vx[i]], cv);
+            } else if (sl[i] != 0xFFFFFFFFu) {
+              b.cval[sl[i]] = cv;
+            }
+            if (i > 0 && ((mk >> i) & 1u)) {  // entry i - 1 ends the segment before
+              const uint32_t q = sk + (uint32_t)__popc(mk & ((2u << i) - 2u)) - 1u;
+              g = sg[q];
+              T = g.w;
+            } else {
+              T = T + L;
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      __syncthreads();  // the tables are rewritten for the next chunk
+    }
+    if (MODE == 0) {
+      uint32_t o[3];
+      brick_origin(a.G, tk.brick, o);
+      const bool shared = tk.ntask > 1;
+      for (uint32_t v = t; v < vol; v += BS) {
+        const uint32_t rr = a.G.db0.div(v), lx = v - rr * a.G.b[0];
+        const uint32_t lz = a.G.db1.div(rr), ly = rr - lz * a.G.b[1];
+        const uint32_t x = o[0] + lx, y = o[1] + ly, z = o[2] + lz;
+        if (x >= a.G.n[0] || y >= a.G.n[1] || z >= a.G.n[2]) continue;
+        const size_t j = (size_t)x + (size_t)a.G.n[0] * ((size_t)y + (size_t)a.G.n[1] * z);
+        const double val = acc_s[v];
+        if (!shared)
+          acc[j] = val;
+        else if (val != 0.0)
+          atomicAdd(&acc[j], val);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Slot of every stream entry in the voxel-major c_ij array (one block per
+// task, the passes' voxel chain): slot = col[j] + a running count per
+// voxel.  The order inside a voxel does not matter to the median or to
+// the sorted trimmed mean.
+template <uint32_t BS>
+__global__ void __launch_bounds__(BS)
+k_slot64s(IterArgs a, const uint64_t* __restrict__ col, uint32_t* __restrict__ fill,
+          uint32_t* __restrict__ slot) {
+  __shared__ uint32_t sv0[BS];
+  __shared__ int stab[16];
+  const uint32_t t = threadIdx.x;
+  fill_step_table(a, stab);
+  const Task tk = a.tasks[blockIdx.x];
+  uint32_t o[3];
+  brick_origin(a.G, tk.brick, o);
+  for (uint32_t c = tk.c0; c < tk.c1; ++c) {
+    const Chunk ch = a.chunks[c];
+    if (t < ch.cnt) sv0[t] = a.seg_v0[ch.q0 + t];
+    const bool act = t * GE < ch.ne;
+    Group gr;
+    if (act) gr.load(a, (ch.base / GE) + t);
+    __syncthreads();
+    if (act) {
+      const uint32_t e0 = t * GE;
+      gr.voxels(stab, [&](uint32_t q) { return sv0[q]; }, [&](int i, uint32_t v) {
+        if (e0 + (uint32_t)i >= ch.ne) return;
+        const uint32_t rr = a.G.db0.div(v), lx = v - rr * a.G.b[0];
+        const uint32_t lz = a.G.db1.div(rr), ly = rr - lz * a.G.b[1];
+        const size_t j = (size_t)(o[0] + lx) +
+                         (size_t)a.G.n[0] * ((size_t)(o[1] + ly) + (size_t)a.G.n[1] * (o[2] + lz));
+        slot[ch.base + e0 + i] = (uint32_t)(col[j] + atomicAdd(&fill[j], 1u));
+      });
+    }
+    __syncthreads();
+  }
+}
+
+// lambda_j + lambda_j^2 / (2 M_j) sum_i c_ij: the mean of S_ij over 2
+// (src/MutoMLSD.cpp:497-500); MAP: lambda_ML to ml
+__global__ void __launch_bounds__(256)
+k_upd64s(size_t nv, const uint32_t* __restrict__ mj, double* __restrict__ acc,
+         double* __restrict__ lam, double* __restrict__ ml) {
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nv) return;
+  const uint32_t m = mj[j];
+  if (m == 0) return;  // made 0 before the run's first pass, as in k_upd32
+  const double l = lam[j];
+  const double out = l + l * l / (2.0 * (double)m) * acc[j];
+  acc[j] = 0.0;
+  if (ml) ml[j] = out; else lam[j] = out;
+}
+
+// the order-statistic update from the sorted c values of one voxel (the
+// reference sorts S_ij = 2 lambda + c_ij lambda^2, the same order for
+// lambda != 0 and all equal for lambda = 0): median (src/MutoMLSD.cpp:
+// 505-515) or the mean of the lowest floor(pct/100 M), at least one
+// (:501-504); `get(k)` returns the k-th smallest c
+template <class Get>
+__device__ __forceinline__ double order_stat_c(double l, uint32_t m, int use_mean, int pct, Get get) {
+  const double l2 = l * l;
+  if (use_mean) {
+    uint32_t jm = x86_u32(pct / 100.0 * (double)m);
+    if (jm < 1) jm = 1;
+    double s = 0.0;
+    for (uint32_t k = 0; k < jm; ++k) s = s + (2.0 * l + get(k) * l2);
+    return s / (double)jm * 0.5;
+  }
+  const double med = 2.0 * l + get(m / 2) * l2;
+  if (m % 2 == 0) return (med + (2.0 * l + get(m / 2 - 1) * l2)) * 0.25;
+  return med * 0.5;
+}
+
+// Bitonic sort of the 64 R values of one wavefront, element e = r 64 + lane
+// in v[r]; partners one register apart (distances >= 64) are exchanged in
+// the lane, the others through a lane shuffle.
+template <int R>
+__device__ __forceinline__ void wave_bitonic(double (&v)[R]) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (uint32_t k = 2; k <= 64u * R; k <<= 1) {
+#pragma unroll
+    for (uint32_t jd = k >> 1; jd > 0; jd >>= 1) {
+      if (jd >= 64) {
+        const uint32_t rj = jd / 64;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int p = r ^ (int)rj;
+          if (p > r) {
+            const bool up = ((r * 64u + lane) & k) == 0;
+            const double lo = fmin(v[r], v[p]), hi = fmax(v[r], v[p]);
+            v[r] = up ? lo : hi;
+            v[p] = up ? hi : lo;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double o = __shfl_xor(v[r], (int)jd, 64);
+          const bool up = ((r * 64u + lane) & k) == 0;
+          const bool lower = (lane & jd) == 0;
+          v[r] = (lower == up) ? fmin(v[r], o) : fmax(v[r], o);
+        }
+      }
+    }
+  }
+}
+
+template <int R>
+__device__ __forceinline__ double wave_elem(const double (&v)[R], uint32_t e) {
+  double x = v[0];
+#pragma unroll
+  for (int r = 1; r < R; ++r)
+    if (e / 64u == (uint32_t)r) x = v[r];
+  return __shfl(x, (int)(e % 64u), 64);
+}
+
+template <int R>
+__device__ __forceinline__ double wave_select(const double* __restrict__ src, uint32_t m, double l,
+                                              int use_mean, int pct) {
+  const uint32_t lane = threadIdx.x & 63u;
+  double v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = r * 64u + lane;
+    v[r] = e < m ? src[e] : INFINITY;
+  }
+  wave_bitonic<R>(v);
+  if (!use_mean) return order_stat_c(l, m, 0, pct, [&](uint32_t k) { return wave_elem<R>(v, k); });
+  // trimmed mean: S of the lowest jm values, summed over the wavefront
+  uint32_t jm = x86_u32(pct / 100.0 * (double)m);
+  if (jm < 1) jm = 1;
+  double s = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (r * 64u + lane < jm) s = s + (2.0 * l + v[r] * (l * l));
+  s = wave_sum(s);
+  return s / (double)jm * 0.5;
+}
+
+// Per voxel, one wavefront (M_j <= 256; longer lists are listed at set-up
+// and go to k_sel64b): the M_j values of c sorted in registers, the
+// update written in place (each voxel reads only its own lambda).  M_j = 0
+// gives 0 (src/MutoMLSD.cpp:494-496).  MAP: lambda_ML to ml.
+__global__ void __launch_bounds__(256)
+k_sel64w(size_t nv, const uint32_t* __restrict__ mj, const uint64_t* __restrict__ col,
+         const double* __restrict__ cval, int use_mean, int pct, double* __restrict__ lam,
+         double* __restrict__ ml) {
+  const size_t nw = (size_t)gridDim.x * (blockDim.x >> 6);
+  for (size_t j = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; j < nv; j += nw) {
+    const uint32_t m = mj[j];
+    double out;
+    if (m == 0) {
+      out = 0.0;
+    } else if (m > 256) {
+      continue;
+    } else {
+      const double l = lam[j];
+      const double* src = cval + col[j];
+      if (m <= 64)
+        out = wave_select<1>(src, m, l, use_mean, pct);
+      else if (m <= 128)
+        out = wave_select<2>(src, m, l, use_mean, pct);
+      else
+        out = wave_select<4>(src, m, l, use_mean, pct);
+    }
+    if ((threadIdx.x & 63u) == 0) {
+      if (ml) ml[j] = out; else lam[j] = out;
+    }
+  }
+}
+
+// Listed voxels with longer lists, one block each: bitonic sort in LDS
+// (M_j <= 4096) or in the voxel's own range of a global scratch.
+__device__ void block_bitonic64(double* a, uint32_t n) {
+  for (uint32_t k = 2; k <= n; k <<= 1) {
+    for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t l = i ^ jj;
+        if (l > i) {
+          const double x = a[i], y = a[l];
+          const bool up = (i & k) == 0;
+          if (up ? (y < x) : (x < y)) { a[i] = y; a[l] = x; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ void __launch_bounds__(1024)
+k_sel64b(uint32_t n_list, const uint32_t* __restrict__ list, const uint64_t* __restrict__ off,
+         const uint32_t* __restrict__ mj, const uint64_t* __restrict__ col,
+         const double* __restrict__ cval, int use_mean, int pct, double* __restrict__ scratch,
+         double* __restrict__ lam, double* __restrict__ ml) {
+  __shared__ double a_s[4096];
+  for (uint32_t b = blockIdx.x; b < n_list; b += gridDim.x) {
+    const uint32_t j = list[b];
+    const uint32_t m = mj[j];
+    uint32_t n = 1;
+    while (n < m) n <<= 1;
+    double* a = n <= 4096 ? a_s : scratch + off[b];
+    const double* src = cval + col[j];
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) a[i] = i < m ? src[i] : INFINITY;
+    __syncthreads();
+    block_bitonic64(a, n);
+    if (threadIdx.x == 0) {
+      const double out = order_stat_c(lam[j], m, use_mean, pct, [&](uint32_t k) { return a[k]; });
+      if (ml) ml[j] = out; else lam[j] = out;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_zero_uncrossed64(size_t nv, const uint32_t* __restrict__ mj, double* __restrict__ lam) {
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nv && mj[j] == 0 && lam[j] != 0.0) lam[j] = 0.0;
+}
+
+// clears the FP64 stream iteration's cost, counters and work counters
+__global__ void k_iter_begin64(double* __restrict__ cost_acc, unsigned long long* __restrict__ cnt,
+                               unsigned int* __restrict__ work) {
+  if (threadIdx.x == 0) cost_acc[0] = 0.0;
+  if (threadIdx.x < 3) cnt[threadIdx.x] = 0ull;
+  if (threadIdx.x < 2) work[threadIdx.x] = 0u;
+}
+
+__global__ void k_copy64(size_t n, const double* __restrict__ a, double* __restrict__ b) {
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) b[j] = a[j];
+}
+
 // Voxels no track crosses become 0 (src/MutoMLSD.cpp:494-496); done once
 // before a run's first mean-update pass, since no later pass touches them.
 __global__ void __launch_bounds__(256)
@@ -1324,6 +1875,8 @@ int fp32c_release(muto_em_plan* pl) {
   Fp32cData& f = pl->f32;
   if (f.graph) (void)hipGraphExecDestroy(f.graph);
   f.graph = nullptr;
+  if (f.graph64) (void)hipGraphExecDestroy(f.graph64);
+  f.graph64 = nullptr;
   return MUTO_OK;
 }
 
@@ -1584,12 +2137,14 @@ int fp32c_build(muto_em_plan* pl) {
     }
     MUTO_CHECK_HIP(ctx, f.seg_trk.alloc(n_seg));
     MUTO_CHECK_HIP(ctx, f.seg_tfx.alloc(n_seg));
+    if (f.with64) MUTO_CHECK_HIP(ctx, f.seg_tail64.alloc(n_seg));
     DBuf<uint64_t> seg_off, seg_off_p;
     MUTO_CHECK_HIP(ctx, seg_off.alloc(n_seg + 1));
     MUTO_CHECK_HIP(ctx, seg_off_p.alloc(n_seg + 1));
     if (n_seg) {
       k_seg_gather<<<grid_for(n_seg, 256), 256, 0, s>>>(n_seg, sval.p, st_tend.p, st_trk.p, rank.p,
-                                                        f.tail_scale, f.seg_trk.p, f.seg_tfx.p);
+                                                        f.tail_scale, f.seg_trk.p, f.seg_tfx.p,
+                                                        f.with64 ? f.seg_tail64.p : nullptr);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(lenq.p + n_seg, 0, sizeof(uint32_t), s));
@@ -1609,6 +2164,12 @@ int fp32c_build(muto_em_plan* pl) {
     MUTO_CHECK_HIP(ctx, f.gcarry.alloc(n_grp + 1));
     MUTO_CHECK_HIP(ctx, f.gv0.alloc(n_grp + 1));
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.ent.p, 0, (n_ent_pad + GE) * sizeof(uint32_t), s));
+    if (f.with64) {  // padding entries: zero length
+      MUTO_CHECK_HIP(ctx, f.len64.alloc(n_ent_pad + GE));
+      MUTO_CHECK_HIP(ctx, f.gtail64.alloc(n_grp + 1));
+      MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.len64.p, 0, (n_ent_pad + GE) * sizeof(double), s));
+      MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.gtail64.p, 0, (n_grp + 1) * sizeof(double), s));
+    }
     {
       // lengths as FP32 in units of U, the power of two above twice the
       // voxel diagonal (an entry of a scattered track may hold both legs in
@@ -1624,7 +2185,9 @@ int fp32c_build(muto_em_plan* pl) {
           constexpr uint32_t BS = decltype(B)::value;
           k_ent_fill<BS><<<f.n_chunks, BS, 0, s>>>(G, f.chunks.p, sval.p, st_start.p, seg_off_p.p,
                                                    pl->vox.p, pl->len.p, inv_unit, f.ent.p,
-                                                   f.seg_v0.p, ncl.p);
+                                                   f.seg_v0.p, ncl.p, pl->tail.p,
+                                                   f.with64 ? f.len64.p : nullptr,
+                                                   f.with64 ? f.gtail64.p : nullptr);
           return hipGetLastError();
         }));
       }
@@ -1672,6 +2235,10 @@ int fp32c_build(muto_em_plan* pl) {
   }
   MUTO_CHECK_HIP(ctx, f.seg_part.alloc(4 * n_seg));
   MUTO_CHECK_HIP(ctx, f.trk_g.alloc(4 * nt));
+  if (f.with64) {
+    MUTO_CHECK_HIP(ctx, f.seg_part64.alloc(4 * n_seg));
+    MUTO_CHECK_HIP(ctx, f.trk_g64.alloc(4 * nt));
+  }
   MUTO_CHECK_HIP(ctx, f.cmax.alloc(1));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.cmax.p, 0, sizeof(unsigned long long), s));
   {
@@ -1834,6 +2401,288 @@ int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done) {
   }
   if (!each && iters > 0) {
     MUTO_CHECK_HIP(ctx, hipMemcpyAsync(sums, f.hist.p, iters * ns * sizeof(double), hipMemcpyDeviceToHost, s));
+    MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    *done = iters;
+  }
+  if (pl->timing && *done)
+    for (int k = 0; k < 5; ++k) pl->t_ms[k] = tsum[k] / *done;
+  return MUTO_OK;
+}
+
+// ---------------------------------------------------------------------
+// FP64 stream (MUTO_OPT_FP64_KERNEL 2): host side
+// ---------------------------------------------------------------------
+static Args64 args64(muto_em_plan* pl) {
+  Fp32cData& f = pl->f32;
+  Args64 b;
+  b.len64 = f.len64.p;
+  b.gtail64 = f.gtail64.p;
+  b.seg_tail64 = f.seg_tail64.p;
+  b.slot = f.slot64.p;
+  b.cval = f.cval.p;
+  return b;
+}
+
+// 256- or 512-thread blocks, FP64 brick and tables within one workgroup's LDS
+static bool fp64s_fits(muto_ctx* ctx, const BrickShape& b) {
+  if (b.block != 256 && b.block != 512) return false;
+  int max_lds = 0;
+  if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess)
+    return false;
+  const size_t fwd = fwd64_static_lds(b.block) + (size_t)b.vol() * sizeof(double);
+  const size_t bwd = bwd64_static_lds(b.block) + (size_t)b.vol() * sizeof(double);
+  return std::max(fwd, bwd) <= (size_t)max_lds;
+}
+
+template <class F>
+static hipError_t by_block64(uint32_t block, F&& f) {
+  if (block == 512) return f(std::integral_constant<uint32_t, 512>{});
+  return f(std::integral_constant<uint32_t, 256>{});
+}
+
+// the brick stream with its FP64 side arrays (rebuilt with them when the
+// stream was built for FP32C alone)
+int fp64s_build(muto_em_plan* pl) {
+  Fp32cData& f = pl->f32;
+  muto_ctx* ctx = pl->ctx;
+  if (f.built && f.with64 && f.grid_fwd64 > 0) return MUTO_OK;
+  if (!fp64s_fits(ctx, f.bs)) {
+    ctx->set_error("FP64 stream: the brick shape needs 256- or 512-thread blocks and its FP64 brick "
+                   "array and tables within one workgroup's LDS");
+    return MUTO_ERR_LIMIT;
+  }
+  if (f.built && !f.with64) f.reset();
+  f.with64 = true;
+  if (int rc = fp32c_build(pl)) return rc;
+  const size_t lds = (size_t)f.bs.vol() * sizeof(double);
+  int of = 1, ob = 1;
+  MUTO_CHECK_HIP(ctx, by_block64(f.bs.block, [&](auto B) {
+    constexpr uint32_t BS = decltype(B)::value;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&of, k_fwd64s<BS>, BS, lds);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob, k_bwd64s<1, BS>, BS, lds);
+  }));
+  f.grid_fwd64 = std::max(1, of) * ctx->n_cu;
+  f.grid_bwd64 = std::max(1, ob) * ctx->n_cu;
+  MUTO_CHECK_HIP(ctx, f.cost_part64.alloc(grid_for(pl->n_trk, 256)));
+  return MUTO_OK;
+}
+
+// voxel-major slots of the per-entry c_ij (median, trimmed mean), and the
+// voxels whose lists are too long for one wavefront
+static int fp64s_slots(muto_em_plan* pl) {
+  Fp32cData& f = pl->f32;
+  if (f.slots64) return MUTO_OK;
+  muto_ctx* ctx = pl->ctx;
+  hipStream_t s = ctx->stream;
+  if (f.n_ent >= 0xFFFFFFFFull) {
+    ctx->set_error("FP64 stream order statistics: more than 2^32 - 1 entries");
+    return MUTO_ERR_LIMIT;
+  }
+  const size_t nv = pl->nv;
+  std::vector<uint32_t> h(nv);
+  MUTO_CHECK_HIP(ctx, copy_sync(ctx, h.data(), pl->mj.p, nv * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  std::vector<uint64_t> col(nv + 1);
+  col[0] = 0;
+  for (size_t j = 0; j < nv; ++j) col[j + 1] = col[j] + h[j];
+  MUTO_CHECK_HIP(ctx, f.col64.alloc(nv + 1));
+  MUTO_CHECK_HIP(ctx, copy_sync(ctx, f.col64.p, col.data(), (nv + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+  MUTO_CHECK_HIP(ctx, f.slot64.alloc(f.n_ent_pad + GE));
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.slot64.p, 0xFF, (f.n_ent_pad + GE) * sizeof(uint32_t), s));
+  MUTO_CHECK_HIP(ctx, f.cval.alloc(std::max<uint64_t>(col[nv], 1)));
+  {
+    DBuf<uint32_t> fill;
+    MUTO_CHECK_HIP(ctx, fill.alloc(nv));
+    MUTO_CHECK_HIP(ctx, hipMemsetAsync(fill.p, 0, nv * sizeof(uint32_t), s));
+    if (f.n_tasks) {
+      const IterArgs a = iter_args(pl);
+      MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
+        constexpr uint32_t BS = decltype(B)::value;
+        k_slot64s<BS><<<f.n_tasks, BS, 0, s>>>(a, f.col64.p, fill.p, f.slot64.p);
+        return hipGetLastError();
+      }));
+    }
+    MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+  }
+  std::vector<uint32_t> ids;
+  std::vector<uint64_t> off;
+  uint64_t tot = 0;
+  for (size_t j = 0; j < nv; ++j) {
+    if (h[j] <= 256) continue;
+    uint64_t n = 1;
+    while (n < h[j]) n <<= 1;
+    ids.push_back((uint32_t)j);
+    off.push_back(n > 4096 ? tot : 0);
+    if (n > 4096) tot += n;
+  }
+  f.n_big64 = (uint32_t)ids.size();
+  MUTO_CHECK_HIP(ctx, f.big64.alloc(ids.size()));
+  MUTO_CHECK_HIP(ctx, f.big64_off.alloc(off.size()));
+  MUTO_CHECK_HIP(ctx, f.big64_scr.alloc(tot));
+  if (!ids.empty()) {
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, f.big64.p, ids.data(), ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, f.big64_off.p, off.data(), off.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  }
+  f.slots64 = true;
+  return MUTO_OK;
+}
+
+// mean of all S (update_with_mean, mean_percentile 100) or an order statistic
+static bool mean_all(const muto_em_plan* pl) {
+  return pl->cfg.update_with_mean && pl->cfg.mean_percentile == 100;
+}
+
+// One FP64 stream iteration on the plan's stream, no host synchronisation
+// and no memset (it is captured as a graph).  ev: kernel boundaries for
+// per-phase timing (5 events), or null.
+static int fp64s_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
+  Fp32cData& f = pl->f32;
+  muto_ctx* ctx = pl->ctx;
+  hipStream_t s = ctx->stream;
+  const IterArgs a = iter_args(pl);
+  const Args64 b = args64(pl);
+  const ErrM E = error_matrix(pl->cfg);
+  const bool map = pl->is_map, mean = mean_all(pl);
+  const size_t lds = (size_t)f.bs.vol() * sizeof(double);
+  k_iter_begin64<<<1, 64, 0, s>>>(pl->cost_acc.p, pl->dcount.p, pl->work.p);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[0], s));
+  if (f.n_tasks) {
+    MUTO_CHECK_HIP(ctx, by_block64(f.bs.block, [&](auto B) {
+      constexpr uint32_t BS = decltype(B)::value;
+      k_fwd64s<BS><<<f.grid_fwd64, BS, lds, s>>>(a, b, pl->lam64.p,
+                                                 reinterpret_cast<double4*>(f.seg_part64.p));
+      return hipGetLastError();
+    }));
+  }
+  if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[1], s));
+  const unsigned nblk = grid_for(pl->n_trk, 256);
+  k_track64s<<<nblk, 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p,
+                                  reinterpret_cast<const double4*>(f.seg_part64.p), f.trk_d.p, E, 0,
+                                  reinterpret_cast<double4*>(f.trk_g64.p), f.cost_part64.p, pl->dcount.p);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  k_sum_parts_f64<<<1, 256, 0, s>>>(f.cost_part64.p, nblk, pl->cost_acc.p);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[2], s));
+  if (f.n_tasks) {
+    MUTO_CHECK_HIP(ctx, by_block64(f.bs.block, [&](auto B) {
+      constexpr uint32_t BS = decltype(B)::value;
+      if (mean)
+        k_bwd64s<0, BS><<<f.grid_bwd64, BS, lds, s>>>(a, b, reinterpret_cast<const double4*>(f.trk_g64.p),
+                                                      pl->acc64.p);
+      else
+        k_bwd64s<1, BS><<<f.grid_bwd64, BS, lds, s>>>(a, b, reinterpret_cast<const double4*>(f.trk_g64.p),
+                                                      nullptr);
+      return hipGetLastError();
+    }));
+  }
+  if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[3], s));
+  double* ml = map ? pl->ml64.p : nullptr;
+  if (mean) {
+    k_upd64s<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->acc64.p, pl->lam64.p, ml);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  } else {
+    const muto_mlsd_cfg& c = pl->cfg;
+    const unsigned g = (unsigned)std::min<size_t>(grid_for(pl->nv, 4), (size_t)ctx->n_cu * 64);
+    k_sel64w<<<g, 256, 0, s>>>(pl->nv, pl->mj.p, f.col64.p, f.cval.p, c.update_with_mean,
+                               c.mean_percentile, pl->lam64.p, ml);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+    if (f.n_big64) {
+      k_sel64b<<<std::min<uint32_t>(f.n_big64, 4096), 1024, 0, s>>>(
+          f.n_big64, f.big64.p, f.big64_off.p, pl->mj.p, f.col64.p, f.cval.p, c.update_with_mean,
+          c.mean_percentile, f.big64_scr.p, pl->lam64.p, ml);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    }
+  }
+  if (map) {
+    if (int rc = map_step(pl, pl->lam64.p, pl->ml64.p, pl->tmp64.p)) return rc;
+    k_copy64<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->tmp64.p, pl->lam64.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  }
+  k_cost_push<<<1, 256, 0, s>>>(1, pl->cost_acc.p, f.hist.p, f.hist_n.p);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[4], s));
+  return MUTO_OK;
+}
+
+static int fp64s_graph(muto_em_plan* pl) {
+  Fp32cData& f = pl->f32;
+  muto_ctx* ctx = pl->ctx;
+  const int mode = (pl->is_map ? 2 : 0) | (mean_all(pl) ? 1 : 0);
+  if (f.graph64 && f.graph64_mode == mode) return MUTO_OK;
+  if (f.graph64) (void)hipGraphExecDestroy(f.graph64);
+  f.graph64 = nullptr;
+  hipGraph_t g = nullptr;
+  MUTO_CHECK_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = fp64s_enqueue(pl, nullptr);
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (rc != MUTO_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  MUTO_CHECK_HIP(ctx, e);
+  const hipError_t ei = hipGraphInstantiate(&f.graph64, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  MUTO_CHECK_HIP(ctx, ei);
+  f.graph64_mode = mode;
+  return MUTO_OK;
+}
+
+// `iters` FP64 stream iterations (any update rule, MLSD or MAP): cost sums
+// into sums[], *done the iterations run; early stop and per-phase timing as
+// in fp32c_run.
+int fp64s_run(muto_em_plan* pl, int iters, double* sums, int* done) {
+  Fp32cData& f = pl->f32;
+  muto_ctx* ctx = pl->ctx;
+  hipStream_t s = ctx->stream;
+  *done = 0;
+  if (int rc = fp64s_build(pl)) return rc;
+  if (!mean_all(pl))
+    if (int rc = fp64s_slots(pl)) return rc;
+  if (pl->is_map && !pl->tmp64.p) MUTO_CHECK_HIP(ctx, pl->tmp64.alloc(pl->nv));
+  if (f.hist_cap < (size_t)iters) {
+    if (f.graph) (void)hipGraphExecDestroy(f.graph);
+    f.graph = nullptr;
+    if (f.graph64) (void)hipGraphExecDestroy(f.graph64);
+    f.graph64 = nullptr;
+    f.hist.release();
+    MUTO_CHECK_HIP(ctx, f.hist.alloc((size_t)iters));
+    f.hist_cap = (size_t)iters;
+  }
+  if (!f.hist_n.p) MUTO_CHECK_HIP(ctx, f.hist_n.alloc(1));
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.hist_n.p, 0, sizeof(unsigned int), s));
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->acc64.p, 0, pl->nv * sizeof(double), s));
+  if (!pl->is_map && iters > 0 && mean_all(pl)) {
+    k_zero_uncrossed64<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->lam64.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  }
+  const bool graph = pl->use_graph && !pl->timing;
+  if (graph)
+    if (int rc = fp64s_graph(pl)) return rc;
+  const bool each = pl->cfg.early_stop || pl->timing;
+  double tsum[5] = {0, 0, 0, 0, 0};
+  double prev = INFINITY;
+  for (int it = 0; it < iters; ++it) {
+    if (graph)
+      MUTO_CHECK_HIP(ctx, hipGraphLaunch(f.graph64, s));
+    else if (int rc = fp64s_enqueue(pl, pl->timing ? pl->ev : nullptr))
+      return rc;
+    if (!each) continue;
+    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(&sums[it], f.hist.p + it, sizeof(double), hipMemcpyDeviceToHost, s));
+    MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    *done = it + 1;
+    if (pl->timing) {
+      float t[4];
+      for (int k = 0; k < 4; ++k) MUTO_CHECK_HIP(ctx, hipEventElapsedTime(&t[k], pl->ev[k], pl->ev[k + 1]));
+      for (int k = 0; k < 4; ++k) tsum[k] += t[k];
+      tsum[4] += t[0] + t[1] + t[2] + t[3];
+    }
+    const double c = sums[it] / (double)pl->n_trk;
+    if (pl->cfg.early_stop && prev < c) break;
+    prev = c;
+  }
+  if (!each && iters > 0) {
+    MUTO_CHECK_HIP(ctx, hipMemcpyAsync(sums, f.hist.p, iters * sizeof(double), hipMemcpyDeviceToHost, s));
     MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
     *done = iters;
   }

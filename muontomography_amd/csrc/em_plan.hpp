@@ -81,6 +81,27 @@ struct Fp32cData {
   bool acc_mode_set = false;  // set through the option call (else environment)
   hipGraphExec_t graph = nullptr;  // one iteration, captured
   bool graph_map = false;          // the MAP flag the graph was captured with
+  // FP64 stream (the brick stream with FP64 lengths and tails), built with
+  // the stream when with64 is set
+  bool with64 = false;
+  DBuf<double> len64;       // [n_ent_pad] exact FP64 length of each entry (0: padding)
+  DBuf<double> gtail64;     // [groups] exact tail of the group's entry 15
+  DBuf<double> seg_tail64;  // [n_seg] exact tail of the segment's last entry
+  DBuf<double> seg_part64;  // [4*n_seg] forward partial sums (FP64, global tail basis)
+  DBuf<double> trk_g64;     // [4*n_trk] p^2 G (g00, g01, g11, -) per rank
+  // its order statistics: per-entry c_ij in voxel-major slots
+  bool slots64 = false;
+  DBuf<uint32_t> slot64;    // [n_ent_pad] slot of each entry (~0u: padding)
+  DBuf<uint64_t> col64;     // [nv+1] first slot of each voxel
+  DBuf<double> cval;        // [nnz] c_ij in slots
+  uint32_t n_big64 = 0;     // voxels above the one-block sort
+  DBuf<uint32_t> big64;     // [n_big64] their indices
+  DBuf<uint64_t> big64_off; // [n_big64] their offsets in big64_scr
+  DBuf<double> big64_scr;
+  DBuf<double> cost_part64; // per-block cost sums of the FP64 stream's track kernel
+  int grid_fwd64 = 0, grid_bwd64 = 0;  // persistent grids of its passes
+  hipGraphExec_t graph64 = nullptr;  // one FP64 stream iteration, captured
+  int graph64_mode = -1;             // the update rule / MAP flag it was captured with
   DBuf<double> hist;               // [hist_cap] cost sum per iteration of a run
   DBuf<unsigned int> hist_n;       // [1] entries in hist
   size_t hist_cap = 0;
@@ -89,6 +110,23 @@ struct Fp32cData {
   void reset() {
     if (graph) (void)hipGraphExecDestroy(graph);
     graph = nullptr;
+    if (graph64) (void)hipGraphExecDestroy(graph64);
+    graph64 = nullptr;
+    graph64_mode = -1;
+    len64.release();
+    gtail64.release();
+    seg_tail64.release();
+    seg_part64.release();
+    trk_g64.release();
+    slots64 = false;
+    slot64.release();
+    col64.release();
+    cval.release();
+    n_big64 = 0;
+    big64.release();
+    big64_off.release();
+    big64_scr.release();
+    cost_part64.release();
     built = false;
     n_seg = n_ent = n_ent_pad = 0;
     n_tasks = n_chunks = 0;
@@ -160,7 +198,8 @@ struct muto_em_plan {
   muto::DBuf<unsigned long long> dcount;  // device counters [8]
   muto::DBuf<unsigned int> work;   // work-queue counters [4]
   int cur_prec = -1;               // which image is current
-  int fp64_kernel = 0;             // FP64 iteration: 0 lane per track, 1 wavefront per track
+  int fp64_kernel = 0;             // FP64 iteration: 0 lane per track, 1 wavefront per track,
+                                   // 2 the FP64 brick stream
   int use_graph = 1;               // FP32C: replay the captured iteration
   int timing = 0;                  // per-kernel events (no graph)
 

@@ -498,6 +498,95 @@ def test_em_fp64_wavefront_kernel(ctx, small_set, mode):
     assert np.array_equal(out == 0, ref == 0)
 
 
+@pytest.mark.parametrize("mode", ["median", "trimmed", "mean"])
+@pytest.mark.parametrize("brick", [(16, 16, 16, 256), (8, 16, 32, 512)])
+def test_em_fp64_stream(ctx, small_set, mode, brick):
+    """The FP64 brick stream (MUTO_OPT_FP64_KERNEL 2): the reference's exact
+    lengths and tails in the brick-major stream, so every W_ij is the
+    reference's and only the association of the covariance and mean sums
+    differs.  Against the oracle on its prepared inputs, 4 iterations:
+    images within 1e-10 relative (zeros where the oracle's are), costs within
+    1e-12; median and trimmed mean sort each voxel's c_ij (one wavefront, or
+    one block for longer lists)."""
+    g, rays, _ = small_set
+    kw = {"median": dict(update_with_mean=0), "trimmed": dict(update_with_mean=1, mean_percentile=60),
+          "mean": dict(update_with_mean=1, mean_percentile=100)}[mode]
+    cfg = _em_cfg(g, **kw)
+    em = oracle.Em(cfg, rays)
+    plan = ctx.em_plan(cfg, prepared=em.prepared())
+    plan.set_option(M.MUTO_OPT_BRICK, M.brick_code(brick))
+    plan.set_option(M.MUTO_OPT_FP64_KERNEL, 2)
+    img = np.full(g.n_vox, 1e-7)
+    plan.set_image(img)
+    ref = img
+    costs = []
+    for _ in range(4):
+        ref, cost, _ns = em.iterate(ref)
+        costs.append(cost)
+    cg = plan.run(4)
+    out = plan.image()
+    assert plan.counters()["n_segments"] > 0
+    np.testing.assert_allclose(cg, costs, rtol=1e-12)
+    np.testing.assert_allclose(out, ref, rtol=1e-10, atol=0)
+    assert np.array_equal(out == 0, ref == 0)
+    # graph replays and eager launches agree (the order statistics bit for bit)
+    plan.set_image(img)
+    plan.set_option(M.MUTO_OPT_GRAPH, 0)
+    plan.run(4)
+    eager = plan.image()
+    if mode == "mean":  # FP64 LDS atomics: order-dependent at rounding level
+        np.testing.assert_allclose(eager, out, rtol=1e-13, atol=0)
+    else:
+        assert np.array_equal(eager.view(np.uint64), out.view(np.uint64))
+    plan.close()
+
+
+@pytest.mark.parametrize("mode", ["median", "trimmed"])
+def test_em_fp64_stream_long_voxel_lists(ctx, mode):
+    """FP64 stream order statistics where voxels hold 2000-6800 entries (a
+    2^3 grid over the 1 m cube, 6e4 muons): the one-block sorts in LDS and in
+    the global scratch, within 1e-10 of the oracle after 3 iterations."""
+    g0, rays, _ = W.muons("small", n_muons=60_000)
+    ext = g0.dx * g0.nx
+    g = make_grid(2, ext / 2, g0.x_min)
+    kw = dict(update_with_mean=0) if mode == "median" else dict(update_with_mean=1, mean_percentile=98)
+    cfg = _em_cfg(g, **kw)
+    em = oracle.Em(cfg, rays)
+    assert (em.mj() > 4096).sum() >= 2
+    plan = ctx.em_plan(cfg, prepared=em.prepared())
+    plan.set_option(M.MUTO_OPT_BRICK, M.brick_code((2, 2, 2, 256)))
+    plan.set_option(M.MUTO_OPT_FP64_KERNEL, 2)
+    img = np.full(g.n_vox, 1e-7)
+    plan.set_image(img)
+    ref = img
+    for _ in range(3):
+        ref, _c, _ns = em.iterate(ref)
+    plan.run(3)
+    np.testing.assert_allclose(plan.image(), ref, rtol=1e-10, atol=0)
+    plan.close()
+
+
+@pytest.mark.parametrize("nb,p", [(0, 2.0), (1, 1.2)])
+def test_map_fp64_stream(ctx, small_set, nb, p):
+    """MAP iterations with the FP64 stream's ML step (mean update) against
+    the reference-order FP64 plan: images within 1e-9, costs within 1e-12."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, beta=1.0, p=p, delta=0.1,
+                  all_neighbours=nb)
+    img = np.full(g.n_vox, 1e-7)
+    ref = ctx.em_plan(cfg, rays=rays)
+    ref.set_image(img)
+    c_ref = ref.run(6, M.MUTO_FP64, map_=True)
+    plan = ctx.em_plan(cfg, rays=rays)
+    plan.set_option(M.MUTO_OPT_FP64_KERNEL, 2)
+    plan.set_image(img)
+    c = plan.run(6, M.MUTO_FP64, map_=True)
+    np.testing.assert_allclose(c, c_ref, rtol=1e-12)
+    np.testing.assert_allclose(plan.image(), ref.image(), rtol=1e-9, atol=0)
+    ref.close()
+    plan.close()
+
+
 def test_mlsd_one_shot(ctx, small_set):
     g, rays, _ = small_set
     cfg = _em_cfg(g, update_with_mean=0, num_iteration=3)
