@@ -58,6 +58,9 @@ extern "C" {
                          the sums are FP64 registers and int64 fixed-point
                          LDS atomics by default, FP32 with two-sum
                          compensation under MUTO_OPT_ACC_MODE 0          */
+#define MUTO_FP64_STREAM 2  /* FP64 on the brick stream (MUTO_OPT_FP64_KERNEL 2):
+                               the reference's W_ij bit for bit, the sums in
+                               another association; every update rule     */
 
 /* Voxel grid: same field order, types and 64-byte layout as VoxelGrid
  * (include/MutoTypes.h:25-37).  Linear voxel index j = x + y*nx + z*nx*ny

@@ -3,8 +3,10 @@
  * handled as in src/MutoMLSD.cpp:226-266 (file with grid adoption, random,
  * constant), the EM iterations on the GPU through muto_mlsd_reconstruct.
  * Extra keys: "device" (default 0) and "precision" ("fp64", the reference's
- * arithmetic, or "fp32c": FP32 entry terms with FP64/fixed-point sums, mean
- * update only).  Library errors throw std::runtime_error. */
+ * arithmetic in its order; "fp64_stream", FP64 on the brick-major stream,
+ * the same W_ij and sums in another association; or "fp32c": FP32 entry
+ * terms with FP64/fixed-point sums, mean update only).  Library errors throw
+ * std::runtime_error. */
 #pragma once
 
 #include <chrono>

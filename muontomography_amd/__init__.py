@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 from . import _abi
-from ._abi import (MUTO_FP32C, MUTO_FP64, RAY_DTYPE, Counters, GenBlock, GenCfg, Grid,
+from ._abi import (MUTO_FP32C, MUTO_FP64, MUTO_FP64_STREAM, RAY_DTYPE, Counters, GenBlock, GenCfg, Grid,
                    make_grid, mlsd_cfg, poca_cfg, ptr)
 
 HERE = os.path.dirname(os.path.abspath(__file__))

@@ -6,6 +6,7 @@ import numpy as np
 MUTO_OK = 0
 MUTO_FP64 = 0
 MUTO_FP32C = 1
+MUTO_FP64_STREAM = 2
 
 
 class Grid(C.Structure):

@@ -691,7 +691,10 @@ static int validate_cfg(muto_ctx* ctx, const muto_mlsd_cfg* cfg) {
     return MUTO_ERR_LIMIT;
   }
   if (!(g.dx > 0 && g.dy > 0 && g.dz > 0)) { ctx->set_error("voxel sizes must be positive"); return MUTO_ERR_ARG; }
-  if (cfg->precision != MUTO_FP64 && cfg->precision != MUTO_FP32C) { ctx->set_error("precision must be MUTO_FP64 or MUTO_FP32C"); return MUTO_ERR_ARG; }
+  if (cfg->precision != MUTO_FP64 && cfg->precision != MUTO_FP32C && cfg->precision != MUTO_FP64_STREAM) {
+    ctx->set_error("precision must be MUTO_FP64, MUTO_FP32C or MUTO_FP64_STREAM");
+    return MUTO_ERR_ARG;
+  }
   return MUTO_OK;
 }
 
@@ -1078,8 +1081,12 @@ extern "C" int muto_map_plan_run(muto_em_plan* pl, int iters, int precision,
 static int plan_run(muto_em_plan* pl, int iters, int precision, double* cost) {
   muto_ctx* ctx = pl->ctx;
   if (iters < 0) { ctx->set_error("muto_em_plan_run: iters < 0"); return MUTO_ERR_ARG; }
+  // MUTO_FP64_STREAM: FP64 on the brick stream for this run, whatever
+  // MUTO_OPT_FP64_KERNEL says
+  const bool stream64 = precision == MUTO_FP64_STREAM || (precision == MUTO_FP64 && pl->fp64_kernel == 2);
+  if (precision == MUTO_FP64_STREAM) precision = MUTO_FP64;
   if (precision != MUTO_FP64 && precision != MUTO_FP32C) {
-    ctx->set_error("muto_em_plan_run: precision must be MUTO_FP64 or MUTO_FP32C");
+    ctx->set_error("muto_em_plan_run: precision must be MUTO_FP64, MUTO_FP32C or MUTO_FP64_STREAM");
     return MUTO_ERR_ARG;
   }
   const muto_mlsd_cfg& c = pl->cfg;
@@ -1111,7 +1118,7 @@ static int plan_run(muto_em_plan* pl, int iters, int precision, double* cost) {
   if (int rc = sync_image(pl, precision)) return rc;
   if (precision == MUTO_FP32C)
     if (int rc = fp32c_build(pl)) return rc;  // counts mj from the brick stream
-  if (precision == MUTO_FP64 && pl->fp64_kernel == 2)
+  if (stream64)
     if (int rc = fp64s_build(pl)) return rc;
   if (int rc = ensure_mj(pl)) return rc;
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->dcount.p, 0, 8 * sizeof(unsigned long long), s));
@@ -1120,7 +1127,7 @@ static int plan_run(muto_em_plan* pl, int iters, int precision, double* cost) {
   int done = 0;
   double tsum[5] = {0, 0, 0, 0, 0};
   const bool time_it = true;
-  if (precision == MUTO_FP32C || pl->fp64_kernel == 2) {
+  if (precision == MUTO_FP32C || stream64) {
     std::vector<double> sums(iters, 0.0);
     if (int rc = precision == MUTO_FP32C ? fp32c_run(pl, iters, sums.data(), &done)
                                          : fp64s_run(pl, iters, sums.data(), &done))

@@ -241,10 +241,13 @@ MutoMLSD::MutoMLSD(json config) : MutoMLSD() {
   const std::string prec = config.value("precision", "fp64");
   if (prec == "fp64")
     fPrecision = MUTO_FP64;
+  else if (prec == "fp64_stream")
+    fPrecision = MUTO_FP64_STREAM;
   else if (prec == "fp32c")
     fPrecision = MUTO_FP32C;
   else
-    throw std::invalid_argument("MutoMLSD: \"precision\" must be \"fp64\" or \"fp32c\"");
+    throw std::invalid_argument(
+        "MutoMLSD: \"precision\" must be \"fp64\", \"fp64_stream\" or \"fp32c\"");
 }
 
 MutoMLSD::~MutoMLSD() {

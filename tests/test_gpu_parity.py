@@ -599,13 +599,16 @@ def test_mlsd_one_shot(ctx, small_set):
     np.testing.assert_allclose(cg, co, rtol=1e-9)
 
 
+@pytest.mark.parametrize("prec", [M.MUTO_FP64, M.MUTO_FP64_STREAM])
 @pytest.mark.parametrize("mean", [0, 1])
-def test_mlsd_reference_config_poca_init_20_iterations(ctx, mean):
+def test_mlsd_reference_config_poca_init_20_iterations(ctx, mean, prec):
     """The CPU reference configuration in full: 40^3 voxels of 25 mm, 1e5
     muons, the PoCA image (length-weighted, two legs) handed over in FP64 as
     the initial image, then 20 MLSD iterations, median (the reference's
-    default) and mean.  Inputs differ from the oracle's only by libm ulps in
-    the angles; after 20 iterations the images agree within 1e-8."""
+    default) and mean, in the reference's order and on the FP64 brick stream
+    (MUTO_FP64_STREAM, through the one-shot call).  Inputs differ from the
+    oracle's only by libm ulps in the angles (and the stream's sums by
+    association); after 20 iterations the images agree within 1e-8."""
     g, rays, _ = W.muons("small")
     assert len(rays) == 100_000
     pc = poca_cfg(g, length_weighted=1, straight_line=0)
@@ -614,6 +617,7 @@ def test_mlsd_reference_config_poca_init_20_iterations(ctx, mean):
     np.testing.assert_allclose(pg, po, rtol=1e-9, atol=0)
     cfg = _em_cfg(g, update_with_mean=mean, mean_percentile=100, num_iteration=20, initialize_method=0)
     io, co, no = oracle.mlsd_reconstruct(cfg, rays, init=po)
+    cfg.precision = prec
     ig, cg, ng = ctx.mlsd_reconstruct(cfg, rays, init=pg)
     assert ng["iterations_done"] == no["iterations_done"] == 20
     assert ng["n_scat_in_grid"] == no["n_scat_in_grid"] and ng["nnz"] == no["nnz"]
@@ -1157,6 +1161,16 @@ def test_cli_poca_then_mlsd_matches_library(ctx, small_set, tmp_path):
     cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, num_iteration=5, initialize_method=0)
     ref, _, _ = ctx.mlsd_reconstruct(cfg, rc, init=pimg)
     np.testing.assert_allclose(mimg, ref.astype(np.float32), rtol=1e-5, atol=0)
+    # the median (the reference's default rule) on the FP64 brick stream,
+    # through the drop-in's "precision" key, against the reference-order run
+    mk2 = dict(mk, update_with_mean=False, precision="fp64_stream")
+    (tmp_path / "mlsd2.json").write_text(json.dumps(dict(common, output_file=str(mlsd_out), mlsd=mk2)))
+    r = subprocess.run([cli, str(tmp_path / "mlsd2.json")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    mimg2, _ = read_image(mlsd_out)
+    cfg2 = _em_cfg(g, update_with_mean=0, num_iteration=5, initialize_method=0)
+    ref2, _, _ = ctx.mlsd_reconstruct(cfg2, rc, init=pimg)
+    np.testing.assert_allclose(mimg2, ref2.astype(np.float32), rtol=1e-5, atol=0)
     # the other Siddon consumers the reference's driver dispatches
     for method, extra, fn in (("pocut", dict(cut_percentile=0.8),
                                lambda c: ctx.pocut_reconstruct(c, rc, 0.8)),
