@@ -217,6 +217,33 @@ def bench_fp64(args, plan, rays, cfg_kw, init, nnz, wall32, cost32):
 
 
 
+def bench_fp64_stream_only(args, ctx, plan, rays, cfg_kw, init, nnz):
+    """The FP64 brick stream alone at one brick shape: --steps mean-update
+    iterations, then --steps median iterations on a second plan."""
+    import muontomography_amd as M
+    brick = tuple(int(v) for v in (args.brick or "16,16,16,256").split(","))
+    out = {"workload": "FP64 stream, 256^3 grid of 10 mm voxels, 1e7 muons", "brick": list(brick),
+           "entries": int(nnz)}
+    kw = dict(cfg_kw)
+    plans = [("mean", plan)]
+    kw["update_with_mean"] = 0
+    plans.append(("median", ctx.em_plan(M.mlsd_cfg(plan.cfg.grid, **kw), rays=rays)))
+    for name, p in plans:
+        p.set_option(M.MUTO_OPT_BRICK, M.brick_code(brick))
+        p.set_option(M.MUTO_OPT_FP64_KERNEL, 2)
+        p.set_image(init)
+        p.run(1, M.MUTO_FP64)
+        p.set_image(init)
+        t0 = time.perf_counter()
+        p.run(args.steps, M.MUTO_FP64)
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        out[name + "_ms_per_step"] = ms
+        log(f"FP64 stream {name}: {ms:.2f} ms/it")
+        p.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
 def bench_scans(args):
     """64 independent scans (grids 32^3-128^3, 10-30 mm voxels, 1e4-1e6
     tracks each) run as one batch: FP32C mean update, one launch per pass
@@ -388,7 +415,8 @@ def bench_map_grad(args):
     nnz = plan.counters()["nnz"]
     res = {}
     for name, pl, prec, rc in (("fp32c", plan, M.MUTO_FP32C, False), ("fp64", plan, M.MUTO_FP64, False),
-                               ("fp64_recompute", lean, M.MUTO_FP64, True)):
+                               ("fp64_recompute", lean, M.MUTO_FP64, True),
+                               ("fp32c_recompute", lean, M.MUTO_FP32C, True)):
         sv = pl.map_forward(lam, prec, recompute=rc)
         gr = sv.backward(1.0)
         sv.close()
@@ -411,7 +439,9 @@ def bench_map_grad(args):
            "entries_per_s": {k: 2 * nnz / v[0] for k, v in res.items()},
            "grad_err_fp32c_vs_fp64_norm_inf": float(np.abs(res["fp32c"][1] - g64).max() / np.abs(g64).max()),
            "grad_err_recompute_vs_fp64_norm_inf": float(np.abs(res["fp64_recompute"][1] - g64).max() /
-                                                         np.abs(g64).max())}
+                                                         np.abs(g64).max()),
+           "grad_err_fp32c_recompute_vs_fp64_norm_inf": float(
+               np.abs(res["fp32c_recompute"][1] - g64).max() / np.abs(g64).max())}
     plan.close()
     lean.close()
     ctx.close()
@@ -436,6 +466,9 @@ def main():
     ap.add_argument("--brick", default=None, metavar="BX,BY,BZ[,BLOCK]",
                     help="em256: use this brick shape (and block size), no tuning (profiling runs)")
     ap.add_argument("--n-scans", type=int, default=64, help="scans64: number of scans")
+    ap.add_argument("--fp64-stream-only", action="store_true",
+                    help="em256: only the FP64 brick stream, mean then median, --steps iterations "
+                         "each at --brick (default 16,16,16,256) (profiling runs)")
     ap.add_argument("--cpu-sample", type=int, default=300_000)
     ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -484,6 +517,9 @@ def main():
     nnz = c["nnz"]
     log(f"plan: {c['n_scat_in_grid']} selected tracks, {nnz} entries "
         f"({nnz / max(c['n_scat_in_grid'], 1):.0f}/track) in {time.perf_counter() - t0:.1f} s")
+
+    if args.fp64_stream_only:
+        return bench_fp64_stream_only(args, ctx, plan, rays, cfg_kw, init, nnz)
 
     # FP32C: brick (LDS tile) shape chosen by the autotuner, warm-up (the
     # stream build and graph capture), then K steps

@@ -308,8 +308,10 @@ int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg, size_t n,
  * called several times.  precision: MUTO_FP64 (track-major, FP64) or
  * MUTO_FP32C (brick stream, FP32 entry terms).  flags:
  * MUTO_MAP_RECOMPUTE_SIDDON walks the paths again instead of reading the
- * stored lists (FP64; implied for plans without lists).  A saved context
- * must be destroyed before its plan. */
+ * stored lists or the brick stream (implied for plans without lists): the
+ * walk and the forward sums are FP64; with MUTO_FP32C the backward adds FP32
+ * entry terms with FP32 atomics.  A saved context must be destroyed before
+ * its plan. */
 typedef struct muto_map_saved muto_map_saved;
 #define MUTO_MAP_RECOMPUTE_SIDDON 1
 /* flag of muto_map_forward: lambda, and grad_lambda of every

@@ -4,9 +4,10 @@ Thin Python handle over libmuto.so (HIP kernels + C ABI, include/muto.h).
 The library is the product; this module only moves numpy arrays across the
 C ABI.  It raises if the library is missing -- there is no fallback.
 """
+import atexit
 import ctypes as C
-import weakref
 import os
+import weakref
 
 import numpy as np
 
@@ -99,6 +100,21 @@ class MutoError(RuntimeError):
     pass
 
 
+_live_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_contexts():
+    """Contexts still open at exit are closed (their batches and plans
+    first) while the HIP runtime is still up, not in whatever order the
+    interpreter's teardown would free them."""
+    for ctx in list(_live_contexts):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One HIP device + stream (muto_ctx)."""
 
@@ -109,6 +125,7 @@ class Context:
         rc = lib().muto_ctx_create(device, C.byref(self._h))
         if rc != 0:
             raise MutoError(f"muto_ctx_create({device}) failed with status {rc}")
+        _live_contexts.add(self)
 
     def close(self):
         """Destroys the context; plans still open on it are closed first

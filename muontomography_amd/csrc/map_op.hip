@@ -53,6 +53,7 @@ struct muto_map_saved {
   DBuf<double> sum;                // [2] cost sum, prior sum
   DBuf<double> part;               // per-block partial sums
   bool dev = false;                // MUTO_MAP_DEVICE_BUFFERS: caller buffers in device memory
+  DBuf<float> acc32;               // FP32C with the paths walked again: [nv] float sums
 };
 
 namespace {
@@ -204,6 +205,39 @@ k_map_bwd64(MapTrackArgs a, const double* __restrict__ gsave, double scale,
                   grad);
 }
 
+// FP32C backward with the paths walked again: the FP64 walk (the voxel
+// indices and lengths of the stored lists), each entry's term
+// scale <W_ij, G_i> rounded to FP32 and added with FP32 atomics.
+__global__ void __launch_bounds__(256)
+k_map_bwd32rc(MapTrackArgs a, const double* __restrict__ gsave, double scale,
+              float* __restrict__ acc) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_trk) return;
+  const double G00 = gsave[4 * i], G01 = gsave[4 * i + 1], G11 = gsave[4 * i + 2];
+  if (G00 == 0.0 && G01 == 0.0 && G11 == 0.0) return;
+  const double P = gsave[4 * i + 3];
+  const float g00 = (float)(scale * G00), g01 = (float)(2.0 * scale * G01), g11 = (float)(scale * G11);
+  walk_path(a.g, a.geo + 9 * i, a.straight, [&](uint32_t j, double L, double s, bool) {
+    if (j >= a.nv) return;
+    // W from the FP64 length and tail, its products with G in FP32
+    const double T = P - s - L;
+    const double LT = L * T;
+    const float w00 = (float)L, w01 = (float)(LT + L * L * 0.5);
+    const float w11 = (float)(L * L * L / 3.0 + (L + T) * LT);
+    atomicAdd(&acc[j], fmaf(w11, g11, fmaf(w01, g01, w00 * g00)));
+  });
+}
+
+__global__ void k_zero_f32(size_t n, float* __restrict__ x) {
+  size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) x[j] = 0.0f;
+}
+
+__global__ void k_widen_f32(size_t n, const float* __restrict__ a, double* __restrict__ b) {
+  size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) b[j] = (double)a[j];
+}
+
 // Forward and backward of one track in one pass, nothing stored per track
 // (muto_map_value_grad_noalloc): the same cost_i, G_i and scatter as the
 // two kernels above.
@@ -309,11 +343,6 @@ extern "C" int muto_map_forward(muto_em_plan* pl, const double* lambda, int prec
   }
   const bool recompute = (flags & MUTO_MAP_RECOMPUTE_SIDDON) || !pl->has_lists;
   const bool dev = (flags & MUTO_MAP_DEVICE_BUFFERS) != 0;
-  if (recompute && precision != MUTO_FP64) {
-    ctx->set_error("muto_map_forward: recomputed paths run in MUTO_FP64 (the FP32C brick stream is "
-                   "built from stored lists)");
-    return MUTO_ERR_ARG;
-  }
   if (recompute && !pl->geo.p && pl->n_trk) {
     ctx->set_error("muto_map_forward: the plan has no track geometry to walk (prepared plan)");
     return MUTO_ERR_STATE;
@@ -342,7 +371,10 @@ extern "C" int muto_map_forward(muto_em_plan* pl, const double* lambda, int prec
   }
   double cost_sum = 0.0;
   if (nt) {
-    if (precision == MUTO_FP64) {
+    // FP64, and FP32C with the paths walked again: the FP64 walk (index
+    // exactness) with FP64 moment sums, G_i saved in FP64; FP32C's
+    // recomputing backward then adds FP32 entry terms
+    if (precision == MUTO_FP64 || recompute) {
       if (sv->g64.alloc(4 * nt) != hipSuccess) { ctx->set_error("muto_map_forward: out of device memory"); return fail(MUTO_ERR_NOMEM); }
       const MapTrackArgs a = track_args(pl);
       if (recompute)
@@ -394,6 +426,20 @@ extern "C" int muto_map_backward(muto_map_saved* sv, double grad_phi, double* gr
         k_map_bwd64<1><<<grid_for(nt, 256), 256, 0, s>>>(a, sv->g64.p, -0.5 * grad_phi, grad);
       else
         k_map_bwd64<0><<<grid_for(nt, 256), 256, 0, s>>>(a, sv->g64.p, -0.5 * grad_phi, grad);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    } else if (sv->recompute) {
+      // FP32C, paths walked again: FP32 entry terms and FP32 atomics into a
+      // float accumulator, then widened into the gradient
+      if (!sv->acc32.p && sv->acc32.alloc(nv) != hipSuccess) {
+        ctx->set_error("muto_map_backward: out of device memory");
+        return MUTO_ERR_NOMEM;
+      }
+      k_zero_f32<<<grid_for(nv, 256), 256, 0, s>>>(nv, sv->acc32.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+      k_map_bwd32rc<<<grid_for(nt, 256), 256, 0, s>>>(track_args(pl), sv->g64.p, -0.5 * grad_phi,
+                                                      sv->acc32.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+      k_widen_f32<<<grid_for(nv, 256), 256, 0, s>>>(nv, sv->acc32.p, grad);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     } else {
       if (int rc = fp32c_map_backward(pl, sv->g32.p, sv->cmax.p, grad)) return rc;

@@ -4,11 +4,24 @@ the same HIP runtime that libmuto.so links (/opt/rocm's libamdhip64), so the
 pointers are valid for the library's context.  (The torch wheel carries a HIP
 runtime of its own; a torch tensor's pointer is only usable when torch and
 libmuto share one runtime.)"""
+import atexit
 import ctypes as C
+import weakref
 
 import numpy as np
 
 _hip = None
+_live = weakref.WeakSet()
+
+
+@atexit.register
+def _free_all():
+    """Arrays still alive at exit are freed while the HIP runtime is up."""
+    for a in list(_live):
+        try:
+            a.free()
+        except Exception:
+            pass
 
 
 def hip():
@@ -44,6 +57,7 @@ class DeviceArray:
         self.dtype = np.dtype(dtype)
         self._p = C.c_void_p()
         _ok(hip().hipMalloc(C.byref(self._p), max(self.n, 1) * self.dtype.itemsize), "hipMalloc")
+        _live.add(self)
 
     @classmethod
     def from_numpy(cls, a):
