@@ -357,7 +357,7 @@ def bench_siddon(args):
     t["rays_from_records"] = ctx.kernel_ms()
     # list output (12 B written per entry as u32 index + f64 length, plus
     # the y, z indices of the CSR API) on a sample that fits host memory
-    ns = 200_000
+    ns = 2_000_000
     row, *_ = ctx.siddon_csr(g, rays["pin"][:ns], P[:ns])
     t_fill = ctx.kernel_ms()
     fill_entries = int(row[-1])

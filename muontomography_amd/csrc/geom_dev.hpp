@@ -30,92 +30,121 @@ __device__ __forceinline__ double smin(double a, double b) {
   return (b < a) ? b : a;
 }
 
-// Siddon voxel walk of segment p1 -> p2 (src/MutoSiddon.cpp:9-141).
-// emit(x, y, z, length) is called for every recorded voxel in order.
-// Returns the number of recorded voxels.  The walk is capped at
-// nx+ny+nz+8 steps, which a finite ray never reaches (the reference has no
-// cap).
+// Siddon voxel walk of segment p1 -> p2 (src/MutoSiddon.cpp:9-141) as a
+// resumable cursor: begin() sets the walk up (returns false for a segment
+// that misses the grid), run(m, emit) takes steps until m voxels have been
+// recorded or the walk ends, calling emit(x, y, z, length) for every
+// recorded voxel in order.  The walk is capped at nx+ny+nz+8 steps, which a
+// finite ray never reaches (the reference has no cap).
+struct SiddonCursor {
+  double length, amax, alpha, alpha_last;
+  double ax[3], da[3];
+  int di[3];
+  uint32_t x, y, z, it, cap;
+  bool live;
+
+  __device__ __forceinline__ bool begin(const GridD& g, const double* p1, const double* p2) {
+    const double eps = 1.0e-6;
+    live = false;
+    double d21[3], a0[3], an[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      d21[a] = p2[a] - p1[a];
+      a0[a] = -0.1;
+      an[a] = 1.1;
+    }
+    length = sqrt((d21[0] * d21[0] + d21[1] * d21[1]) + d21[2] * d21[2]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double hi = g.mn[a] + g.d[a] * (double)g.n[a];
+      if (eps < fabs(d21[a])) {
+        a0[a] = (g.mn[a] - p1[a]) / d21[a];
+        an[a] = (hi - p1[a]) / d21[a];
+      } else if (p1[a] < g.mn[a] || hi <= p1[a]) {
+        return false;
+      }
+    }
+    double lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const bool sw = an[a] < a0[a];
+      lo[a] = sw ? an[a] : a0[a];
+      hi[a] = sw ? a0[a] : an[a];
+    }
+    const double amin = smax(smax(smax(lo[0], lo[1]), lo[2]), 0.0);
+    amax = smin(smin(smin(hi[0], hi[1]), hi[2]), 1.0);
+    if (amax <= amin) return false;
+    alpha = amin;
+    uint32_t idx[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      idx[a] = x86_u32((alpha * d21[a] + p1[a] - g.mn[a]) / g.d[a] +
+                       (d21[a] <= 0 ? -eps : eps));
+      di[a] = 0;
+      da[a] = 0.0;
+      if (eps < fabs(d21[a])) {
+        da[a] = g.d[a] / (fabs(d21[a]));
+        di[a] = d21[a] < 0 ? -1 : 1;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      ax[a] = 1.1;
+      if (fabs(amin - lo[a]) < eps) {
+        ax[a] = amin + da[a];
+      } else if (di[a] != 0) {
+        ax[a] = ((double)idx[a] * g.d[a] + g.mn[a] - p1[a]) / d21[a] +
+                (di[a] == 1 ? da[a] : 0.0);
+      }
+    }
+    alpha_last = alpha;
+    it = 0;
+    cap = g.n[0] + g.n[1] + g.n[2] + 8;
+    x = idx[0];
+    y = idx[1];
+    z = idx[2];
+    live = true;
+    return true;
+  }
+
+  template <class Emit>
+  __device__ __forceinline__ uint32_t run(uint32_t max_emit, Emit&& emit) {
+    const double eps = 1.0e-6;
+    uint32_t cnt = 0;
+    while (live && cnt < max_emit) {
+      if (!(alpha < amax && it < cap)) {
+        live = false;
+        break;
+      }
+      ++it;
+      const uint32_t cx = x, cy = y, cz = z;
+      if (ax[0] < ax[1] && ax[0] < ax[2]) {
+        x += (uint32_t)di[0]; alpha = ax[0]; ax[0] += da[0];
+      } else if (ax[1] < ax[2]) {
+        y += (uint32_t)di[1]; alpha = ax[1]; ax[1] += da[1];
+      } else {
+        z += (uint32_t)di[2]; alpha = ax[2]; ax[2] += da[2];
+      }
+      const double dalpha = smin(alpha, amax) - alpha_last;
+      if (eps < dalpha) {
+        emit(cx, cy, cz, dalpha * length);
+        ++cnt;
+        alpha_last = alpha;
+      }
+    }
+    return cnt;
+  }
+};
+
+// The whole walk in one call; returns the number of recorded voxels.
 template <class Emit>
 __device__ __forceinline__ uint32_t siddon_walk(const GridD& g,
                                                 const double* p1,
                                                 const double* p2,
                                                 Emit&& emit) {
-  const double eps = 1.0e-6;
-  double d21[3], a0[3], an[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    d21[a] = p2[a] - p1[a];
-    a0[a] = -0.1;
-    an[a] = 1.1;
-  }
-  const double length =
-      sqrt((d21[0] * d21[0] + d21[1] * d21[1]) + d21[2] * d21[2]);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const double hi = g.mn[a] + g.d[a] * (double)g.n[a];
-    if (eps < fabs(d21[a])) {
-      a0[a] = (g.mn[a] - p1[a]) / d21[a];
-      an[a] = (hi - p1[a]) / d21[a];
-    } else if (p1[a] < g.mn[a] || hi <= p1[a]) {
-      return 0;
-    }
-  }
-  double lo[3], hi[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const bool sw = an[a] < a0[a];
-    lo[a] = sw ? an[a] : a0[a];
-    hi[a] = sw ? a0[a] : an[a];
-  }
-  const double amin = smax(smax(smax(lo[0], lo[1]), lo[2]), 0.0);
-  const double amax = smin(smin(smin(hi[0], hi[1]), hi[2]), 1.0);
-  if (amax <= amin) return 0;
-  double alpha = amin;
-  uint32_t idx[3];
-  int di[3];
-  double da[3], ax[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    idx[a] = x86_u32((alpha * d21[a] + p1[a] - g.mn[a]) / g.d[a] +
-                     (d21[a] <= 0 ? -eps : eps));
-    di[a] = 0;
-    da[a] = 0.0;
-    if (eps < fabs(d21[a])) {
-      da[a] = g.d[a] / (fabs(d21[a]));
-      di[a] = d21[a] < 0 ? -1 : 1;
-    }
-  }
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    ax[a] = 1.1;
-    if (fabs(amin - lo[a]) < eps) {
-      ax[a] = amin + da[a];
-    } else if (di[a] != 0) {
-      ax[a] = ((double)idx[a] * g.d[a] + g.mn[a] - p1[a]) / d21[a] +
-              (di[a] == 1 ? da[a] : 0.0);
-    }
-  }
-  double alpha_last = alpha;
-  uint32_t cnt = 0;
-  const uint32_t cap = g.n[0] + g.n[1] + g.n[2] + 8;
-  uint32_t x = idx[0], y = idx[1], z = idx[2];
-  for (uint32_t it = 0; alpha < amax && it < cap; ++it) {
-    const uint32_t cx = x, cy = y, cz = z;
-    if (ax[0] < ax[1] && ax[0] < ax[2]) {
-      x += (uint32_t)di[0]; alpha = ax[0]; ax[0] += da[0];
-    } else if (ax[1] < ax[2]) {
-      y += (uint32_t)di[1]; alpha = ax[1]; ax[1] += da[1];
-    } else {
-      z += (uint32_t)di[2]; alpha = ax[2]; ax[2] += da[2];
-    }
-    const double dalpha = smin(alpha, amax) - alpha_last;
-    if (eps < dalpha) {
-      emit(cx, cy, cz, dalpha * length);
-      ++cnt;
-      alpha_last = alpha;
-    }
-  }
-  return cnt;
+  SiddonCursor c;
+  if (!c.begin(g, p1, p2)) return 0;
+  return c.run(0xFFFFFFFFu, emit);
 }
 
 // Eigen 3-vector dot: (x0*y0 + x1*y1) + x2*y2

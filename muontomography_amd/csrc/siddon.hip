@@ -18,27 +18,72 @@ k_siddon_count(GridD g, const double* __restrict__ p1,
   counts[i] = siddon_walk(g, a, b, [](uint32_t, uint32_t, uint32_t, double) {});
 }
 
-__global__ void __launch_bounds__(256)
+// Fill pass: each lane walks its own segment, FILL_STAGE recorded voxels at
+// a time, into an LDS stage (slot k of thread t at k * FILL_BS + t); then
+// the wavefront writes every lane's staged run together, FILL_STAGE lanes
+// per source lane, so that each store instruction writes 64 / FILL_STAGE
+// runs of consecutive entries instead of 64 scattered words (the lanes'
+// lists lie far apart).
+// Every lane of a wavefront takes part in every flush (the loop runs while
+// any lane's walk is live).
+constexpr int FILL_STAGE = 16;
+constexpr uint32_t FILL_BS = 128;
+__global__ void __launch_bounds__(FILL_BS)
 k_siddon_fill(GridD g, const double* __restrict__ p1,
               const double* __restrict__ p2, size_t n,
               const uint64_t* __restrict__ row_ptr, uint32_t* __restrict__ vx,
               uint32_t* __restrict__ vy, uint32_t* __restrict__ vz,
               double* __restrict__ len) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double a[3] = {p1[3 * i], p1[3 * i + 1], p1[3 * i + 2]};
-  double b[3] = {p2[3 * i], p2[3 * i + 1], p2[3 * i + 2]};
-  uint64_t o = row_ptr[i];
-  const uint64_t end = row_ptr[i + 1];
-  siddon_walk(g, a, b, [&](uint32_t x, uint32_t y, uint32_t z, double L) {
-    if (o < end) {
-      vx[o] = x;
-      if (vy) vy[o] = y;
-      if (vz) vz[o] = z;
-      if (len) len[o] = L;
+  __shared__ uint32_t st_x[FILL_STAGE * FILL_BS], st_y[FILL_STAGE * FILL_BS], st_z[FILL_STAGE * FILL_BS];
+  __shared__ double st_l[FILL_STAGE * FILL_BS];
+  const uint32_t tid = threadIdx.x;
+  const int lane = (int)(tid & 63u);
+  const uint32_t wbase = tid & ~63u;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + tid;
+  SiddonCursor c;
+  bool live = false;
+  uint64_t o = 0, e = 0;
+  if (i < n) {
+    const double a[3] = {p1[3 * i], p1[3 * i + 1], p1[3 * i + 2]};
+    const double b[3] = {p2[3 * i], p2[3 * i + 1], p2[3 * i + 2]};
+    o = row_ptr[i];
+    e = row_ptr[i + 1];
+    live = c.begin(g, a, b);
+  }
+  while (__any(live)) {
+    int cnt = 0;
+    if (live) {
+      c.run(FILL_STAGE, [&](uint32_t x, uint32_t y, uint32_t z, double L) {
+        const uint32_t sl = (uint32_t)cnt * FILL_BS + tid;
+        st_x[sl] = x;
+        st_y[sl] = y;
+        st_z[sl] = z;
+        st_l[sl] = L;
+        ++cnt;
+      });
+      live = c.live;
     }
-    ++o;
-  });
+    // the wavefront's flush, FILL_STAGE rounds of R = 64 / FILL_STAGE source
+    // lanes: source lane src's k-th staged entry is written by lane
+    // (src % R) * FILL_STAGE + k of round src / R
+#pragma unroll
+    for (int r = 0; r < FILL_STAGE; ++r) {
+      const int src = r * (64 / FILL_STAGE) + lane / FILL_STAGE, k = lane % FILL_STAGE;
+      const int ns = __shfl(cnt, src, 64);
+      const uint64_t os = (uint64_t)(uint32_t)__shfl((int)(uint32_t)o, src, 64) |
+                          ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(o >> 32), src, 64) << 32);
+      const uint64_t es = (uint64_t)(uint32_t)__shfl((int)(uint32_t)e, src, 64) |
+                          ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(e >> 32), src, 64) << 32);
+      if (k < ns && os + (uint64_t)k < es) {
+        const uint32_t sl = (uint32_t)k * FILL_BS + wbase + (uint32_t)src;
+        vx[os + k] = st_x[sl];
+        if (vy) vy[os + k] = st_y[sl];
+        if (vz) vz[os + k] = st_z[sl];
+        if (len) len[os + k] = st_l[sl];
+      }
+    }
+    o += (uint64_t)cnt;
+  }
 }
 
 // Per-segment count and digest of the list: FNV-1a over 64-bit words, three
@@ -75,8 +120,8 @@ hipError_t siddon_fill(const GridD& g, const double* p1, const double* p2,
                        uint32_t* vy, uint32_t* vz, double* len,
                        hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_siddon_fill<<<grid_for(n, 256), 256, 0, s>>>(g, p1, p2, n, row_ptr, vx,
-                                                 vy, vz, len);
+  k_siddon_fill<<<grid_for(n, FILL_BS), FILL_BS, 0, s>>>(g, p1, p2, n, row_ptr, vx,
+                                                         vy, vz, len);
   return hipGetLastError();
 }
 
