@@ -1138,6 +1138,14 @@ struct Args64 {
   const double* seg_tail64;
   const uint32_t* slot;  // median / trimmed mean: slot of each entry
   double* cval;          // ... and the per-entry c_ij, voxel-major
+  // the order statistics' backprojection takes its tasks from one queue per
+  // XCD (workgroup b on queue b % nq): small tasks of one brick at a time,
+  // so that the brick's slot range stays in that XCD's L2 while its
+  // scattered c_ij writes land
+  const Task* qtask;
+  const uint32_t* qoff;  // [nq + 1]
+  unsigned int* qwork;   // [nq]
+  uint32_t nq;
 };
 
 constexpr size_t fwd64_static_lds(uint32_t bs) {
@@ -1344,14 +1352,15 @@ k_bwd64s(IterArgs a, Args64 b, const double4* __restrict__ trk_g, double* __rest
   const uint32_t t = threadIdx.x;
   fill_step_table(a, stab);
   const uint32_t vol = a.G.b[0] * a.G.b[1] * a.G.b[2];
+  const uint32_t qi = MODE == 1 ? blockIdx.x % b.nq : 0u;
   for (;;) {
-    if (t == 0) sh = atomicAdd(&a.work[1], 1u);
+    if (t == 0) sh = MODE == 1 ? b.qoff[qi] + atomicAdd(&b.qwork[qi], 1u) : atomicAdd(&a.work[1], 1u);
     if (MODE == 0)
       for (uint32_t v = t; v < vol; v += BS) acc_s[v] = 0.0;
     __syncthreads();
     const uint32_t ti = sh;
-    if (ti >= a.n_tasks) break;
-    const Task tk = a.tasks[ti];
+    if (MODE == 1 ? ti >= b.qoff[qi + 1] : ti >= a.n_tasks) break;
+    const Task tk = MODE == 1 ? b.qtask[ti] : a.tasks[ti];
     for (uint32_t c = tk.c0; c < tk.c1; ++c) {
       const Chunk ch = a.chunks[c];
       if (t < ch.cnt) {
@@ -1554,17 +1563,23 @@ __device__ __forceinline__ double wave_elem(const double (&v)[R], uint32_t e) {
   return __shfl(x, (int)(e % 64u), 64);
 }
 
+// rank of the order statistic that is kept as the next pass's pivot: the
+// lower median, or the last value of the trimmed mean
+__device__ __forceinline__ uint32_t pivot_rank(uint32_t m, int use_mean, int pct) {
+  if (!use_mean) return (m - 1) / 2;
+  uint32_t jm = x86_u32(pct / 100.0 * (double)m);
+  if (jm < 1) jm = 1;
+  return jm - 1;
+}
+
+// The order statistic from a full sort of the wavefront's values; *piv gets
+// the value at pivot_rank.
 template <int R>
-__device__ __forceinline__ double wave_select(const double* __restrict__ src, uint32_t m, double l,
-                                              int use_mean, int pct) {
+__device__ __forceinline__ double wave_select(double (&v)[R], uint32_t m, double l, int use_mean,
+                                              int pct, double* piv) {
   const uint32_t lane = threadIdx.x & 63u;
-  double v[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const uint32_t e = r * 64u + lane;
-    v[r] = e < m ? src[e] : INFINITY;
-  }
   wave_bitonic<R>(v);
+  *piv = wave_elem<R>(v, pivot_rank(m, use_mean, pct));
   if (!use_mean) return order_stat_c(l, m, 0, pct, [&](uint32_t k) { return wave_elem<R>(v, k); });
   // trimmed mean: S of the lowest jm values, summed over the wavefront
   uint32_t jm = x86_u32(pct / 100.0 * (double)m);
@@ -1577,18 +1592,167 @@ __device__ __forceinline__ double wave_select(const double* __restrict__ src, ui
   return s / (double)jm * 0.5;
 }
 
+template <int R, class P>
+__device__ __forceinline__ uint32_t wave_count_if(const double (&v)[R], P pred) {
+  uint32_t n = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) n += (uint32_t)__popcll(__ballot(pred(v[r])));
+  return n;
+}
+
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+constexpr int PIVOT_STEPS = 16;  // distinct values searched past the pivot before a full sort
+
+// The value of rank k among the wavefront's values (+inf padding beyond m
+// never counts: k < m), from a pivot p with cl values below it and ce equal
+// to it: the distinct values between p and the answer are visited one at a
+// time (the largest below, or the smallest above, the last one).  false
+// when more than PIVOT_STEPS are needed.
+template <int R>
+__device__ __forceinline__ bool rank_from_pivot(const double (&v)[R], uint32_t k, double p, uint32_t cl,
+                                                uint32_t ce, double* out) {
+  if (cl <= k && k < cl + ce) { *out = p; return true; }
+  double t = p;
+  if (k < cl) {
+    uint32_t have = cl;  // the values below t hold ranks [0, have)
+    for (int it = 0; it < PIVOT_STEPS; ++it) {
+      double mx = -INFINITY;
+#pragma unroll
+      for (int r = 0; r < R; ++r) mx = fmax(mx, v[r] < t ? v[r] : -INFINITY);
+      const double t2 = wave_max(mx);
+      const uint32_t n2 = wave_count_if(v, [&](double x) { return x == t2; });
+      if (have - n2 <= k) { *out = t2; return true; }
+      have -= n2;
+      t = t2;
+    }
+    return false;
+  }
+  uint32_t have = cl + ce;  // the values up to t hold ranks [0, have)
+  for (int it = 0; it < PIVOT_STEPS; ++it) {
+    double mn = INFINITY;
+#pragma unroll
+    for (int r = 0; r < R; ++r) mn = fmin(mn, v[r] > t ? v[r] : INFINITY);
+    const double t2 = wave_min(mn);
+    const uint32_t n2 = wave_count_if(v, [&](double x) { return x == t2; });
+    if (k < have + n2) { *out = t2; return true; }
+    have += n2;
+    t = t2;
+  }
+  return false;
+}
+
+// The same order statistic from the previous pass's pivot p (the voxel's
+// values move little from pass to pass, so the answer is a few distinct
+// values away from it); false when it is not (a full sort then runs).
+template <int R>
+__device__ __forceinline__ bool wave_select_pivot(const double (&v)[R], uint32_t m, double l, int use_mean,
+                                                  int pct, double p, double* out, double* piv) {
+  const uint32_t lane = threadIdx.x & 63u;
+  if (!(p == p)) return false;  // no pivot yet
+  const uint32_t cl = wave_count_if(v, [&](double x) { return x < p; });
+  const uint32_t ce = wave_count_if(v, [&](double x) { return x == p; });
+  const double l2 = l * l;
+  if (!use_mean) {
+    const uint32_t klo = (m - 1) / 2, khi = m / 2;
+    double elo, ehi;
+    if (!rank_from_pivot(v, klo, p, cl, ce, &elo)) return false;
+    if (khi == klo) {
+      ehi = elo;
+    } else if (!rank_from_pivot(v, khi, p, cl, ce, &ehi)) {
+      return false;
+    }
+    *piv = elo;
+    const double med = 2.0 * l + ehi * l2;  // S of rank m/2, as order_stat_c
+    *out = m % 2 == 0 ? (med + (2.0 * l + elo * l2)) * 0.25 : med * 0.5;
+    return true;
+  }
+  // trimmed mean: the sum of the lowest jm values from the sum below p,
+  // adjusted by the distinct values between p and rank jm - 1
+  uint32_t jm = x86_u32(pct / 100.0 * (double)m);
+  if (jm < 1) jm = 1;
+  double sl = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (v[r] < p) sl = sl + v[r];
+  double sc = wave_sum(sl);
+  (void)lane;
+  double t = p;
+  if (jm < cl) {
+    uint32_t have = cl;
+    int it = 0;
+    while (have > jm) {
+      if (++it > PIVOT_STEPS) return false;
+      double mx = -INFINITY;
+#pragma unroll
+      for (int r = 0; r < R; ++r) mx = fmax(mx, v[r] < t ? v[r] : -INFINITY);
+      t = wave_max(mx);
+      const uint32_t n2 = wave_count_if(v, [&](double x) { return x == t; });
+      const uint32_t take = min(n2, have - jm);  // values equal to t dropped from the top
+      sc = sc - (double)take * t;
+      have -= take;
+    }
+  } else if (jm <= cl + ce) {
+    sc = sc + (double)(jm - cl) * p;
+  } else {
+    sc = sc + (double)ce * p;
+    uint32_t have = cl + ce;
+    int it = 0;
+    while (have < jm) {
+      if (++it > PIVOT_STEPS) return false;
+      double mn = INFINITY;
+#pragma unroll
+      for (int r = 0; r < R; ++r) mn = fmin(mn, v[r] > t ? v[r] : INFINITY);
+      t = wave_min(mn);
+      const uint32_t n2 = wave_count_if(v, [&](double x) { return x == t; });
+      const uint32_t take = min(n2, jm - have);
+      sc = sc + (double)take * t;
+      have += take;
+    }
+  }
+  *piv = t;  // near rank jm - 1 (a pivot only has to be close)
+  *out = (2.0 * l * (double)jm + l2 * sc) / (double)jm * 0.5;
+  return true;
+}
+
+template <int R>
+__device__ __forceinline__ double wave_update(const double* __restrict__ src, uint32_t m, double l,
+                                              int use_mean, int pct, double p, double* piv) {
+  const uint32_t lane = threadIdx.x & 63u;
+  double v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = r * 64u + lane;
+    v[r] = e < m ? src[e] : INFINITY;
+  }
+  double out;
+  if (wave_select_pivot<R>(v, m, l, use_mean, pct, p, &out, piv)) return out;
+  return wave_select<R>(v, m, l, use_mean, pct, piv);
+}
+
 // Per voxel, one wavefront (M_j <= 256; longer lists are listed at set-up
-// and go to k_sel64b): the M_j values of c sorted in registers, the
-// update written in place (each voxel reads only its own lambda).  M_j = 0
-// gives 0 (src/MutoMLSD.cpp:494-496).  MAP: lambda_ML to ml.
+// and go to k_sel64b): the order statistic of the voxel's M_j values of c,
+// from the previous pass's pivot when it is close, else by a sort in
+// registers; the update written in place (each voxel reads only its own
+// lambda) and the new pivot kept.  M_j = 0 gives 0 (src/MutoMLSD.cpp:
+// 494-496).  MAP: lambda_ML to ml.
 __global__ void __launch_bounds__(256)
 k_sel64w(size_t nv, const uint32_t* __restrict__ mj, const uint64_t* __restrict__ col,
          const double* __restrict__ cval, int use_mean, int pct, double* __restrict__ lam,
-         double* __restrict__ ml) {
+         double* __restrict__ ml, double* __restrict__ pivot) {
   const size_t nw = (size_t)gridDim.x * (blockDim.x >> 6);
   for (size_t j = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; j < nv; j += nw) {
     const uint32_t m = mj[j];
-    double out;
+    double out, piv = NAN;
     if (m == 0) {
       out = 0.0;
     } else if (m > 256) {
@@ -1596,17 +1760,24 @@ k_sel64w(size_t nv, const uint32_t* __restrict__ mj, const uint64_t* __restrict_
     } else {
       const double l = lam[j];
       const double* src = cval + col[j];
+      const double p = pivot[j];
       if (m <= 64)
-        out = wave_select<1>(src, m, l, use_mean, pct);
+        out = wave_update<1>(src, m, l, use_mean, pct, p, &piv);
       else if (m <= 128)
-        out = wave_select<2>(src, m, l, use_mean, pct);
+        out = wave_update<2>(src, m, l, use_mean, pct, p, &piv);
       else
-        out = wave_select<4>(src, m, l, use_mean, pct);
+        out = wave_update<4>(src, m, l, use_mean, pct, p, &piv);
     }
     if ((threadIdx.x & 63u) == 0) {
       if (ml) ml[j] = out; else lam[j] = out;
+      pivot[j] = piv;
     }
   }
+}
+
+__global__ void k_fill_nan(size_t n, double* __restrict__ x) {
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) x[j] = NAN;
 }
 
 // Listed voxels with longer lists, one block each: bitonic sort in LDS
@@ -1658,10 +1829,12 @@ __global__ void k_zero_uncrossed64(size_t nv, const uint32_t* __restrict__ mj, d
 
 // clears the FP64 stream iteration's cost, counters and work counters
 __global__ void k_iter_begin64(double* __restrict__ cost_acc, unsigned long long* __restrict__ cnt,
-                               unsigned int* __restrict__ work) {
+                               unsigned int* __restrict__ work, unsigned int* __restrict__ qwork,
+                               uint32_t nq) {
   if (threadIdx.x == 0) cost_acc[0] = 0.0;
   if (threadIdx.x < 3) cnt[threadIdx.x] = 0ull;
   if (threadIdx.x < 2) work[threadIdx.x] = 0u;
+  if (qwork && threadIdx.x < nq) qwork[threadIdx.x] = 0u;
 }
 
 __global__ void k_copy64(size_t n, const double* __restrict__ a, double* __restrict__ b) {
@@ -2420,6 +2593,10 @@ static Args64 args64(muto_em_plan* pl) {
   b.seg_tail64 = f.seg_tail64.p;
   b.slot = f.slot64.p;
   b.cval = f.cval.p;
+  b.qtask = f.qtask64.p;
+  b.qoff = f.qoff64.p;
+  b.qwork = f.qwork64.p;
+  b.nq = f.nq64;
   return b;
 }
 
@@ -2482,14 +2659,32 @@ static int fp64s_slots(muto_em_plan* pl) {
   const size_t nv = pl->nv;
   std::vector<uint32_t> h(nv);
   MUTO_CHECK_HIP(ctx, copy_sync(ctx, h.data(), pl->mj.p, nv * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  // first slot of each voxel, the voxels taken brick by brick (so that
+  // the entries of one brick, which one backprojection task writes, land in
+  // one contiguous range); col[nv] is the total
   std::vector<uint64_t> col(nv + 1);
-  col[0] = 0;
-  for (size_t j = 0; j < nv; ++j) col[j + 1] = col[j] + h[j];
+  {
+    const uint32_t n0 = pl->g.n[0], n1 = pl->g.n[1], n2 = pl->g.n[2];
+    const uint32_t b0 = f.bs.bx, b1 = f.bs.by, b2 = f.bs.bz;
+    uint64_t run = 0;
+    for (uint32_t oz = 0; oz < n2; oz += b2)
+      for (uint32_t oy = 0; oy < n1; oy += b1)
+        for (uint32_t ox = 0; ox < n0; ox += b0)
+          for (uint32_t z = oz; z < std::min(oz + b2, n2); ++z)
+            for (uint32_t y = oy; y < std::min(oy + b1, n1); ++y)
+              for (uint32_t x = ox; x < std::min(ox + b0, n0); ++x) {
+                const size_t j = (size_t)x + (size_t)n0 * ((size_t)y + (size_t)n1 * z);
+                col[j] = run;
+                run += h[j];
+              }
+    col[nv] = run;
+  }
   MUTO_CHECK_HIP(ctx, f.col64.alloc(nv + 1));
   MUTO_CHECK_HIP(ctx, copy_sync(ctx, f.col64.p, col.data(), (nv + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
   MUTO_CHECK_HIP(ctx, f.slot64.alloc(f.n_ent_pad + GE));
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.slot64.p, 0xFF, (f.n_ent_pad + GE) * sizeof(uint32_t), s));
   MUTO_CHECK_HIP(ctx, f.cval.alloc(std::max<uint64_t>(col[nv], 1)));
+  MUTO_CHECK_HIP(ctx, f.pivot64.alloc(nv));
   {
     DBuf<uint32_t> fill;
     MUTO_CHECK_HIP(ctx, fill.alloc(nv));
@@ -2514,6 +2709,62 @@ static int fp64s_slots(muto_em_plan* pl) {
     ids.push_back((uint32_t)j);
     off.push_back(n > 4096 ? tot : 0);
     if (n > 4096) tot += n;
+  }
+  {
+    // per-XCD task queues for the order statistics' backprojection: each
+    // brick's chunks cut into tasks of QCHUNKS chunks, the bricks dealt to
+    // NQ queues (largest first, to the least loaded), a queue's tasks brick
+    // by brick
+    constexpr uint32_t NQ = 8, QCHUNKS = 1;
+    std::vector<Task> tk(f.n_tasks);
+    if (f.n_tasks)
+      MUTO_CHECK_HIP(ctx, copy_sync(ctx, tk.data(), f.tasks.p, f.n_tasks * sizeof(Task), hipMemcpyDeviceToHost));
+    std::vector<Chunk> ch(f.n_chunks);
+    if (f.n_chunks)
+      MUTO_CHECK_HIP(ctx, copy_sync(ctx, ch.data(), f.chunks.p, f.n_chunks * sizeof(Chunk), hipMemcpyDeviceToHost));
+    struct BR { uint32_t brick, c0, c1; uint64_t ne; };
+    std::vector<BR> br;
+    {
+      std::sort(tk.begin(), tk.end(), [](const Task& x, const Task& y) {
+        return x.brick != y.brick ? x.brick < y.brick : x.c0 < y.c0;
+      });
+      for (const Task& t : tk) {
+        if (!br.empty() && br.back().brick == t.brick) {
+          br.back().c0 = std::min(br.back().c0, t.c0);
+          br.back().c1 = std::max(br.back().c1, t.c1);
+        } else {
+          br.push_back({t.brick, t.c0, t.c1, 0});
+        }
+      }
+      for (BR& x : br)
+        for (uint32_t c = x.c0; c < x.c1; ++c) x.ne += ch[c].ne;
+    }
+    std::vector<size_t> order(br.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return br[x].ne > br[y].ne; });
+    std::vector<std::vector<Task>> q(NQ);
+    std::vector<uint64_t> load(NQ, 0);
+    for (size_t k : order) {
+      const uint32_t qi = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[qi] += br[k].ne;
+      for (uint32_t c = br[k].c0; c < br[k].c1; c += QCHUNKS)
+        q[qi].push_back(Task{br[k].brick, 1u, c, std::min(c + QCHUNKS, br[k].c1)});
+    }
+    std::vector<Task> all;
+    std::vector<uint32_t> off(NQ + 1, 0);
+    for (uint32_t qi = 0; qi < NQ; ++qi) {
+      off[qi] = (uint32_t)all.size();
+      all.insert(all.end(), q[qi].begin(), q[qi].end());
+    }
+    off[NQ] = (uint32_t)all.size();
+    f.nq64 = NQ;
+    MUTO_CHECK_HIP(ctx, f.qtask64.alloc(all.size()));
+    MUTO_CHECK_HIP(ctx, f.qoff64.alloc(NQ + 1));
+    MUTO_CHECK_HIP(ctx, f.qwork64.alloc(NQ));
+    if (!all.empty())
+      MUTO_CHECK_HIP(ctx, copy_sync(ctx, f.qtask64.p, all.data(), all.size() * sizeof(Task), hipMemcpyHostToDevice));
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, f.qoff64.p, off.data(), (NQ + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+    MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.qwork64.p, 0, NQ * sizeof(unsigned int), s));
   }
   f.n_big64 = (uint32_t)ids.size();
   MUTO_CHECK_HIP(ctx, f.big64.alloc(ids.size()));
@@ -2544,7 +2795,7 @@ static int fp64s_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
   const ErrM E = error_matrix(pl->cfg);
   const bool map = pl->is_map, mean = mean_all(pl);
   const size_t lds = (size_t)f.bs.vol() * sizeof(double);
-  k_iter_begin64<<<1, 64, 0, s>>>(pl->cost_acc.p, pl->dcount.p, pl->work.p);
+  k_iter_begin64<<<1, 64, 0, s>>>(pl->cost_acc.p, pl->dcount.p, pl->work.p, f.qwork64.p, b.nq);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[0], s));
   if (f.n_tasks) {
@@ -2585,7 +2836,7 @@ static int fp64s_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
     const muto_mlsd_cfg& c = pl->cfg;
     const unsigned g = (unsigned)std::min<size_t>(grid_for(pl->nv, 4), (size_t)ctx->n_cu * 64);
     k_sel64w<<<g, 256, 0, s>>>(pl->nv, pl->mj.p, f.col64.p, f.cval.p, c.update_with_mean,
-                               c.mean_percentile, pl->lam64.p, ml);
+                               c.mean_percentile, pl->lam64.p, ml, f.pivot64.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     if (f.n_big64) {
       k_sel64b<<<std::min<uint32_t>(f.n_big64, 4096), 1024, 0, s>>>(
@@ -2654,6 +2905,10 @@ int fp64s_run(muto_em_plan* pl, int iters, double* sums, int* done) {
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(pl->acc64.p, 0, pl->nv * sizeof(double), s));
   if (!pl->is_map && iters > 0 && mean_all(pl)) {
     k_zero_uncrossed64<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->lam64.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  }
+  if (!mean_all(pl)) {  // every run starts without pivots: the same work, run to run
+    k_fill_nan<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, f.pivot64.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
   const bool graph = pl->use_graph && !pl->timing;

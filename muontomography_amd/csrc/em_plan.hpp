@@ -94,6 +94,11 @@ struct Fp32cData {
   DBuf<uint32_t> slot64;    // [n_ent_pad] slot of each entry (~0u: padding)
   DBuf<uint64_t> col64;     // [nv+1] first slot of each voxel
   DBuf<double> cval;        // [nnz] c_ij in slots
+  DBuf<double> pivot64;     // [nv] last pass's order statistic per voxel (NaN: none)
+  DBuf<Task> qtask64;       // per-XCD task queues of its backprojection
+  DBuf<uint32_t> qoff64;    // [nq64 + 1]
+  DBuf<unsigned int> qwork64;  // [nq64]
+  uint32_t nq64 = 1;
   uint32_t n_big64 = 0;     // voxels above the one-block sort
   DBuf<uint32_t> big64;     // [n_big64] their indices
   DBuf<uint64_t> big64_off; // [n_big64] their offsets in big64_scr
@@ -122,6 +127,11 @@ struct Fp32cData {
     slot64.release();
     col64.release();
     cval.release();
+    pivot64.release();
+    qtask64.release();
+    qoff64.release();
+    qwork64.release();
+    nq64 = 1;
     n_big64 = 0;
     big64.release();
     big64_off.release();
