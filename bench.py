@@ -124,6 +124,22 @@ def bench_fp64(args, plan, rays, cfg_kw, init, nnz, wall32, cost32):
     cost32n = plan.run(n, M.MUTO_FP32C)
     img32 = plan.image()
     e32 = _rel(img32, img64, mj)
+    # FP32C with FP32 two-sum sums (MUTO_OPT_ACC_MODE 0) for the same iterations
+    plan.set_option(M.MUTO_OPT_ACC_MODE, 0)
+    plan.set_image(init)
+    plan.run(1, M.MUTO_FP32C)  # graph capture
+    plan.set_image(init)
+    t0 = time.perf_counter()
+    plan.run(n, M.MUTO_FP32C)
+    w2s = (time.perf_counter() - t0) / n
+    e2s = _rel(plan.image(), img64, mj)
+    plan.set_option(M.MUTO_OPT_TIMING, 1)
+    plan.set_image(init)
+    plan.run(args.timing_steps, M.MUTO_FP32C)
+    plan.set_option(M.MUTO_OPT_TIMING, 0)
+    t2s = plan.timing()
+    plan.set_option(M.MUTO_OPT_ACC_MODE, 1)
+    log(f"FP32C two-sum sums: {w2s * 1e3:.2f} ms/it ({t2s}), err max {e2s[0]}")
     # FP64 brick stream: its own brick shape (FP64 bricks and tables in LDS),
     # the fastest of a few over 2 iterations
     plan.set_option(M.MUTO_OPT_FP64_KERNEL, 2)
@@ -198,6 +214,9 @@ def bench_fp64(args, plan, rays, cfg_kw, init, nnz, wall32, cost32):
         "fp64_wave_value": nnz / w64w,
         "fp32c_speedup_vs_fp64_wave": w64w * 1e3 / ms32,
         "fp64_wave_vs_fp64_err_max_rel": _rel(img64w, img64, mj)[0],
+        "fp32c_two_sum": {"ms_per_step": w2s * 1e3, "value": nnz / w2s,
+                          "kernel_ms": {k: round(v, 4) for k, v in t2s.items()},
+                          "err_vs_fp64_max_rel": e2s[0], "err_vs_fp64_rms_rel": e2s[1]},
         "fp64_stream": {"ms_per_step": w64s * 1e3, "value": nnz / w64s, "brick": list(b64),
                         "kernel_ms": {k: round(v, 4) for k, v in t64s.items()},
                         "brick_ms": {",".join(map(str, k)): round(v, 3) for k, v in tune.items()},
@@ -484,8 +503,17 @@ def main():
         b = torch.empty_like(a)
         for _ in range(3):
             b.copy_(a)
+        # a read-only kernel (a reduction: reads the array, writes a few
+        # bytes) and a write-only one (a fill), same size, to check that the
+        # copy's counter factors hold for reads and writes alone
+        for _ in range(3):
+            s = a.sum()
+        for _ in range(3):
+            b.fill_(2.0)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3,
+                          "sum_bytes_read": 4 * n, "fill_bytes_written": 4 * n,
+                          "sum": float(s)}))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "

@@ -106,87 +106,135 @@ k_compact(GridD g, const muto_ray* __restrict__ rays, size_t n,
 }
 
 // Merged two-leg path of each selected track (MutoMLSD.cpp:382-415).
-// FILL = false: entry counts only.  FILL = true: voxel indices and lengths.
-// Each lane walks its own track; the entries it emits are staged in LDS
-// and written PATH_STAGE at a time, so the lines of a lane's list are
-// written in short bursts instead of one word per walk step (the lists of
-// the 64 lanes of a wavefront lie far apart).
+// k_paths_count: entry counts only.  k_paths_fill: voxel indices and
+// lengths.  Each lane walks its own track, one leg after the other on one
+// cursor, up to PATH_STAGE entries at a time into an LDS stage; then the
+// wavefront writes every lane's staged run together, PATH_STAGE lanes per
+// source lane.  As in the Siddon CSR fill, a window ends where the lane's
+// output position reaches a multiple of PATH_STAGE and its entries sit at
+// their position modulo PATH_STAGE, so that later windows write whole
+// aligned runs.
 constexpr int PATH_STAGE = 16;
-template <bool FILL>
+constexpr uint32_t PATH_BS = 64;
+
 __global__ void __launch_bounds__(256)
-k_paths(GridD g, const muto_ray* __restrict__ rays,
-        const uint32_t* __restrict__ sel, const double* __restrict__ sel_p,
-        size_t n_trk, int straight, uint64_t* __restrict__ counts,
-        const uint64_t* __restrict__ row_ptr, uint32_t* __restrict__ vox,
-        double* __restrict__ len, unsigned long long* __restrict__ cnt) {
-  // entry-major staging (slot n of lane t at n * 256 + t: no bank conflicts
-  // when the lanes emit in step)
-  __shared__ uint32_t st_j[FILL ? PATH_STAGE * 256 : 1];
-  __shared__ double st_l[FILL ? PATH_STAGE * 256 : 1];
+k_paths_count(GridD g, const muto_ray* __restrict__ rays,
+              const uint32_t* __restrict__ sel, const double* __restrict__ sel_p,
+              size_t n_trk, int straight, uint64_t* __restrict__ counts) {
+  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_trk) return;
+  muto_ray r;
+  load_ray(rays, sel[s], r);
+  const double P[3] = {sel_p[3 * s], sel_p[3 * s + 1], sel_p[3 * s + 2]};
+  uint64_t c;
+  if (straight) {
+    c = siddon_walk(g, r.pin, r.pout, [](uint32_t, uint32_t, uint32_t, double) {});
+  } else {
+    uint64_t ki = siddon_walk(g, r.pin, P, [](uint32_t, uint32_t, uint32_t, double) {});
+    uint64_t ko = siddon_walk(g, P, r.pout, [](uint32_t, uint32_t, uint32_t, double) {});
+    c = (ki ? ki : 0) + (ko ? ko - 1 : 0);
+  }
+  counts[s] = c;
+}
+
+__global__ void __launch_bounds__(PATH_BS)
+k_paths_fill(GridD g, const muto_ray* __restrict__ rays,
+             const uint32_t* __restrict__ sel, const double* __restrict__ sel_p,
+             size_t n_trk, int straight, const uint64_t* __restrict__ row_ptr,
+             uint32_t* __restrict__ vox, double* __restrict__ len,
+             unsigned long long* __restrict__ cnt_out) {
+  // slot k of thread t at k * PATH_BS + t
+  __shared__ uint32_t st_j[PATH_STAGE * PATH_BS];
+  __shared__ double st_l[PATH_STAGE * PATH_BS];
   const uint32_t tid = threadIdx.x;
-  size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = (int)(tid & 63u);
+  const uint32_t wbase = tid & ~63u;
+  const size_t s = (size_t)blockIdx.x * PATH_BS + tid;
+  const uint32_t nx = g.n[0], nxy = g.n[0] * g.n[1];
   uint32_t nout = 0;
+  SiddonCursor c;
+  bool live = false;
+  bool in_leg = false;      // the cursor walks the in-leg (the out-leg follows)
+  bool merge_next = false;  // the next recorded voxel is the out-leg's first
+  uint32_t ki = 0;          // entries the in-leg recorded
+  uint64_t o = 0, e = 0;
+  double P[3] = {0.0, 0.0, 0.0}, pout[3] = {0.0, 0.0, 0.0};
   if (s < n_trk) {
     muto_ray r;
     load_ray(rays, sel[s], r);
-    const double P[3] = {sel_p[3 * s], sel_p[3 * s + 1], sel_p[3 * s + 2]};
-    const uint32_t nx = g.n[0], nxy = g.n[0] * g.n[1];
-    if (!FILL) {
-      uint64_t c;
-      if (straight) {
-        c = siddon_walk(g, r.pin, r.pout, [](uint32_t, uint32_t, uint32_t, double) {});
-      } else {
-        uint64_t ki = siddon_walk(g, r.pin, P, [](uint32_t, uint32_t, uint32_t, double) {});
-        uint64_t ko = siddon_walk(g, P, r.pout, [](uint32_t, uint32_t, uint32_t, double) {});
-        c = (ki ? ki : 0) + (ko ? ko - 1 : 0);
-      }
-      counts[s] = c;
+    for (int k = 0; k < 3; ++k) {
+      P[k] = sel_p[3 * s + k];
+      pout[k] = r.pout[k];
+    }
+    o = row_ptr[s];
+    e = row_ptr[s + 1];
+    if (straight) {
+      live = c.begin(g, r.pin, r.pout);
     } else {
-      const uint64_t b = row_ptr[s], e = row_ptr[s + 1];
-      uint64_t o = b;  // list position of the first staged entry
-      int n = 0;       // staged entries
-      auto flush = [&]() {
-        for (int k = 0; k < n; ++k) {
-          if (o + k < e) {
-            vox[o + k] = st_j[k * 256 + tid];
-            len[o + k] = st_l[k * 256 + tid];
-          }
-        }
-        o += n;
-        n = 0;
-      };
-      auto put = [&](uint32_t x, uint32_t y, uint32_t z, double L) {
-        if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) ++nout;
-        st_j[n * 256 + tid] = x + y * nx + z * nxy;  // xyz2index, u32 arithmetic
-        st_l[n * 256 + tid] = L;
-        if (++n == PATH_STAGE) flush();
-      };
-      if (straight) {
-        siddon_walk(g, r.pin, r.pout, put);
-      } else {
-        const uint32_t ki = siddon_walk(g, r.pin, P, put);
-        bool first = true;
-        siddon_walk(g, P, r.pout, [&](uint32_t x, uint32_t y, uint32_t z, double L) {
-          if (first) {
-            first = false;
-            // in.back().length += out[0].length: the in-leg's last entry is
-            // the last one staged, or already written when the stage was
-            // just flushed
-            if (ki) {
-              if (n > 0)
-                st_l[(n - 1) * 256 + tid] += L;
-              else if (o > b)
-                len[o - 1] += L;
-            }
-            return;
-          }
-          put(x, y, z, L);
-        });
+      in_leg = c.begin(g, r.pin, P);
+      live = in_leg;
+      if (!live) {  // an empty in-leg: the out-leg's first entry is dropped
+        merge_next = true;
+        live = c.begin(g, P, pout);
       }
-      flush();
     }
   }
-  if (FILL && nout) atomicAdd(&cnt[C_OUT], (unsigned long long)nout);
+  while (__any(live)) {
+    const int sh = (int)(o % (uint64_t)PATH_STAGE);
+    int cnt = 0;
+    auto put = [&](uint32_t x, uint32_t y, uint32_t z, double L) {
+      if (merge_next) {
+        // in.back().length += out[0].length; out[0] is dropped.  The
+        // in-leg's last entry is staged in this window, or was written by
+        // an earlier flush of this wavefront (then wait for that store)
+        merge_next = false;
+        if (ki) {
+          if (cnt > 0) {
+            st_l[(uint32_t)(sh + cnt - 1) * PATH_BS + tid] += L;
+          } else {
+            __threadfence_block();
+            len[o - 1] += L;
+          }
+        }
+        return;
+      }
+      if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) ++nout;
+      const uint32_t sl = (uint32_t)(sh + cnt) * PATH_BS + tid;
+      st_j[sl] = x + y * nx + z * nxy;  // xyz2index, u32 arithmetic
+      st_l[sl] = L;
+      ++cnt;
+      if (in_leg) ++ki;
+    };
+    while (live && cnt < PATH_STAGE - sh) {
+      c.run((uint32_t)(PATH_STAGE - sh - cnt), put);
+      if (!c.live) {
+        if (in_leg) {
+          in_leg = false;
+          merge_next = true;
+          live = c.begin(g, P, pout);
+        } else {
+          live = false;
+        }
+      }
+    }
+    const uint64_t rem = e > o ? e - o : 0;
+    const int hi = sh + ((uint64_t)cnt < rem ? cnt : (int)rem);
+    const uint64_t ob = o - (uint64_t)sh;
+#pragma unroll
+    for (int rd = 0; rd < PATH_STAGE; ++rd) {
+      const int src = rd * (64 / PATH_STAGE) + lane / PATH_STAGE, k = lane % PATH_STAGE;
+      const int kse = __shfl(sh | (hi << 8), src, 64), ks = kse & 255, ke = kse >> 8;
+      const uint64_t os = (uint64_t)(uint32_t)__shfl((int)(uint32_t)ob, src, 64) |
+                          ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(ob >> 32), src, 64) << 32);
+      if (k >= ks && k < ke) {
+        const uint32_t sl = (uint32_t)k * PATH_BS + wbase + (uint32_t)src;
+        vox[os + k] = st_j[sl];
+        len[os + k] = st_l[sl];
+      }
+    }
+    o += (uint64_t)cnt;
+  }
+  if (nout) atomicAdd(&cnt_out[C_OUT], (unsigned long long)nout);
 }
 
 // Tails T_k = sum of the lengths after entry k (MutoMLSD.cpp:403-414, the
@@ -777,8 +825,8 @@ extern "C" int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     MUTO_CHECK_HIP(ctx, counts.alloc(nsel + 1));
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(counts.p, 0, (nsel + 1) * sizeof(uint64_t), s));
-    k_paths<false><<<grid_for(nsel, 256), 256, 0, s>>>(pl->g, dr.p, sel.p, sel_p.p, nsel,
-        cfg->straight_line, counts.p, nullptr, nullptr, nullptr, nullptr);
+    k_paths_count<<<grid_for(nsel, 256), 256, 0, s>>>(pl->g, dr.p, sel.p, sel_p.p, nsel,
+        cfg->straight_line, counts.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     MUTO_CHECK_HIP(ctx, pl->row_ptr.alloc(nsel + 1));
     MUTO_CHECK_HIP(ctx, exclusive_scan_u64(counts.p, pl->row_ptr.p, nsel + 1, s));
@@ -796,8 +844,8 @@ extern "C" int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
     MUTO_CHECK_HIP(ctx, pl->vox.alloc(pl->nnz));
     MUTO_CHECK_HIP(ctx, pl->len.alloc(pl->nnz));
     MUTO_CHECK_HIP(ctx, pl->tail.alloc(pl->nnz));
-    k_paths<true><<<grid_for(nsel, 256), 256, 0, s>>>(pl->g, dr.p, sel.p, sel_p.p, nsel,
-        cfg->straight_line, nullptr, pl->row_ptr.p, pl->vox.p, pl->len.p, pl->dcount.p);
+    k_paths_fill<<<grid_for(nsel, PATH_BS), PATH_BS, 0, s>>>(pl->g, dr.p, sel.p, sel_p.p, nsel,
+        cfg->straight_line, pl->row_ptr.p, pl->vox.p, pl->len.p, pl->dcount.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     k_tails<<<grid_for(nsel, 256), 256, 0, s>>>(nsel, pl->row_ptr.p, pl->len.p, pl->tail.p,
                                                 pl->dcount.p);

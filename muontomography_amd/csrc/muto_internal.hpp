@@ -114,9 +114,11 @@ hipError_t sum_f64(const double* in, double* out, size_t n, hipStream_t s);
 // Siddon over a batch of segments (siddon.hip)
 hipError_t siddon_count(const GridD& g, const double* p1, const double* p2,
                         size_t n, uint64_t* counts, hipStream_t s);
+// unpackable: null for the unpacked form; else the packed-staging form,
+// which sets *unpackable when a recorded index does not fit 11/11/10 bits
 hipError_t siddon_fill(const GridD& g, const double* p1, const double* p2,
                        size_t n, const uint64_t* row_ptr, uint32_t* vx,
                        uint32_t* vy, uint32_t* vz, double* len,
-                       hipStream_t s);
+                       uint32_t* unpackable, hipStream_t s);
 
 }  // namespace muto

@@ -78,6 +78,8 @@ def _check_siddon(ctx, g, p1, p2):
     make_grid(100, 2.0, -100.0),                 # test/SiddonTest.cpp:11-14
     make_grid((40, 30, 20), (25.0, 10.0, 7.5), (-500.0, -150.0, -80.0)),
     make_grid((7, 1, 3), (1.0, 3.0, 0.5), (0.0, 0.0, 0.0)),
+    # nx above 2047: the fill stages x, y, z unpacked
+    make_grid((2100, 3, 2), (0.1, 60.0, 90.0), (-105.0, -90.0, -90.0)),
 ])
 def test_siddon_bitexact(ctx, grid):
     p1, p2 = _segments_siddon_test(5000)
