@@ -18,7 +18,11 @@
 //     16-entry boundary and is padded to whole 16-entry groups with zero
 //     words, and each group has a descriptor (segment-start mask, the
 //     segment holding its first entry, its first voxel, the length of its
-//     last segment after the group): about 4.6 bytes per entry in all.
+//     last segment after the group): about 4.6 bytes per entry in all;
+//   - every segment starts on a SEG_ALIGN-entry boundary: a segment is
+//     padded to a multiple of SEG_ALIGN entries with padding words (step
+//     code 7, zero length payload; zero FP64 length), so the walks test for
+//     a segment start at one entry in SEG_ALIGN only.
 // The passes over entries are entry-parallel: thread t of a block takes the
 // chunk's group t (16-byte vector loads) and walks it backwards, the tail of
 // each entry inside its segment following from the group's carry; every
@@ -66,6 +70,16 @@ namespace {
 // shape's `block`); a chunk holds up to one segment per thread and GE
 // entries per thread
 constexpr int GE = 16;                   // entries per thread and chunk
+// segments start on multiples of SEG_ALIGN entries (padding words inside
+// the segment's last SEG_ALIGN run): the walks' start tests, voxel selects
+// and first-voxel reads then exist at one entry position in SEG_ALIGN
+constexpr int SEG_ALIGN = 2;
+static_assert(GE % SEG_ALIGN == 0, "groups hold whole alignment runs");
+constexpr uint32_t ENT_PAD_CODE = 7u;
+constexpr uint32_t ENT_PAD = ENT_PAD_CODE << 29;  // a padding word
+__host__ __device__ constexpr uint32_t seg_padded(uint32_t n) {
+  return (n + (uint32_t)SEG_ALIGN - 1u) / (uint32_t)SEG_ALIGN * (uint32_t)SEG_ALIGN;
+}
 constexpr uint32_t DIR_BINS = 8;         // track-order direction bins per unit slope
 // backprojection sums (ACC 1) are int64 fixed point: an entry is at most
 // 2^fx_bits, fx_bits = min(FX_BITS, 62 - bits(max M_j)) per plan; a task
@@ -367,6 +381,12 @@ k_seg_len(uint64_t n_seg, const uint32_t* __restrict__ sval,
   if (q == 0 || (uint32_t)(skey[q - 1] >> 32) != b) bstart[b] = q;
 }
 
+// a segment's length rounded up to whole alignment runs (its stream size)
+__global__ void k_pad_len(uint64_t n_seg, uint32_t* __restrict__ lenq) {
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n_seg) lenq[q] = seg_padded(lenq[q]);
+}
+
 // Track rank and tail after the segment, as unsigned fixed point with
 // 2^-tail_bits mm resolution (tails are >= 0 and below the fixed-point
 // range, which the host sizes from the grid diagonal).
@@ -424,7 +444,10 @@ k_group_desc(const Chunk* __restrict__ chunks, const uint64_t* __restrict__ seg_
     s = k;
   }
   double cr = 0.0;  // in units of U
-  for (uint32_t e = e0 + GE; e < so[s + 1]; ++e) cr += (double)ent_len(ent[ch.base + e]);
+  for (uint32_t e = e0 + GE; e < so[s + 1]; ++e) {
+    const uint32_t w = ent[ch.base + e];
+    if (ent_code(w) != ENT_PAD_CODE) cr += (double)ent_len(w);  // padding: zero length
+  }
   // voxel of entry e0: its segment's first voxel plus the steps up to it
   uint32_t v = seg_v0[ch.q0 + lo];
   for (uint32_t e = so[lo] + 1; e <= e0; ++e) v += ec.step(ent_code(ent[ch.base + e]));
@@ -442,13 +465,14 @@ k_group_desc(const Chunk* __restrict__ chunks, const uint64_t* __restrict__ seg_
 template <uint32_t BS>
 __global__ void __launch_bounds__(BS)
 k_ent_fill(BrickGeo G, const Chunk* __restrict__ chunks, const uint32_t* __restrict__ sval,
-           const uint64_t* __restrict__ st_start, const uint64_t* __restrict__ seg_off_p,
+           const uint64_t* __restrict__ st_start, const uint32_t* __restrict__ st_len,
+           const uint64_t* __restrict__ seg_off_p,
            const uint32_t* __restrict__ vox, const double* __restrict__ len, double inv_unit,
            uint32_t* __restrict__ ent, uint16_t* __restrict__ seg_v0,
            unsigned long long* __restrict__ n_clamped, const double* __restrict__ tail,
            double* __restrict__ len64, double* __restrict__ gtail64) {
   __shared__ uint64_t s_src[BS];
-  __shared__ uint32_t s_off[BS];
+  __shared__ uint32_t s_off[BS], s_len[BS];
   __shared__ uint16_t s_id[GE * BS];
   const Chunk ch = chunks[blockIdx.x];
   const uint32_t t = threadIdx.x;
@@ -458,6 +482,7 @@ k_ent_fill(BrickGeo G, const Chunk* __restrict__ chunks, const uint32_t* __restr
     const uint32_t end = t + 1 < ch.cnt ? (uint32_t)(seg_off_p[q + 1] - ch.base) : ch.ne;
     s_src[t] = st_start[sval[q]];
     s_off[t] = off;
+    s_len[t] = st_len[sval[q]];  // real entries; the rest up to `end` is padding
     for (uint32_t e = off; e < end; ++e) s_id[e] = (uint16_t)t;
   }
   __syncthreads();
@@ -466,6 +491,15 @@ k_ent_fill(BrickGeo G, const Chunk* __restrict__ chunks, const uint32_t* __restr
   for (uint32_t e = t; e < ch.ne; e += BS) {
     const uint32_t i = s_id[e];
     const uint32_t k = e - s_off[i];
+    if (k >= s_len[i]) {
+      // padding of the segment's last alignment run: no step, zero length
+      // (len64 stays 0); a group anchored here takes the tail of the
+      // segment's last real entry (the same tail after a zero length)
+      ent[ch.base + e] = ENT_PAD;
+      if (len64 && ((e % GE) == GE - 1 || e + 1 == ch.ne))
+        gtail64[(ch.base + e) / GE] = tail[s_src[i] + s_len[i] - 1];
+      continue;
+    }
     const uint64_t src = s_src[i] + k;
     const uint32_t j = vox[src];
     uint32_t code = 0;
@@ -645,6 +679,12 @@ __device__ __forceinline__ uint32_t bit_mask(uint32_t m, int i) {
   return (uint32_t)((int)(m << (31 - i)) >> 31);
 }
 
+// all ones when entry i of a group starts a segment (i a constant of the
+// unrolled walk: zero without a test where no segment can start)
+__device__ __forceinline__ uint32_t start_mask(uint32_t m, int i) {
+  return i % SEG_ALIGN == 0 ? bit_mask(m, i) : 0u;
+}
+
 // One thread's group of 16 consecutive entries of a chunk (chunks start on
 // 16-entry boundaries, so the loads are 16-byte vectors), with the group's
 // descriptor: bit i of `mask` is set when entry i starts a segment, `s0` is
@@ -675,9 +715,10 @@ struct Group {
   // voxel (inside the brick) of every entry, handed to sink(i, v): entry 0
   // from the descriptor, a segment start from the chunk's first-voxel
   // table, else the previous voxel plus the step of the entry's code (LDS
-  // table of 8 steps).  Both reads are made for every entry and the choice
-  // is a bitwise select, so the reads of a half group issue together and
-  // the chain is three VALU ops per entry.  Halves are kept apart
+  // table of 8 steps).  At the entries where a segment may start (one in
+  // SEG_ALIGN) both reads are made and the choice is a bitwise select, so
+  // the reads of a half group issue together and the chain is three VALU
+  // ops per entry; elsewhere the chain is one add.  Halves are kept apart
   // (scheduling barrier) so that fewer reads are in flight per thread.
   template <class First, class Sink>
   __device__ __forceinline__ void voxels(const int* stab, First first, Sink sink) const {
@@ -691,14 +732,17 @@ struct Group {
       for (int k = 0; k < GE / 2; ++k) {
         const int i = h + k;
         if (i == 0) continue;
-        f[k] = first(seg(i));
+        if (i % SEG_ALIGN == 0) f[k] = first(seg(i));
         d[k] = stab[ent_code(w[i])];
       }
 #pragma unroll
       for (int k = 0; k < GE / 2; ++k) {
         const int i = h + k;
         if (i == 0) continue;
-        v = bit_select(bit_mask(mask, i), f[k], v + (uint32_t)d[k]);  // a start takes f
+        if (i % SEG_ALIGN == 0)
+          v = bit_select(bit_mask(mask, i), f[k], v + (uint32_t)d[k]);  // a start takes f
+        else
+          v += (uint32_t)d[k];  // no segment starts here
         sink(i, v);
       }
       __builtin_amdgcn_sched_barrier(0);
@@ -792,7 +836,7 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
         for (int i = GE - 1; i >= 0; --i) {
           // W = [[L, L(L/2 + T)], [., L(L^2/3 + L T + T^2)]]
           const float L = ent_len_v(gr.w[i], base_v);
-          const uint32_t st = bit_mask(mk, i);  // entry i starts a segment
+          const uint32_t st = start_mask(mk, i);  // entry i starts a segment
           const float av = lam_e[i] * L;
           const float h = fmaf(0.5f, L, T);
           const float k = fmaf(L, fmaf(L, 1.0f / 3.0f, T), T * T);
@@ -1067,7 +1111,7 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
         for (int i = GE - 1; i >= 0; --i) {
           // <W, G> = L (g00 + (L + 2T) g01 + (L (L/3 + T) + T^2) g11)
           const float L = ent_len_v(gr.w[i], base_v);
-          const uint32_t st = bit_mask(mk, i);  // entry i starts a segment
+          const uint32_t st = start_mask(mk, i);  // entry i starts a segment
           const float k = fmaf(L, fmaf(L, 1.0f / 3.0f, T), T * T);
           const float cv = L * fmaf(k, g.z, fmaf(fmaf(2.0f, T, L), g.y, g.x));
           const uint32_t v = vx[i];
@@ -1239,7 +1283,7 @@ k_fwd64s(IterArgs a, Args64 b, const double* __restrict__ lam, double4* __restri
             d0 = d0 + lj * L;
             d1 = d1 + lj * w01;
             d2 = d2 + lj * w11;
-            if (i > 0 && ((mk >> i) & 1u)) {  // entry i starts segment seg(i)
+            if (i > 0 && i % SEG_ALIGN == 0 && ((mk >> i) & 1u)) {  // entry i starts segment seg(i)
               const uint32_t q = sk + (uint32_t)__popc(mk & ((2u << i) - 2u));
               spc[q] = make_double4(d0, d1, d2, (double)(e0 + i));
               d0 = d1 = d2 = 0.0;
@@ -1413,7 +1457,7 @@ k_bwd64s(IterArgs a, Args64 b, const double4* __restrict__ trk_g, double* __rest
             } else if (sl[i] != 0xFFFFFFFFu) {
               b.cval[sl[i]] = cv;
             }
-            if (i > 0 && ((mk >> i) & 1u)) {  // entry i - 1 ends the segment before
+            if (i > 0 && i % SEG_ALIGN == 0 && ((mk >> i) & 1u)) {  // entry i - 1 ends the segment before
               const uint32_t q = sk + (uint32_t)__popc(mk & ((2u << i) - 2u)) - 1u;
               g = sg[q];
               T = g.w;
@@ -1472,7 +1516,7 @@ k_slot64s(IterArgs a, const uint64_t* __restrict__ col, uint32_t* __restrict__ f
     if (act) {
       const uint32_t e0 = t * GE;
       gr.voxels(stab, [&](uint32_t q) { return sv0[q]; }, [&](int i, uint32_t v) {
-        if (e0 + (uint32_t)i >= ch.ne) return;
+        if (e0 + (uint32_t)i >= ch.ne || ent_code(gr.w[i]) == ENT_PAD_CODE) return;  // padding: no slot
         const uint32_t rr = a.G.db0.div(v), lx = v - rr * a.G.b[0];
         const uint32_t lz = a.G.db1.div(rr), ly = rr - lz * a.G.b[1];
         const size_t j = (size_t)(o[0] + lx) +
@@ -1949,7 +1993,7 @@ k_count32(IterArgs a, uint32_t* __restrict__ mj) {
     if (act) {
       const uint32_t e0 = t * GE;
       gr.voxels(stab, [&](uint32_t q) { return sv0[q]; }, [&](int i, uint32_t v) {
-        if (e0 + (uint32_t)i < ch.ne) atomicAdd(&cnt_s[v], 1u);
+        if (e0 + (uint32_t)i < ch.ne && ent_code(gr.w[i]) != ENT_PAD_CODE) atomicAdd(&cnt_s[v], 1u);
       });
     }
     __syncthreads();
@@ -2222,8 +2266,8 @@ int fp32c_build(muto_em_plan* pl) {
     for (uint64_t b = nbk; b-- > 0;)
       if (hb[b] == ~0ull) hb[b] = hb[b + 1];  // empty brick
     uint64_t n_ent = 0;
-    uint32_t max_len = 0;
-    for (uint32_t v : hl) { n_ent += v; max_len = std::max(max_len, v); }
+    uint32_t max_len = 0;  // padded
+    for (uint32_t v : hl) { n_ent += v; max_len = std::max(max_len, seg_padded(v)); }
     const uint32_t BSZ = f.bs.block, ECAP = BSZ * GE;  // chunk capacity
     if (max_len > ECAP) {
       ctx->set_error("a brick segment is longer than the staging capacity");
@@ -2245,7 +2289,7 @@ int fp32c_build(muto_em_plan* pl) {
       const uint64_t s0 = hb[b], s1 = hb[b + 1];
       if (s1 == s0) continue;
       uint64_t eb = 0;
-      for (uint64_t q = s0; q < s1; ++q) eb += hl[q];
+      for (uint64_t q = s0; q < s1; ++q) eb += seg_padded(hl[q]);
       const uint64_t k = std::max<uint64_t>(1, (eb + target - 1) / target);
       const uint64_t per = (eb + k - 1) / k;
       uint64_t q = s0;
@@ -2260,7 +2304,11 @@ int fp32c_build(muto_em_plan* pl) {
           Chunk ch{};
           ch.q0 = q;
           uint32_t ce = 0, cn = 0;
-          while (q < s1 && cn < chunk_seg_cap(BSZ) && ce + hl[q] <= ECAP) { ce += hl[q]; ++cn; ++q; }
+          while (q < s1 && cn < chunk_seg_cap(BSZ) && ce + seg_padded(hl[q]) <= ECAP) {
+            ce += seg_padded(hl[q]);
+            ++cn;
+            ++q;
+          }
           ch.cnt = cn;
           ch.ne = ce;
           ch.base = n_ent_pad;
@@ -2320,6 +2368,11 @@ int fp32c_build(muto_em_plan* pl) {
                                                         f.with64 ? f.seg_tail64.p : nullptr);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
+    // segment offsets from the padded lengths
+    if (n_seg) {
+      k_pad_len<<<grid_for(n_seg, 256), 256, 0, s>>>(n_seg, lenq.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    }
     MUTO_CHECK_HIP(ctx, hipMemsetAsync(lenq.p + n_seg, 0, sizeof(uint32_t), s));
     MUTO_CHECK_HIP(ctx, exclusive_scan_u32_to_u64(lenq.p, seg_off.p, n_seg + 1, s));
     f.n_ent = n_ent;
@@ -2356,7 +2409,7 @@ int fp32c_build(muto_em_plan* pl) {
       if (f.n_chunks) {
         MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
           constexpr uint32_t BS = decltype(B)::value;
-          k_ent_fill<BS><<<f.n_chunks, BS, 0, s>>>(G, f.chunks.p, sval.p, st_start.p, seg_off_p.p,
+          k_ent_fill<BS><<<f.n_chunks, BS, 0, s>>>(G, f.chunks.p, sval.p, st_start.p, st_len.p, seg_off_p.p,
                                                    pl->vox.p, pl->len.p, inv_unit, f.ent.p,
                                                    f.seg_v0.p, ncl.p, pl->tail.p,
                                                    f.with64 ? f.len64.p : nullptr,
