@@ -135,8 +135,9 @@ int muto_abi_version(void);
 /* GPU time (ms, HIP events on the context's stream) of the kernels of the
  * last muto_siddon_csr (fill pass), muto_siddon_digest, muto_poca_points,
  * muto_poca_reconstruct, muto_pocut_reconstruct or muto_slr_reconstruct call
- * on this context, without the host copies (PoCut: with its host-side list
- * of long voxel lists).  No
+ * on this context, without the host copies (the images: with the calls'
+ * few waits for sizes, and on a context's first such call the allocation
+ * of its workspaces).  No
  * counterpart in the reference, which times whole runs with a stopwatch
  * (MuToTest.cpp:25,80-81). */
 int muto_ctx_kernel_ms(muto_ctx* ctx, double* ms);
@@ -180,7 +181,12 @@ int muto_poca_points(muto_ctx* ctx, const muto_grid* grid, size_t n,
                      double* p_scat, uint32_t* vox, uint8_t* accept,
                      muto_counters* counters);
 
-/* MutoPoCA::reconstruct (MutoPoCA.cpp:51-133).  img: host, nx*ny*nz FP64. */
+/* MutoPoCA::reconstruct (MutoPoCA.cpp:51-133).  img: host, nx*ny*nz FP64.
+ * The weight sums are made per 16^3-voxel tile in LDS from records of
+ * where each leg's walk enters a tile; the context keeps those device
+ * workspaces (about 40 B per tile visit and 120 B per ray: some 8 GB at
+ * 1e7 muons on 200^3 voxels) for its later PoCA, PoCut and SLR calls and
+ * frees them in muto_ctx_destroy. */
 int muto_poca_reconstruct(muto_ctx* ctx, const muto_poca_cfg* cfg, size_t n,
                           const muto_ray* rays, double* img,
                           muto_counters* counters);
@@ -198,7 +204,11 @@ int muto_pocut_reconstruct(muto_ctx* ctx, const muto_poca_cfg* cfg,
 
 /* MutoSLR::reconstruct (src/MutoSLR.cpp:47-103): each track's theta^2 / 2
  * spread along the straight chord between the grid's top and bottom planes
- * (cfg->straight_line is not used). */
+ * (cfg->straight_line is not used).  Tiled as muto_poca_reconstruct; an
+ * entry's value is computed as f L^2 with the track's f = theta^2 / 2 /
+ * ltot^2 (the reference: theta^2 / 2 (L / ltot)^2) and the weight as
+ * (sum L) / dz per voxel (the reference: sum of L / dz), a few ulps per
+ * term apart. */
 int muto_slr_reconstruct(muto_ctx* ctx, const muto_poca_cfg* cfg, size_t n,
                          const muto_ray* rays, double* img,
                          muto_counters* counters);

@@ -44,6 +44,8 @@ extern "C" int muto_ctx_destroy(muto_ctx* ctx) {
   }
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  for (void* w : ctx->ws)
+    if (w) (void)hipFree(w);
   delete ctx;
   return MUTO_OK;
 }

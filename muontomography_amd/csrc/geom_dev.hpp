@@ -134,6 +134,65 @@ struct SiddonCursor {
     }
     return cnt;
   }
+
+  // The same steps as run(), until emit(x, y, z, length) returns false
+  // (that voxel is then not taken and the cursor is not used again) or the
+  // walk ends.
+  template <class Emit>
+  __device__ __forceinline__ void run_while(Emit&& emit) {
+    const double eps = 1.0e-6;
+    while (live) {
+      if (!(alpha < amax && it < cap)) {
+        live = false;
+        break;
+      }
+      ++it;
+      const uint32_t cx = x, cy = y, cz = z;
+      if (ax[0] < ax[1] && ax[0] < ax[2]) {
+        x += (uint32_t)di[0]; alpha = ax[0]; ax[0] += da[0];
+      } else if (ax[1] < ax[2]) {
+        y += (uint32_t)di[1]; alpha = ax[1]; ax[1] += da[1];
+      } else {
+        z += (uint32_t)di[2]; alpha = ax[2]; ax[2] += da[2];
+      }
+      const double dalpha = smin(alpha, amax) - alpha_last;
+      if (eps < dalpha) {
+        if (!emit(cx, cy, cz, dalpha * length)) return;
+        alpha_last = alpha;
+      }
+    }
+  }
+
+  // Inside emit(cx, cy, cz, ...) of run(): the plane crossings as they were
+  // before the step that recorded (cx, cy, cz).  Only the stepped axis's
+  // index moved, and its crossing was taken as alpha before the increment
+  // (a step along an axis the segment does not cross moves nothing and
+  // adds nothing).  alpha_last is still the value before that step.
+  __device__ __forceinline__ void snapshot(uint32_t cx, uint32_t cy, uint32_t cz,
+                                           double* ax_pre) const {
+    ax_pre[0] = x != cx ? alpha : ax[0];
+    ax_pre[1] = y != cy ? alpha : ax[1];
+    ax_pre[2] = z != cz ? alpha : ax[2];
+  }
+
+  // Right after begin() on the same segment: continue from a snapshot, so
+  // that the next recorded voxel is (cx, cy, cz) with the same arithmetic
+  // as the walk that took the snapshot.  The step count follows from the
+  // index moves (every step before the last moves one index by one), and
+  // the loop test of the first step passes as it did there (alpha_last is
+  // at most the alpha of the step before, which was below amax).
+  __device__ __forceinline__ void resume(uint32_t cx, uint32_t cy, uint32_t cz,
+                                         const double* ax_pre, double al) {
+    it = (cx - x) * (uint32_t)di[0] + (cy - y) * (uint32_t)di[1] + (cz - z) * (uint32_t)di[2];
+    x = cx;
+    y = cy;
+    z = cz;
+    ax[0] = ax_pre[0];
+    ax[1] = ax_pre[1];
+    ax[2] = ax_pre[2];
+    alpha = al;
+    alpha_last = al;
+  }
 };
 
 // The whole walk in one call; returns the number of recorded voxels.

@@ -28,6 +28,29 @@ struct muto_ctx {
   // read by muto_ctx_kernel_ms
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool ev_valid = false, ev_begun = false;
+  // device workspaces of the one-shot image calls (PoCA, PoCut, SLR), kept
+  // for the next call and grown when one needs more; freed with the context
+  void* ws[2] = {nullptr, nullptr};
+  size_t ws_bytes[2] = {0, 0};
+  hipError_t workspace(int k, size_t bytes, void** out) {
+    if (bytes > ws_bytes[k]) {
+      if (ws[k]) {
+        const hipError_t e = hipFree(ws[k]);
+        ws[k] = nullptr;
+        ws_bytes[k] = 0;
+        if (e != hipSuccess) return e;
+      }
+      const size_t b = bytes + bytes / 4;
+      const hipError_t e = hipMalloc(&ws[k], b);
+      if (e != hipSuccess) {
+        ws[k] = nullptr;
+        return e;
+      }
+      ws_bytes[k] = b;
+    }
+    *out = ws[k];
+    return hipSuccess;
+  }
   void set_error(const std::string& m) { err = m; }
   void mark_begin() {
     ev_valid = false;
@@ -109,7 +132,20 @@ hipError_t exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out,
                                      size_t n, hipStream_t s);
 hipError_t sort_pairs_u64_u32(uint64_t* keys, uint32_t* vals, size_t n,
                               int begin_bit, int end_bit, hipStream_t s);
+hipError_t sort_pairs_u32_u32(uint32_t* keys, uint32_t* vals, size_t n,
+                              int begin_bit, int end_bit, hipStream_t s);
 hipError_t sum_f64(const double* in, double* out, size_t n, hipStream_t s);
+// Workspace forms: no allocation and no synchronisation.  With temp ==
+// nullptr they only set *temp_bytes.  The sorts leave the result in (keys,
+// vals) or, when *in_alt, in (keys_alt, vals_alt).
+hipError_t sort_pairs_u64_u32_ws(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
+                                 uint32_t* vals_alt, size_t n, int begin_bit, int end_bit,
+                                 void* temp, size_t* temp_bytes, bool* in_alt, hipStream_t s);
+hipError_t sort_pairs_u32_u32_ws(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                                 uint32_t* vals_alt, size_t n, int begin_bit, int end_bit,
+                                 void* temp, size_t* temp_bytes, bool* in_alt, hipStream_t s);
+hipError_t exclusive_scan_u32_to_u64_ws(const uint32_t* in, uint64_t* out, size_t n, void* temp,
+                                        size_t* temp_bytes, hipStream_t s);
 
 // Siddon over a batch of segments (siddon.hip)
 hipError_t siddon_count(const GridD& g, const double* p1, const double* p2,

@@ -7,7 +7,9 @@
     the brick stream) against the oracle's grad Phi on the host;
   64 independent scans (32^3-128^3 grids of 10-30 mm voxels, 1e4-1e6 tracks
     each) in one batch for 30 iterations, each scan against its own FP64
-    plan voxel by voxel.
+    plan voxel by voxel;
+  the PoCA, PoCut and SLR images of the 200^3 Siddon workload's grid (2e6 of
+    its muons) against the oracle.
 
 Relative image errors are taken over voxels that tracks cross (M_j > 0) and
 that the FP64 image holds above zero; max and RMS are printed with each
@@ -97,6 +99,38 @@ def test_map128_gradient_against_oracle(ctx):
         sv.close()
     plan.close()
     lean.close()
+
+
+def test_tiled_images_200_grid_against_oracle(ctx):
+    """The PoCA, PoCut and SLR images at the 200^3 x 10 mm grid of the
+    Siddon workload with 2e6 of its muons (a tile then holds up to about
+    3e4 leg records, so the busy tiles are cut into several tasks whose
+    sums meet in the image by atomics): each within 1e-9 of the oracle
+    (the FP64 sums' order and atan2 ulps, as in the small-set tests), the
+    counters equal; the second call on the context (workspaces reused)
+    gives the same images bit for bit."""
+    g, rays, _ = W.muons("siddon200", n_muons=2_000_000)
+    for lw in (1, 0):
+        cfg = M.poca_cfg(g, length_weighted=lw, straight_line=0)
+        io, co = oracle.poca_reconstruct(cfg, rays)
+        ig, cg = ctx.poca_reconstruct(cfg, rays)
+        for k in ("n_scat_in_grid", "n_not_parallel", "n_out_of_grid"):
+            assert co[k] == cg[k], (lw, k)
+        assert np.array_equal(io == 0, ig == 0)
+        np.testing.assert_allclose(ig, io, rtol=1e-9, atol=0)
+        ig2, _ = ctx.poca_reconstruct(cfg, rays)
+        assert np.array_equal(ig2, ig) or np.allclose(ig2, ig, rtol=1e-13, atol=0)
+        print(f"poca200 lw={lw}: {cg['n_scat_in_grid']} rays in the grid, max rel "
+              f"{float(np.max(np.abs(ig - io) / np.maximum(np.abs(io), 1e-300))):.2e}", flush=True)
+    cfg = M.poca_cfg(g, length_weighted=1)
+    io, _ = oracle.pocut_reconstruct(cfg, rays, 0.9)
+    ig, _ = ctx.pocut_reconstruct(cfg, rays, 0.9)
+    np.testing.assert_allclose(ig, io, rtol=1e-9, atol=0)
+    io, co = oracle.slr_reconstruct(cfg, rays)
+    ig, cg = ctx.slr_reconstruct(cfg, rays)
+    assert co["n_out_of_grid"] == cg["n_out_of_grid"]
+    assert np.array_equal(io == 0, ig == 0)
+    np.testing.assert_allclose(ig, io, rtol=1e-9, atol=0)
 
 
 def test_scans64_batch_against_each_fp64_plan(ctx):
