@@ -457,6 +457,34 @@ def bench_map_grad(args):
         t = (time.perf_counter() - t0) / args.steps
         res[name] = (t, gr, sv.phi)
         log(f"{name}: {t * 1e3:.2f} ms per value+gradient")
+    # the same with lambda and the gradient in device memory (no host
+    # copies: MUTO_MAP_DEVICE_BUFFERS), and the FP64 call that allocates
+    # nothing (muto_map_value_grad_noalloc)
+    from muontomography_amd.devmem import DeviceArray
+    lam_d = DeviceArray.from_numpy(lam)
+    grad_d = DeviceArray(g.n_vox)
+    phi_d = DeviceArray(1)
+    dev = {}
+    for name, pl, prec, rc in (("fp32c", plan, M.MUTO_FP32C, False), ("fp64", plan, M.MUTO_FP64, False),
+                               ("fp64_recompute", lean, M.MUTO_FP64, True)):
+        for k in range(args.steps + 1):
+            if k == 1:
+                t0 = time.perf_counter()
+            sv = pl.map_forward(lam_d, prec, recompute=rc)
+            sv.backward(1.0, out=grad_d)
+            sv.close()
+        dev[name] = (time.perf_counter() - t0) / args.steps
+        log(f"{name} on device buffers: {dev[name] * 1e3:.2f} ms per value+gradient")
+    for name, pl, rc in (("fp64_noalloc", plan, False), ("fp64_recompute_noalloc", lean, True)):
+        for k in range(args.steps + 1):
+            if k == 1:
+                t0 = time.perf_counter()
+            pl.map_value_grad_noalloc(lam_d, grad_d, phi_d, recompute=rc)
+        dev[name] = (time.perf_counter() - t0) / args.steps
+        log(f"{name}: {dev[name] * 1e3:.2f} ms per value+gradient")
+    g_dev = grad_d.numpy()
+    for a in (lam_d, grad_d, phi_d):
+        a.free()
     g64 = res["fp64"][1]
     out = {"metric": "MAP objective value+gradient evaluations/s",
            "value": 1.0 / res["fp32c"][0], "unit": "evaluations/s", "n_gpus": 1,
@@ -465,6 +493,9 @@ def bench_map_grad(args):
            "config": {"workload": "MAP gradient, 128^3 grid of 15 mm voxels, 1e6 muons",
                       "entries": int(nnz)},
            "ms": {k: v[0] * 1e3 for k, v in res.items()},
+           "ms_device_buffers": {k: v * 1e3 for k, v in dev.items()},
+           "device_value": 1.0 / dev["fp32c"],
+           "grad_err_noalloc_recompute_vs_fp64_norm_inf": float(np.abs(g_dev - g64).max() / np.abs(g64).max()),
            "entries_per_s": {k: 2 * nnz / v[0] for k, v in res.items()},
            "grad_err_fp32c_vs_fp64_norm_inf": float(np.abs(res["fp32c"][1] - g64).max() / np.abs(g64).max()),
            "grad_err_recompute_vs_fp64_norm_inf": float(np.abs(res["fp64_recompute"][1] - g64).max() /

@@ -594,20 +594,25 @@ __global__ void k_copy_f64_to_f32(size_t n, const double* __restrict__ a,
 // ------------------------------------------------------------------ //
 // MAP voxel step (src/MutoMAP.cpp:162-184, 207-407), one lane / voxel //
 // ------------------------------------------------------------------ //
-__device__ double map_surrogate(const MapCfgD& c, uint32_t mj, double ljn,
-                                double ljml, double lj, const double* w,
-                                const double* lm, int nn) {
+// pw[k] = pow(t_k^2 + delta^2, (p - 2) / 2) of each neighbour, the factor
+// the coefficients and every candidate share (computed once per voxel;
+// the expressions are the reference's, in its order)
+template <int NN>
+__device__ __forceinline__ double map_surrogate(const MapCfgD& c, uint32_t mj, double ljn,
+                                                double ljml, double lj, const double (&lm)[NN],
+                                                const double (&pw)[NN], double pw_obj) {
   double value = 0.0;  // surrogateFunctionValue, MutoMAP.cpp:261-281
   if (lj < 0) return 1.7976931348623157e308;
   value += 0 < lj ? (double)mj * (log(lj) + ljml / lj) : 0.0;
-  const double ex = (c.p - 2.0) / 2.0, dd = c.delta * c.delta;
-  for (int k = 0; k < nn; ++k) {
-    const double t = ljn - lm[k];
+  const double w2 = 1.0 / sqrt(2.0), w3 = 1.0 / sqrt(3.0);
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    const double w = k < 6 ? 1.0 : (k < 18 ? w2 : w3);
     const double u = 2.0 * lj - ljn - lm[k];
-    value += 0.5 * c.beta * c.p * w[k] * pow(t * t + dd, ex) * (u * u);
+    value += 0.5 * c.beta * c.p * w * pw[k] * (u * u);
   }
   if (c.obj_prior)
-    value += 0.5 * c.beta * c.p * lj * lj * pow(ljn * ljn + dd, ex);
+    value += 0.5 * c.beta * c.p * lj * lj * pw_obj;
   return value;
 }
 
@@ -644,6 +649,7 @@ __device__ uint32_t cubic_solve(double* roots, double p3, double p2, double p1,
   return 3;
 }
 
+template <int NN>
 __global__ void __launch_bounds__(256)
 k_map_step(MapCfgD c, size_t nv, const uint32_t* __restrict__ mj,
            const double* __restrict__ last, const double* __restrict__ ml,
@@ -656,17 +662,21 @@ k_map_step(MapCfgD c, size_t nv, const uint32_t* __restrict__ mj,
       out[j] = 0.0;
     } else {
       double p0 = -ml[j] * MAP_SCALE, p1 = 1.0, p2 = 0.0, p3 = 0.0;
-      double w[26], lm[26];
-      const int nn = map_neighbours(c, last, (uint32_t)j, w, lm);
+      // the neighbours' l and pow factors in registers (NN fixed, loops
+      // unrolled)
+      double lm[NN], pw[NN];
       const double lj = last[j] * MAP_SCALE;
       const double ex = (c.p - 2.0) / 2.0, dd = c.delta * c.delta;
-      for (int k = 0; k < nn; ++k) {
+      for_each_neighbour<NN>(c, (uint32_t)j, [&](int k, size_t idx, double w) {
+        lm[k] = last[idx] * MAP_SCALE;
         const double t = lj - lm[k];
-        const double t1 = w[k] * c.p * pow(t * t + dd, ex);
+        pw[k] = pow(t * t + dd, ex);
+        const double t1 = w * c.p * pw[k];
         p2 += t1 * 2.0 * (lj + lm[k]);
         p3 += t1 * 4.0;
-      }
-      if (c.obj_prior) p3 += c.p * pow(lj * lj + dd, ex);
+      });
+      const double pw_obj = c.obj_prior ? pow(lj * lj + dd, ex) : 0.0;
+      if (c.obj_prior) p3 += c.p * pw_obj;
       p2 *= -c.beta / (double)m;
       p3 *= c.beta / (double)m;
       double roots[3];
@@ -676,7 +686,7 @@ k_map_step(MapCfgD c, size_t nv, const uint32_t* __restrict__ mj,
       uint32_t bi = 0;
       for (uint32_t r = 0; r <= nr; ++r) {
         const double cand = r < nr ? roots[r] : ljml;
-        const double f = map_surrogate(c, m, ljn, ljml, cand, w, lm, nn);
+        const double f = map_surrogate<NN>(c, m, ljn, ljml, cand, lm, pw, pw_obj);
         if (r == 0 || f < best) { best = f; bi = r; }
       }
       out[j] = bi < nr ? roots[bi] / MAP_SCALE : ml[j];
@@ -1103,8 +1113,13 @@ static int iterate64(muto_em_plan* pl, double* dst, bool time_it) {
 namespace muto {
 int map_step(muto_em_plan* pl, const double* last, const double* ml, double* out) {
   muto_ctx* ctx = pl->ctx;
-  k_map_step<<<grid_for(pl->nv, 256), 256, 0, ctx->stream>>>(map_cfg(pl), pl->nv, pl->mj.p, last, ml,
-                                                            out, pl->dcount.p);
+  const MapCfgD mc = map_cfg(pl);
+  if (mc.all_nb)
+    k_map_step<26><<<grid_for(pl->nv, 256), 256, 0, ctx->stream>>>(mc, pl->nv, pl->mj.p, last, ml, out,
+                                                                   pl->dcount.p);
+  else
+    k_map_step<6><<<grid_for(pl->nv, 256), 256, 0, ctx->stream>>>(mc, pl->nv, pl->mj.p, last, ml, out,
+                                                                  pl->dcount.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   return MUTO_OK;
 }

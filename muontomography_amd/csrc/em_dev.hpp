@@ -99,45 +99,35 @@ struct MapCfgD {
 
 #define MAP_SCALE 1.0e7
 
-// clamp-to-edge neighbours of voxel j (index2xyz, MutoMAP.cpp:283-290;
-// getSixNeighbours / getAllNeighbours, :292-350) and their weights
-// (1, 1/sqrt2, 1/sqrt3, :210-212); returns 6 or 26
-__device__ __forceinline__ int map_neighbour_index(const MapCfgD& c, uint32_t j, size_t* idx,
-                                                   double* w) {
+// The clamp-to-edge neighbours of voxel j (index2xyz, MutoMAP.cpp:283-290;
+// getSixNeighbours / getAllNeighbours, :292-350, in their order) with their
+// weights (1, 1/sqrt2, 1/sqrt3, :210-212): one call f(k, index, weight)
+// each, NN (6 or 26) fixed at compile time, so that the offsets fold into
+// the unrolled loop and nothing is kept in per-lane arrays (which the
+// compiler would place in scratch memory).
+template <int NN, class F>
+__device__ __forceinline__ void for_each_neighbour(const MapCfgD& c, uint32_t j, F&& f) {
   const uint32_t nx = c.g.n[0], ny = c.g.n[1], nz = c.g.n[2];
   const uint32_t x = j % (nx * ny) % nx;
   const uint32_t y = j % (nx * ny) / nx;
   const uint32_t z = j / (nx * ny);
-  const uint32_t xp = x < nx - 1 ? x + 1 : nx - 1;
-  const uint32_t yp = y < ny - 1 ? y + 1 : ny - 1;
-  const uint32_t zp = z < nz - 1 ? z + 1 : nz - 1;
-  const uint32_t xm = 0 < x ? x - 1 : 0;
-  const uint32_t ym = 0 < y ? y - 1 : 0;
-  const uint32_t zm = 0 < z ? z - 1 : 0;
-  const uint32_t nb[26][3] = {
-      {xp, y, z},   {xm, y, z},   {x, yp, z},   {x, ym, z},   {x, y, zp},
-      {x, y, zm},   {xp, yp, z},  {xm, yp, z},  {xp, ym, z},  {xm, ym, z},
-      {x, yp, zp},  {x, ym, zp},  {x, yp, zm},  {x, ym, zm},  {xp, y, zp},
-      {xm, y, zp},  {xp, y, zm},  {xm, y, zm},  {xp, yp, zp}, {xm, yp, zp},
-      {xp, ym, zp}, {xm, ym, zp}, {xp, yp, zm}, {xm, yp, zm}, {xp, ym, zm},
-      {xm, ym, zm}};
-  const int nn = c.all_nb ? 26 : 6;
+  const uint32_t X[3] = {0 < x ? x - 1 : 0, x, x < nx - 1 ? x + 1 : nx - 1};
+  const uint32_t Y[3] = {0 < y ? y - 1 : 0, y, y < ny - 1 ? y + 1 : ny - 1};
+  const uint32_t Z[3] = {0 < z ? z - 1 : 0, z, z < nz - 1 ? z + 1 : nz - 1};
+  // the offsets (x, y, z) as indices 0 (-1), 1 (0), 2 (+1)
+  constexpr int8_t O[26][3] = {
+      {2, 1, 1}, {0, 1, 1}, {1, 2, 1}, {1, 0, 1}, {1, 1, 2}, {1, 1, 0}, {2, 2, 1},
+      {0, 2, 1}, {2, 0, 1}, {0, 0, 1}, {1, 2, 2}, {1, 0, 2}, {1, 2, 0}, {1, 0, 0},
+      {2, 1, 2}, {0, 1, 2}, {2, 1, 0}, {0, 1, 0}, {2, 2, 2}, {0, 2, 2}, {2, 0, 2},
+      {0, 0, 2}, {2, 2, 0}, {0, 2, 0}, {2, 0, 0}, {0, 0, 0}};
   const double w2 = 1.0 / sqrt(2.0), w3 = 1.0 / sqrt(3.0);
-  for (int k = 0; k < nn; ++k) {
-    w[k] = k < 6 ? 1.0 : (k < 18 ? w2 : w3);
-    idx[k] = (size_t)nb[k][0] + (size_t)nx * ((size_t)nb[k][1] + (size_t)ny * nb[k][2]);
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    const size_t idx = (size_t)X[O[k][0]] + (size_t)nx * ((size_t)Y[O[k][1]] + (size_t)ny * Z[O[k][2]]);
+    f(k, idx, k < 6 ? 1.0 : (k < 18 ? w2 : w3));
   }
-  return nn;
 }
 
-// the neighbours' l = 1e7 lambda (MutoMAP.cpp:207-237)
-__device__ __forceinline__ int map_neighbours(const MapCfgD& c, const double* __restrict__ img,
-                                              uint32_t j, double* w, double* lm) {
-  size_t idx[26];
-  const int nn = map_neighbour_index(c, j, idx, w);
-  for (int k = 0; k < nn; ++k) lm[k] = img[idx[k]] * MAP_SCALE;
-  return nn;
-}
 
 MapCfgD map_cfg(const muto_em_plan* pl);
 

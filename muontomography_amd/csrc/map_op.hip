@@ -264,16 +264,17 @@ __device__ __forceinline__ double dpsi(double t, double p, double dd) {
   return p * t * pow(t * t + dd, (p - 2.0) / 2.0);
 }
 
-// prior R(lambda), summed over all voxels
+// prior R(lambda), summed over all voxels (NN: 6 or 26 neighbours)
+template <int NN>
 __global__ void __launch_bounds__(256)
 k_prior_phi(MapCfgD c, size_t nv, const double* __restrict__ lam, double* __restrict__ part) {
   size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0.0;
   if (j < nv) {
-    double w[26], lm[26];
-    const int nn = map_neighbours(c, lam, (uint32_t)j, w, lm);
     const double lj = lam[j] * MAP_SCALE, dd = c.delta * c.delta;
-    for (int k = 0; k < nn; ++k) v += w[k] * psi(lj - lm[k], c.p, dd);
+    for_each_neighbour<NN>(c, (uint32_t)j, [&](int, size_t idx, double w) {
+      v += w * psi(lj - lam[idx] * MAP_SCALE, c.p, dd);
+    });
     if (c.obj_prior) v += psi(lj, c.p, dd);
   }
   block_sum_to(part, v);
@@ -289,24 +290,37 @@ k_sum_parts(const double* __restrict__ part, size_t n, double* __restrict__ out)
 
 // grad += scale dR/dlambda (scale = grad_phi beta): each pair term
 // w psi(l_j - l_k) adds to j and subtracts from k (FP64 atomics)
+template <int NN>
 __global__ void __launch_bounds__(256)
 k_prior_grad(MapCfgD c, size_t nv, const double* __restrict__ lam, double scale,
              double* __restrict__ grad) {
   size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nv) return;
-  size_t idx[26];
-  double w[26];
-  const int nn = map_neighbour_index(c, (uint32_t)j, idx, w);
   const double lj = lam[j] * MAP_SCALE, dd = c.delta * c.delta;
   const double s = scale * MAP_SCALE;
   double own = 0.0;
-  for (int k = 0; k < nn; ++k) {
-    const double dv = w[k] * dpsi(lj - lam[idx[k]] * MAP_SCALE, c.p, dd);
+  for_each_neighbour<NN>(c, (uint32_t)j, [&](int, size_t idx, double w) {
+    const double dv = w * dpsi(lj - lam[idx] * MAP_SCALE, c.p, dd);
     own += s * dv;
-    atomicAdd(&grad[idx[k]], -s * dv);
-  }
+    atomicAdd(&grad[idx], -s * dv);
+  });
   if (c.obj_prior) own += s * dpsi(lj, c.p, dd);
   atomicAdd(&grad[j], own);
+}
+
+// the prior kernels for the plan's neighbourhood
+static void prior_phi(const MapCfgD& c, size_t nv, const double* lam, double* part, hipStream_t s) {
+  if (c.all_nb)
+    k_prior_phi<26><<<grid_for(nv, 256), 256, 0, s>>>(c, nv, lam, part);
+  else
+    k_prior_phi<6><<<grid_for(nv, 256), 256, 0, s>>>(c, nv, lam, part);
+}
+static void prior_grad(const MapCfgD& c, size_t nv, const double* lam, double scale, double* grad,
+                       hipStream_t s) {
+  if (c.all_nb)
+    k_prior_grad<26><<<grid_for(nv, 256), 256, 0, s>>>(c, nv, lam, scale, grad);
+  else
+    k_prior_grad<6><<<grid_for(nv, 256), 256, 0, s>>>(c, nv, lam, scale, grad);
 }
 
 __global__ void k_scale(size_t n, double a, double* __restrict__ x) {
@@ -396,7 +410,7 @@ extern "C" int muto_map_forward(muto_em_plan* pl, const double* lambda, int prec
     }
   }
   double prior = 0.0;
-  k_prior_phi<<<grid_for(nv, 256), 256, 0, s>>>(map_cfg(pl), nv, sv->lam.p, sv->part.p);
+  prior_phi(map_cfg(pl), nv, sv->lam.p, sv->part.p, s);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   k_sum_parts<<<1, 256, 0, s>>>(sv->part.p, grid_for(nv, 256), sv->sum.p + 1);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
@@ -447,8 +461,7 @@ extern "C" int muto_map_backward(muto_map_saved* sv, double grad_phi, double* gr
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
   }
-  k_prior_grad<<<grid_for(nv, 256), 256, 0, s>>>(map_cfg(pl), nv, sv->lam.p, grad_phi * pl->cfg.beta,
-                                                 grad);
+  prior_grad(map_cfg(pl), nv, sv->lam.p, grad_phi * pl->cfg.beta, grad, s);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (!sv->dev)
     MUTO_CHECK_HIP(ctx, hipMemcpyAsync(grad_lambda, grad, nv * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -553,11 +566,11 @@ extern "C" int muto_map_value_grad_noalloc(muto_em_plan* pl, const double* lambd
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
   const MapCfgD mc = map_cfg(pl);
-  k_prior_phi<<<grid_for(nv, 256), 256, 0, s>>>(mc, nv, lambda, part_p);
+  prior_phi(mc, nv, lambda, part_p, s);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   k_sum_parts<<<1, 256, 0, s>>>(part_p, grid_for(nv, 256), sums + 1);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
-  k_prior_grad<<<grid_for(nv, 256), 256, 0, s>>>(mc, nv, lambda, pl->cfg.beta, grad_lambda);
+  prior_grad(mc, nv, lambda, pl->cfg.beta, grad_lambda, s);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   k_phi_combine<<<1, 64, 0, s>>>(sums, pl->cfg.beta, phi);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
