@@ -238,42 +238,61 @@ k_paths_fill(GridD g, const muto_ray* __restrict__ rays,
 }
 
 // Tails T_k = sum of the lengths after entry k (MutoMLSD.cpp:403-414, the
-// reference's backward running sum, same order), one lane per track, read
-// and written TAIL_BURST entries at a time so that a lane's loads and
-// stores of one burst share lines.
-constexpr int TAIL_BURST = 16;
-__global__ void __launch_bounds__(256)
+// reference's backward running sum, same order), one lane per track.  Each
+// lane takes its list from the end in windows that stop at 16-entry
+// boundaries of the list array; the wavefront reads every lane's window
+// into LDS together (16 lanes per window, whole aligned runs), each lane
+// turns its own window's lengths into tails in place, and the wavefront
+// writes them back the same way.
+constexpr int TAIL_STAGE = 16;
+constexpr uint32_t TAIL_BS = 256;
+__global__ void __launch_bounds__(TAIL_BS)
 k_tails(size_t n_trk, const uint64_t* __restrict__ row_ptr,
         const double* __restrict__ len, double* __restrict__ tail,
         unsigned long long* __restrict__ cnt) {
-  size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double st[TAIL_STAGE * TAIL_BS];  // slot k of thread t at k * TAIL_BS + t
+  const uint32_t tid = threadIdx.x;
+  const int lane = (int)(tid & 63u);
+  const uint32_t wbase = tid & ~63u;
+  const size_t s = (size_t)blockIdx.x * blockDim.x + tid;
+  uint64_t b = 0, p = 0;  // entries [b, p) still to do
   bool empty = false;
   if (s < n_trk) {
-    const uint64_t b = row_ptr[s], e = row_ptr[s + 1];
-    empty = (e == b);
-    if (e > b) {
-      double run = 0.0;  // tail[k] while k walks down
-      tail[e - 1] = 0.0;
-      uint64_t k = e - 1;  // entries k-1 .. b still to write
-      while (k - b >= (uint64_t)TAIL_BURST) {
-        double l[TAIL_BURST];
+    b = row_ptr[s];
+    p = row_ptr[s + 1];
+    empty = (p == b);
+  }
+  double run = 0.0;  // the tail of entry p - 1
+  while (__any(p > b)) {
+    // the window [lo, p): within the aligned run holding entry p - 1
+    const bool act = p > b;
+    const uint64_t base = act ? (p - 1) / TAIL_STAGE * TAIL_STAGE : 0;
+    const uint64_t lo = act ? (base > b ? base : b) : 0;
+    const int ks = act ? (int)(lo - base) : 0, ke = act ? (int)(p - base) : 0;
 #pragma unroll
-        for (int m = 0; m < TAIL_BURST; ++m) l[m] = len[k - m];
-        double t[TAIL_BURST];
-#pragma unroll
-        for (int m = 0; m < TAIL_BURST; ++m) {
-          run = run + l[m];  // tail[k-m-1] = tail[k-m] + len[k-m]
-          t[m] = run;
-        }
-#pragma unroll
-        for (int m = 0; m < TAIL_BURST; ++m) tail[k - m - 1] = t[m];
-        k -= TAIL_BURST;
-      }
-      for (; k > b; --k) {
-        run = run + len[k];
-        tail[k - 1] = run;
+    for (int r = 0; r < TAIL_STAGE; ++r) {
+      const int src = r * (64 / TAIL_STAGE) + lane / TAIL_STAGE, k = lane % TAIL_STAGE;
+      const int kse = __shfl(ks | (ke << 8), src, 64), k0 = kse & 255, k1 = kse >> 8;
+      const uint64_t bs = (uint64_t)(uint32_t)__shfl((int)(uint32_t)base, src, 64) |
+                          ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(base >> 32), src, 64) << 32);
+      if (k >= k0 && k < k1) st[(uint32_t)k * TAIL_BS + wbase + (uint32_t)src] = len[bs + k];
+    }
+    if (act) {
+      for (int k = ke - 1; k >= ks; --k) {  // tail[k - 1] = tail[k] + len[k]
+        const double l = st[(uint32_t)k * TAIL_BS + tid];
+        st[(uint32_t)k * TAIL_BS + tid] = run;
+        run = run + l;
       }
     }
+#pragma unroll
+    for (int r = 0; r < TAIL_STAGE; ++r) {
+      const int src = r * (64 / TAIL_STAGE) + lane / TAIL_STAGE, k = lane % TAIL_STAGE;
+      const int kse = __shfl(ks | (ke << 8), src, 64), k0 = kse & 255, k1 = kse >> 8;
+      const uint64_t bs = (uint64_t)(uint32_t)__shfl((int)(uint32_t)base, src, 64) |
+                          ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(base >> 32), src, 64) << 32);
+      if (k >= k0 && k < k1) tail[bs + k] = st[(uint32_t)k * TAIL_BS + wbase + (uint32_t)src];
+    }
+    p = lo;
   }
   wave_count(&cnt[C_EMPTY], empty);
 }
@@ -857,7 +876,7 @@ extern "C" int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
     k_paths_fill<<<grid_for(nsel, PATH_BS), PATH_BS, 0, s>>>(pl->g, dr.p, sel.p, sel_p.p, nsel,
         cfg->straight_line, pl->row_ptr.p, pl->vox.p, pl->len.p, pl->dcount.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
-    k_tails<<<grid_for(nsel, 256), 256, 0, s>>>(nsel, pl->row_ptr.p, pl->len.p, pl->tail.p,
+    k_tails<<<grid_for(nsel, TAIL_BS), TAIL_BS, 0, s>>>(nsel, pl->row_ptr.p, pl->len.p, pl->tail.p,
                                                 pl->dcount.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
@@ -916,7 +935,7 @@ extern "C" int muto_em_plan_create_prepared(
     MUTO_CHECK_HIP(ctx, copy_sync(ctx, pl->len.p, len, nnz * sizeof(double), hipMemcpyHostToDevice));
   }
   if (n_trk) {
-    k_tails<<<grid_for(n_trk, 256), 256, 0, s>>>(n_trk, pl->row_ptr.p, pl->len.p, pl->tail.p,
+    k_tails<<<grid_for(n_trk, TAIL_BS), TAIL_BS, 0, s>>>(n_trk, pl->row_ptr.p, pl->len.p, pl->tail.p,
                                                  pl->dcount.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
