@@ -25,6 +25,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+# LDS int64 atomic adds: 5.9 lane-ops per cycle per CU (a microbenchmark on
+# this GPU), 256 CUs at 2.4 GHz
+LDS_ATOMIC_PEAK = 5.9 * 256 * 2.4e9
 BYTES_PER_UPDATE_FP32C = 8.0  # u32 voxel index + f32 length per entry
 BYTES_PER_UPDATE_FP64 = 12.0  # u32 voxel index + f64 length per entry
 
@@ -678,6 +681,12 @@ def main():
                    "parallelism": "1 GPU"},
         "kernel_ms": {k: round(v, 4) for k, v in tk.items()},
         "roofline": roof,
+        # the backprojection's scatter-adds (one int64 fixed-point LDS atomic
+        # per entry) against the LDS atomic rate a microbenchmark measured
+        # on this GPU (ds_add_u64: 5.9 lane-ops per cycle per CU)
+        "scatter": {"ops_per_s": nnz / (tk["backward"] * 1e-3), "kind": "ds_add_u64 (LDS)",
+                    "peak_ops_per_s": LDS_ATOMIC_PEAK,
+                    "frac": nnz / (tk["backward"] * 1e-3) / LDS_ATOMIC_PEAK},
         "cpu_baseline": cpu,
     }
     out.update(extra)
