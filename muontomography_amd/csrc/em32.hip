@@ -629,6 +629,35 @@ struct IterArgs {
   EntCode ec;
 };
 
+// Thread t's segment of a chunk, and its group.  Threads past the chunk's
+// segments (groups) read the chunk's last segment (first group) again and
+// never use it: the loads are then unconditional, so the compiler counts
+// exactly which of them a later use has to wait for, and the repeated
+// addresses add no memory traffic (a wavefront's equal addresses are one
+// request).
+__device__ __forceinline__ uint64_t seg_slot(const IterArgs& a, const Chunk& ch, uint32_t t) {
+  const uint32_t last = ch.cnt > 0 ? ch.cnt - 1 : 0u;
+  return min(ch.q0 + min(t, last), a.n_seg - 1);
+}
+__device__ __forceinline__ uint64_t group_slot(const Chunk& ch, uint32_t t) {
+  return ch.base / GE + (t * GE < ch.ne ? t : 0u);
+}
+
+// A chunk's descriptor, read through the constant address space: with a
+// wave-uniform index that is a scalar load, which takes no vector registers
+// and is not ordered with the vector loads, so the forward pass issues the
+// next chunk's while it walks the current one.
+__device__ __forceinline__ Chunk chunk_at(const IterArgs& a, uint32_t c) {
+  typedef const __attribute__((address_space(4))) Chunk* ConstChunkPtr;
+  const ConstChunkPtr p = (ConstChunkPtr)a.chunks + c;
+  Chunk r;
+  r.q0 = p->q0;
+  r.base = p->base;
+  r.cnt = p->cnt;
+  r.ne = p->ne;
+  return r;
+}
+
 __device__ __forceinline__ void brick_origin(const BrickGeo& G, uint32_t brick,
                                              uint32_t* o) {
   const uint32_t bx = brick % G.nb[0];
@@ -701,11 +730,16 @@ struct Group {
       const uint4 v = pw[k];
       w[4 * k] = v.x; w[4 * k + 1] = v.y; w[4 * k + 2] = v.z; w[4 * k + 3] = v.w;
     }
-    const uint32_t d = a.gdesc[gi];
-    mask = d & 0xFFFFu;
-    s0 = d >> 16;
+    mask = a.gdesc[gi];  // raw until decode()
     carry = a.gcarry[gi];
     v0 = a.gv0[gi];
+  }
+  // Splits the descriptor word into the start mask and the first segment.
+  // Kept apart from load() so that a pass can issue its other loads before
+  // it waits for the group's.
+  __device__ __forceinline__ void decode() {
+    s0 = mask >> 16;
+    mask &= 0xFFFFu;
   }
   __device__ __forceinline__ bool start(int i) const { return (mask >> i) & 1u; }
   // chunk segment of entry i: s0 plus the segment starts in 1..i
@@ -801,20 +835,30 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
     const uint32_t ti = sh;
     if (ti >= a.n_tasks) break;
     const Task tk = a.tasks[ti];
-    load_brick(a.G, tk.brick, lam, lds_lam);
-    __syncthreads();
+    load_brick(a.G, tk.brick, lam, lds_lam);  // (the first chunk's barrier orders it)
+    Chunk nx = chunk_at(a, tk.c0);
+    // a segment partial of the chunk before, stored once the next chunk's
+    // loads are issued (so that the next chunk's address arithmetic does not
+    // wait for the store to read its registers)
+    float4 pend = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint64_t pend_q = ~0ull;
     for (uint32_t c = tk.c0; c < tk.c1; ++c) {
-      const Chunk ch = a.chunks[c];
-      uint32_t tfx = 0;
-      if (t < ch.cnt) {
-        tfx = a.seg_tfx[ch.q0 + t];
-        sv0[t] = a.seg_v0[ch.q0 + t];
-      }
+      // One memory round trip per chunk: the chunk's descriptor came with
+      // the chunk before, and its segment words and groups load together.
+      const Chunk ch = nx;
+      if (c + 1 < tk.c1) nx = chunk_at(a, c + 1);
+      const uint64_t q = seg_slot(a, ch, t);
+      const uint32_t tfx = a.seg_tfx[q];
+      const uint32_t v0w = a.seg_v0[q];
       const bool act = t * GE < ch.ne;
       Group gr;
-      if (act) gr.load(a, (ch.base / GE) + t);
+      gr.load(a, group_slot(ch, t));
+      if (pend_q != ~0ull) part[pend_q] = pend;
+      pend_q = ~0ull;
+      if (t < ch.cnt) sv0[t] = v0w;  // waits for the segment words only
       __syncthreads();
       if (act) {
+        gr.decode();
         float lam_e[GE];
         gr.voxels(stab, [&](uint32_t q) { return sv0[q]; },
                   [&](int i, uint32_t v) { lam_e[i] = lds_lam[v]; });
@@ -875,10 +919,11 @@ k_fwd32(IterArgs a, const float* __restrict__ lam, float4* __restrict__ part) {
           y += hp.y;
           z += hp.z;
         }
-        part[ch.q0 + t] = make_float4((float)(x * U), (float)(y * U2), (float)(z * U3),
-                                      __uint_as_float(tfx));
+        pend = make_float4((float)(x * U), (float)(y * U2), (float)(z * U3), __uint_as_float(tfx));
+        pend_q = ch.q0 + t;
       }
     }
+    if (pend_q != ~0ull) part[pend_q] = pend;
   }
 }
 
@@ -1081,6 +1126,9 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
     if (ti >= a.n_tasks) break;
     const Task tk = a.tasks[ti];
     for (uint32_t c0 = tk.c0; c0 < tk.c1; c0 += SUB) {
+      // (the chunk's descriptor, then its segment words and the G' gather,
+      // then the stream's groups: measured faster here than issuing the
+      // groups first or the descriptor a chunk ahead)
       Chunk ch{};  // no chunk (cnt = ne = 0) past the task's last
       if (c0 + sub < tk.c1) ch = a.chunks[c0 + sub];
       float4* sgb = sg + sub * TAB;
@@ -1100,6 +1148,7 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
       if (act) gr.load(a, (ch.base / GE) + tl);
       __syncthreads();
       if (act) {
+        gr.decode();
         uint32_t vx[GE];
         gr.voxels(stab, [&](uint32_t q) { return __float_as_uint(sgb[q].w); },
                   [&](int i, uint32_t v) { vx[i] = v; });
@@ -1254,6 +1303,7 @@ k_fwd64s(IterArgs a, Args64 b, const double* __restrict__ lam, double4* __restri
       if (act) gr.load(a, gi);
       __syncthreads();
       if (act) {
+        gr.decode();
         uint32_t vx[GE];
         gr.voxels(stab, [&](uint32_t q) { return sv0[q]; }, [&](int i, uint32_t v) { vx[i] = v; });
         uint32_t e0 = t * GE, mk = gr.mask, sk = gr.s0;
@@ -1419,6 +1469,7 @@ k_bwd64s(IterArgs a, Args64 b, const double4* __restrict__ trk_g, double* __rest
       if (act) gr.load(a, gi);
       __syncthreads();
       if (act) {
+        gr.decode();
         uint32_t vx[GE];
         gr.voxels(stab, [&](uint32_t q) { return sv0[q]; }, [&](int i, uint32_t v) { vx[i] = v; });
         uint32_t mk = gr.mask, sk = gr.s0;
@@ -1514,6 +1565,7 @@ k_slot64s(IterArgs a, const uint64_t* __restrict__ col, uint32_t* __restrict__ f
     if (act) gr.load(a, (ch.base / GE) + t);
     __syncthreads();
     if (act) {
+      gr.decode();
       const uint32_t e0 = t * GE;
       gr.voxels(stab, [&](uint32_t q) { return sv0[q]; }, [&](int i, uint32_t v) {
         if (e0 + (uint32_t)i >= ch.ne || ent_code(gr.w[i]) == ENT_PAD_CODE) return;  // padding: no slot
@@ -1991,6 +2043,7 @@ k_count32(IterArgs a, uint32_t* __restrict__ mj) {
     if (act) gr.load(a, (ch.base / GE) + t);
     __syncthreads();
     if (act) {
+      gr.decode();
       const uint32_t e0 = t * GE;
       gr.voxels(stab, [&](uint32_t q) { return sv0[q]; }, [&](int i, uint32_t v) {
         if (e0 + (uint32_t)i < ch.ne && ent_code(gr.w[i]) != ENT_PAD_CODE) atomicAdd(&cnt_s[v], 1u);
