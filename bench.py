@@ -95,13 +95,16 @@ def _rel(a, ref, mj):
             float(np.abs(a - ref).max() / np.abs(ref).max()))
 
 
-def bench_fp64(args, plan, rays, cfg_kw, init, nnz, wall32, cost32):
+def bench_fp64(args, plan, rays, cfg_kw, init, nnz, wall32, cost32, brick32):
     """The FP64 paths beside FP32C, each for N_COMPARE iterations from the
     same image: the reference-order kernel (one lane per track), the
     wavefront-per-track kernel, and the FP64 brick stream; the FP32C and
     FP64-stream images against the reference-order one.  Then the
     reference's default update rule, the median (src/MutoMLSD.cpp:505-515):
-    reference-order FP64 for 3 iterations beside the FP64 stream."""
+    reference-order FP64 for 3 iterations beside the FP64 stream and FP32C
+    (c_ij from FP32 entry terms in each voxel's slot, the median selected
+    among the FP32 values), and FP32C against the FP64 stream for
+    N_COMPARE iterations."""
     import muontomography_amd as M
     n = N_COMPARE
     res = {}
@@ -199,9 +202,31 @@ def bench_fp64(args, plan, rays, cfg_kw, init, nnz, wall32, cost32):
     med.set_image(init)
     med.run(args.timing_steps, M.MUTO_FP64)
     tm = med.timing()
-    med.close()
+    med.set_option(M.MUTO_OPT_TIMING, 0)
+    med.set_image(init)
+    med.run(n, M.MUTO_FP64)
+    imgm_s = med.image()
     log(f"median: reference order {wm_ref * 1e3:.1f} ms/it, FP64 stream {wm * 1e3:.2f} ms/it "
         f"(phases {tm}); err after {nm} it {em_}")
+    # FP32C median on the FP32C brick shape
+    med.set_option(M.MUTO_OPT_BRICK, M.brick_code(brick32))
+    med.set_image(init)
+    med.run(nm, M.MUTO_FP32C)  # stream, slots, graph
+    e32m = _rel(med.image(), imgm_ref, mj)
+    med.set_image(init)
+    med.run(n, M.MUTO_FP32C)
+    e32ms = _rel(med.image(), imgm_s, mj)
+    med.set_image(init)
+    t0 = time.perf_counter()
+    med.run(args.steps, M.MUTO_FP32C)
+    wm32 = (time.perf_counter() - t0) / args.steps
+    med.set_option(M.MUTO_OPT_TIMING, 1)
+    med.set_image(init)
+    med.run(args.timing_steps, M.MUTO_FP32C)
+    tm32 = med.timing()
+    med.close()
+    log(f"median FP32C: {wm32 * 1e3:.2f} ms/it (phases {tm32}); err after {nm} it vs reference order "
+        f"{e32m}, after {n} it vs the FP64 stream {e32ms}")
     ms32 = wall32 / args.steps * 1e3
     # the FP64 stream's iteration against its HBM roofline (12 B per update)
     bw64s = BYTES_PER_UPDATE_FP64 * nnz / (t64s["iteration"] * 1e-3)
@@ -234,7 +259,13 @@ def bench_fp64(args, plan, rays, cfg_kw, init, nnz, wall32, cost32):
                                 "frac": BYTES_PER_UPDATE_FP64 * nnz / wm / HBM_PEAK},
                    "fp64_stream_value": nnz / wm, "speedup": wm_ref / wm,
                    "fp64_stream_kernel_ms": {k: round(v, 4) for k, v in tm.items()},
-                   "iterations_compared": nm, "err_max_rel": em_[0], "err_rms_rel": em_[1]},
+                   "iterations_compared": nm, "err_max_rel": em_[0], "err_rms_rel": em_[1],
+                   "fp32c_ms_per_step": wm32 * 1e3, "fp32c_value": nnz / wm32,
+                   "fp32c_kernel_ms": {k: round(v, 4) for k, v in tm32.items()},
+                   "fp32c_speedup_vs_fp64": wm_ref / wm32, "fp32c_speedup_vs_fp64_stream": wm / wm32,
+                   "fp32c_err_vs_fp64_max_rel": e32m[0], "fp32c_err_vs_fp64_rms_rel": e32m[1],
+                   "fp32c_vs_fp64_stream_iterations": n, "fp32c_err_vs_fp64_stream_max_rel": e32ms[0],
+                   "fp32c_err_vs_fp64_stream_rms_rel": e32ms[1]},
     }
 
 
@@ -650,7 +681,7 @@ def main():
 
     extra = {}
     if not args.no_fp64:
-        extra = bench_fp64(args, plan, rays, cfg_kw, init, nnz, wall, cost32)
+        extra = bench_fp64(args, plan, rays, cfg_kw, init, nnz, wall, cost32, brick)
     plan.close()
     ctx.close()
 

@@ -55,9 +55,11 @@ extern "C" {
 /* arithmetic of the EM / MAP iterations */
 #define MUTO_FP64 0   /* reference order, FP64 throughout                    */
 #define MUTO_FP32C 1  /* FP64 Siddon, 2x2 algebra and sums; FP32 entry terms;
-                         the sums are FP64 registers and int64 fixed-point
-                         LDS atomics by default, FP32 with two-sum
-                         compensation under MUTO_OPT_ACC_MODE 0          */
+                         the mean's sums are FP64 registers and int64
+                         fixed-point LDS atomics by default, FP32 with
+                         two-sum compensation under MUTO_OPT_ACC_MODE 0;
+                         the median and trimmed mean select among the
+                         FP32 c_ij of each voxel                        */
 #define MUTO_FP64_STREAM 2  /* FP64 on the brick stream (MUTO_OPT_FP64_KERNEL 2):
                                the reference's W_ij bit for bit, the sums in
                                another association; every update rule     */

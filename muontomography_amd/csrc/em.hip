@@ -1172,11 +1172,6 @@ static int plan_run(muto_em_plan* pl, int iters, int precision, double* cost) {
     return MUTO_ERR_ARG;
   }
   const muto_mlsd_cfg& c = pl->cfg;
-  if (precision == MUTO_FP32C && !(c.update_with_mean && c.mean_percentile == 100)) {
-    ctx->set_error("MUTO_FP32C runs the mean update (update_with_mean, mean_percentile 100); "
-                   "median and trimmed mean run in MUTO_FP64");
-    return MUTO_ERR_ARG;
-  }
   if (pl->cur_prec < 0) { ctx->set_error("muto_em_plan_run: no image set"); return MUTO_ERR_STATE; }
   if (!pl->has_lists) {
     ctx->set_error("muto_em_plan_run: the plan keeps no path lists (they exceeded its list budget); "

@@ -943,7 +943,7 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
           const uint32_t* __restrict__ tseg_pos,
           const float4* __restrict__ part, const double* __restrict__ trk_d,
           ErrM E, double tail_scale, double lmax, double ubound, int grad_mode,
-          float* __restrict__ trk_g, unsigned long long* __restrict__ cmax,
+          float* __restrict__ trk_g, double4* __restrict__ trk_g64, unsigned long long* __restrict__ cmax,
           const uint16_t* __restrict__ trk_scan,
           double* __restrict__ cost_acc, unsigned long long* __restrict__ cnt) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1020,6 +1020,11 @@ k_track32(size_t n_trk, const uint32_t* __restrict__ tseg_ptr,
     if (grad_mode && sing) g.x = g.y = g.z = 0.0f;  // cost_i = 0: no gradient
     g.w = __uint_as_float(tcx);
     *reinterpret_cast<float4*>(trk_g + 4 * r) = g;
+    if (trk_g64) {  // the order statistics' backprojection: G' before its rounding to FP32
+      const bool zero = grad_mode && sing;
+      trk_g64[r] = make_double4(zero ? 0.0 : pp * (0.5 * M00 - i00), zero ? 0.0 : pp * (0.5 * M01 - i01),
+                                zero ? 0.0 : pp * (0.5 * M11 - i11), (double)tcx);
+    }
     // bound on |c_ij| over the track's entries for the backprojection's
     // fixed-point sums: an entry lies at most R = max|tau_q - t_c| + ubound
     // from t_c and is at most lmax long, so |<W, G'>| <= lmax (|g00| +
@@ -1542,6 +1547,79 @@ k_bwd64s(IterArgs a, Args64 b, const double4* __restrict__ trk_g, double* __rest
   }
 }
 
+// Order-statistic backprojection, FP32C (median, trimmed mean): c_ij =
+// <W_ij, G_i> per entry from the stream's FP32 lengths, with the track's
+// FP64 G' moved to the segment's basis and the tail and the product in FP64
+// (a selection takes single values, whose errors do not average out over
+// the voxel's entries as the mean's sums do), rounded once to FP32 and
+// written to the entry's voxel-major slot (the FP64 stream's slots: the two streams
+// share their layout) for the selection that follows.  Tasks of one chunk
+// from one queue per XCD, brick by brick, so that a brick's slot range
+// stays in that XCD's L2 while its scattered writes land.  No voxel chain:
+// the slot names the destination.
+template <uint32_t BS>
+__global__ void __launch_bounds__(BS)
+k_bwd32s(IterArgs a, Args64 b, const double4* __restrict__ trk_g, float* __restrict__ cval) {
+  __shared__ double4 sg[BS];
+  __shared__ uint32_t sh_[4];
+  uint32_t& sh = sh_[0];
+  const uint32_t t = threadIdx.x;
+  const double U = a.ec.unit, U2 = U * U, U3 = U2 * U;
+  const uint32_t qi = blockIdx.x % b.nq;
+  for (;;) {
+    if (t == 0) sh = b.qoff[qi] + atomicAdd(&b.qwork[qi], 1u);
+    __syncthreads();
+    const uint32_t ti = sh;
+    if (ti >= b.qoff[qi + 1]) break;
+    const Task tk = b.qtask[ti];
+    for (uint32_t c = tk.c0; c < tk.c1; ++c) {
+      const Chunk ch = a.chunks[c];
+      if (t < ch.cnt) {
+        const uint64_t q = ch.q0 + t;
+        const uint32_t tfx = a.seg_tfx[q];
+        const double4 g = trk_g[a.seg_trk[q]];
+        // A^T G' A with A = [[1, 0], [s, 1]], s = tau_q - t_c (exact), in
+        // units of U for the entry terms, all in FP64: rounding G' before
+        // this shift would be amplified where s is large
+        const double shf = (double)((int64_t)tfx - (int64_t)g.w) * a.inv_scale;
+        const double gy = fma(shf, g.z, g.y);
+        sg[t] = make_double4(fma(shf, gy + g.y, g.x) * U, gy * U2, g.z * U3, 0.0);
+      }
+      const bool act = t * GE < ch.ne;
+      const uint64_t gi = (ch.base / GE) + t;
+      Group gr;
+      if (act) gr.load(a, gi);
+      __syncthreads();
+      if (act) {
+        gr.decode();
+        uint32_t sl[GE];  // slots of the group's entries (padding: none, all ones)
+        const uint4* ps = reinterpret_cast<const uint4*>(b.slot + gi * GE);
+#pragma unroll
+        for (int k = 0; k < GE / 4; ++k) {
+          const uint4 v4 = ps[k];
+          sl[4 * k] = v4.x; sl[4 * k + 1] = v4.y; sl[4 * k + 2] = v4.z; sl[4 * k + 3] = v4.w;
+        }
+        double T = gr.carry;
+        uint32_t mk = gr.mask, sk = gr.s0, base_v = ENT_BASE;  // opaque copies, as in k_bwd32
+        asm volatile("" : "+v"(mk), "+v"(sk), "+v"(base_v));
+        double4 g = sg[sk + (uint32_t)__popc(mk & 0xFFFEu)];
+#pragma unroll
+        for (int i = GE - 1; i >= 0; --i) {
+          // the FP32 length; the tail and <W, G> in FP64
+          const double L = ent_len_v(gr.w[i], base_v);
+          const uint32_t st = start_mask(mk, i);  // entry i starts a segment
+          const double k = fma(L, fma(L, 1.0 / 3.0, T), T * T);
+          const double cv = L * fma(k, g.z, fma(fma(2.0, T, L), g.y, g.x));
+          if (sl[i] != 0xFFFFFFFFu) cval[sl[i]] = (float)cv;
+          if (i > 0 && st != 0u) g = sg[sk + (uint32_t)__popc(mk & ((2u << i) - 2u)) - 1u];
+          T = st != 0u ? 0.0 : T + L;
+        }
+      }
+      __syncthreads();  // the table is rewritten for the next chunk
+    }
+  }
+}
+
 // Slot of every stream entry in the voxel-major c_ij array (one block per
 // task, the passes' voxel chain): slot = col[j] + a running count per
 // voxel.  The order inside a voxel does not matter to the median or to
@@ -1820,15 +1898,15 @@ __device__ __forceinline__ bool wave_select_pivot(const double (&v)[R], uint32_t
   return true;
 }
 
-template <int R>
-__device__ __forceinline__ double wave_update(const double* __restrict__ src, uint32_t m, double l,
+template <int R, class V>
+__device__ __forceinline__ double wave_update(const V* __restrict__ src, uint32_t m, double l,
                                               int use_mean, int pct, double p, double* piv) {
   const uint32_t lane = threadIdx.x & 63u;
   double v[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t e = r * 64u + lane;
-    v[r] = e < m ? src[e] : INFINITY;
+    v[r] = e < m ? (double)src[e] : INFINITY;  // (FP32C values widen exactly)
   }
   double out;
   if (wave_select_pivot<R>(v, m, l, use_mean, pct, p, &out, piv)) return out;
@@ -1840,10 +1918,12 @@ __device__ __forceinline__ double wave_update(const double* __restrict__ src, ui
 // from the previous pass's pivot when it is close, else by a sort in
 // registers; the update written in place (each voxel reads only its own
 // lambda) and the new pivot kept.  M_j = 0 gives 0 (src/MutoMLSD.cpp:
-// 494-496).  MAP: lambda_ML to ml.
+// 494-496).  MAP: lambda_ML to ml.  V: double (FP64 stream) or float
+// (FP32C) values of c.
+template <class V>
 __global__ void __launch_bounds__(256)
 k_sel64w(size_t nv, const uint32_t* __restrict__ mj, const uint64_t* __restrict__ col,
-         const double* __restrict__ cval, int use_mean, int pct, double* __restrict__ lam,
+         const V* __restrict__ cval, int use_mean, int pct, double* __restrict__ lam,
          double* __restrict__ ml, double* __restrict__ pivot) {
   const size_t nw = (size_t)gridDim.x * (blockDim.x >> 6);
   for (size_t j = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; j < nv; j += nw) {
@@ -1855,7 +1935,7 @@ k_sel64w(size_t nv, const uint32_t* __restrict__ mj, const uint64_t* __restrict_
       continue;
     } else {
       const double l = lam[j];
-      const double* src = cval + col[j];
+      const V* src = cval + col[j];
       const double p = pivot[j];
       if (m <= 64)
         out = wave_update<1>(src, m, l, use_mean, pct, p, &piv);
@@ -1894,10 +1974,11 @@ __device__ void block_bitonic64(double* a, uint32_t n) {
   }
 }
 
+template <class V>
 __global__ void __launch_bounds__(1024)
 k_sel64b(uint32_t n_list, const uint32_t* __restrict__ list, const uint64_t* __restrict__ off,
          const uint32_t* __restrict__ mj, const uint64_t* __restrict__ col,
-         const double* __restrict__ cval, int use_mean, int pct, double* __restrict__ scratch,
+         const V* __restrict__ cval, int use_mean, int pct, double* __restrict__ scratch,
          double* __restrict__ lam, double* __restrict__ ml) {
   __shared__ double a_s[4096];
   for (uint32_t b = blockIdx.x; b < n_list; b += gridDim.x) {
@@ -1906,8 +1987,8 @@ k_sel64b(uint32_t n_list, const uint32_t* __restrict__ list, const uint64_t* __r
     uint32_t n = 1;
     while (n < m) n <<= 1;
     double* a = n <= 4096 ? a_s : scratch + off[b];
-    const double* src = cval + col[j];
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) a[i] = i < m ? src[i] : INFINITY;
+    const V* src = cval + col[j];
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) a[i] = i < m ? (double)src[i] : INFINITY;
     __syncthreads();
     block_bitonic64(a, n);
     if (threadIdx.x == 0) {
@@ -1995,10 +2076,13 @@ __global__ void k_fx_to_f64(size_t nv, double* __restrict__ acc,
 // clears the iteration's cost sum and per-iteration counters
 __global__ void k_iter_begin(uint32_t n_scans, double* __restrict__ cost_acc,
                              unsigned long long* __restrict__ cnt,
-                             unsigned long long* __restrict__ cmax) {
+                             unsigned long long* __restrict__ cmax, unsigned int* __restrict__ work,
+                             unsigned int* __restrict__ qwork, uint32_t nq) {
   for (uint32_t k = threadIdx.x; k < n_scans; k += blockDim.x) cost_acc[k] = 0.0;
   if (threadIdx.x < 3) cnt[threadIdx.x] = 0ull;
   if (threadIdx.x == 0) cmax[0] = 0ull;  // the track kernel sets this pass's |c| bound
+  if (threadIdx.x < 2) work[threadIdx.x] = 0u;
+  if (qwork && threadIdx.x < nq) qwork[threadIdx.x] = 0u;  // order statistics: per-XCD queues
 }
 
 // largest M_j (a grid-stride loop per thread, one atomic per block)
@@ -2558,19 +2642,28 @@ int fp32c_build(muto_em_plan* pl) {
   return MUTO_OK;
 }
 
+static bool mean_all(const muto_em_plan* pl);
+static Args64 args64(muto_em_plan* pl);
+static int fp64s_slots(muto_em_plan* pl);
+
 // One FP32C iteration enqueued on the plan's stream with no host
 // synchronisation and no memset, so it can be captured into a graph: a
 // kernel clears the per-iteration sums and counters, the last kernel appends
 // the iteration's cost sum to the device cost history.  With events (5),
-// kernel boundaries are recorded for per-kernel timing.
+// kernel boundaries are recorded for per-kernel timing.  The mean update
+// backprojects into per-voxel sums (k_bwd32, k_upd32); the median and the
+// trimmed mean write each c_ij to its voxel's slot (k_bwd32s) and select
+// per voxel (k_sel64w / k_sel64b on the float values), as the FP64 stream
+// does.
 static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
   Fp32cData& f = pl->f32;
   muto_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream;
   const IterArgs a = iter_args(pl);
   const ErrM E = error_matrix(pl->cfg);
-  const bool map = pl->is_map;
-  k_iter_begin<<<1, 256, 0, s>>>(pl->n_scans, pl->cost_acc.p, pl->dcount.p, f.cmax.p);
+  const bool map = pl->is_map, mean = mean_all(pl);
+  k_iter_begin<<<1, 256, 0, s>>>(pl->n_scans, pl->cost_acc.p, pl->dcount.p, f.cmax.p, pl->work.p,
+                                 mean ? nullptr : f.qwork64.p, f.nq64);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[0], s));
   if (f.n_tasks)
@@ -2579,18 +2672,46 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
   k_track32<<<grid_for(pl->n_trk, 256), 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p,
                                                       reinterpret_cast<const float4*>(f.seg_part.p),
                                                       f.trk_d.p, E, f.tail_scale, f.lmax, f.ubound, 0,
-                                                      f.trk_g.p, f.cmax.p,
-                                                      pl->n_scans > 1 ? f.trk_scan.p : nullptr,
+                                                      f.trk_g.p, mean ? nullptr : reinterpret_cast<double4*>(f.trk_g64.p),
+                                                      f.cmax.p, pl->n_scans > 1 ? f.trk_scan.p : nullptr,
                                                       pl->cost_acc.p, pl->dcount.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[2], s));
-  if (f.n_tasks) MUTO_CHECK_HIP(ctx, launch_bwd(f, a, f.trk_g.p, f.cmax.p, pl->acc64.p, s));
+  float* cval32 = reinterpret_cast<float*>(f.cval.p);  // order statistics: the slots' values
+  if (f.n_tasks && mean) MUTO_CHECK_HIP(ctx, launch_bwd(f, a, f.trk_g.p, f.cmax.p, pl->acc64.p, s));
+  if (f.n_tasks && !mean) {
+    const Args64 b = args64(pl);
+    MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
+      constexpr uint32_t BS = decltype(B)::value;
+      k_bwd32s<BS><<<f.grid_bwd32s, BS, 0, s>>>(a, b, reinterpret_cast<const double4*>(f.trk_g64.p), cval32);
+      return hipGetLastError();
+    }));
+  }
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[3], s));
-  k_upd32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->acc64.p, pl->lam64.p,
-                                                pl->lam32.p, map ? pl->ml64.p : nullptr,
-                                                pl->work.p, f.cmax.p, f.acc_mode == 1 ? 1 : 0,
-                                                f.fx_bits);
-  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  if (mean) {
+    k_upd32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->acc64.p, pl->lam64.p,
+                                                  pl->lam32.p, map ? pl->ml64.p : nullptr,
+                                                  pl->work.p, f.cmax.p, f.acc_mode == 1 ? 1 : 0,
+                                                  f.fx_bits);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  } else {
+    const muto_mlsd_cfg& c = pl->cfg;
+    double* ml = map ? pl->ml64.p : nullptr;
+    const unsigned g = (unsigned)std::min<size_t>(grid_for(pl->nv, 4), (size_t)ctx->n_cu * 64);
+    k_sel64w<float><<<g, 256, 0, s>>>(pl->nv, pl->mj.p, f.col64.p, cval32, c.update_with_mean,
+                                      c.mean_percentile, pl->lam64.p, ml, f.pivot64.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+    if (f.n_big64) {
+      k_sel64b<float><<<std::min<uint32_t>(f.n_big64, 4096), 1024, 0, s>>>(
+          f.n_big64, f.big64.p, f.big64_off.p, pl->mj.p, f.col64.p, cval32, c.update_with_mean,
+          c.mean_percentile, f.big64_scr.p, pl->lam64.p, ml);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    }
+    if (!map) {  // the forward pass reads the float mirror
+      k_mirror32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->lam64.p, pl->lam32.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    }
+  }
   if (map) {
     if (int rc = map_step(pl, pl->lam64.p, pl->ml64.p, pl->tmp64.p)) return rc;
     k_commit<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->tmp64.p, pl->lam64.p, pl->lam32.p);
@@ -2607,7 +2728,7 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
 static int fp32c_graph(muto_em_plan* pl) {
   Fp32cData& f = pl->f32;
   muto_ctx* ctx = pl->ctx;
-  if (f.graph && f.graph_map == pl->is_map) return MUTO_OK;
+  if (f.graph && f.graph_map == pl->is_map && f.graph_mean == mean_all(pl)) return MUTO_OK;
   if (f.graph) (void)hipGraphExecDestroy(f.graph);
   f.graph = nullptr;
   hipGraph_t g = nullptr;
@@ -2623,6 +2744,7 @@ static int fp32c_graph(muto_em_plan* pl) {
   (void)hipGraphDestroy(g);
   MUTO_CHECK_HIP(ctx, ei);
   f.graph_map = pl->is_map;
+  f.graph_mean = mean_all(pl);
   return MUTO_OK;
 }
 
@@ -2650,6 +2772,23 @@ int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done) {
   MUTO_CHECK_HIP(ctx, hipMemsetAsync(f.hist_n.p, 0, sizeof(unsigned int), s));
   if (!pl->is_map && iters > 0) {
     k_zero_uncrossed<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->lam64.p, pl->lam32.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+  }
+  if (!mean_all(pl)) {
+    // median / trimmed mean: the FP64 stream's voxel-major slots and per-XCD
+    // queues (built once per stream), float values in its c_ij array; every
+    // run starts without pivots
+    if (int rc = fp64s_slots(pl)) return rc;
+    if (!f.trk_g64.p) MUTO_CHECK_HIP(ctx, f.trk_g64.alloc(4 * pl->n_trk));
+    if (f.grid_bwd32s == 0) {
+      int ob = 1;
+      MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
+        constexpr uint32_t BS = decltype(B)::value;
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob, k_bwd32s<BS>, BS, 0);
+      }));
+      f.grid_bwd32s = std::max(1, ob) * ctx->n_cu;
+    }
+    k_fill_nan<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, f.pivot64.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   }
   const bool graph = pl->use_graph && !pl->timing;
@@ -2941,11 +3080,11 @@ static int fp64s_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
   } else {
     const muto_mlsd_cfg& c = pl->cfg;
     const unsigned g = (unsigned)std::min<size_t>(grid_for(pl->nv, 4), (size_t)ctx->n_cu * 64);
-    k_sel64w<<<g, 256, 0, s>>>(pl->nv, pl->mj.p, f.col64.p, f.cval.p, c.update_with_mean,
+    k_sel64w<double><<<g, 256, 0, s>>>(pl->nv, pl->mj.p, f.col64.p, f.cval.p, c.update_with_mean,
                                c.mean_percentile, pl->lam64.p, ml, f.pivot64.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     if (f.n_big64) {
-      k_sel64b<<<std::min<uint32_t>(f.n_big64, 4096), 1024, 0, s>>>(
+      k_sel64b<double><<<std::min<uint32_t>(f.n_big64, 4096), 1024, 0, s>>>(
           f.n_big64, f.big64.p, f.big64_off.p, pl->mj.p, f.col64.p, f.cval.p, c.update_with_mean,
           c.mean_percentile, f.big64_scr.p, pl->lam64.p, ml);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
@@ -3076,7 +3215,7 @@ int fp32c_map_forward(muto_em_plan* pl, const double* lam, float* g_out,
   k_track32<<<grid_for(pl->n_trk, 256), 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p,
                                                       reinterpret_cast<const float4*>(f.seg_part.p),
                                                       f.trk_d.p, E, f.tail_scale, f.lmax, f.ubound, 1,
-                                                      f.trk_g.p, f.cmax.p,
+                                                      f.trk_g.p, nullptr, f.cmax.p,
                                                       pl->n_scans > 1 ? f.trk_scan.p : nullptr,
                                                       pl->cost_acc.p, pl->dcount.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());

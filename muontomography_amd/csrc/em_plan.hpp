@@ -81,6 +81,8 @@ struct Fp32cData {
   bool acc_mode_set = false;  // set through the option call (else environment)
   hipGraphExec_t graph = nullptr;  // one iteration, captured
   bool graph_map = false;          // the MAP flag the graph was captured with
+  bool graph_mean = true;          // ... and the update rule (mean, or an order statistic)
+  int grid_bwd32s = 0;             // persistent grid of the order statistics' backprojection
   // FP64 stream (the brick stream with FP64 lengths and tails), built with
   // the stream when with64 is set
   bool with64 = false;
@@ -124,6 +126,7 @@ struct Fp32cData {
     seg_part64.release();
     trk_g64.release();
     slots64 = false;
+    grid_bwd32s = 0;
     slot64.release();
     col64.release();
     cval.release();
