@@ -297,6 +297,30 @@ def bench_fp64_stream_only(args, ctx, plan, rays, cfg_kw, init, nnz):
     print(json.dumps(out))
 
 
+def bench_median_fp32c_only(args, ctx, plan, rays, cfg_kw, init, nnz):
+    """The FP32C median alone at one brick shape (--brick, default the
+    benchmark's 16 x 16 x 32 with 512-thread blocks): --steps iterations
+    after a warm-up one (stream, slots, graph)."""
+    import muontomography_amd as M
+    brick = tuple(int(v) for v in (args.brick or "16,16,32,512").split(","))
+    kw = dict(cfg_kw)
+    kw["update_with_mean"] = 0
+    med = ctx.em_plan(M.mlsd_cfg(plan.cfg.grid, **kw), rays=rays)
+    plan.close()
+    med.set_option(M.MUTO_OPT_BRICK, M.brick_code(brick))
+    med.set_image(init)
+    med.run(1, M.MUTO_FP32C)
+    med.set_image(init)
+    t0 = time.perf_counter()
+    med.run(args.steps, M.MUTO_FP32C)
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    log(f"FP32C median: {ms:.2f} ms/it")
+    med.close()
+    ctx.close()
+    print(json.dumps({"workload": "FP32C median, 256^3 grid of 10 mm voxels, 1e7 muons", "brick": list(brick),
+                      "entries": int(nnz), "ms_per_step": ms}))
+
+
 def bench_scans(args):
     """64 independent scans (grids 32^3-128^3, 10-30 mm voxels, 1e4-1e6
     tracks each) run as one batch: FP32C mean update, one launch per pass
@@ -560,6 +584,8 @@ def main():
     ap.add_argument("--brick", default=None, metavar="BX,BY,BZ[,BLOCK]",
                     help="em256: use this brick shape (and block size), no tuning (profiling runs)")
     ap.add_argument("--n-scans", type=int, default=64, help="scans64: number of scans")
+    ap.add_argument("--median-fp32c-only", action="store_true",
+                    help="time only the FP32C median (the command of its profile)")
     ap.add_argument("--fp64-stream-only", action="store_true",
                     help="em256: only the FP64 brick stream, mean then median, --steps iterations "
                          "each at --brick (default 16,16,16,256) (profiling runs)")
@@ -623,6 +649,8 @@ def main():
 
     if args.fp64_stream_only:
         return bench_fp64_stream_only(args, ctx, plan, rays, cfg_kw, init, nnz)
+    if args.median_fp32c_only:
+        return bench_median_fp32c_only(args, ctx, plan, rays, cfg_kw, init, nnz)
 
     # FP32C: brick (LDS tile) shape chosen by the autotuner, warm-up (the
     # stream build and graph capture), then K steps
