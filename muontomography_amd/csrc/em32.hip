@@ -1294,18 +1294,40 @@ k_fwd64s(IterArgs a, Args64 b, const double* __restrict__ lam, double4* __restri
     const uint32_t ti = sh;
     if (ti >= a.n_tasks) break;
     const Task tk = a.tasks[ti];
-    load_brick64(a.G, tk.brick, lam, lds_lam);
-    __syncthreads();
+    load_brick64(a.G, tk.brick, lam, lds_lam);  // (the first chunk's barrier orders it)
+    Chunk nx = chunk_at(a, tk.c0);
+    // as in k_fwd32: one memory round trip per chunk (the descriptor a chunk
+    // ahead; segment words, groups, the group's exact tail and lengths issued
+    // together), the partials stored after the next chunk's loads
+    double4 pend = make_double4(0.0, 0.0, 0.0, 0.0);
+    uint64_t pend_q = ~0ull;
     for (uint32_t c = tk.c0; c < tk.c1; ++c) {
-      const Chunk ch = a.chunks[c];
-      if (t < ch.cnt) {
-        sv0[t] = a.seg_v0[ch.q0 + t];
-        stl[t] = b.seg_tail64[ch.q0 + t];
-      }
+      const Chunk ch = nx;
+      if (c + 1 < tk.c1) nx = chunk_at(a, c + 1);
+      const uint64_t q = seg_slot(a, ch, t);
+      const uint32_t v0w = a.seg_v0[q];
+      const double tlw = b.seg_tail64[q];
       const bool act = t * GE < ch.ne;
-      const uint64_t gi = (ch.base / GE) + t;
+      const uint64_t gi = group_slot(ch, t);
       Group gr;
-      if (act) gr.load(a, gi);
+      gr.load(a, gi);
+      double T = b.gtail64[gi];
+      double L16[GE];
+      {
+        const double2* pl2 = reinterpret_cast<const double2*>(b.len64 + gi * GE);
+#pragma unroll
+        for (int k = 0; k < GE / 2; ++k) {
+          const double2 v2 = pl2[k];
+          L16[2 * k] = v2.x;
+          L16[2 * k + 1] = v2.y;
+        }
+      }
+      if (pend_q != ~0ull) part[pend_q] = pend;
+      pend_q = ~0ull;
+      if (t < ch.cnt) {  // waits for the segment words only
+        sv0[t] = v0w;
+        stl[t] = tlw;
+      }
       __syncthreads();
       if (act) {
         gr.decode();
@@ -1313,24 +1335,16 @@ k_fwd64s(IterArgs a, Args64 b, const double* __restrict__ lam, double4* __restri
         gr.voxels(stab, [&](uint32_t q) { return sv0[q]; }, [&](int i, uint32_t v) { vx[i] = v; });
         uint32_t e0 = t * GE, mk = gr.mask, sk = gr.s0;
         asm volatile("" : "+v"(e0), "+v"(mk), "+v"(sk));
-        double T = b.gtail64[gi];
         double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-        const double2* pl2 = reinterpret_cast<const double2*>(b.len64 + gi * GE);
 #pragma unroll
         for (int h = GE / 2; h >= 0; h -= GE / 2) {
-          double L8[GE / 2], l8[GE / 2];
-#pragma unroll
-          for (int k = 0; k < GE / 4; ++k) {
-            const double2 v2 = pl2[h / 2 + k];
-            L8[2 * k] = v2.x;
-            L8[2 * k + 1] = v2.y;
-          }
+          double l8[GE / 2];
 #pragma unroll
           for (int k = 0; k < GE / 2; ++k) l8[k] = lds_lam[vx[h + k]];
 #pragma unroll
           for (int k = GE / 2 - 1; k >= 0; --k) {
             const int i = h + k;
-            const double L = L8[k], lj = l8[k];
+            const double L = L16[i], lj = l8[k];
             // generateWeightMatrix (MutoMLSD.cpp:417-430), the reference's own expressions
             const double LT = L * T;
             const double w01 = LT + L * L * 0.5;
@@ -1366,9 +1380,11 @@ k_fwd64s(IterArgs a, Args64 b, const double* __restrict__ lam, double4* __restri
           y = y + hp.y;
           z = z + hp.z;
         }
-        part[ch.q0 + t] = make_double4(x, y, z, 0.0);
+        pend = make_double4(x, y, z, 0.0);
+        pend_q = ch.q0 + t;
       }
     }
+    if (pend_q != ~0ull) part[pend_q] = pend;
   }
 }
 
@@ -1468,10 +1484,34 @@ k_bwd64s(IterArgs a, Args64 b, const double4* __restrict__ trk_g, double* __rest
         sg[t] = make_double4(g.x, g.y, g.z, b.seg_tail64[q]);
         sv0[t] = a.seg_v0[q];
       }
+      // MODE 1: the group, its exact tail, lengths and slots issued together
+      // before the barrier, after the G' gathers above (25.1 -> 23.0 ms at
+      // the benchmark size); MODE 0 measured faster loading the tail and
+      // lengths in the walk (5.34 against 5.64 ms)
+      constexpr bool PRE = MODE == 1;
       const bool act = t * GE < ch.ne;
-      const uint64_t gi = (ch.base / GE) + t;
+      const uint64_t gi = PRE ? group_slot(ch, t) : (ch.base / GE) + t;
       Group gr;
-      if (act) gr.load(a, gi);
+      if (PRE || act) gr.load(a, gi);
+      double T = 0.0;
+      double L16[GE];
+      uint32_t sl[GE];
+      if (PRE) {
+        T = b.gtail64[gi];
+        const double2* pl2 = reinterpret_cast<const double2*>(b.len64 + gi * GE);
+#pragma unroll
+        for (int k = 0; k < GE / 2; ++k) {
+          const double2 v2 = pl2[k];
+          L16[2 * k] = v2.x;
+          L16[2 * k + 1] = v2.y;
+        }
+        const uint4* ps = reinterpret_cast<const uint4*>(b.slot + gi * GE);
+#pragma unroll
+        for (int k = 0; k < GE / 4; ++k) {
+          const uint4 v4 = ps[k];
+          sl[4 * k] = v4.x; sl[4 * k + 1] = v4.y; sl[4 * k + 2] = v4.z; sl[4 * k + 3] = v4.w;
+        }
+      }
       __syncthreads();
       if (act) {
         gr.decode();
@@ -1479,31 +1519,23 @@ k_bwd64s(IterArgs a, Args64 b, const double4* __restrict__ trk_g, double* __rest
         gr.voxels(stab, [&](uint32_t q) { return sv0[q]; }, [&](int i, uint32_t v) { vx[i] = v; });
         uint32_t mk = gr.mask, sk = gr.s0;
         asm volatile("" : "+v"(mk), "+v"(sk));
-        double T = b.gtail64[gi];
+        if (!PRE) T = b.gtail64[gi];
         double4 g = sg[sk + (uint32_t)__popc(mk & 0xFFFEu)];
         const double2* pl2 = reinterpret_cast<const double2*>(b.len64 + gi * GE);
-        uint32_t sl[GE];
-        if (MODE == 1) {
-          const uint4* ps = reinterpret_cast<const uint4*>(b.slot + gi * GE);
-#pragma unroll
-          for (int k = 0; k < GE / 4; ++k) {
-            const uint4 v4 = ps[k];
-            sl[4 * k] = v4.x; sl[4 * k + 1] = v4.y; sl[4 * k + 2] = v4.z; sl[4 * k + 3] = v4.w;
-          }
-        }
 #pragma unroll
         for (int h = GE / 2; h >= 0; h -= GE / 2) {
-          double L8[GE / 2];
+          if (!PRE) {
 #pragma unroll
-          for (int k = 0; k < GE / 4; ++k) {
-            const double2 v2 = pl2[h / 2 + k];
-            L8[2 * k] = v2.x;
-            L8[2 * k + 1] = v2.y;
+            for (int k = 0; k < GE / 4; ++k) {
+              const double2 v2 = pl2[h / 2 + k];
+              L16[h + 2 * k] = v2.x;
+              L16[h + 2 * k + 1] = v2.y;
+            }
           }
 #pragma unroll
           for (int k = GE / 2 - 1; k >= 0; --k) {
             const int i = h + k;
-            const double L = L8[k];
+            const double L = L16[i];
             const double LT = L * T;
             const double w01 = LT + L * L * 0.5;
             const double w11 = L * L * L / 3.0 + (L + T) * LT;
