@@ -385,11 +385,20 @@ int muto_batch_destroy(muto_batch* batch);
  *                     per-entry c_ij in voxel-major slots and a per-voxel
  *                     sort (median, trimmed mean); every W_ij is the
  *                     reference's, only the association of the sums differs
+ *   MUTO_OPT_MEDIAN_BRACKETS FP32C median: 1 (default) each voxel keeps a
+ *                     bracket of c values around its last median; one pass
+ *                     counts the values below it and keeps those inside,
+ *                     the median is picked from those, and only the voxels
+ *                     whose median left the bracket collect their whole
+ *                     lists (the same values and the same pick as 0, so
+ *                     the images are identical); 0 every c_ij is written
+ *                     to its voxel's slot and each voxel's list is searched
  * No counterpart in the reference (which has no device state). */
 #define MUTO_OPT_GRAPH 1
 #define MUTO_OPT_TIMING 2
 #define MUTO_OPT_ACC_MODE 3
 #define MUTO_OPT_FP64_KERNEL 5
+#define MUTO_OPT_MEDIAN_BRACKETS 6
 #define MUTO_OPT_BRICK 4  /* FP32C brick (LDS tile) shape and block size:
                              bx | by << 8 | bz << 16 | log2(block / 256) << 24,
                              at most 16384 voxels, block 256, 512 or 1024

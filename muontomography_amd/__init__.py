@@ -73,6 +73,7 @@ MUTO_OPT_TIMING = 2
 MUTO_OPT_ACC_MODE = 3
 MUTO_OPT_BRICK = 4
 MUTO_OPT_FP64_KERNEL = 5
+MUTO_OPT_MEDIAN_BRACKETS = 6
 # the autotuner's built-in candidate list (muto_em_plan_autotune), in order
 def brick_code(shape):
     """MUTO_OPT_BRICK / tuner code of (bx, by, bz) or (bx, by, bz, block):

@@ -1285,6 +1285,14 @@ extern "C" int muto_em_plan_set_option(muto_em_plan* pl, int option, int value) 
       if (value < 0 || value > 2) break;
       pl->fp64_kernel = value;
       return MUTO_OK;
+    case MUTO_OPT_MEDIAN_BRACKETS:
+      if (value != 0 && value != 1) break;
+      if (pl->f32.med_brk != (value != 0) && pl->f32.graph) {
+        (void)hipGraphExecDestroy(pl->f32.graph);
+        pl->f32.graph = nullptr;
+      }
+      pl->f32.med_brk = value != 0;
+      return MUTO_OK;
     case MUTO_OPT_ACC_MODE:
       if (value != 0 && value != 1) break;
       if (pl->f32.acc_mode != value && pl->f32.graph) {

@@ -80,6 +80,10 @@ constexpr uint32_t ENT_PAD = ENT_PAD_CODE << 29;  // a padding word
 __host__ __device__ constexpr uint32_t seg_padded(uint32_t n) {
   return (n + (uint32_t)SEG_ALIGN - 1u) / (uint32_t)SEG_ALIGN * (uint32_t)SEG_ALIGN;
 }
+// FP32C median by brackets: values kept inside a voxel's bracket, and ranks
+// kept on either side of the median for the next bracket
+constexpr uint32_t MED_K = 32;
+constexpr uint32_t MED_H = 6;
 constexpr uint32_t DIR_BINS = 8;         // track-order direction bins per unit slope
 // backprojection sums (ACC 1) are int64 fixed point: an entry is at most
 // 2^fx_bits, fx_bits = min(FX_BITS, 62 - bits(max M_j)) per plan; a task
@@ -1983,13 +1987,28 @@ k_sel64w(size_t nv, const uint32_t* __restrict__ mj, const uint64_t* __restrict_
   }
 }
 
+__global__ void k_fill_u8(size_t n, uint8_t* __restrict__ x, uint8_t v) {
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) x[j] = v;
+}
+
+// a voxel's first bracket: everything when M_j <= MED_K, else none (a miss,
+// so the first pass collects the whole list)
+__global__ void k_brk_init(size_t nv, const uint32_t* __restrict__ mj, float2* __restrict__ brk) {
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nv)
+    brk[j] = mj[j] <= MED_K ? make_float2(-INFINITY, INFINITY) : make_float2(INFINITY, -INFINITY);
+}
+
 __global__ void k_fill_nan(size_t n, double* __restrict__ x) {
   const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j < n) x[j] = NAN;
 }
 
 // Listed voxels with longer lists, one block each: bitonic sort in LDS
-// (M_j <= 4096) or in the voxel's own range of a global scratch.
+// (M_j <= 4096) or in the voxel's own range of a global scratch.  With
+// `miss`, only the voxels it marks, and the next bracket to `brk` (the FP32C
+// median by brackets).
 __device__ void block_bitonic64(double* a, uint32_t n) {
   for (uint32_t k = 2; k <= n; k <<= 1) {
     for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
@@ -2011,10 +2030,12 @@ __global__ void __launch_bounds__(1024)
 k_sel64b(uint32_t n_list, const uint32_t* __restrict__ list, const uint64_t* __restrict__ off,
          const uint32_t* __restrict__ mj, const uint64_t* __restrict__ col,
          const V* __restrict__ cval, int use_mean, int pct, double* __restrict__ scratch,
-         double* __restrict__ lam, double* __restrict__ ml) {
+         double* __restrict__ lam, double* __restrict__ ml, const uint8_t* __restrict__ miss,
+         float2* __restrict__ brk) {
   __shared__ double a_s[4096];
   for (uint32_t b = blockIdx.x; b < n_list; b += gridDim.x) {
     const uint32_t j = list[b];
+    if (miss && !miss[j]) continue;  // uniform over the block
     const uint32_t m = mj[j];
     uint32_t n = 1;
     while (n < m) n <<= 1;
@@ -2026,8 +2047,272 @@ k_sel64b(uint32_t n_list, const uint32_t* __restrict__ list, const uint64_t* __r
     if (threadIdx.x == 0) {
       const double out = order_stat_c(lam[j], m, use_mean, pct, [&](uint32_t k) { return a[k]; });
       if (ml) ml[j] = out; else lam[j] = out;
+      if (brk) {
+        const uint32_t r1 = m / 2, r0 = m % 2 == 0 ? r1 - 1 : r1;
+        brk[j] = make_float2((float)a[r0 >= MED_H ? r0 - MED_H : 0u], (float)a[min(r1 + MED_H, m - 1)]);
+      }
     }
     __syncthreads();
+  }
+}
+
+// ---------------- FP32C median by brackets --------------------------------
+// The median of a voxel's c_ij moves little from one pass to the next, so
+// each voxel keeps a bracket [lo, hi] of c values around its last median
+// (MED_H ranks either side, or everything when M_j <= MED_K).  A pass over
+// the stream, one task per brick (so that a brick's counts and write
+// positions are its own), computes each c_ij exactly as the slot pass does
+// and counts per voxel in LDS the values below the bracket and those inside
+// it, writing up to MED_K inside values to the voxel's buffer.  When both
+// middle ranks fall inside the bracket the median is selected from those
+// values, the same FP32 values the slot pass would store, so the update is
+// bit-identical to it; the other voxels (misses) and their bricks go
+// through a second pass that collects their whole lists into their slot
+// ranges, and a sort.
+
+struct MedArgs {
+  const Task* btask;   // one task per brick, largest first
+  uint32_t n_btask;
+  unsigned int* work;  // task counter of the pass
+  float2* brk;         // [nv] bracket (lo, hi)
+  uint32_t* bcnt;      // [nv] values below | values inside << 16
+  float* buf;          // [nv * MED_K] values inside
+  uint8_t* miss;       // [nv] the median left the bracket
+  uint8_t* bmiss;      // [bricks] the brick holds a miss
+  const uint64_t* col; // [nv + 1] slot ranges (misses' whole lists)
+  float* cval;
+};
+
+// FB = false: the bracket pass; FB = true: the bricks with misses, whose
+// missed voxels' values all go to their slot ranges.
+template <uint32_t BS, bool FB>
+__global__ void __launch_bounds__(BS)
+k_bwd32m(IterArgs a, MedArgs m, const double4* __restrict__ trk_g) {
+  extern __shared__ __align__(16) double lds_dyn[];
+  const uint32_t vol = a.G.b[0] * a.G.b[1] * a.G.b[2];
+  float2* brk_s = reinterpret_cast<float2*>(lds_dyn);
+  uint32_t* cnt_s = reinterpret_cast<uint32_t*>(brk_s + vol);
+  __shared__ double4 sg[BS];
+  __shared__ uint32_t sv0[BS];
+  __shared__ int stab[16];
+  __shared__ uint32_t sh_[4];
+  uint32_t& sh = sh_[0];
+  const uint32_t t = threadIdx.x;
+  fill_step_table(a, stab);
+  const double U = a.ec.unit, U2 = U * U, U3 = U2 * U;
+  for (;;) {
+    if (t == 0) sh = atomicAdd(m.work, 1u);
+    __syncthreads();
+    const uint32_t ti = sh;
+    if (ti >= m.n_btask) break;
+    const Task tk = m.btask[ti];
+    if (FB && !m.bmiss[tk.brick]) {
+      __syncthreads();  // every thread has read sh before it is rewritten
+      continue;
+    }
+    uint32_t o[3];
+    brick_origin(a.G, tk.brick, o);
+    for (uint32_t v = t; v < vol; v += BS) {
+      const uint32_t rr = a.G.db0.div(v), lx = v - rr * a.G.b[0];
+      const uint32_t lz = a.G.db1.div(rr), ly = rr - lz * a.G.b[1];
+      const uint32_t x = o[0] + lx, y = o[1] + ly, z = o[2] + lz;
+      float2 br = make_float2(INFINITY, -INFINITY);  // outside the grid: no values
+      if (x < a.G.n[0] && y < a.G.n[1] && z < a.G.n[2]) {
+        const size_t j = (size_t)x + (size_t)a.G.n[0] * ((size_t)y + (size_t)a.G.n[1] * z);
+        if (FB)
+          br = m.miss[j] ? make_float2(-INFINITY, INFINITY) : make_float2(INFINITY, -INFINITY);
+        else
+          br = m.brk[j];
+      }
+      brk_s[v] = br;
+      cnt_s[v] = 0u;
+    }
+    for (uint32_t c = tk.c0; c < tk.c1; ++c) {
+      const Chunk ch = a.chunks[c];
+      if (t < ch.cnt) {
+        const uint64_t q = ch.q0 + t;
+        const uint32_t tfx = a.seg_tfx[q];
+        const double4 g = trk_g[a.seg_trk[q]];
+        // as in k_bwd32s: A^T G' A in FP64, in units of U
+        const double shf = (double)((int64_t)tfx - (int64_t)g.w) * a.inv_scale;
+        const double gy = fma(shf, g.z, g.y);
+        sg[t] = make_double4(fma(shf, gy + g.y, g.x) * U, gy * U2, g.z * U3, 0.0);
+        sv0[t] = a.seg_v0[q];
+      }
+      const bool act = t * GE < ch.ne;
+      Group gr;
+      if (act) gr.load(a, (ch.base / GE) + t);
+      __syncthreads();
+      if (act) {
+        gr.decode();
+        uint32_t vx[GE];
+        gr.voxels(stab, [&](uint32_t q) { return sv0[q]; }, [&](int i, uint32_t v) { vx[i] = v; });
+        const uint32_t e0 = t * GE;
+        double T = gr.carry;
+        uint32_t mk = gr.mask, sk = gr.s0, base_v = ENT_BASE;  // opaque copies, as in k_bwd32
+        asm volatile("" : "+v"(mk), "+v"(sk), "+v"(base_v));
+        double4 g = sg[sk + (uint32_t)__popc(mk & 0xFFFEu)];
+#pragma unroll
+        for (int i = GE - 1; i >= 0; --i) {
+          // c_ij exactly as k_bwd32s stores it
+          const double L = ent_len_v(gr.w[i], base_v);
+          const uint32_t st = start_mask(mk, i);
+          const double k = fma(L, fma(L, 1.0 / 3.0, T), T * T);
+          const double cv = L * fma(k, g.z, fma(fma(2.0, T, L), g.y, g.x));
+          const float cf = (float)cv;
+          if (e0 + (uint32_t)i < ch.ne && ent_code(gr.w[i]) != ENT_PAD_CODE) {
+            const uint32_t v = vx[i];
+            const float2 br = brk_s[v];
+            if (cf < br.x) {
+              atomicAdd(&cnt_s[v], 1u);
+            } else if (!(cf > br.y)) {
+              const uint32_t pos = atomicAdd(&cnt_s[v], 0x10000u) >> 16;
+              const uint32_t rr = a.G.db0.div(v), lx = v - rr * a.G.b[0];
+              const uint32_t lz = a.G.db1.div(rr), ly = rr - lz * a.G.b[1];
+              const size_t j = (size_t)(o[0] + lx) +
+                               (size_t)a.G.n[0] * ((size_t)(o[1] + ly) + (size_t)a.G.n[1] * (o[2] + lz));
+              if (FB)
+                m.cval[m.col[j] + pos] = cf;  // pos < M_j
+              else if (pos < MED_K)
+                m.buf[j * MED_K + pos] = cf;
+            }
+          }
+          if (i > 0 && st != 0u) g = sg[sk + (uint32_t)__popc(mk & ((2u << i) - 2u)) - 1u];
+          T = st != 0u ? 0.0 : T + L;
+        }
+      }
+      __syncthreads();  // the tables are rewritten for the next chunk
+    }
+    if (!FB) {
+      for (uint32_t v = t; v < vol; v += BS) {
+        const uint32_t rr = a.G.db0.div(v), lx = v - rr * a.G.b[0];
+        const uint32_t lz = a.G.db1.div(rr), ly = rr - lz * a.G.b[1];
+        const uint32_t x = o[0] + lx, y = o[1] + ly, z = o[2] + lz;
+        if (x >= a.G.n[0] || y >= a.G.n[1] || z >= a.G.n[2]) continue;
+        m.bcnt[(size_t)x + (size_t)a.G.n[0] * ((size_t)y + (size_t)a.G.n[1] * z)] = cnt_s[v];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// the median (order_stat_c with update_with_mean 0) of two middle values
+__device__ __forceinline__ double median_update(double l, uint32_t m, double c_lo, double c_hi) {
+  const double l2 = l * l;
+  const double med = 2.0 * l + c_hi * l2;
+  if (m % 2 == 0) return (med + (2.0 * l + c_lo * l2)) * 0.25;
+  return med * 0.5;
+}
+
+// Per voxel (one thread): the median from the bracket's values when both
+// middle ranks (m/2 - 1 and m/2 for even m, m/2 for odd) fall inside it, with
+// the next bracket MED_H ranks either side (kept where the values run out);
+// else a miss, marked on the voxel and its brick.  M_j = 0 gives 0.
+__global__ void __launch_bounds__(256)
+k_sel_brk(BrickGeo G, size_t nv, const uint32_t* __restrict__ mj, MedArgs m,
+          double* __restrict__ lam, double* __restrict__ ml) {
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nv) return;
+  const uint32_t mm = mj[j];
+  if (mm == 0) {
+    if (ml) ml[j] = 0.0; else lam[j] = 0.0;
+    m.miss[j] = 0;
+    return;
+  }
+  const uint32_t pk = m.bcnt[j], nlo = pk & 0xFFFFu, nin = pk >> 16;
+  const uint32_t r1 = mm / 2, r0 = mm % 2 == 0 ? r1 - 1 : r1;
+  if (!(nin <= MED_K && nlo <= r0 && r1 < nlo + nin)) {
+    m.miss[j] = 1;
+    const uint32_t tq = G.dn0.div((uint32_t)j), x = (uint32_t)j - tq * G.n[0];
+    const uint32_t z = G.dn1.div(tq), y = tq - z * G.n[1];
+    m.bmiss[G.db0.div(x) + G.nb[0] * (G.db1.div(y) + G.nb[1] * G.db2.div(z))] = 1;
+    return;
+  }
+  m.miss[j] = 0;
+  float w[MED_K];
+  const float4* src = reinterpret_cast<const float4*>(m.buf + j * MED_K);
+#pragma unroll
+  for (uint32_t k = 0; k < MED_K / 4; ++k) {
+    const float4 q = src[k];
+    w[4 * k] = 4 * k < nin ? q.x : INFINITY;
+    w[4 * k + 1] = 4 * k + 1 < nin ? q.y : INFINITY;
+    w[4 * k + 2] = 4 * k + 2 < nin ? q.z : INFINITY;
+    w[4 * k + 3] = 4 * k + 3 < nin ? q.w : INFINITY;
+  }
+  // bitonic sorting network, ascending
+#pragma unroll
+  for (uint32_t kk = 2; kk <= MED_K; kk <<= 1) {
+#pragma unroll
+    for (uint32_t jd = kk >> 1; jd > 0; jd >>= 1) {
+#pragma unroll
+      for (uint32_t i = 0; i < MED_K; ++i) {
+        const uint32_t p = i ^ jd;
+        if (p > i) {
+          const float lo = fminf(w[i], w[p]), hi = fmaxf(w[i], w[p]);
+          const bool up = (i & kk) == 0;
+          w[i] = up ? lo : hi;
+          w[p] = up ? hi : lo;
+        }
+      }
+    }
+  }
+  const uint32_t ia = r0 - nlo, ib = r1 - nlo;
+  const float2 br = m.brk[j];
+  const uint32_t il = ia >= MED_H ? ia - MED_H : MED_K, ih = ib + MED_H < nin ? ib + MED_H : MED_K;
+  float ca = 0.f, cb = 0.f, nl = br.x, nh = br.y;
+#pragma unroll
+  for (uint32_t k = 0; k < MED_K; ++k) {
+    if (k == ia) ca = w[k];
+    if (k == ib) cb = w[k];
+    if (k == il) nl = w[k];
+    if (k == ih) nh = w[k];
+  }
+  const double out = median_update(lam[j], mm, (double)ca, (double)cb);
+  if (ml) ml[j] = out; else lam[j] = out;
+  if (mm > MED_K) m.brk[j] = make_float2(nl, nh);
+}
+
+// The missed voxels (M_j <= 256; longer ones go to k_sel64b): one wavefront
+// per voxel sorts its whole list, as k_sel64w without a pivot, and sets the
+// next bracket MED_H ranks either side of the median.
+template <int R>
+__device__ __forceinline__ void miss_select(const float* __restrict__ src, uint32_t mm, double l,
+                                            double* out, float2* nb) {
+  const uint32_t lane = threadIdx.x & 63u;
+  double v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = r * 64u + lane;
+    v[r] = e < mm ? (double)src[e] : INFINITY;
+  }
+  wave_bitonic<R>(v);
+  const uint32_t r1 = mm / 2, r0 = mm % 2 == 0 ? r1 - 1 : r1;
+  *out = median_update(l, mm, wave_elem<R>(v, r0), wave_elem<R>(v, r1));
+  *nb = make_float2((float)wave_elem<R>(v, r0 >= MED_H ? r0 - MED_H : 0u),
+                    (float)wave_elem<R>(v, min(r1 + MED_H, mm - 1)));
+}
+__global__ void __launch_bounds__(256)
+k_sel_miss(size_t nv, const uint32_t* __restrict__ mj, MedArgs m, double* __restrict__ lam,
+           double* __restrict__ ml) {
+  const size_t nw = (size_t)gridDim.x * (blockDim.x >> 6);
+  for (size_t j = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; j < nv; j += nw) {
+    if (!m.miss[j]) continue;
+    const uint32_t mm = mj[j];
+    if (mm > 256) continue;
+    const float* src = m.cval + m.col[j];
+    const double l = lam[j];
+    double out;
+    float2 nb;
+    if (mm <= 64)
+      miss_select<1>(src, mm, l, &out, &nb);
+    else if (mm <= 128)
+      miss_select<2>(src, mm, l, &out, &nb);
+    else
+      miss_select<4>(src, mm, l, &out, &nb);
+    if ((threadIdx.x & 63u) == 0) {
+      if (ml) ml[j] = out; else lam[j] = out;
+      if (mm > MED_K) m.brk[j] = nb;
+    }
   }
 }
 
@@ -2113,7 +2398,7 @@ __global__ void k_iter_begin(uint32_t n_scans, double* __restrict__ cost_acc,
   for (uint32_t k = threadIdx.x; k < n_scans; k += blockDim.x) cost_acc[k] = 0.0;
   if (threadIdx.x < 3) cnt[threadIdx.x] = 0ull;
   if (threadIdx.x == 0) cmax[0] = 0ull;  // the track kernel sets this pass's |c| bound
-  if (threadIdx.x < 2) work[threadIdx.x] = 0u;
+  if (threadIdx.x < 4) work[threadIdx.x] = 0u;  // (2, 3: the median's bracket passes)
   if (qwork && threadIdx.x < nq) qwork[threadIdx.x] = 0u;  // order statistics: per-XCD queues
 }
 
@@ -2659,6 +2944,7 @@ int fp32c_build(muto_em_plan* pl) {
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     unsigned int max_mj = 0;
     MUTO_CHECK_HIP(ctx, copy_sync(ctx, &max_mj, mx.p, sizeof max_mj, hipMemcpyDeviceToHost));
+    f.max_mj = max_mj;
     f.fx_bits = std::min(FX_BITS, 62 - bits_for(std::max<uint64_t>(max_mj, 1)));
   }
   {
@@ -2677,6 +2963,98 @@ int fp32c_build(muto_em_plan* pl) {
 static bool mean_all(const muto_em_plan* pl);
 static Args64 args64(muto_em_plan* pl);
 static int fp64s_slots(muto_em_plan* pl);
+
+// whether this plan's order statistic runs by brackets: the median (not
+// the trimmed mean), the option on, M_j within the 16-bit counts, and the
+// arrays built (med_setup)
+static bool fp32c_brackets(const muto_em_plan* pl) {
+  const Fp32cData& f = pl->f32;
+  return !pl->cfg.update_with_mean && f.med_brk && f.med_ready;
+}
+
+static MedArgs med_args(muto_em_plan* pl, unsigned int* work) {
+  Fp32cData& f = pl->f32;
+  MedArgs m;
+  m.btask = f.btask.p;
+  m.n_btask = f.n_btask;
+  m.work = work;
+  m.brk = f.brk.p;
+  m.bcnt = f.bcnt.p;
+  m.buf = f.bbuf.p;
+  m.miss = f.bmiss_v.p;
+  m.bmiss = f.bmiss_b.p;
+  m.col = f.col64.p;
+  m.cval = reinterpret_cast<float*>(f.cval.p);
+  return m;
+}
+
+// One task per brick (its chunks in order), largest first; the bracket
+// arrays, every voxel starting without a bracket unless M_j <= MED_K.  Needs
+// the slot ranges (fp64s_slots).  Skipped (the slot pass then runs) when a
+// voxel has 65536 entries or more.
+static int med_setup(muto_em_plan* pl) {
+  Fp32cData& f = pl->f32;
+  if (f.med_ready || !f.med_brk || pl->cfg.update_with_mean || f.max_mj >= 65536u) return MUTO_OK;
+  muto_ctx* ctx = pl->ctx;
+  hipStream_t s = ctx->stream;
+  // the pass's LDS: bracket and counts per brick voxel, and its tables
+  const size_t lds = (size_t)f.bs.vol() * (sizeof(float2) + sizeof(uint32_t));
+  int max_lds = 0;
+  MUTO_CHECK_HIP(ctx, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+  if (lds + (size_t)f.bs.block * (sizeof(double4) + sizeof(uint32_t)) + 128 > (size_t)max_lds) return MUTO_OK;
+  int ob = 0;
+  MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
+    constexpr uint32_t BS = decltype(B)::value;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob, k_bwd32m<BS, false>, BS, lds);
+  }));
+  if (ob < 1) return MUTO_OK;
+  f.grid_med = ob * ctx->n_cu;
+  std::vector<Task> tk(f.n_tasks);
+  if (f.n_tasks)
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, tk.data(), f.tasks.p, f.n_tasks * sizeof(Task), hipMemcpyDeviceToHost));
+  std::vector<Chunk> ch(f.n_chunks);
+  if (f.n_chunks)
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, ch.data(), f.chunks.p, f.n_chunks * sizeof(Chunk), hipMemcpyDeviceToHost));
+  std::sort(tk.begin(), tk.end(), [](const Task& x, const Task& y) {
+    return x.brick != y.brick ? x.brick < y.brick : x.c0 < y.c0;
+  });
+  std::vector<Task> bt;
+  std::vector<uint64_t> ne;
+  for (const Task& t : tk) {
+    if (!bt.empty() && bt.back().brick == t.brick) {
+      bt.back().c0 = std::min(bt.back().c0, t.c0);
+      bt.back().c1 = std::max(bt.back().c1, t.c1);
+    } else {
+      bt.push_back(Task{t.brick, 1u, t.c0, t.c1});
+    }
+  }
+  for (const Task& t : bt) {
+    uint64_t n = 0;
+    for (uint32_t c = t.c0; c < t.c1; ++c) n += ch[c].ne;
+    ne.push_back(n);
+  }
+  std::vector<size_t> order(bt.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return ne[x] > ne[y]; });
+  std::vector<Task> sorted(bt.size());
+  for (size_t i = 0; i < order.size(); ++i) sorted[i] = bt[order[i]];
+  const size_t nv = pl->nv;
+  const uint64_t nbk = (uint64_t)f.nb[0] * f.nb[1] * f.nb[2];
+  MUTO_CHECK_HIP(ctx, f.btask.alloc(std::max<size_t>(sorted.size(), 1)));
+  if (!sorted.empty())
+    MUTO_CHECK_HIP(ctx, copy_sync(ctx, f.btask.p, sorted.data(), sorted.size() * sizeof(Task), hipMemcpyHostToDevice));
+  f.n_btask = (uint32_t)sorted.size();
+  MUTO_CHECK_HIP(ctx, f.brk.alloc(nv));
+  MUTO_CHECK_HIP(ctx, f.bcnt.alloc(nv));
+  MUTO_CHECK_HIP(ctx, f.bbuf.alloc(nv * MED_K));
+  MUTO_CHECK_HIP(ctx, f.bmiss_v.alloc(nv));
+  MUTO_CHECK_HIP(ctx, f.bmiss_b.alloc(std::max<uint64_t>(nbk, 1)));
+  k_brk_init<<<grid_for(nv, 256), 256, 0, s>>>(nv, pl->mj.p, f.brk.p);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+  f.med_ready = true;
+  return MUTO_OK;
+}
 
 // One FP32C iteration enqueued on the plan's stream with no host
 // synchronisation and no memset, so it can be captured into a graph: a
@@ -2710,8 +3088,23 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[2], s));
   float* cval32 = reinterpret_cast<float*>(f.cval.p);  // order statistics: the slots' values
+  const bool brk = !mean && fp32c_brackets(pl);
+  const size_t lds_med = (size_t)f.bs.vol() * (sizeof(float2) + sizeof(uint32_t));
+  const double4* g64 = reinterpret_cast<const double4*>(f.trk_g64.p);
   if (f.n_tasks && mean) MUTO_CHECK_HIP(ctx, launch_bwd(f, a, f.trk_g.p, f.cmax.p, pl->acc64.p, s));
-  if (f.n_tasks && !mean) {
+  if (brk) {
+    // counts below / inside each voxel's bracket and the values inside
+    const uint64_t nbk = (uint64_t)f.nb[0] * f.nb[1] * f.nb[2];
+    k_fill_u8<<<grid_for(nbk, 256), 256, 0, s>>>(nbk, f.bmiss_b.p, 0);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+    const MedArgs m = med_args(pl, pl->work.p + 2);
+    MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
+      constexpr uint32_t BS = decltype(B)::value;
+      k_bwd32m<BS, false><<<f.grid_med, BS, lds_med, s>>>(a, m, g64);
+      return hipGetLastError();
+    }));
+  }
+  if (f.n_tasks && !mean && !brk) {
     const Args64 b = args64(pl);
     MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
       constexpr uint32_t BS = decltype(B)::value;
@@ -2726,6 +3119,31 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
                                                   pl->work.p, f.cmax.p, f.acc_mode == 1 ? 1 : 0,
                                                   f.fx_bits);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
+  } else if (brk) {
+    // medians inside their brackets; then the misses: their bricks' pass
+    // collects their whole lists into their slot ranges, and a sort
+    double* ml = map ? pl->ml64.p : nullptr;
+    MedArgs m = med_args(pl, pl->work.p + 3);
+    k_sel_brk<<<grid_for(pl->nv, 256), 256, 0, s>>>(a.G, pl->nv, pl->mj.p, m, pl->lam64.p, ml);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+    MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
+      constexpr uint32_t BS = decltype(B)::value;
+      k_bwd32m<BS, true><<<f.grid_med, BS, lds_med, s>>>(a, m, g64);
+      return hipGetLastError();
+    }));
+    const unsigned g = (unsigned)std::min<size_t>(grid_for(pl->nv, 4), (size_t)ctx->n_cu * 64);
+    k_sel_miss<<<g, 256, 0, s>>>(pl->nv, pl->mj.p, m, pl->lam64.p, ml);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+    if (f.n_big64) {
+      k_sel64b<float><<<std::min<uint32_t>(f.n_big64, 4096), 1024, 0, s>>>(
+          f.n_big64, f.big64.p, f.big64_off.p, pl->mj.p, f.col64.p, cval32, 0, 100, f.big64_scr.p,
+          pl->lam64.p, ml, f.bmiss_v.p, f.brk.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    }
+    if (!map) {  // the forward pass reads the float mirror
+      k_mirror32<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->lam64.p, pl->lam32.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    }
   } else {
     const muto_mlsd_cfg& c = pl->cfg;
     double* ml = map ? pl->ml64.p : nullptr;
@@ -2736,7 +3154,7 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
     if (f.n_big64) {
       k_sel64b<float><<<std::min<uint32_t>(f.n_big64, 4096), 1024, 0, s>>>(
           f.n_big64, f.big64.p, f.big64_off.p, pl->mj.p, f.col64.p, cval32, c.update_with_mean,
-          c.mean_percentile, f.big64_scr.p, pl->lam64.p, ml);
+          c.mean_percentile, f.big64_scr.p, pl->lam64.p, ml, nullptr, nullptr);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
     if (!map) {  // the forward pass reads the float mirror
@@ -2760,7 +3178,8 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
 static int fp32c_graph(muto_em_plan* pl) {
   Fp32cData& f = pl->f32;
   muto_ctx* ctx = pl->ctx;
-  if (f.graph && f.graph_map == pl->is_map && f.graph_mean == mean_all(pl)) return MUTO_OK;
+  if (f.graph && f.graph_map == pl->is_map && f.graph_mean == mean_all(pl) && f.graph_brk == fp32c_brackets(pl))
+    return MUTO_OK;
   if (f.graph) (void)hipGraphExecDestroy(f.graph);
   f.graph = nullptr;
   hipGraph_t g = nullptr;
@@ -2777,6 +3196,7 @@ static int fp32c_graph(muto_em_plan* pl) {
   MUTO_CHECK_HIP(ctx, ei);
   f.graph_map = pl->is_map;
   f.graph_mean = mean_all(pl);
+  f.graph_brk = fp32c_brackets(pl);
   return MUTO_OK;
 }
 
@@ -2822,6 +3242,7 @@ int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done) {
     }
     k_fill_nan<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, f.pivot64.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
+    if (int rc = med_setup(pl)) return rc;
   }
   const bool graph = pl->use_graph && !pl->timing;
   if (graph)
@@ -3118,7 +3539,7 @@ static int fp64s_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
     if (f.n_big64) {
       k_sel64b<double><<<std::min<uint32_t>(f.n_big64, 4096), 1024, 0, s>>>(
           f.n_big64, f.big64.p, f.big64_off.p, pl->mj.p, f.col64.p, f.cval.p, c.update_with_mean,
-          c.mean_percentile, f.big64_scr.p, pl->lam64.p, ml);
+          c.mean_percentile, f.big64_scr.p, pl->lam64.p, ml, nullptr, nullptr);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
   }

@@ -109,6 +109,21 @@ struct Fp32cData {
   int grid_fwd64 = 0, grid_bwd64 = 0;  // persistent grids of its passes
   hipGraphExec_t graph64 = nullptr;  // one FP64 stream iteration, captured
   int graph64_mode = -1;             // the update rule / MAP flag it was captured with
+  // FP32C median by brackets (MUTO_OPT_MEDIAN_BRACKETS): one task per
+  // brick; per voxel a bracket of c values around its last median, the
+  // counts below and inside it, up to MED_K values inside, and miss marks
+  // (voxels, bricks) for the voxels whose median left the bracket
+  bool med_brk = true;      // the option
+  bool med_ready = false;   // the arrays below are built for this stream
+  uint32_t max_mj = 0;      // largest M_j (brackets count in 16 bits)
+  DBuf<Task> btask;
+  uint32_t n_btask = 0;
+  int grid_med = 0;
+  DBuf<float2> brk;
+  DBuf<uint32_t> bcnt;
+  DBuf<float> bbuf;
+  DBuf<uint8_t> bmiss_v, bmiss_b;
+  bool graph_brk = false;   // the bracket mode the graph was captured with
   DBuf<double> hist;               // [hist_cap] cost sum per iteration of a run
   DBuf<unsigned int> hist_n;       // [1] entries in hist
   size_t hist_cap = 0;
@@ -162,6 +177,15 @@ struct Fp32cData {
     hist.release();
     hist_n.release();
     hist_cap = 0;
+    med_ready = false;
+    btask.release();
+    n_btask = 0;
+    grid_med = 0;
+    brk.release();
+    bcnt.release();
+    bbuf.release();
+    bmiss_v.release();
+    bmiss_b.release();
   }
 };
 

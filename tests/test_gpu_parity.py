@@ -719,8 +719,59 @@ def test_fp32c_order_statistics_long_voxel_lists(ctx, mode):
     p32.run(3, M.MUTO_FP32C)
     mx, rms, norm = _rel_stats(p32.image(), p64.image(), p64.voxel_counts())
     assert mx <= 1e-4, (mx, rms, norm)
+    if mode == "median":
+        # brackets (the default) and whole lists in slots pick the same values
+        a = p32.image()
+        p32.set_option(M.MUTO_OPT_MEDIAN_BRACKETS, 0)
+        p32.set_image(img)
+        p32.run(3, M.MUTO_FP32C)
+        assert np.array_equal(p32.image().view(np.uint64), a.view(np.uint64))
     p64.close()
     p32.close()
+
+
+@pytest.mark.parametrize("brick", [(16, 16, 16, 256), (8, 16, 32, 512), (8, 8, 16, 1024)])
+def test_fp32c_median_brackets_equal_slots(ctx, small_set, brick):
+    """FP32C median by brackets (MUTO_OPT_MEDIAN_BRACKETS 1: each voxel's
+    median picked among the values inside a bracket around its last one, the
+    voxels whose median left it through their whole lists) against every
+    c_ij written to its slot (option 0): both pick the same FP32 values with
+    the same arithmetic, so the images agree bit for bit after each of 12
+    iterations (src/MutoMLSD.cpp:505-515), in a second run that starts from
+    the first run's brackets, without the graph, and for MAP iterations."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=0, beta=1.0, p=1.2, delta=0.1, all_neighbours=1,
+                  add_object_prior=0)
+    pa = ctx.em_plan(cfg, rays=rays)
+    pb = ctx.em_plan(cfg, rays=rays)
+    for p in (pa, pb):
+        p.set_option(M.MUTO_OPT_BRICK, M.brick_code(brick))
+    pb.set_option(M.MUTO_OPT_MEDIAN_BRACKETS, 0)
+    img = np.full(g.n_vox, 1e-7)
+    pa.set_image(img)
+    pb.set_image(img)
+    for it in range(12):
+        ca = pa.run(1, M.MUTO_FP32C)
+        cb = pb.run(1, M.MUTO_FP32C)
+        assert np.array_equal(pa.image().view(np.uint64), pb.image().view(np.uint64)), it
+        np.testing.assert_allclose(ca, cb, rtol=1e-12)  # (per-block cost sums meet in any order)
+    assert np.abs(pa.image()).max() > 0
+    ref = pb.image()
+    pb.set_image(img)
+    pb.run(12, M.MUTO_FP32C)
+    assert np.array_equal(pb.image().view(np.uint64), ref.view(np.uint64))
+    for graph in (1, 0):
+        pa.set_option(M.MUTO_OPT_GRAPH, graph)
+        pa.set_image(img)
+        pa.run(12, M.MUTO_FP32C)
+        assert np.array_equal(pa.image().view(np.uint64), ref.view(np.uint64)), graph
+    pa.set_image(img)
+    pb.set_image(img)
+    pa.run(4, M.MUTO_FP32C, map_=True)
+    pb.run(4, M.MUTO_FP32C, map_=True)
+    assert np.array_equal(pa.image().view(np.uint64), pb.image().view(np.uint64))
+    pa.close()
+    pb.close()
 
 
 def test_map_fp32c_median_close_to_fp64(ctx, small_set):
