@@ -2084,10 +2084,13 @@ struct MedArgs {
 };
 
 // FB = false: the bracket pass; FB = true: the bricks with misses, whose
-// missed voxels' values all go to their slot ranges.
-template <uint32_t BS, bool FB>
-__global__ void __launch_bounds__(BS)
+// missed voxels' values all go to their slot ranges.  A workgroup of SUB
+// chunk widths CW takes SUB chunks of its brick at a time (its LDS arrays
+// are per brick, so two chunks cost no more of them).
+template <uint32_t CW, uint32_t SUB, bool FB>
+__global__ void __launch_bounds__(CW * SUB)
 k_bwd32m(IterArgs a, MedArgs m, const double4* __restrict__ trk_g) {
+  constexpr uint32_t BS = CW * SUB;
   extern __shared__ __align__(16) double lds_dyn[];
   const uint32_t vol = a.G.b[0] * a.G.b[1] * a.G.b[2];
   float2* brk_s = reinterpret_cast<float2*>(lds_dyn);
@@ -2098,6 +2101,8 @@ k_bwd32m(IterArgs a, MedArgs m, const double4* __restrict__ trk_g) {
   __shared__ uint32_t sh_[4];
   uint32_t& sh = sh_[0];
   const uint32_t t = threadIdx.x;
+  const uint32_t sub = SUB == 1 ? 0u : t / CW;  // this thread's chunk of the SUB (whole wavefronts)
+  const uint32_t tl = SUB == 1 ? t : t % CW;    // and its thread in that chunk
   fill_step_table(a, stab);
   const double U = a.ec.unit, U2 = U * U, U3 = U2 * U;
   for (;;) {
@@ -2127,31 +2132,34 @@ k_bwd32m(IterArgs a, MedArgs m, const double4* __restrict__ trk_g) {
       brk_s[v] = br;
       cnt_s[v] = 0u;
     }
-    for (uint32_t c = tk.c0; c < tk.c1; ++c) {
-      const Chunk ch = a.chunks[c];
-      if (t < ch.cnt) {
-        const uint64_t q = ch.q0 + t;
+    for (uint32_t c0 = tk.c0; c0 < tk.c1; c0 += SUB) {
+      Chunk ch{};  // no chunk (cnt = ne = 0) past the task's last
+      if (c0 + sub < tk.c1) ch = a.chunks[c0 + sub];
+      double4* sgb = sg + sub * CW;
+      uint32_t* svb = sv0 + sub * CW;
+      if (tl < ch.cnt) {
+        const uint64_t q = ch.q0 + tl;
         const uint32_t tfx = a.seg_tfx[q];
         const double4 g = trk_g[a.seg_trk[q]];
         // as in k_bwd32s: A^T G' A in FP64, in units of U
         const double shf = (double)((int64_t)tfx - (int64_t)g.w) * a.inv_scale;
         const double gy = fma(shf, g.z, g.y);
-        sg[t] = make_double4(fma(shf, gy + g.y, g.x) * U, gy * U2, g.z * U3, 0.0);
-        sv0[t] = a.seg_v0[q];
+        sgb[tl] = make_double4(fma(shf, gy + g.y, g.x) * U, gy * U2, g.z * U3, 0.0);
+        svb[tl] = a.seg_v0[q];
       }
-      const bool act = t * GE < ch.ne;
+      const bool act = tl * GE < ch.ne;
       Group gr;
-      if (act) gr.load(a, (ch.base / GE) + t);
+      if (act) gr.load(a, (ch.base / GE) + tl);
       __syncthreads();
       if (act) {
         gr.decode();
         uint32_t vx[GE];
-        gr.voxels(stab, [&](uint32_t q) { return sv0[q]; }, [&](int i, uint32_t v) { vx[i] = v; });
-        const uint32_t e0 = t * GE;
+        gr.voxels(stab, [&](uint32_t q) { return svb[q]; }, [&](int i, uint32_t v) { vx[i] = v; });
+        const uint32_t e0 = tl * GE;
         double T = gr.carry;
         uint32_t mk = gr.mask, sk = gr.s0, base_v = ENT_BASE;  // opaque copies, as in k_bwd32
         asm volatile("" : "+v"(mk), "+v"(sk), "+v"(base_v));
-        double4 g = sg[sk + (uint32_t)__popc(mk & 0xFFFEu)];
+        double4 g = sgb[sk + (uint32_t)__popc(mk & 0xFFFEu)];
 #pragma unroll
         for (int i = GE - 1; i >= 0; --i) {
           // c_ij exactly as k_bwd32s stores it
@@ -2177,11 +2185,11 @@ k_bwd32m(IterArgs a, MedArgs m, const double4* __restrict__ trk_g) {
                 m.buf[j * MED_K + pos] = cf;
             }
           }
-          if (i > 0 && st != 0u) g = sg[sk + (uint32_t)__popc(mk & ((2u << i) - 2u)) - 1u];
+          if (i > 0 && st != 0u) g = sgb[sk + (uint32_t)__popc(mk & ((2u << i) - 2u)) - 1u];
           T = st != 0u ? 0.0 : T + L;
         }
       }
-      __syncthreads();  // the tables are rewritten for the next chunk
+      __syncthreads();  // the tables are rewritten for the next chunks
     }
     if (!FB) {
       for (uint32_t v = t; v < vol; v += BS) {
@@ -2998,14 +3006,17 @@ static int med_setup(muto_em_plan* pl) {
   muto_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream;
   // the pass's LDS: bracket and counts per brick voxel, and its tables
+  // (two chunks per workgroup below 1024-thread blocks)
   const size_t lds = (size_t)f.bs.vol() * (sizeof(float2) + sizeof(uint32_t));
+  const size_t threads = f.bs.block <= 512 ? 2 * (size_t)f.bs.block : (size_t)f.bs.block;
   int max_lds = 0;
   MUTO_CHECK_HIP(ctx, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
-  if (lds + (size_t)f.bs.block * (sizeof(double4) + sizeof(uint32_t)) + 128 > (size_t)max_lds) return MUTO_OK;
+  if (lds + threads * (sizeof(double4) + sizeof(uint32_t)) + 128 > (size_t)max_lds) return MUTO_OK;
   int ob = 0;
   MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
-    constexpr uint32_t BS = decltype(B)::value;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob, k_bwd32m<BS, false>, BS, lds);
+    constexpr uint32_t CW = decltype(B)::value;
+    constexpr uint32_t SUB = CW <= 512 ? 2 : 1;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob, k_bwd32m<CW, SUB, false>, CW * SUB, lds);
   }));
   if (ob < 1) return MUTO_OK;
   f.grid_med = ob * ctx->n_cu;
@@ -3099,8 +3110,9 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     const MedArgs m = med_args(pl, pl->work.p + 2);
     MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
-      constexpr uint32_t BS = decltype(B)::value;
-      k_bwd32m<BS, false><<<f.grid_med, BS, lds_med, s>>>(a, m, g64);
+      constexpr uint32_t CW = decltype(B)::value;
+      constexpr uint32_t SUB = CW <= 512 ? 2 : 1;
+      k_bwd32m<CW, SUB, false><<<f.grid_med, CW * SUB, lds_med, s>>>(a, m, g64);
       return hipGetLastError();
     }));
   }
@@ -3127,8 +3139,9 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
     k_sel_brk<<<grid_for(pl->nv, 256), 256, 0, s>>>(a.G, pl->nv, pl->mj.p, m, pl->lam64.p, ml);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
-      constexpr uint32_t BS = decltype(B)::value;
-      k_bwd32m<BS, true><<<f.grid_med, BS, lds_med, s>>>(a, m, g64);
+      constexpr uint32_t CW = decltype(B)::value;
+      constexpr uint32_t SUB = CW <= 512 ? 2 : 1;
+      k_bwd32m<CW, SUB, true><<<f.grid_med, CW * SUB, lds_med, s>>>(a, m, g64);
       return hipGetLastError();
     }));
     const unsigned g = (unsigned)std::min<size_t>(grid_for(pl->nv, 4), (size_t)ctx->n_cu * 64);
