@@ -120,7 +120,9 @@ typedef struct muto_counters {
   int64_t nnz;              /* CSR entries (track-voxel pairs)                */
   int64_t n_segments;       /* FP32C: brick segments                          */
   int64_t iterations_done;
-  int64_t _reserved[5];
+  int64_t n_median_miss;    /* FP32C median by brackets: voxels whose median
+                               left their bracket (last it)                   */
+  int64_t _reserved[4];
 } muto_counters;
 
 /* ------------------------------------------------------------------ */

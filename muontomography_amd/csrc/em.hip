@@ -784,6 +784,7 @@ int muto_read_counters(muto_em_plan* pl) {
   pl->counters.n_newton_cap = (int64_t)c[C_NEWTON];
   pl->counters.n_out_of_grid = (int64_t)c[C_OUT];
   pl->counters.n_empty_path = (int64_t)c[C_EMPTY];
+  pl->counters.n_median_miss = (int64_t)c[C_MISS];
   return MUTO_OK;
 }
 

@@ -1993,11 +1993,24 @@ __global__ void k_fill_u8(size_t n, uint8_t* __restrict__ x, uint8_t v) {
 }
 
 // a voxel's first bracket: everything when M_j <= MED_K, else none (a miss,
-// so the first pass collects the whole list)
-__global__ void k_brk_init(size_t nv, const uint32_t* __restrict__ mj, float2* __restrict__ brk) {
+// so the first pass collects the whole list); no last median
+__global__ void k_brk_init(size_t nv, const uint32_t* __restrict__ mj, float2* __restrict__ brk,
+                           float* __restrict__ bmed) {
   const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < nv)
+  if (j < nv) {
     brk[j] = mj[j] <= MED_K ? make_float2(-INFINITY, INFINITY) : make_float2(INFINITY, -INFINITY);
+    bmed[j] = NAN;
+  }
+}
+
+// The next bracket: the values lo and hi MED_H ranks (or more) either side of
+// the median value cm, widened towards where the median is heading (cm plus
+// its last move, the same spread around it), since the medians drift the
+// same way pass after pass while the image converges.
+__device__ __forceinline__ float2 next_bracket(float lo, float hi, float cm, float last) {
+  if (!(last == last)) return make_float2(lo, hi);  // no last median
+  const float pred = cm + (cm - last);
+  return make_float2(fminf(lo, pred - (cm - lo)), fmaxf(hi, pred + (hi - cm)));
 }
 
 __global__ void k_fill_nan(size_t n, double* __restrict__ x) {
@@ -2031,7 +2044,7 @@ k_sel64b(uint32_t n_list, const uint32_t* __restrict__ list, const uint64_t* __r
          const uint32_t* __restrict__ mj, const uint64_t* __restrict__ col,
          const V* __restrict__ cval, int use_mean, int pct, double* __restrict__ scratch,
          double* __restrict__ lam, double* __restrict__ ml, const uint8_t* __restrict__ miss,
-         float2* __restrict__ brk) {
+         float2* __restrict__ brk, float* __restrict__ bmed) {
   __shared__ double a_s[4096];
   for (uint32_t b = blockIdx.x; b < n_list; b += gridDim.x) {
     const uint32_t j = list[b];
@@ -2047,9 +2060,12 @@ k_sel64b(uint32_t n_list, const uint32_t* __restrict__ list, const uint64_t* __r
     if (threadIdx.x == 0) {
       const double out = order_stat_c(lam[j], m, use_mean, pct, [&](uint32_t k) { return a[k]; });
       if (ml) ml[j] = out; else lam[j] = out;
-      if (brk) {
+      if (brk) {  // as k_sel_miss
         const uint32_t r1 = m / 2, r0 = m % 2 == 0 ? r1 - 1 : r1;
-        brk[j] = make_float2((float)a[r0 >= MED_H ? r0 - MED_H : 0u], (float)a[min(r1 + MED_H, m - 1)]);
+        const float cm = (float)a[r1];
+        brk[j] = next_bracket((float)a[r0 >= 2 * MED_H ? r0 - 2 * MED_H : 0u],
+                              (float)a[min(r1 + 2 * MED_H, m - 1)], cm, bmed[j]);
+        bmed[j] = cm;
       }
     }
     __syncthreads();
@@ -2075,12 +2091,14 @@ struct MedArgs {
   uint32_t n_btask;
   unsigned int* work;  // task counter of the pass
   float2* brk;         // [nv] bracket (lo, hi)
+  float* bmed;         // [nv] last median value (upper middle value; NaN: none yet)
   uint32_t* bcnt;      // [nv] values below | values inside << 16
   float* buf;          // [nv * MED_K] values inside
   uint8_t* miss;       // [nv] the median left the bracket
   uint8_t* bmiss;      // [bricks] the brick holds a miss
   const uint64_t* col; // [nv + 1] slot ranges (misses' whole lists)
   float* cval;
+  unsigned long long* nmiss;  // misses of the pass (a plan counter)
 };
 
 // FB = false: the bracket pass; FB = true: the bricks with misses, whose
@@ -2216,16 +2234,23 @@ __device__ __forceinline__ double median_update(double l, uint32_t m, double c_l
 // middle ranks (m/2 - 1 and m/2 for even m, m/2 for odd) fall inside it, with
 // the next bracket MED_H ranks either side (kept where the values run out);
 // else a miss, marked on the voxel and its brick.  M_j = 0 gives 0.
+__device__ __forceinline__ bool sel_brk_voxel(const BrickGeo& G, size_t j, uint32_t mm, const MedArgs& m,
+                                              double* __restrict__ lam, double* __restrict__ ml);
 __global__ void __launch_bounds__(256)
 k_sel_brk(BrickGeo G, size_t nv, const uint32_t* __restrict__ mj, MedArgs m,
           double* __restrict__ lam, double* __restrict__ ml) {
   const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nv) return;
-  const uint32_t mm = mj[j];
+  const bool miss = j < nv && sel_brk_voxel(G, j, mj[j], m, lam, ml);
+  const unsigned long long bm = __ballot(miss);
+  if ((threadIdx.x & 63u) == 0 && bm) atomicAdd(m.nmiss, (unsigned long long)__popcll(bm));
+}
+// one voxel of k_sel_brk; true on a miss
+__device__ __forceinline__ bool sel_brk_voxel(const BrickGeo& G, size_t j, uint32_t mm, const MedArgs& m,
+                                              double* __restrict__ lam, double* __restrict__ ml) {
   if (mm == 0) {
     if (ml) ml[j] = 0.0; else lam[j] = 0.0;
     m.miss[j] = 0;
-    return;
+    return false;
   }
   const uint32_t pk = m.bcnt[j], nlo = pk & 0xFFFFu, nin = pk >> 16;
   const uint32_t r1 = mm / 2, r0 = mm % 2 == 0 ? r1 - 1 : r1;
@@ -2234,7 +2259,7 @@ k_sel_brk(BrickGeo G, size_t nv, const uint32_t* __restrict__ mj, MedArgs m,
     const uint32_t tq = G.dn0.div((uint32_t)j), x = (uint32_t)j - tq * G.n[0];
     const uint32_t z = G.dn1.div(tq), y = tq - z * G.n[1];
     m.bmiss[G.db0.div(x) + G.nb[0] * (G.db1.div(y) + G.nb[1] * G.db2.div(z))] = 1;
-    return;
+    return true;
   }
   m.miss[j] = 0;
   float w[MED_K];
@@ -2277,7 +2302,11 @@ k_sel_brk(BrickGeo G, size_t nv, const uint32_t* __restrict__ mj, MedArgs m,
   }
   const double out = median_update(lam[j], mm, (double)ca, (double)cb);
   if (ml) ml[j] = out; else lam[j] = out;
-  if (mm > MED_K) m.brk[j] = make_float2(nl, nh);
+  if (mm > MED_K) {
+    m.brk[j] = next_bracket(nl, nh, cb, m.bmed[j]);
+    m.bmed[j] = cb;
+  }
+  return false;
 }
 
 // The missed voxels (M_j <= 256; longer ones go to k_sel64b): one wavefront
@@ -2285,7 +2314,7 @@ k_sel_brk(BrickGeo G, size_t nv, const uint32_t* __restrict__ mj, MedArgs m,
 // next bracket MED_H ranks either side of the median.
 template <int R>
 __device__ __forceinline__ void miss_select(const float* __restrict__ src, uint32_t mm, double l,
-                                            double* out, float2* nb) {
+                                            double* out, float2* nb, float* cm) {
   const uint32_t lane = threadIdx.x & 63u;
   double v[R];
 #pragma unroll
@@ -2295,31 +2324,44 @@ __device__ __forceinline__ void miss_select(const float* __restrict__ src, uint3
   }
   wave_bitonic<R>(v);
   const uint32_t r1 = mm / 2, r0 = mm % 2 == 0 ? r1 - 1 : r1;
-  *out = median_update(l, mm, wave_elem<R>(v, r0), wave_elem<R>(v, r1));
-  *nb = make_float2((float)wave_elem<R>(v, r0 >= MED_H ? r0 - MED_H : 0u),
-                    (float)wave_elem<R>(v, min(r1 + MED_H, mm - 1)));
+  const double cb = wave_elem<R>(v, r1);
+  *out = median_update(l, mm, wave_elem<R>(v, r0), cb);
+  // a voxel that missed gets twice the ranks either side
+  *nb = make_float2((float)wave_elem<R>(v, r0 >= 2 * MED_H ? r0 - 2 * MED_H : 0u),
+                    (float)wave_elem<R>(v, min(r1 + 2 * MED_H, mm - 1)));
+  *cm = (float)cb;
 }
 __global__ void __launch_bounds__(256)
 k_sel_miss(size_t nv, const uint32_t* __restrict__ mj, MedArgs m, double* __restrict__ lam,
            double* __restrict__ ml) {
+  // each wavefront looks at 64 voxels at a time and sorts the ones missed
   const size_t nw = (size_t)gridDim.x * (blockDim.x >> 6);
-  for (size_t j = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; j < nv; j += nw) {
-    if (!m.miss[j]) continue;
-    const uint32_t mm = mj[j];
-    if (mm > 256) continue;
-    const float* src = m.cval + m.col[j];
-    const double l = lam[j];
-    double out;
-    float2 nb;
-    if (mm <= 64)
-      miss_select<1>(src, mm, l, &out, &nb);
-    else if (mm <= 128)
-      miss_select<2>(src, mm, l, &out, &nb);
-    else
-      miss_select<4>(src, mm, l, &out, &nb);
-    if ((threadIdx.x & 63u) == 0) {
-      if (ml) ml[j] = out; else lam[j] = out;
-      if (mm > MED_K) m.brk[j] = nb;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (size_t base = (((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64; base < nv; base += nw * 64) {
+    const size_t jl = base + lane;
+    unsigned long long bm = __ballot(jl < nv && m.miss[jl] && mj[jl] <= 256);
+    while (bm) {
+      const size_t j = base + (size_t)(__ffsll((long long)bm) - 1);
+      bm &= bm - 1;
+      const uint32_t mm = mj[j];
+      const float* src = m.cval + m.col[j];
+      const double l = lam[j];
+      double out;
+      float2 nb;
+      float cm;
+      if (mm <= 64)
+        miss_select<1>(src, mm, l, &out, &nb, &cm);
+      else if (mm <= 128)
+        miss_select<2>(src, mm, l, &out, &nb, &cm);
+      else
+        miss_select<4>(src, mm, l, &out, &nb, &cm);
+      if (lane == 0) {
+        if (ml) ml[j] = out; else lam[j] = out;
+        if (mm > MED_K) {
+          m.brk[j] = next_bracket(nb.x, nb.y, cm, m.bmed[j]);
+          m.bmed[j] = cm;
+        }
+      }
     }
   }
 }
@@ -2404,7 +2446,7 @@ __global__ void k_iter_begin(uint32_t n_scans, double* __restrict__ cost_acc,
                              unsigned long long* __restrict__ cmax, unsigned int* __restrict__ work,
                              unsigned int* __restrict__ qwork, uint32_t nq) {
   for (uint32_t k = threadIdx.x; k < n_scans; k += blockDim.x) cost_acc[k] = 0.0;
-  if (threadIdx.x < 3) cnt[threadIdx.x] = 0ull;
+  if (threadIdx.x < 3 || threadIdx.x == C_MISS) cnt[threadIdx.x] = 0ull;
   if (threadIdx.x == 0) cmax[0] = 0ull;  // the track kernel sets this pass's |c| bound
   if (threadIdx.x < 4) work[threadIdx.x] = 0u;  // (2, 3: the median's bracket passes)
   if (qwork && threadIdx.x < nq) qwork[threadIdx.x] = 0u;  // order statistics: per-XCD queues
@@ -2987,12 +3029,14 @@ static MedArgs med_args(muto_em_plan* pl, unsigned int* work) {
   m.n_btask = f.n_btask;
   m.work = work;
   m.brk = f.brk.p;
+  m.bmed = f.bmed.p;
   m.bcnt = f.bcnt.p;
   m.buf = f.bbuf.p;
   m.miss = f.bmiss_v.p;
   m.bmiss = f.bmiss_b.p;
   m.col = f.col64.p;
   m.cval = reinterpret_cast<float*>(f.cval.p);
+  m.nmiss = pl->dcount.p + C_MISS;
   return m;
 }
 
@@ -3056,11 +3100,12 @@ static int med_setup(muto_em_plan* pl) {
     MUTO_CHECK_HIP(ctx, copy_sync(ctx, f.btask.p, sorted.data(), sorted.size() * sizeof(Task), hipMemcpyHostToDevice));
   f.n_btask = (uint32_t)sorted.size();
   MUTO_CHECK_HIP(ctx, f.brk.alloc(nv));
+  MUTO_CHECK_HIP(ctx, f.bmed.alloc(nv));
   MUTO_CHECK_HIP(ctx, f.bcnt.alloc(nv));
   MUTO_CHECK_HIP(ctx, f.bbuf.alloc(nv * MED_K));
   MUTO_CHECK_HIP(ctx, f.bmiss_v.alloc(nv));
   MUTO_CHECK_HIP(ctx, f.bmiss_b.alloc(std::max<uint64_t>(nbk, 1)));
-  k_brk_init<<<grid_for(nv, 256), 256, 0, s>>>(nv, pl->mj.p, f.brk.p);
+  k_brk_init<<<grid_for(nv, 256), 256, 0, s>>>(nv, pl->mj.p, f.brk.p, f.bmed.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
   f.med_ready = true;
@@ -3144,13 +3189,13 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
       k_bwd32m<CW, SUB, true><<<f.grid_med, CW * SUB, lds_med, s>>>(a, m, g64);
       return hipGetLastError();
     }));
-    const unsigned g = (unsigned)std::min<size_t>(grid_for(pl->nv, 4), (size_t)ctx->n_cu * 64);
+    const unsigned g = (unsigned)std::min<size_t>(grid_for(pl->nv, 256), (size_t)ctx->n_cu * 16);
     k_sel_miss<<<g, 256, 0, s>>>(pl->nv, pl->mj.p, m, pl->lam64.p, ml);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     if (f.n_big64) {
       k_sel64b<float><<<std::min<uint32_t>(f.n_big64, 4096), 1024, 0, s>>>(
           f.n_big64, f.big64.p, f.big64_off.p, pl->mj.p, f.col64.p, cval32, 0, 100, f.big64_scr.p,
-          pl->lam64.p, ml, f.bmiss_v.p, f.brk.p);
+          pl->lam64.p, ml, f.bmiss_v.p, f.brk.p, f.bmed.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
     if (!map) {  // the forward pass reads the float mirror
@@ -3167,7 +3212,7 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
     if (f.n_big64) {
       k_sel64b<float><<<std::min<uint32_t>(f.n_big64, 4096), 1024, 0, s>>>(
           f.n_big64, f.big64.p, f.big64_off.p, pl->mj.p, f.col64.p, cval32, c.update_with_mean,
-          c.mean_percentile, f.big64_scr.p, pl->lam64.p, ml, nullptr, nullptr);
+          c.mean_percentile, f.big64_scr.p, pl->lam64.p, ml, nullptr, nullptr, nullptr);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
     if (!map) {  // the forward pass reads the float mirror
@@ -3552,7 +3597,7 @@ static int fp64s_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
     if (f.n_big64) {
       k_sel64b<double><<<std::min<uint32_t>(f.n_big64, 4096), 1024, 0, s>>>(
           f.n_big64, f.big64.p, f.big64_off.p, pl->mj.p, f.col64.p, f.cval.p, c.update_with_mean,
-          c.mean_percentile, f.big64_scr.p, pl->lam64.p, ml, nullptr, nullptr);
+          c.mean_percentile, f.big64_scr.p, pl->lam64.p, ml, nullptr, nullptr, nullptr);
       MUTO_CHECK_HIP(ctx, hipGetLastError());
     }
   }

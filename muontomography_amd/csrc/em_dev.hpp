@@ -89,7 +89,7 @@ __device__ __forceinline__ void wave_count(unsigned long long* dst, bool v) {
 }
 
 // device counter slots
-enum { C_SINGULAR = 0, C_CUBIC = 1, C_NEWTON = 2, C_OUT = 3, C_EMPTY = 4 };
+enum { C_SINGULAR = 0, C_CUBIC = 1, C_NEWTON = 2, C_OUT = 3, C_EMPTY = 4, C_MISS = 5 };
 
 struct MapCfgD {
   GridD g;

@@ -120,6 +120,7 @@ struct Fp32cData {
   uint32_t n_btask = 0;
   int grid_med = 0;
   DBuf<float2> brk;
+  DBuf<float> bmed;
   DBuf<uint32_t> bcnt;
   DBuf<float> bbuf;
   DBuf<uint8_t> bmiss_v, bmiss_b;
@@ -182,6 +183,7 @@ struct Fp32cData {
     n_btask = 0;
     grid_med = 0;
     brk.release();
+    bmed.release();
     bcnt.release();
     bbuf.release();
     bmiss_v.release();
