@@ -1234,6 +1234,25 @@ k_bwd32(IterArgs a, const float* __restrict__ trk_g,
 // own backward recurrence T_{k-1} = T_k + L_k inside a segment.  So every
 // W_ij (src/MutoMLSD.cpp:417-430) is bit-identical to the reference's; the
 // sums over a track differ from its running order only in association.
+// FP32C and FP64 stream medians by brackets (k_bwd32m, k_bwd64s modes 2-3,
+// k_sel_brk): the arrays of one pass
+struct MedArgs {
+  const Task* btask;   // one task per brick, largest first
+  uint32_t n_btask;
+  unsigned int* work;  // task counter of the pass
+  float2* brk;         // [nv] bracket (lo, hi)
+  float* bmed;         // [nv] last median value (upper middle value; NaN: none yet)
+  uint32_t* bcnt;      // [nv] values below | values inside << 16
+  float* buf;          // [nv * MED_K] values inside (FP32C)
+  double* buf64;       // ... (FP64 stream)
+  uint8_t* miss;       // [nv] the median left the bracket
+  uint8_t* bmiss;      // [bricks] the brick holds a miss
+  const uint64_t* col; // [nv + 1] slot ranges (misses' whole lists)
+  float* cval;         // (FP32C)
+  double* cval64;      // (FP64 stream)
+  unsigned long long* nmiss;  // misses of the pass (a plan counter)
+};
+
 struct Args64 {
   const double* len64;
   const double* gtail64;
@@ -1460,9 +1479,13 @@ k_track64s(size_t n_trk, const uint32_t* __restrict__ tseg_ptr, const uint32_t* 
 // the entry's voxel-major slot.
 template <int MODE, uint32_t BS>
 __global__ void __launch_bounds__(BS)
-k_bwd64s(IterArgs a, Args64 b, const double4* __restrict__ trk_g, double* __restrict__ acc) {
+k_bwd64s(IterArgs a, Args64 b, MedArgs md, const double4* __restrict__ trk_g, double* __restrict__ acc) {
+  // MODE 2: the median's bracket pass, MODE 3: its bricks with misses (as
+  // k_bwd32m, on the exact FP64 values)
+  constexpr bool MED = MODE >= 2;
   extern __shared__ __align__(16) double lds_dyn[];
   double* acc_s = lds_dyn;
+  float2* brk_s = reinterpret_cast<float2*>(lds_dyn);
   __shared__ double4 sg[BS];  // per segment: p^2 G of its track (x, y, z), tail of its last entry (w)
   __shared__ uint32_t sv0[BS];
   __shared__ int stab[8];
@@ -1471,15 +1494,40 @@ k_bwd64s(IterArgs a, Args64 b, const double4* __restrict__ trk_g, double* __rest
   const uint32_t t = threadIdx.x;
   fill_step_table(a, stab);
   const uint32_t vol = a.G.b[0] * a.G.b[1] * a.G.b[2];
+  uint32_t* cnt_s = reinterpret_cast<uint32_t*>(brk_s + vol);
   const uint32_t qi = MODE == 1 ? blockIdx.x % b.nq : 0u;
   for (;;) {
-    if (t == 0) sh = MODE == 1 ? b.qoff[qi] + atomicAdd(&b.qwork[qi], 1u) : atomicAdd(&a.work[1], 1u);
+    if (t == 0)
+      sh = MED ? atomicAdd(md.work, 1u) : MODE == 1 ? b.qoff[qi] + atomicAdd(&b.qwork[qi], 1u) : atomicAdd(&a.work[1], 1u);
     if (MODE == 0)
       for (uint32_t v = t; v < vol; v += BS) acc_s[v] = 0.0;
     __syncthreads();
     const uint32_t ti = sh;
-    if (MODE == 1 ? ti >= b.qoff[qi + 1] : ti >= a.n_tasks) break;
-    const Task tk = MODE == 1 ? b.qtask[ti] : a.tasks[ti];
+    if (MED ? ti >= md.n_btask : MODE == 1 ? ti >= b.qoff[qi + 1] : ti >= a.n_tasks) break;
+    const Task tk = MED ? md.btask[ti] : MODE == 1 ? b.qtask[ti] : a.tasks[ti];
+    if (MODE == 3 && !md.bmiss[tk.brick]) {
+      __syncthreads();  // every thread has read sh before it is rewritten
+      continue;
+    }
+    uint32_t o[3];
+    brick_origin(a.G, tk.brick, o);
+    if (MED) {
+      for (uint32_t v = t; v < vol; v += BS) {
+        const uint32_t rr = a.G.db0.div(v), lx = v - rr * a.G.b[0];
+        const uint32_t lz = a.G.db1.div(rr), ly = rr - lz * a.G.b[1];
+        const uint32_t x = o[0] + lx, y = o[1] + ly, z = o[2] + lz;
+        float2 br = make_float2(INFINITY, -INFINITY);
+        if (x < a.G.n[0] && y < a.G.n[1] && z < a.G.n[2]) {
+          const size_t j = (size_t)x + (size_t)a.G.n[0] * ((size_t)y + (size_t)a.G.n[1] * z);
+          if (MODE == 3)
+            br = md.miss[j] ? make_float2(-INFINITY, INFINITY) : make_float2(INFINITY, -INFINITY);
+          else
+            br = md.brk[j];
+        }
+        brk_s[v] = br;
+        cnt_s[v] = 0u;
+      }
+    }
     for (uint32_t c = tk.c0; c < tk.c1; ++c) {
       const Chunk ch = a.chunks[c];
       if (t < ch.cnt) {
@@ -1546,8 +1594,25 @@ k_bwd64s(IterArgs a, Args64 b, const double4* __restrict__ trk_g, double* __rest
             const double cv = L * g.x + 2.0 * w01 * g.y + w11 * g.z;
             if (MODE == 0) {
               atomicAdd(&acc_s[vx[i]], cv);
-            } else if (sl[i] != 0xFFFFFFFFu) {
-              b.cval[sl[i]] = cv;
+            } else if (MODE == 1) {
+              if (sl[i] != 0xFFFFFFFFu) b.cval[sl[i]] = cv;
+            } else if (t * GE + (uint32_t)i < ch.ne && ent_code(gr.w[i]) != ENT_PAD_CODE) {
+              // the bracket's FP32 bounds against the exact value
+              const uint32_t v = vx[i];
+              const float2 br = brk_s[v];
+              if (cv < (double)br.x) {
+                atomicAdd(&cnt_s[v], 1u);
+              } else if (!(cv > (double)br.y)) {
+                const uint32_t pos = atomicAdd(&cnt_s[v], 0x10000u) >> 16;
+                const uint32_t rr = a.G.db0.div(v), lx = v - rr * a.G.b[0];
+                const uint32_t lz = a.G.db1.div(rr), ly = rr - lz * a.G.b[1];
+                const size_t j = (size_t)(o[0] + lx) +
+                                 (size_t)a.G.n[0] * ((size_t)(o[1] + ly) + (size_t)a.G.n[1] * (o[2] + lz));
+                if (MODE == 3)
+                  md.cval64[md.col[j] + pos] = cv;
+                else if (pos < MED_K)
+                  md.buf64[j * MED_K + pos] = cv;
+              }
             }
             if (i > 0 && i % SEG_ALIGN == 0 && ((mk >> i) & 1u)) {  // entry i - 1 ends the segment before
               const uint32_t q = sk + (uint32_t)__popc(mk & ((2u << i) - 2u)) - 1u;
@@ -1562,9 +1627,16 @@ k_bwd64s(IterArgs a, Args64 b, const double4* __restrict__ trk_g, double* __rest
       }
       __syncthreads();  // the tables are rewritten for the next chunk
     }
+    if (MODE == 2) {
+      for (uint32_t v = t; v < vol; v += BS) {
+        const uint32_t rr = a.G.db0.div(v), lx = v - rr * a.G.b[0];
+        const uint32_t lz = a.G.db1.div(rr), ly = rr - lz * a.G.b[1];
+        const uint32_t x = o[0] + lx, y = o[1] + ly, z = o[2] + lz;
+        if (x >= a.G.n[0] || y >= a.G.n[1] || z >= a.G.n[2]) continue;
+        md.bcnt[(size_t)x + (size_t)a.G.n[0] * ((size_t)y + (size_t)a.G.n[1] * z)] = cnt_s[v];
+      }
+    }
     if (MODE == 0) {
-      uint32_t o[3];
-      brick_origin(a.G, tk.brick, o);
       const bool shared = tk.ntask > 1;
       for (uint32_t v = t; v < vol; v += BS) {
         const uint32_t rr = a.G.db0.div(v), lx = v - rr * a.G.b[0];
@@ -2003,6 +2075,12 @@ __global__ void k_brk_init(size_t nv, const uint32_t* __restrict__ mj, float2* _
   }
 }
 
+// a value as a bracket bound: FP32, rounded outwards (exact for FP32C values)
+__device__ __forceinline__ float bound_lo(float x) { return x; }
+__device__ __forceinline__ float bound_hi(float x) { return x; }
+__device__ __forceinline__ float bound_lo(double x) { return __double2float_rd(x); }
+__device__ __forceinline__ float bound_hi(double x) { return __double2float_ru(x); }
+
 // The next bracket: the values lo and hi MED_H ranks (or more) either side of
 // the median value cm, widened towards where the median is heading (cm plus
 // its last move, the same spread around it), since the medians drift the
@@ -2063,8 +2141,8 @@ k_sel64b(uint32_t n_list, const uint32_t* __restrict__ list, const uint64_t* __r
       if (brk) {  // as k_sel_miss
         const uint32_t r1 = m / 2, r0 = m % 2 == 0 ? r1 - 1 : r1;
         const float cm = (float)a[r1];
-        brk[j] = next_bracket((float)a[r0 >= 2 * MED_H ? r0 - 2 * MED_H : 0u],
-                              (float)a[min(r1 + 2 * MED_H, m - 1)], cm, bmed[j]);
+        brk[j] = next_bracket(bound_lo(a[r0 >= 2 * MED_H ? r0 - 2 * MED_H : 0u]),
+                              bound_hi(a[min(r1 + 2 * MED_H, m - 1)]), cm, bmed[j]);
         bmed[j] = cm;
       }
     }
@@ -2086,20 +2164,6 @@ k_sel64b(uint32_t n_list, const uint32_t* __restrict__ list, const uint64_t* __r
 // through a second pass that collects their whole lists into their slot
 // ranges, and a sort.
 
-struct MedArgs {
-  const Task* btask;   // one task per brick, largest first
-  uint32_t n_btask;
-  unsigned int* work;  // task counter of the pass
-  float2* brk;         // [nv] bracket (lo, hi)
-  float* bmed;         // [nv] last median value (upper middle value; NaN: none yet)
-  uint32_t* bcnt;      // [nv] values below | values inside << 16
-  float* buf;          // [nv * MED_K] values inside
-  uint8_t* miss;       // [nv] the median left the bracket
-  uint8_t* bmiss;      // [bricks] the brick holds a miss
-  const uint64_t* col; // [nv + 1] slot ranges (misses' whole lists)
-  float* cval;
-  unsigned long long* nmiss;  // misses of the pass (a plan counter)
-};
 
 // FB = false: the bracket pass; FB = true: the bricks with misses, whose
 // missed voxels' values all go to their slot ranges.  A workgroup of SUB
@@ -2234,17 +2298,21 @@ __device__ __forceinline__ double median_update(double l, uint32_t m, double c_l
 // middle ranks (m/2 - 1 and m/2 for even m, m/2 for odd) fall inside it, with
 // the next bracket MED_H ranks either side (kept where the values run out);
 // else a miss, marked on the voxel and its brick.  M_j = 0 gives 0.
+template <class V>
 __device__ __forceinline__ bool sel_brk_voxel(const BrickGeo& G, size_t j, uint32_t mm, const MedArgs& m,
                                               double* __restrict__ lam, double* __restrict__ ml);
+// V: float (FP32C, values in m.buf) or double (FP64 stream, m.buf64)
+template <class V>
 __global__ void __launch_bounds__(256)
 k_sel_brk(BrickGeo G, size_t nv, const uint32_t* __restrict__ mj, MedArgs m,
           double* __restrict__ lam, double* __restrict__ ml) {
   const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool miss = j < nv && sel_brk_voxel(G, j, mj[j], m, lam, ml);
+  const bool miss = j < nv && sel_brk_voxel<V>(G, j, mj[j], m, lam, ml);
   const unsigned long long bm = __ballot(miss);
   if ((threadIdx.x & 63u) == 0 && bm) atomicAdd(m.nmiss, (unsigned long long)__popcll(bm));
 }
 // one voxel of k_sel_brk; true on a miss
+template <class V>
 __device__ __forceinline__ bool sel_brk_voxel(const BrickGeo& G, size_t j, uint32_t mm, const MedArgs& m,
                                               double* __restrict__ lam, double* __restrict__ ml) {
   if (mm == 0) {
@@ -2262,15 +2330,25 @@ __device__ __forceinline__ bool sel_brk_voxel(const BrickGeo& G, size_t j, uint3
     return true;
   }
   m.miss[j] = 0;
-  float w[MED_K];
-  const float4* src = reinterpret_cast<const float4*>(m.buf + j * MED_K);
+  V w[MED_K];
+  if constexpr (std::is_same<V, float>::value) {
+    const float4* src = reinterpret_cast<const float4*>(m.buf + j * MED_K);
 #pragma unroll
-  for (uint32_t k = 0; k < MED_K / 4; ++k) {
-    const float4 q = src[k];
-    w[4 * k] = 4 * k < nin ? q.x : INFINITY;
-    w[4 * k + 1] = 4 * k + 1 < nin ? q.y : INFINITY;
-    w[4 * k + 2] = 4 * k + 2 < nin ? q.z : INFINITY;
-    w[4 * k + 3] = 4 * k + 3 < nin ? q.w : INFINITY;
+    for (uint32_t k = 0; k < MED_K / 4; ++k) {
+      const float4 q = src[k];
+      w[4 * k] = 4 * k < nin ? q.x : INFINITY;
+      w[4 * k + 1] = 4 * k + 1 < nin ? q.y : INFINITY;
+      w[4 * k + 2] = 4 * k + 2 < nin ? q.z : INFINITY;
+      w[4 * k + 3] = 4 * k + 3 < nin ? q.w : INFINITY;
+    }
+  } else {
+    const double2* src = reinterpret_cast<const double2*>(m.buf64 + j * MED_K);
+#pragma unroll
+    for (uint32_t k = 0; k < MED_K / 2; ++k) {
+      const double2 q = src[k];
+      w[2 * k] = 2 * k < nin ? q.x : INFINITY;
+      w[2 * k + 1] = 2 * k + 1 < nin ? q.y : INFINITY;
+    }
   }
   // bitonic sorting network, ascending
 #pragma unroll
@@ -2281,7 +2359,7 @@ __device__ __forceinline__ bool sel_brk_voxel(const BrickGeo& G, size_t j, uint3
       for (uint32_t i = 0; i < MED_K; ++i) {
         const uint32_t p = i ^ jd;
         if (p > i) {
-          const float lo = fminf(w[i], w[p]), hi = fmaxf(w[i], w[p]);
+          const V lo = fmin(w[i], w[p]), hi = fmax(w[i], w[p]);
           const bool up = (i & kk) == 0;
           w[i] = up ? lo : hi;
           w[p] = up ? hi : lo;
@@ -2292,19 +2370,20 @@ __device__ __forceinline__ bool sel_brk_voxel(const BrickGeo& G, size_t j, uint3
   const uint32_t ia = r0 - nlo, ib = r1 - nlo;
   const float2 br = m.brk[j];
   const uint32_t il = ia >= MED_H ? ia - MED_H : MED_K, ih = ib + MED_H < nin ? ib + MED_H : MED_K;
-  float ca = 0.f, cb = 0.f, nl = br.x, nh = br.y;
+  V ca = 0, cb = 0;
+  float nl = br.x, nh = br.y;
 #pragma unroll
   for (uint32_t k = 0; k < MED_K; ++k) {
     if (k == ia) ca = w[k];
     if (k == ib) cb = w[k];
-    if (k == il) nl = w[k];
-    if (k == ih) nh = w[k];
+    if (k == il) nl = bound_lo(w[k]);
+    if (k == ih) nh = bound_hi(w[k]);
   }
   const double out = median_update(lam[j], mm, (double)ca, (double)cb);
   if (ml) ml[j] = out; else lam[j] = out;
   if (mm > MED_K) {
-    m.brk[j] = next_bracket(nl, nh, cb, m.bmed[j]);
-    m.bmed[j] = cb;
+    m.brk[j] = next_bracket(nl, nh, (float)cb, m.bmed[j]);
+    m.bmed[j] = (float)cb;
   }
   return false;
 }
@@ -2312,8 +2391,8 @@ __device__ __forceinline__ bool sel_brk_voxel(const BrickGeo& G, size_t j, uint3
 // The missed voxels (M_j <= 256; longer ones go to k_sel64b): one wavefront
 // per voxel sorts its whole list, as k_sel64w without a pivot, and sets the
 // next bracket MED_H ranks either side of the median.
-template <int R>
-__device__ __forceinline__ void miss_select(const float* __restrict__ src, uint32_t mm, double l,
+template <int R, class V>
+__device__ __forceinline__ void miss_select(const V* __restrict__ src, uint32_t mm, double l,
                                             double* out, float2* nb, float* cm) {
   const uint32_t lane = threadIdx.x & 63u;
   double v[R];
@@ -2327,10 +2406,11 @@ __device__ __forceinline__ void miss_select(const float* __restrict__ src, uint3
   const double cb = wave_elem<R>(v, r1);
   *out = median_update(l, mm, wave_elem<R>(v, r0), cb);
   // a voxel that missed gets twice the ranks either side
-  *nb = make_float2((float)wave_elem<R>(v, r0 >= 2 * MED_H ? r0 - 2 * MED_H : 0u),
-                    (float)wave_elem<R>(v, min(r1 + 2 * MED_H, mm - 1)));
+  *nb = make_float2(bound_lo(wave_elem<R>(v, r0 >= 2 * MED_H ? r0 - 2 * MED_H : 0u)),
+                    bound_hi(wave_elem<R>(v, min(r1 + 2 * MED_H, mm - 1))));
   *cm = (float)cb;
 }
+template <class V>
 __global__ void __launch_bounds__(256)
 k_sel_miss(size_t nv, const uint32_t* __restrict__ mj, MedArgs m, double* __restrict__ lam,
            double* __restrict__ ml) {
@@ -2344,17 +2424,17 @@ k_sel_miss(size_t nv, const uint32_t* __restrict__ mj, MedArgs m, double* __rest
       const size_t j = base + (size_t)(__ffsll((long long)bm) - 1);
       bm &= bm - 1;
       const uint32_t mm = mj[j];
-      const float* src = m.cval + m.col[j];
+      const V* src = std::is_same<V, float>::value ? (const V*)(m.cval + m.col[j]) : (const V*)(m.cval64 + m.col[j]);
       const double l = lam[j];
       double out;
       float2 nb;
       float cm;
       if (mm <= 64)
-        miss_select<1>(src, mm, l, &out, &nb, &cm);
+        miss_select<1, V>(src, mm, l, &out, &nb, &cm);
       else if (mm <= 128)
-        miss_select<2>(src, mm, l, &out, &nb, &cm);
+        miss_select<2, V>(src, mm, l, &out, &nb, &cm);
       else
-        miss_select<4>(src, mm, l, &out, &nb, &cm);
+        miss_select<4, V>(src, mm, l, &out, &nb, &cm);
       if (lane == 0) {
         if (ml) ml[j] = out; else lam[j] = out;
         if (mm > MED_K) {
@@ -2376,8 +2456,8 @@ __global__ void k_iter_begin64(double* __restrict__ cost_acc, unsigned long long
                                unsigned int* __restrict__ work, unsigned int* __restrict__ qwork,
                                uint32_t nq) {
   if (threadIdx.x == 0) cost_acc[0] = 0.0;
-  if (threadIdx.x < 3) cnt[threadIdx.x] = 0ull;
-  if (threadIdx.x < 2) work[threadIdx.x] = 0u;
+  if (threadIdx.x < 3 || threadIdx.x == C_MISS) cnt[threadIdx.x] = 0ull;
+  if (threadIdx.x < 4) work[threadIdx.x] = 0u;  // (2, 3: the median's bracket passes)
   if (qwork && threadIdx.x < nq) qwork[threadIdx.x] = 0u;
 }
 
@@ -3019,7 +3099,18 @@ static int fp64s_slots(muto_em_plan* pl);
 // arrays built (med_setup)
 static bool fp32c_brackets(const muto_em_plan* pl) {
   const Fp32cData& f = pl->f32;
-  return !pl->cfg.update_with_mean && f.med_brk && f.med_ready;
+  return !pl->cfg.update_with_mean && f.med_brk && f.med_ready && f.bbuf.p && f.grid_med > 0;
+}
+static bool fp64s_brackets(const muto_em_plan* pl) {
+  const Fp32cData& f = pl->f32;
+  return !pl->cfg.update_with_mean && f.med_brk && f.med_ready && f.bbuf64.p && f.grid_med64 > 0;
+}
+
+// the FP64 stream's passes run 256- or 512-thread blocks
+template <class F>
+static hipError_t by_block64(uint32_t block, F&& f) {
+  if (block == 512) return f(std::integral_constant<uint32_t, 512>{});
+  return f(std::integral_constant<uint32_t, 256>{});
 }
 
 static MedArgs med_args(muto_em_plan* pl, unsigned int* work) {
@@ -3032,10 +3123,12 @@ static MedArgs med_args(muto_em_plan* pl, unsigned int* work) {
   m.bmed = f.bmed.p;
   m.bcnt = f.bcnt.p;
   m.buf = f.bbuf.p;
+  m.buf64 = f.bbuf64.p;
   m.miss = f.bmiss_v.p;
   m.bmiss = f.bmiss_b.p;
   m.col = f.col64.p;
   m.cval = reinterpret_cast<float*>(f.cval.p);
+  m.cval64 = f.cval.p;
   m.nmiss = pl->dcount.p + C_MISS;
   return m;
 }
@@ -3044,71 +3137,88 @@ static MedArgs med_args(muto_em_plan* pl, unsigned int* work) {
 // arrays, every voxel starting without a bracket unless M_j <= MED_K.  Needs
 // the slot ranges (fp64s_slots).  Skipped (the slot pass then runs) when a
 // voxel has 65536 entries or more.
-static int med_setup(muto_em_plan* pl) {
+static int med_setup(muto_em_plan* pl, bool fp64) {
   Fp32cData& f = pl->f32;
-  if (f.med_ready || !f.med_brk || pl->cfg.update_with_mean || f.max_mj >= 65536u) return MUTO_OK;
+  if (!f.med_brk || pl->cfg.update_with_mean || f.max_mj >= 65536u) return MUTO_OK;
+  if (fp64 ? (f.bbuf64.p != nullptr) : (f.bbuf.p != nullptr)) return MUTO_OK;
   muto_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream;
+  const size_t nv = pl->nv;
   // the pass's LDS: bracket and counts per brick voxel, and its tables
-  // (two chunks per workgroup below 1024-thread blocks)
+  // (FP32C: two chunks per workgroup below 1024-thread blocks)
   const size_t lds = (size_t)f.bs.vol() * (sizeof(float2) + sizeof(uint32_t));
-  const size_t threads = f.bs.block <= 512 ? 2 * (size_t)f.bs.block : (size_t)f.bs.block;
+  const size_t threads = !fp64 && f.bs.block <= 512 ? 2 * (size_t)f.bs.block : (size_t)f.bs.block;
   int max_lds = 0;
   MUTO_CHECK_HIP(ctx, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
   if (lds + threads * (sizeof(double4) + sizeof(uint32_t)) + 128 > (size_t)max_lds) return MUTO_OK;
   int ob = 0;
-  MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
-    constexpr uint32_t CW = decltype(B)::value;
-    constexpr uint32_t SUB = CW <= 512 ? 2 : 1;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob, k_bwd32m<CW, SUB, false>, CW * SUB, lds);
-  }));
+  if (fp64) {
+    MUTO_CHECK_HIP(ctx, by_block64(f.bs.block, [&](auto B) {
+      constexpr uint32_t BS = decltype(B)::value;
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob, k_bwd64s<2, BS>, BS, lds);
+    }));
+  } else {
+    MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
+      constexpr uint32_t CW = decltype(B)::value;
+      constexpr uint32_t SUB = CW <= 512 ? 2 : 1;
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob, k_bwd32m<CW, SUB, false>, CW * SUB, lds);
+    }));
+  }
   if (ob < 1) return MUTO_OK;
-  f.grid_med = ob * ctx->n_cu;
-  std::vector<Task> tk(f.n_tasks);
-  if (f.n_tasks)
-    MUTO_CHECK_HIP(ctx, copy_sync(ctx, tk.data(), f.tasks.p, f.n_tasks * sizeof(Task), hipMemcpyDeviceToHost));
-  std::vector<Chunk> ch(f.n_chunks);
-  if (f.n_chunks)
-    MUTO_CHECK_HIP(ctx, copy_sync(ctx, ch.data(), f.chunks.p, f.n_chunks * sizeof(Chunk), hipMemcpyDeviceToHost));
-  std::sort(tk.begin(), tk.end(), [](const Task& x, const Task& y) {
-    return x.brick != y.brick ? x.brick < y.brick : x.c0 < y.c0;
-  });
-  std::vector<Task> bt;
-  std::vector<uint64_t> ne;
-  for (const Task& t : tk) {
-    if (!bt.empty() && bt.back().brick == t.brick) {
-      bt.back().c0 = std::min(bt.back().c0, t.c0);
-      bt.back().c1 = std::max(bt.back().c1, t.c1);
-    } else {
-      bt.push_back(Task{t.brick, 1u, t.c0, t.c1});
+  if (!f.med_ready) {
+    // one task per brick, largest first, and the brackets (shared by both
+    // precisions: any bracket gives the same medians)
+    std::vector<Task> tk(f.n_tasks);
+    if (f.n_tasks)
+      MUTO_CHECK_HIP(ctx, copy_sync(ctx, tk.data(), f.tasks.p, f.n_tasks * sizeof(Task), hipMemcpyDeviceToHost));
+    std::vector<Chunk> ch(f.n_chunks);
+    if (f.n_chunks)
+      MUTO_CHECK_HIP(ctx, copy_sync(ctx, ch.data(), f.chunks.p, f.n_chunks * sizeof(Chunk), hipMemcpyDeviceToHost));
+    std::sort(tk.begin(), tk.end(), [](const Task& x, const Task& y) {
+      return x.brick != y.brick ? x.brick < y.brick : x.c0 < y.c0;
+    });
+    std::vector<Task> bt;
+    std::vector<uint64_t> ne;
+    for (const Task& t : tk) {
+      if (!bt.empty() && bt.back().brick == t.brick) {
+        bt.back().c0 = std::min(bt.back().c0, t.c0);
+        bt.back().c1 = std::max(bt.back().c1, t.c1);
+      } else {
+        bt.push_back(Task{t.brick, 1u, t.c0, t.c1});
+      }
     }
+    for (const Task& t : bt) {
+      uint64_t n = 0;
+      for (uint32_t c = t.c0; c < t.c1; ++c) n += ch[c].ne;
+      ne.push_back(n);
+    }
+    std::vector<size_t> order(bt.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return ne[x] > ne[y]; });
+    std::vector<Task> sorted(bt.size());
+    for (size_t i = 0; i < order.size(); ++i) sorted[i] = bt[order[i]];
+    const uint64_t nbk = (uint64_t)f.nb[0] * f.nb[1] * f.nb[2];
+    MUTO_CHECK_HIP(ctx, f.btask.alloc(std::max<size_t>(sorted.size(), 1)));
+    if (!sorted.empty())
+      MUTO_CHECK_HIP(ctx, copy_sync(ctx, f.btask.p, sorted.data(), sorted.size() * sizeof(Task), hipMemcpyHostToDevice));
+    f.n_btask = (uint32_t)sorted.size();
+    MUTO_CHECK_HIP(ctx, f.brk.alloc(nv));
+    MUTO_CHECK_HIP(ctx, f.bmed.alloc(nv));
+    MUTO_CHECK_HIP(ctx, f.bcnt.alloc(nv));
+    MUTO_CHECK_HIP(ctx, f.bmiss_v.alloc(nv));
+    MUTO_CHECK_HIP(ctx, f.bmiss_b.alloc(std::max<uint64_t>(nbk, 1)));
+    k_brk_init<<<grid_for(nv, 256), 256, 0, s>>>(nv, pl->mj.p, f.brk.p, f.bmed.p);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+    f.med_ready = true;
   }
-  for (const Task& t : bt) {
-    uint64_t n = 0;
-    for (uint32_t c = t.c0; c < t.c1; ++c) n += ch[c].ne;
-    ne.push_back(n);
+  if (fp64) {
+    MUTO_CHECK_HIP(ctx, f.bbuf64.alloc(nv * MED_K));
+    f.grid_med64 = ob * ctx->n_cu;
+  } else {
+    MUTO_CHECK_HIP(ctx, f.bbuf.alloc(nv * MED_K));
+    f.grid_med = ob * ctx->n_cu;
   }
-  std::vector<size_t> order(bt.size());
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return ne[x] > ne[y]; });
-  std::vector<Task> sorted(bt.size());
-  for (size_t i = 0; i < order.size(); ++i) sorted[i] = bt[order[i]];
-  const size_t nv = pl->nv;
-  const uint64_t nbk = (uint64_t)f.nb[0] * f.nb[1] * f.nb[2];
-  MUTO_CHECK_HIP(ctx, f.btask.alloc(std::max<size_t>(sorted.size(), 1)));
-  if (!sorted.empty())
-    MUTO_CHECK_HIP(ctx, copy_sync(ctx, f.btask.p, sorted.data(), sorted.size() * sizeof(Task), hipMemcpyHostToDevice));
-  f.n_btask = (uint32_t)sorted.size();
-  MUTO_CHECK_HIP(ctx, f.brk.alloc(nv));
-  MUTO_CHECK_HIP(ctx, f.bmed.alloc(nv));
-  MUTO_CHECK_HIP(ctx, f.bcnt.alloc(nv));
-  MUTO_CHECK_HIP(ctx, f.bbuf.alloc(nv * MED_K));
-  MUTO_CHECK_HIP(ctx, f.bmiss_v.alloc(nv));
-  MUTO_CHECK_HIP(ctx, f.bmiss_b.alloc(std::max<uint64_t>(nbk, 1)));
-  k_brk_init<<<grid_for(nv, 256), 256, 0, s>>>(nv, pl->mj.p, f.brk.p, f.bmed.p);
-  MUTO_CHECK_HIP(ctx, hipGetLastError());
   MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
-  f.med_ready = true;
   return MUTO_OK;
 }
 
@@ -3181,7 +3291,7 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
     // collects their whole lists into their slot ranges, and a sort
     double* ml = map ? pl->ml64.p : nullptr;
     MedArgs m = med_args(pl, pl->work.p + 3);
-    k_sel_brk<<<grid_for(pl->nv, 256), 256, 0, s>>>(a.G, pl->nv, pl->mj.p, m, pl->lam64.p, ml);
+    k_sel_brk<float><<<grid_for(pl->nv, 256), 256, 0, s>>>(a.G, pl->nv, pl->mj.p, m, pl->lam64.p, ml);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     MUTO_CHECK_HIP(ctx, by_block(f.bs.block, [&](auto B) {
       constexpr uint32_t CW = decltype(B)::value;
@@ -3190,7 +3300,7 @@ static int fp32c_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
       return hipGetLastError();
     }));
     const unsigned g = (unsigned)std::min<size_t>(grid_for(pl->nv, 256), (size_t)ctx->n_cu * 16);
-    k_sel_miss<<<g, 256, 0, s>>>(pl->nv, pl->mj.p, m, pl->lam64.p, ml);
+    k_sel_miss<float><<<g, 256, 0, s>>>(pl->nv, pl->mj.p, m, pl->lam64.p, ml);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
     if (f.n_big64) {
       k_sel64b<float><<<std::min<uint32_t>(f.n_big64, 4096), 1024, 0, s>>>(
@@ -3300,7 +3410,7 @@ int fp32c_run(muto_em_plan* pl, int iters, double* sums, int* done) {
     }
     k_fill_nan<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, f.pivot64.p);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
-    if (int rc = med_setup(pl)) return rc;
+    if (int rc = med_setup(pl, false)) return rc;
   }
   const bool graph = pl->use_graph && !pl->timing;
   if (graph)
@@ -3365,12 +3475,6 @@ static bool fp64s_fits(muto_ctx* ctx, const BrickShape& b) {
   const size_t fwd = fwd64_static_lds(b.block) + (size_t)b.vol() * sizeof(double);
   const size_t bwd = bwd64_static_lds(b.block) + (size_t)b.vol() * sizeof(double);
   return std::max(fwd, bwd) <= (size_t)max_lds;
-}
-
-template <class F>
-static hipError_t by_block64(uint32_t block, F&& f) {
-  if (block == 512) return f(std::integral_constant<uint32_t, 512>{});
-  return f(std::integral_constant<uint32_t, 256>{});
 }
 
 // the brick stream with its FP64 side arrays (rebuilt with them when the
@@ -3571,21 +3675,51 @@ static int fp64s_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
   k_sum_parts_f64<<<1, 256, 0, s>>>(f.cost_part64.p, nblk, pl->cost_acc.p);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[2], s));
-  if (f.n_tasks) {
+  const bool brk = !mean && fp64s_brackets(pl);
+  const size_t lds_med = (size_t)f.bs.vol() * (sizeof(float2) + sizeof(uint32_t));
+  const double4* g64 = reinterpret_cast<const double4*>(f.trk_g64.p);
+  if (brk) {
+    // as the FP32C median: counts below / inside each bracket, values inside
+    const uint64_t nbk = (uint64_t)f.nb[0] * f.nb[1] * f.nb[2];
+    k_fill_u8<<<grid_for(nbk, 256), 256, 0, s>>>(nbk, f.bmiss_b.p, 0);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+    const MedArgs md = med_args(pl, pl->work.p + 2);
+    MUTO_CHECK_HIP(ctx, by_block64(f.bs.block, [&](auto B) {
+      constexpr uint32_t BS = decltype(B)::value;
+      k_bwd64s<2, BS><<<f.grid_med64, BS, lds_med, s>>>(a, b, md, g64, nullptr);
+      return hipGetLastError();
+    }));
+  } else if (f.n_tasks) {
     MUTO_CHECK_HIP(ctx, by_block64(f.bs.block, [&](auto B) {
       constexpr uint32_t BS = decltype(B)::value;
       if (mean)
-        k_bwd64s<0, BS><<<f.grid_bwd64, BS, lds, s>>>(a, b, reinterpret_cast<const double4*>(f.trk_g64.p),
-                                                      pl->acc64.p);
+        k_bwd64s<0, BS><<<f.grid_bwd64, BS, lds, s>>>(a, b, MedArgs{}, g64, pl->acc64.p);
       else
-        k_bwd64s<1, BS><<<f.grid_bwd64, BS, lds, s>>>(a, b, reinterpret_cast<const double4*>(f.trk_g64.p),
-                                                      nullptr);
+        k_bwd64s<1, BS><<<f.grid_bwd64, BS, lds, s>>>(a, b, MedArgs{}, g64, nullptr);
       return hipGetLastError();
     }));
   }
   if (ev) MUTO_CHECK_HIP(ctx, hipEventRecord(ev[3], s));
   double* ml = map ? pl->ml64.p : nullptr;
-  if (mean) {
+  if (brk) {
+    const MedArgs md = med_args(pl, pl->work.p + 3);
+    k_sel_brk<double><<<grid_for(pl->nv, 256), 256, 0, s>>>(a.G, pl->nv, pl->mj.p, md, pl->lam64.p, ml);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+    MUTO_CHECK_HIP(ctx, by_block64(f.bs.block, [&](auto B) {
+      constexpr uint32_t BS = decltype(B)::value;
+      k_bwd64s<3, BS><<<f.grid_med64, BS, lds_med, s>>>(a, b, md, g64, nullptr);
+      return hipGetLastError();
+    }));
+    const unsigned g = (unsigned)std::min<size_t>(grid_for(pl->nv, 256), (size_t)ctx->n_cu * 16);
+    k_sel_miss<double><<<g, 256, 0, s>>>(pl->nv, pl->mj.p, md, pl->lam64.p, ml);
+    MUTO_CHECK_HIP(ctx, hipGetLastError());
+    if (f.n_big64) {
+      k_sel64b<double><<<std::min<uint32_t>(f.n_big64, 4096), 1024, 0, s>>>(
+          f.n_big64, f.big64.p, f.big64_off.p, pl->mj.p, f.col64.p, f.cval.p, 0, 100, f.big64_scr.p,
+          pl->lam64.p, ml, f.bmiss_v.p, f.brk.p, f.bmed.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    }
+  } else if (mean) {
     k_upd64s<<<grid_for(pl->nv, 256), 256, 0, s>>>(pl->nv, pl->mj.p, pl->acc64.p, pl->lam64.p, ml);
     MUTO_CHECK_HIP(ctx, hipGetLastError());
   } else {
@@ -3615,7 +3749,7 @@ static int fp64s_enqueue(muto_em_plan* pl, hipEvent_t* ev) {
 static int fp64s_graph(muto_em_plan* pl) {
   Fp32cData& f = pl->f32;
   muto_ctx* ctx = pl->ctx;
-  const int mode = (pl->is_map ? 2 : 0) | (mean_all(pl) ? 1 : 0);
+  const int mode = (pl->is_map ? 2 : 0) | (mean_all(pl) ? 1 : 0) | (fp64s_brackets(pl) ? 4 : 0);
   if (f.graph64 && f.graph64_mode == mode) return MUTO_OK;
   if (f.graph64) (void)hipGraphExecDestroy(f.graph64);
   f.graph64 = nullptr;
@@ -3644,8 +3778,10 @@ int fp64s_run(muto_em_plan* pl, int iters, double* sums, int* done) {
   hipStream_t s = ctx->stream;
   *done = 0;
   if (int rc = fp64s_build(pl)) return rc;
-  if (!mean_all(pl))
+  if (!mean_all(pl)) {
     if (int rc = fp64s_slots(pl)) return rc;
+    if (int rc = med_setup(pl, true)) return rc;
+  }
   if (pl->is_map && !pl->tmp64.p) MUTO_CHECK_HIP(ctx, pl->tmp64.alloc(pl->nv));
   if (f.hist_cap < (size_t)iters) {
     if (f.graph) (void)hipGraphExecDestroy(f.graph);

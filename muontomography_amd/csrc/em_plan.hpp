@@ -122,7 +122,9 @@ struct Fp32cData {
   DBuf<float2> brk;
   DBuf<float> bmed;
   DBuf<uint32_t> bcnt;
-  DBuf<float> bbuf;
+  DBuf<float> bbuf;         // FP32C values inside the brackets
+  DBuf<double> bbuf64;      // FP64 stream values inside the brackets
+  int grid_med64 = 0;
   DBuf<uint8_t> bmiss_v, bmiss_b;
   bool graph_brk = false;   // the bracket mode the graph was captured with
   DBuf<double> hist;               // [hist_cap] cost sum per iteration of a run
@@ -186,6 +188,8 @@ struct Fp32cData {
     bmed.release();
     bcnt.release();
     bbuf.release();
+    bbuf64.release();
+    grid_med64 = 0;
     bmiss_v.release();
     bmiss_b.release();
   }

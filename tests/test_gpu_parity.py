@@ -774,6 +774,38 @@ def test_fp32c_median_brackets_equal_slots(ctx, small_set, brick):
     pb.close()
 
 
+@pytest.mark.parametrize("brick", [(8, 16, 32, 512), (16, 16, 16, 256)])
+def test_fp64_stream_median_brackets_equal_slots(ctx, small_set, brick):
+    """The FP64 stream's median by brackets (the exact FP64 c_ij classified
+    against FP32 bounds rounded outwards) against every value in its slot:
+    the same images bit for bit after each of 8 iterations and for MAP
+    iterations."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=0, beta=1.0, p=1.2, delta=0.1, all_neighbours=1,
+                  add_object_prior=0)
+    pa = ctx.em_plan(cfg, rays=rays)
+    pb = ctx.em_plan(cfg, rays=rays)
+    for p in (pa, pb):
+        p.set_option(M.MUTO_OPT_BRICK, M.brick_code(brick))
+        p.set_option(M.MUTO_OPT_FP64_KERNEL, 2)
+    pb.set_option(M.MUTO_OPT_MEDIAN_BRACKETS, 0)
+    img = np.full(g.n_vox, 1e-7)
+    pa.set_image(img)
+    pb.set_image(img)
+    for it in range(8):
+        pa.run(1, M.MUTO_FP64)
+        pb.run(1, M.MUTO_FP64)
+        assert np.array_equal(pa.image().view(np.uint64), pb.image().view(np.uint64)), it
+    assert np.abs(pa.image()).max() > 0
+    pa.set_image(img)
+    pb.set_image(img)
+    pa.run(4, M.MUTO_FP64, map_=True)
+    pb.run(4, M.MUTO_FP64, map_=True)
+    assert np.array_equal(pa.image().view(np.uint64), pb.image().view(np.uint64))
+    pa.close()
+    pb.close()
+
+
 def test_map_fp32c_median_close_to_fp64(ctx, small_set):
     """MAP iterations whose ML step is the median (the reference's default
     update rule) in FP32C against FP64: images within 1e-4 relative after 8
