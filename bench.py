@@ -198,15 +198,25 @@ def bench_fp64(args, plan, rays, cfg_kw, init, nnz, wall32, cost32, brick32):
     t0 = time.perf_counter()
     med.run(args.steps, M.MUTO_FP64)
     wm = (time.perf_counter() - t0) / args.steps
+    miss64 = med.counters()["n_median_miss"]
     med.set_option(M.MUTO_OPT_TIMING, 1)
     med.set_image(init)
     med.run(args.timing_steps, M.MUTO_FP64)
     tm = med.timing()
     med.set_option(M.MUTO_OPT_TIMING, 0)
+    # the same median with every c_ij written to its slot (no brackets)
+    med.set_option(M.MUTO_OPT_MEDIAN_BRACKETS, 0)
+    med.set_image(init)
+    med.run(1, M.MUTO_FP64)
+    t0 = time.perf_counter()
+    med.run(args.steps, M.MUTO_FP64)
+    wm_slots = (time.perf_counter() - t0) / args.steps
+    med.set_option(M.MUTO_OPT_MEDIAN_BRACKETS, 1)
     med.set_image(init)
     med.run(n, M.MUTO_FP64)
     imgm_s = med.image()
-    log(f"median: reference order {wm_ref * 1e3:.1f} ms/it, FP64 stream {wm * 1e3:.2f} ms/it "
+    log(f"median: reference order {wm_ref * 1e3:.1f} ms/it, FP64 stream {wm * 1e3:.2f} ms/it by brackets "
+        f"({miss64} misses in the last iteration; {wm_slots * 1e3:.2f} with every value in its slot) "
         f"(phases {tm}); err after {nm} it {em_}")
     # FP32C median on the FP32C brick shape
     med.set_option(M.MUTO_OPT_BRICK, M.brick_code(brick32))
@@ -220,13 +230,23 @@ def bench_fp64(args, plan, rays, cfg_kw, init, nnz, wall32, cost32, brick32):
     t0 = time.perf_counter()
     med.run(args.steps, M.MUTO_FP32C)
     wm32 = (time.perf_counter() - t0) / args.steps
+    miss32 = med.counters()["n_median_miss"]
     med.set_option(M.MUTO_OPT_TIMING, 1)
     med.set_image(init)
     med.run(args.timing_steps, M.MUTO_FP32C)
     tm32 = med.timing()
+    med.set_option(M.MUTO_OPT_TIMING, 0)
+    med.set_option(M.MUTO_OPT_MEDIAN_BRACKETS, 0)
+    med.set_image(init)
+    med.run(1, M.MUTO_FP32C)
+    t0 = time.perf_counter()
+    med.run(args.steps, M.MUTO_FP32C)
+    wm32_slots = (time.perf_counter() - t0) / args.steps
+    med.set_option(M.MUTO_OPT_MEDIAN_BRACKETS, 1)
     med.close()
-    log(f"median FP32C: {wm32 * 1e3:.2f} ms/it (phases {tm32}); err after {nm} it vs reference order "
-        f"{e32m}, after {n} it vs the FP64 stream {e32ms}")
+    log(f"median FP32C: {wm32 * 1e3:.2f} ms/it by brackets ({miss32} misses in the last iteration; "
+        f"{wm32_slots * 1e3:.2f} with every value in its slot) (phases {tm32}); err after {nm} it vs "
+        f"reference order {e32m}, after {n} it vs the FP64 stream {e32ms}")
     ms32 = wall32 / args.steps * 1e3
     # the FP64 stream's iteration against its HBM roofline (12 B per update)
     bw64s = BYTES_PER_UPDATE_FP64 * nnz / (t64s["iteration"] * 1e-3)
@@ -258,9 +278,11 @@ def bench_fp64(args, plan, rays, cfg_kw, init, nnz, wall32, cost32, brick32):
                                 "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                                 "frac": BYTES_PER_UPDATE_FP64 * nnz / wm / HBM_PEAK},
                    "fp64_stream_value": nnz / wm, "speedup": wm_ref / wm,
+                   "fp64_stream_slots_ms_per_step": wm_slots * 1e3, "fp64_stream_misses_last": miss64,
                    "fp64_stream_kernel_ms": {k: round(v, 4) for k, v in tm.items()},
                    "iterations_compared": nm, "err_max_rel": em_[0], "err_rms_rel": em_[1],
                    "fp32c_ms_per_step": wm32 * 1e3, "fp32c_value": nnz / wm32,
+                   "fp32c_slots_ms_per_step": wm32_slots * 1e3, "fp32c_misses_last": miss32,
                    "fp32c_kernel_ms": {k: round(v, 4) for k, v in tm32.items()},
                    "fp32c_speedup_vs_fp64": wm_ref / wm32, "fp32c_speedup_vs_fp64_stream": wm / wm32,
                    "fp32c_err_vs_fp64_max_rel": e32m[0], "fp32c_err_vs_fp64_rms_rel": e32m[1],
