@@ -2238,12 +2238,21 @@ k_bwd32m(IterArgs a, MedArgs m, const double4* __restrict__ trk_g) {
         uint32_t vx[GE];
         gr.voxels(stab, [&](uint32_t q) { return svb[q]; }, [&](int i, uint32_t v) { vx[i] = v; });
         const uint32_t e0 = tl * GE;
+        // each half group's brackets read before its FP64 terms, so the LDS
+        // reads overlap the arithmetic
+        float2 brs[GE];
+#pragma unroll
+        for (int i = GE / 2; i < GE; ++i) brs[i] = brk_s[vx[i]];
         double T = gr.carry;
         uint32_t mk = gr.mask, sk = gr.s0, base_v = ENT_BASE;  // opaque copies, as in k_bwd32
         asm volatile("" : "+v"(mk), "+v"(sk), "+v"(base_v));
         double4 g = sgb[sk + (uint32_t)__popc(mk & 0xFFFEu)];
 #pragma unroll
         for (int i = GE - 1; i >= 0; --i) {
+          if (i == GE / 2 - 1) {
+#pragma unroll
+            for (int k = 0; k < GE / 2; ++k) brs[k] = brk_s[vx[k]];
+          }
           // c_ij exactly as k_bwd32s stores it
           const double L = ent_len_v(gr.w[i], base_v);
           const uint32_t st = start_mask(mk, i);
@@ -2252,7 +2261,7 @@ k_bwd32m(IterArgs a, MedArgs m, const double4* __restrict__ trk_g) {
           const float cf = (float)cv;
           if (e0 + (uint32_t)i < ch.ne && ent_code(gr.w[i]) != ENT_PAD_CODE) {
             const uint32_t v = vx[i];
-            const float2 br = brk_s[v];
+            const float2 br = brs[i];
             if (cf < br.x) {
               atomicAdd(&cnt_s[v], 1u);
             } else if (!(cf > br.y)) {
