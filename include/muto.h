@@ -410,7 +410,9 @@ int muto_em_plan_set_option(muto_em_plan* plan, int option, int value);
 /* FP32C autotuner: builds the brick stream for each candidate tile shape
  * (shapes[k] coded as for MUTO_OPT_BRICK; NULL/0: a built-in list of 4096-
  * to 16384-voxel bricks), times `iters` iterations of each from the plan's image
- * (restored after every trial) and keeps the fastest.  ms[k] (may be NULL)
+ * under the plan's update rule (after one untimed iteration, which for the
+ * median also sets its brackets; the image is restored after every trial)
+ * and keeps the fastest.  ms[k] (may be NULL)
  * receives each shape's ms per iteration, *best (may be NULL) the chosen
  * shape.  No counterpart in the reference. */
 int muto_em_plan_autotune(muto_em_plan* plan, int iters, int n_shapes, const uint32_t* shapes,

@@ -1328,10 +1328,6 @@ extern "C" int muto_em_plan_autotune(muto_em_plan* pl, int iters, int n_cand,
   }
   if (pl->cur_prec < 0) { ctx->set_error("muto_em_plan_autotune: no image set"); return MUTO_ERR_STATE; }
   if (!pl->has_lists || pl->n_trk == 0) { ctx->set_error("muto_em_plan_autotune: nothing to tune"); return MUTO_ERR_STATE; }
-  if (!(pl->cfg.update_with_mean && pl->cfg.mean_percentile == 100)) {
-    ctx->set_error("muto_em_plan_autotune: FP32C runs the mean update");
-    return MUTO_ERR_ARG;
-  }
   std::vector<double> t(n_cand);
   pl->is_map = false;
   if (int rc = fp32c_autotune(pl, iters, n_cand, shapes, t.data())) return rc;

@@ -1440,12 +1440,15 @@ def test_fp32c_block_sizes(ctx, small_set, shape):
     plan.close()
 
 
-def test_autotune_keeps_image_and_picks_a_candidate(ctx, small_set):
-    """The brick-shape autotuner leaves the image as it was, picks one of
-    the candidates it timed, and the FP32C run with the chosen shape matches
-    the same shape set by the option (and stays within 1e-4 of FP64)."""
+@pytest.mark.parametrize("rule", ["mean", "median"])
+def test_autotune_keeps_image_and_picks_a_candidate(ctx, small_set, rule):
+    """The brick-shape autotuner (timing the plan's own update rule: the mean,
+    or the median by brackets) leaves the image as it was, picks one of the
+    candidates it timed, and the FP32C run with the chosen shape matches the
+    same shape set by the option (and stays within 1e-4 of FP64)."""
     g, rays, _ = small_set
-    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100)
+    kw = dict(update_with_mean=1, mean_percentile=100) if rule == "mean" else dict(update_with_mean=0)
+    cfg = _em_cfg(g, **kw)
     plan = ctx.em_plan(cfg, rays=rays)
     img = np.full(g.n_vox, 1e-7)
     plan.set_image(img)
@@ -1463,7 +1466,12 @@ def test_autotune_keeps_image_and_picks_a_candidate(ctx, small_set):
     np.testing.assert_allclose(a_t, other.image(), rtol=1e-6, atol=1e-6 * np.abs(a_t).max())
     other.set_image(img)
     other.run(10, M.MUTO_FP64)
-    mx, _, _ = _rel_stats(a_t, other.image(), other.voxel_counts())
-    assert mx <= 1e-4, mx
+    mj = other.voxel_counts()
+    mx, _, _ = _rel_stats(a_t, other.image(), mj)
+    if rule == "mean":
+        assert mx <= 1e-4, mx
+    else:  # as test_fp32c_order_statistics_match_fp64: few-track voxels within 5e-4
+        mx8, _, _ = _rel_stats(a_t, other.image(), np.where(mj >= 8, mj, 0))
+        assert mx <= 5e-4 and mx8 <= 1e-4, (mx, mx8)
     plan.close()
     other.close()
