@@ -2339,12 +2339,14 @@ __device__ __forceinline__ bool sel_brk_voxel(const BrickGeo& G, size_t j, uint3
     return true;
   }
   m.miss[j] = 0;
+  // only the filled part of the buffer is read (about half of it)
   V w[MED_K];
   if constexpr (std::is_same<V, float>::value) {
     const float4* src = reinterpret_cast<const float4*>(m.buf + j * MED_K);
 #pragma unroll
     for (uint32_t k = 0; k < MED_K / 4; ++k) {
-      const float4 q = src[k];
+      float4 q = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
+      if (4 * k < nin) q = src[k];
       w[4 * k] = 4 * k < nin ? q.x : INFINITY;
       w[4 * k + 1] = 4 * k + 1 < nin ? q.y : INFINITY;
       w[4 * k + 2] = 4 * k + 2 < nin ? q.z : INFINITY;
@@ -2354,7 +2356,8 @@ __device__ __forceinline__ bool sel_brk_voxel(const BrickGeo& G, size_t j, uint3
     const double2* src = reinterpret_cast<const double2*>(m.buf64 + j * MED_K);
 #pragma unroll
     for (uint32_t k = 0; k < MED_K / 2; ++k) {
-      const double2 q = src[k];
+      double2 q = make_double2(INFINITY, INFINITY);
+      if (2 * k < nin) q = src[k];
       w[2 * k] = 2 * k < nin ? q.x : INFINITY;
       w[2 * k + 1] = 2 * k + 1 < nin ? q.y : INFINITY;
     }
