@@ -2083,11 +2083,13 @@ __device__ __forceinline__ float bound_hi(double x) { return __double2float_ru(x
 
 // The next bracket: the values lo and hi MED_H ranks (or more) either side of
 // the median value cm, widened towards where the median is heading (cm plus
-// its last move, the same spread around it), since the medians drift the
-// same way pass after pass while the image converges.
+// twice its last move, the same spread around it), since the medians drift
+// the same way pass after pass while the image converges.  (Twice: 1.1e3
+// misses per pass at most over 50 passes at 256^3 x 1e7 muons, against 1.5e3
+// and rising with the last move once.)
 __device__ __forceinline__ float2 next_bracket(float lo, float hi, float cm, float last) {
   if (!(last == last)) return make_float2(lo, hi);  // no last median
-  const float pred = cm + (cm - last);
+  const float pred = cm + 2.0f * (cm - last);
   return make_float2(fminf(lo, pred - (cm - lo)), fmaxf(hi, pred + (hi - cm)));
 }
 
