@@ -519,16 +519,24 @@ def bench_map_grad(args):
     cfg = mlsd_cfg(g, update_with_mean=1, mean_percentile=100, error_xy=0.5, error_dz_outer=500.0,
                    error_dz_inner=500.0, beta=1.0, p=1.2, delta=0.1, all_neighbours=1)
     plan = ctx.em_plan(cfg, rays=rays)
-    lean = ctx.em_plan(cfg, rays=rays, list_budget=0)
-    lam = 1e-7 * (0.5 + np.random.default_rng(1).random(g.n_vox))
     nnz = plan.counters()["nnz"]
+    # plans without lists: a budget below the lists' 20 bytes per entry that
+    # holds the backward's tile records, and one of 0 (one atomic per entry)
+    lean = ctx.em_plan(cfg, rays=rays, list_budget=20 * nnz - 1)
+    lean0 = ctx.em_plan(cfg, rays=rays, list_budget=0)
+    lam = 1e-7 * (0.5 + np.random.default_rng(1).random(g.n_vox))
     res = {}
+    first = {}
     for name, pl, prec, rc in (("fp32c", plan, M.MUTO_FP32C, False), ("fp64", plan, M.MUTO_FP64, False),
                                ("fp64_recompute", lean, M.MUTO_FP64, True),
-                               ("fp32c_recompute", lean, M.MUTO_FP32C, True)):
+                               ("fp32c_recompute", lean, M.MUTO_FP32C, True),
+                               ("fp64_recompute_per_entry", lean0, M.MUTO_FP64, True),
+                               ("fp32c_recompute_per_entry", lean0, M.MUTO_FP32C, True)):
+        t0 = time.perf_counter()
         sv = pl.map_forward(lam, prec, recompute=rc)
         gr = sv.backward(1.0)
         sv.close()
+        first[name] = (time.perf_counter() - t0) * 1e3  # (the tile records are made here)
         t0 = time.perf_counter()
         for _ in range(args.steps):
             sv = pl.map_forward(lam, prec, recompute=rc)
@@ -573,6 +581,8 @@ def bench_map_grad(args):
            "config": {"workload": "MAP gradient, 128^3 grid of 15 mm voxels, 1e6 muons",
                       "entries": int(nnz)},
            "ms": {k: v[0] * 1e3 for k, v in res.items()},
+           "first_call_ms": first,
+           "map_tile_records": int(lean.counters()["n_map_tile_records"]),
            "ms_device_buffers": {k: v * 1e3 for k, v in dev.items()},
            "device_value": 1.0 / dev["fp32c"],
            "grad_err_noalloc_recompute_vs_fp64_norm_inf": float(np.abs(g_dev - g64).max() / np.abs(g64).max()),
@@ -581,9 +591,13 @@ def bench_map_grad(args):
            "grad_err_recompute_vs_fp64_norm_inf": float(np.abs(res["fp64_recompute"][1] - g64).max() /
                                                          np.abs(g64).max()),
            "grad_err_fp32c_recompute_vs_fp64_norm_inf": float(
-               np.abs(res["fp32c_recompute"][1] - g64).max() / np.abs(g64).max())}
+               np.abs(res["fp32c_recompute"][1] - g64).max() / np.abs(g64).max()),
+           "grad_err_recompute_tiled_vs_per_entry_norm_inf": float(
+               np.abs(res["fp64_recompute"][1] - res["fp64_recompute_per_entry"][1]).max() /
+               np.abs(g64).max())}
     plan.close()
     lean.close()
+    lean0.close()
     ctx.close()
     print(json.dumps(out))
 

@@ -104,7 +104,7 @@ class Counters(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "n_total", "n_not_parallel", "n_scat_in_grid", "n_empty_path", "n_out_of_grid",
         "n_singular", "n_cubic_warn", "n_newton_cap", "nnz", "n_segments", "iterations_done",
-        "n_median_miss", "r0", "r1", "r2", "r3")]
+        "n_median_miss", "n_map_tile_records", "r1", "r2", "r3")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_ if not n.startswith("r")}

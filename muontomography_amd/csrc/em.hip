@@ -807,6 +807,7 @@ extern "C" int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg,
   pl->cfg = *cfg;
   pl->g = to_dev(cfg->grid);
   pl->nv = n_vox(cfg->grid);
+  pl->list_budget = list_budget;
   int rc = muto_plan_alloc_common(pl);
   if (rc) { delete pl; return rc; }
   hipStream_t s = ctx->stream;
@@ -1341,6 +1342,7 @@ extern "C" int muto_em_plan_counters(const muto_em_plan* pl, muto_counters* out)
   if (!pl || !out) return MUTO_ERR_ARG;
   *out = pl->counters;
   out->n_segments = (int64_t)pl->f32.n_seg;
+  out->n_map_tile_records = (int64_t)pl->mt_n_rec;
   return MUTO_OK;
 }
 

@@ -195,6 +195,23 @@ struct Fp32cData {
   }
 };
 
+// The recomputing MAP backward's record of one run of a track's path
+// entries inside one tile: the Siddon walk's state before the step that
+// recorded the run's first voxel, and the path length before that entry.
+struct MapRec {
+  double ax[3];       // plane crossings before the step
+  double alpha_last;
+  double s;           // path length before the entry
+  uint32_t x, y, z;   // the step's voxel coordinates
+  uint32_t tl;        // track << 1 | leg (1: the out-leg of a kinked path)
+};
+static_assert(sizeof(MapRec) == 56, "tile record");
+
+// records [q0, q1) of tile `tile`, which `ntask` tasks share
+struct MapTask {
+  uint32_t tile, q0, q1, ntask;
+};
+
 }  // namespace muto
 
 struct muto_em_plan {
@@ -219,6 +236,17 @@ struct muto_em_plan {
   uint32_t n_scans = 1;         // > 1: a batch of scans laid side by side in one grid
   muto::DBuf<uint16_t> trk_scan;  // [n_trk] scan of each track (batches only)
   bool has_lists = true;        // row_ptr/vox/len/tail hold the paths
+  uint64_t list_budget = UINT64_MAX;  // bytes allowed for the lists (and, without them, the tile records)
+  // the recomputing MAP backward's tile records, sorted by tile, made at
+  // its first call: mt_state 0 not made yet, 1 made, -1 not kept (over the
+  // budget, or no memory: the backward then adds one atomic per entry)
+  int mt_state = 0;
+  muto::DBuf<muto::MapRec> mt_rec;
+  muto::DBuf<muto::MapTask> mt_task;
+  muto::DBuf<unsigned int> mt_work;  // [1] task counter
+  uint64_t mt_n_rec = 0;
+  uint32_t mt_n_task = 0;
+  int mt_grid = 0;
   // mj counted: by the FP32C stream build when that comes first (LDS counts
   // per brick), else by one atomic per entry (ensure_mj)
   bool mj_ready = false;

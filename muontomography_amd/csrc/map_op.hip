@@ -228,6 +228,332 @@ k_map_bwd32rc(MapTrackArgs a, const double* __restrict__ gsave, double scale,
   });
 }
 
+// ---------------- the recomputing backward, per tile -------------------
+// One global atomic per entry runs at about 2.4e10 entries/s on this GPU
+// (FP64 or FP32, scattered voxels), so the backward that walks the paths
+// again makes its sums per tile of MT_E^3 voxels in LDS instead:
+//   1. once per plan, every track's merged path is walked and each run of
+//      its entries inside one tile gets a record (MapRec: the Siddon walk's
+//      state before the step of the run's first entry, and the path length
+//      before that entry); the records are sorted by tile (and inside a
+//      tile by leg and run length, see mt_key), stably, and cut into tasks
+//      of at most MT_TASK records;
+//   2. per backward call, one workgroup per task resumes each record's walk,
+//      adds the terms of the run's entries into the tile in LDS, and adds
+//      the tile into the gradient (stores it when one task covers the tile).
+// The entries (voxel, length, and path length s before them, so the tail
+// T = P - s - L) are walk_path's own, bit for bit; only the order of the
+// sums differs from one atomic per entry.  A record takes 56 bytes per run
+// of entries (about a dozen at 16^3 tiles), against 20 bytes per entry for
+// the stored lists.
+constexpr uint32_t MT_E = 16;
+constexpr uint32_t MT_V = MT_E * MT_E * MT_E;
+constexpr uint32_t MT_BS = 256;
+constexpr uint32_t MT_TASK = 1024;
+
+struct MtGeo {
+  uint32_t nt[3];  // tiles per axis
+};
+
+// Tile of entry voxel j (j < nv) and the voxel's index in it, from the
+// walk's coordinates or, where those lie outside the grid, from j (which
+// then names another voxel, as the reference's xyz2index does)
+__device__ __forceinline__ uint32_t mt_tile(const GridD& g, const MtGeo& T, uint32_t x, uint32_t y,
+                                            uint32_t z, uint32_t j, uint32_t& lv) {
+  if (!(x < g.n[0] && y < g.n[1] && z < g.n[2])) {
+    const uint32_t r = j / g.n[0];
+    x = j - r * g.n[0];
+    y = r % g.n[1];
+    z = r / g.n[1];
+  }
+  lv = (x % MT_E) + MT_E * ((y % MT_E) + MT_E * (z % MT_E));
+  return x / MT_E + T.nt[0] * (y / MT_E + T.nt[1] * (z / MT_E));
+}
+
+// Every entry of a track's merged path in walk_path's order and arithmetic,
+// f(j, s, leg, x, y, z, ax_pre, alpha_pre) with s the path length before
+// the entry and the state of the step that recorded the entry's voxel on
+// leg `leg` (0: the in-leg, or the whole straight path; 1: the out-leg).
+// The in-leg's last entry, whose length takes the out-leg's first length on
+// top, carries its own step's state.
+template <class F>
+__device__ __forceinline__ void walk_entries(const GridD& g, const double* geo, int straight, F&& f) {
+  const double* pin = geo;
+  const double* pout = geo + 3;
+  const double* P = geo + 6;
+  const uint32_t nx = g.n[0], nxy = g.n[0] * g.n[1];
+  double s = 0.0;
+  if (straight) {
+    SiddonCursor c;
+    if (!c.begin(g, pin, pout)) return;
+    c.run(0xFFFFFFFFu, [&](uint32_t x, uint32_t y, uint32_t z, double L) {
+      double ap[3];
+      c.snapshot(x, y, z, ap);
+      f(x + y * nx + z * nxy, s, 0u, x, y, z, ap, c.alpha_last);
+      s += L;
+    });
+    return;
+  }
+  bool have = false;
+  uint32_t px = 0, py = 0, pz = 0;
+  double pL = 0.0, pal = 0.0, pap[3] = {0.0, 0.0, 0.0};
+  {
+    SiddonCursor c;
+    if (c.begin(g, pin, P))
+      c.run(0xFFFFFFFFu, [&](uint32_t x, uint32_t y, uint32_t z, double L) {
+        if (have) {
+          f(px + py * nx + pz * nxy, s, 0u, px, py, pz, pap, pal);
+          s += pL;
+        }
+        px = x;
+        py = y;
+        pz = z;
+        pL = L;
+        c.snapshot(x, y, z, pap);
+        pal = c.alpha_last;
+        have = true;
+      });
+  }
+  SiddonCursor c2;
+  bool first = true;
+  if (c2.begin(g, P, pout))
+    c2.run(0xFFFFFFFFu, [&](uint32_t x, uint32_t y, uint32_t z, double L) {
+      if (first) {
+        first = false;
+        if (have) {
+          pL += L;
+          f(px + py * nx + pz * nxy, s, 0u, px, py, pz, pap, pal);
+          s += pL;
+          have = false;
+        }
+        return;
+      }
+      double ap[3];
+      c2.snapshot(x, y, z, ap);
+      f(x + y * nx + z * nxy, s, 1u, x, y, z, ap, c2.alpha_last);
+      s += L;
+    });
+  if (have) f(px + py * nx + pz * nxy, s, 0u, px, py, pz, pap, pal);
+}
+
+// The entries of track geo's merged path from record R's entry on, in path
+// order, f(j, x, y, z, L, s) for each until it returns false or the path
+// ends; the same voxels, lengths and s as walk_path.
+template <class F>
+__device__ __forceinline__ void resume_entries(const GridD& g, const double* geo, int straight,
+                                               const MapRec& R, F&& f) {
+  const double* pin = geo;
+  const double* pout = geo + 3;
+  const double* P = geo + 6;
+  const uint32_t nx = g.n[0], nxy = g.n[0] * g.n[1];
+  double s = R.s;
+  bool go = true;
+  if (straight || (R.tl & 1u)) {
+    SiddonCursor c;
+    c.begin(g, straight ? pin : P, pout);  // succeeded when the record was taken
+    c.resume(R.x, R.y, R.z, R.ax, R.alpha_last);
+    c.run_while([&](uint32_t x, uint32_t y, uint32_t z, double L) -> bool {
+      if (!f(x + y * nx + z * nxy, x, y, z, L, s)) return false;
+      s += L;
+      return true;
+    });
+    return;
+  }
+  // the in-leg: each entry taken when the next step is found, the last one
+  // with the out-leg's first length added
+  bool have = false;
+  uint32_t px = 0, py = 0, pz = 0;
+  double pL = 0.0;
+  {
+    SiddonCursor c;
+    c.begin(g, pin, P);
+    c.resume(R.x, R.y, R.z, R.ax, R.alpha_last);
+    c.run_while([&](uint32_t x, uint32_t y, uint32_t z, double L) -> bool {
+      if (have) {
+        if (!f(px + py * nx + pz * nxy, px, py, pz, pL, s)) {
+          go = false;
+          return false;
+        }
+        s += pL;
+      }
+      px = x;
+      py = y;
+      pz = z;
+      pL = L;
+      have = true;
+      return true;
+    });
+  }
+  if (!go) return;
+  SiddonCursor c2;
+  bool first = true;
+  if (c2.begin(g, P, pout))
+    c2.run_while([&](uint32_t x, uint32_t y, uint32_t z, double L) -> bool {
+      if (first) {
+        first = false;
+        if (have) {
+          pL += L;
+          if (!f(px + py * nx + pz * nxy, px, py, pz, pL, s)) {
+            go = false;
+            return false;
+          }
+          s += pL;
+          have = false;
+        }
+        return true;
+      }
+      if (!f(x + y * nx + z * nxy, x, y, z, L, s)) {
+        go = false;
+        return false;
+      }
+      s += L;
+      return true;
+    });
+  if (go && have) f(px + py * nx + pz * nxy, px, py, pz, pL, s);
+}
+
+// Sort key of a record: its tile, then its leg and its run's length
+// (longest first, 63 entries and more alike), so that the lanes of a
+// wavefront, which take consecutive records, walk runs of one kind and of
+// similar lengths.  Tiles below 2^25.
+constexpr int MT_SUB_BITS = 7;
+__device__ __forceinline__ uint32_t mt_key(uint32_t tile, uint32_t leg, uint32_t run) {
+  return tile << MT_SUB_BITS | leg << 6 | (63u - min(run, 63u));
+}
+
+// Step 1 per track: the runs of its entries (voxel below nv) in one tile.
+// COUNT: their number into cnt[i]; else the records at roff[i], each with
+// its sort key and its own index as the value.
+template <bool COUNT>
+__global__ void __launch_bounds__(256)
+k_mt_split(MapTrackArgs a, MtGeo T, const uint64_t* __restrict__ roff, uint32_t* __restrict__ cnt,
+           MapRec* __restrict__ rec, uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_trk) return;
+  const uint64_t b = COUNT ? 0 : roff[i];
+  uint32_t cur = 0xFFFFFFFFu, k = 0, run = 0, run_leg = 0;
+  walk_entries(a.g, a.geo + 9 * i, a.straight,
+               [&](uint32_t j, double s, uint32_t leg, uint32_t x, uint32_t y, uint32_t z,
+                   const double* ap, double al) {
+                 if (j >= a.nv) return;
+                 uint32_t lv;
+                 const uint32_t t = mt_tile(a.g, T, x, y, z, j, lv);
+                 if (t == cur) {
+                   ++run;
+                   return;
+                 }
+                 if (!COUNT && k > 0) key[b + k - 1] = mt_key(cur, run_leg, run);
+                 cur = t;
+                 run = 1;
+                 run_leg = leg;
+                 if (!COUNT) {
+                   MapRec R;
+                   R.ax[0] = ap[0];
+                   R.ax[1] = ap[1];
+                   R.ax[2] = ap[2];
+                   R.alpha_last = al;
+                   R.s = s;
+                   R.x = x;
+                   R.y = y;
+                   R.z = z;
+                   R.tl = (uint32_t)(i << 1) | leg;
+                   rec[b + k] = R;
+                   idx[b + k] = (uint32_t)(b + k);
+                 }
+                 ++k;
+               });
+  if (COUNT)
+    cnt[i] = k;
+  else if (k > 0)
+    key[b + k - 1] = mt_key(cur, run_leg, run);
+}
+
+__global__ void k_mt_gather(size_t n, const uint32_t* __restrict__ idx, const MapRec* __restrict__ src,
+                            MapRec* __restrict__ dst) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) dst[q] = src[idx[q]];
+}
+
+// first and end record of every tile in the sorted keys
+__global__ void k_mt_bounds(size_t n, const uint32_t* __restrict__ key, uint32_t* __restrict__ ts,
+                            uint32_t* __restrict__ te) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const uint32_t t = key[q] >> MT_SUB_BITS;
+  if (q == 0 || (key[q - 1] >> MT_SUB_BITS) != t) ts[t] = (uint32_t)q;
+  if (q + 1 == n || (key[q + 1] >> MT_SUB_BITS) != t) te[t] = (uint32_t)(q + 1);
+}
+
+__global__ void k_mt_reset(unsigned int* __restrict__ work) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) work[0] = 0u;
+}
+
+// Step 2: grad_j += sum over the tile's runs of scale <W_ij, G_i>, with
+// the terms of k_map_bwd64<1> (F32 false) or of k_map_bwd32rc (F32 true:
+// W and its product with G in FP32), summed in FP64 in LDS; persistent
+// workgroups over the tasks.
+template <bool F32>
+__global__ void __launch_bounds__(MT_BS)
+k_mt_bwd(MapTrackArgs a, MtGeo T, const MapTask* __restrict__ tasks, uint32_t n_tasks,
+         unsigned int* __restrict__ work, const MapRec* __restrict__ rec,
+         const double* __restrict__ gsave, double scale, double* __restrict__ grad) {
+  __shared__ double acc[MT_V];
+  __shared__ uint32_t sh;
+  const uint32_t t = threadIdx.x;
+  for (;;) {
+    if (t == 0) sh = atomicAdd(work, 1u);
+    for (uint32_t v = t; v < MT_V; v += MT_BS) acc[v] = 0.0;
+    __syncthreads();
+    const uint32_t ti = sh;
+    if (ti >= n_tasks) break;
+    const MapTask tk = tasks[ti];
+    for (uint32_t q = tk.q0 + t; q < tk.q1; q += MT_BS) {
+      const MapRec R = rec[q];
+      const size_t i = R.tl >> 1;
+      const double G00 = gsave[4 * i], G01 = gsave[4 * i + 1], G11 = gsave[4 * i + 2];
+      if (G00 == 0.0 && G01 == 0.0 && G11 == 0.0) continue;
+      const double P = gsave[4 * i + 3];
+      const float g00 = (float)(scale * G00), g01 = (float)(2.0 * scale * G01),
+                  g11 = (float)(scale * G11);
+      resume_entries(a.g, a.geo + 9 * i, a.straight, R,
+                     [&](uint32_t j, uint32_t x, uint32_t y, uint32_t z, double L, double s) -> bool {
+                       if (j >= a.nv) return true;
+                       uint32_t lv;
+                       if (mt_tile(a.g, T, x, y, z, j, lv) != tk.tile) return false;
+                       const double Tl = P - s - L;
+                       double term;
+                       if (F32) {
+                         const double LT = L * Tl;
+                         const float w00 = (float)L, w01 = (float)(LT + L * L * 0.5);
+                         const float w11 = (float)(L * L * L / 3.0 + (L + Tl) * LT);
+                         term = (double)fmaf(w11, g11, fmaf(w01, g01, w00 * g00));
+                       } else {
+                         Mat2 W;
+                         weight(L, Tl, W);
+                         term = scale * (W.m00 * G00 + 2.0 * W.m01 * G01 + W.m11 * G11);
+                       }
+                       atomicAdd(&acc[lv], term);
+                       return true;
+                     });
+    }
+    __syncthreads();
+    const uint32_t tyz = tk.tile / T.nt[0];
+    const uint32_t o0 = (tk.tile - tyz * T.nt[0]) * MT_E, o1 = (tyz % T.nt[1]) * MT_E,
+                   o2 = (tyz / T.nt[1]) * MT_E;
+    for (uint32_t v = t; v < MT_V; v += MT_BS) {
+      const double val = acc[v];
+      if (val == 0.0) continue;  // (a nonzero sum lies inside the grid)
+      const uint32_t x = o0 + v % MT_E, y = o1 + (v / MT_E) % MT_E, z = o2 + v / (MT_E * MT_E);
+      const size_t j = (size_t)x + (size_t)a.g.n[0] * ((size_t)y + (size_t)a.g.n[1] * z);
+      if (tk.ntask == 1)
+        grad[j] = val;
+      else
+        atomicAdd(&grad[j], val);
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void k_zero_f32(size_t n, float* __restrict__ x) {
   size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j < n) x[j] = 0.0f;
@@ -339,6 +665,99 @@ __global__ void k_phi_combine(const double* __restrict__ sums, double beta, doub
   if (threadIdx.x == 0 && blockIdx.x == 0) phi[0] = 0.5 * sums[0] + beta * sums[1];
 }
 
+MtGeo mt_geo(const muto_em_plan* pl) {
+  MtGeo T;
+  for (int a = 0; a < 3; ++a) T.nt[a] = (pl->g.n[a] + MT_E - 1) / MT_E;
+  return T;
+}
+
+// The tile records and tasks of the recomputing backward, made at its
+// first call on the plan.  Returns false when they are not kept: more than
+// 2^31 records or tracks, records over the list budget of a plan that keeps
+// no lists (56 bytes each), or no device memory; the backward then makes
+// one atomic per entry.  Kept or not, the decision holds for the plan's
+// life.
+bool mt_build(muto_em_plan* pl) {
+  if (pl->mt_state) return pl->mt_state > 0;
+  pl->mt_state = -1;
+  muto_ctx* ctx = pl->ctx;
+  hipStream_t s = ctx->stream;
+  const size_t nt = pl->n_trk;
+  if (!nt || !pl->geo.p || nt >= (1ull << 31)) return false;
+  const MtGeo T = mt_geo(pl);
+  const uint64_t n_tiles = (uint64_t)T.nt[0] * T.nt[1] * T.nt[2];
+  if (n_tiles >= (1ull << (32 - MT_SUB_BITS))) return false;
+  const MapTrackArgs a = track_args(pl);
+  auto fail = [&]() {
+    (void)hipStreamSynchronize(s);
+    (void)hipGetLastError();  // an allocation that failed leaves its status
+    pl->mt_rec.release();
+    pl->mt_task.release();
+    pl->mt_work.release();
+    return false;
+  };
+  DBuf<uint32_t> cnt;
+  DBuf<uint64_t> roff;
+  if (cnt.alloc(nt + 1) != hipSuccess || roff.alloc(nt + 1) != hipSuccess) return fail();
+  if (hipMemsetAsync(cnt.p + nt, 0, sizeof(uint32_t), s) != hipSuccess) return fail();
+  k_mt_split<true><<<grid_for(nt, 256), 256, 0, s>>>(a, T, nullptr, cnt.p, nullptr, nullptr, nullptr);
+  if (hipGetLastError() != hipSuccess) return fail();
+  if (exclusive_scan_u32_to_u64(cnt.p, roff.p, nt + 1, s) != hipSuccess) return fail();
+  uint64_t n_rec = 0;
+  if (copy_sync(ctx, &n_rec, roff.p + nt, sizeof n_rec, hipMemcpyDeviceToHost) != hipSuccess) return fail();
+  cnt.release();
+  if (n_rec == 0 || n_rec >= (1ull << 31)) return fail();
+  if (!pl->has_lists && (double)n_rec * sizeof(MapRec) > (double)pl->list_budget) return fail();
+  {
+    DBuf<MapRec> rec0;
+    DBuf<uint32_t> key, idx;
+    if (rec0.alloc(n_rec) != hipSuccess || key.alloc(n_rec) != hipSuccess || idx.alloc(n_rec) != hipSuccess)
+      return fail();
+    k_mt_split<false><<<grid_for(nt, 256), 256, 0, s>>>(a, T, roff.p, nullptr, rec0.p, key.p, idx.p);
+    if (hipGetLastError() != hipSuccess) return fail();
+    int bits = 1;
+    while ((1ull << bits) < n_tiles) ++bits;
+    if (sort_pairs_u32_u32(key.p, idx.p, n_rec, 0, bits + MT_SUB_BITS, s) != hipSuccess) return fail();
+    if (pl->mt_rec.alloc(n_rec) != hipSuccess) return fail();
+    k_mt_gather<<<grid_for(n_rec, 256), 256, 0, s>>>(n_rec, idx.p, rec0.p, pl->mt_rec.p);
+    if (hipGetLastError() != hipSuccess) return fail();
+    rec0.release();
+    idx.release();
+    DBuf<uint32_t> ts, te;
+    if (ts.alloc(n_tiles) != hipSuccess || te.alloc(n_tiles) != hipSuccess) return fail();
+    if (hipMemsetAsync(ts.p, 0, n_tiles * sizeof(uint32_t), s) != hipSuccess ||
+        hipMemsetAsync(te.p, 0, n_tiles * sizeof(uint32_t), s) != hipSuccess)
+      return fail();
+    k_mt_bounds<<<grid_for(n_rec, 256), 256, 0, s>>>(n_rec, key.p, ts.p, te.p);
+    if (hipGetLastError() != hipSuccess) return fail();
+    std::vector<uint32_t> hs(n_tiles), he(n_tiles);
+    if (copy_sync(ctx, hs.data(), ts.p, n_tiles * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+        copy_sync(ctx, he.data(), te.p, n_tiles * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+      return fail();
+    // each tile's records cut into tasks of nearly equal size
+    std::vector<MapTask> tasks;
+    for (uint64_t t = 0; t < n_tiles; ++t) {
+      const uint32_t m = he[t] - hs[t];
+      if (m == 0) continue;
+      const uint32_t k = (m + MT_TASK - 1) / MT_TASK, per = (m + k - 1) / k;
+      for (uint32_t q = 0; q < k; ++q)
+        tasks.push_back(MapTask{(uint32_t)t, hs[t] + q * per, std::min(hs[t] + (q + 1) * per, he[t]), k});
+    }
+    if (pl->mt_task.alloc(tasks.size()) != hipSuccess || pl->mt_work.alloc(1) != hipSuccess) return fail();
+    if (copy_sync(ctx, pl->mt_task.p, tasks.data(), tasks.size() * sizeof(MapTask), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      return fail();
+    pl->mt_n_task = (uint32_t)tasks.size();
+  }
+  int ob = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob, k_mt_bwd<false>, MT_BS, 0) != hipSuccess || ob < 1)
+    return fail();
+  pl->mt_grid = (int)std::min<uint64_t>((uint64_t)ob * ctx->n_cu, pl->mt_n_task);
+  pl->mt_n_rec = n_rec;
+  pl->mt_state = 1;
+  return true;
+}
+
 }  // namespace
 
 extern "C" int muto_map_forward(muto_em_plan* pl, const double* lambda, int precision, int flags,
@@ -434,7 +853,21 @@ extern "C" int muto_map_backward(muto_map_saved* sv, double grad_phi, double* gr
   k_zero_f64<<<grid_for(nv, 256), 256, 0, s>>>(nv, grad);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (nt) {
-    if (sv->precision == MUTO_FP64) {
+    if (sv->recompute && mt_build(pl)) {
+      // paths walked again, summed per tile in LDS (FP64 or FP32 terms)
+      k_mt_reset<<<1, 64, 0, s>>>(pl->mt_work.p);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+      const MapTrackArgs a = track_args(pl);
+      if (sv->precision == MUTO_FP64)
+        k_mt_bwd<false><<<pl->mt_grid, MT_BS, 0, s>>>(a, mt_geo(pl), pl->mt_task.p, pl->mt_n_task,
+                                                      pl->mt_work.p, pl->mt_rec.p, sv->g64.p,
+                                                      -0.5 * grad_phi, grad);
+      else
+        k_mt_bwd<true><<<pl->mt_grid, MT_BS, 0, s>>>(a, mt_geo(pl), pl->mt_task.p, pl->mt_n_task,
+                                                     pl->mt_work.p, pl->mt_rec.p, sv->g64.p,
+                                                     -0.5 * grad_phi, grad);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    } else if (sv->precision == MUTO_FP64) {
       const MapTrackArgs a = track_args(pl);
       if (sv->recompute)
         k_map_bwd64<1><<<grid_for(nt, 256), 256, 0, s>>>(a, sv->g64.p, -0.5 * grad_phi, grad);

@@ -806,6 +806,40 @@ def test_fp64_stream_median_brackets_equal_slots(ctx, small_set, brick):
     pb.close()
 
 
+@pytest.mark.parametrize("prec", ["fp32c", "fp64_stream"])
+def test_median_voxels_beyond_bracket_counts(ctx, prec):
+    """Voxels of more than 65535 entries (a 1x1x2 grid over the 1 m cube, 3e5
+    muons: about 8.4e4 entries each), beyond the 16-bit counts of the
+    brackets: the median runs on every value in its slot whatever
+    MUTO_OPT_MEDIAN_BRACKETS says, so both settings give the same image, with
+    no misses counted, within the precision's bound of the oracle after 3
+    iterations (src/MutoMLSD.cpp:505-515)."""
+    g0, rays, _ = W.muons("small", n_muons=300_000)
+    ext = g0.dx * g0.nx
+    g = make_grid((1, 1, 2), (ext, ext, ext / 2), g0.x_min)
+    cfg = _em_cfg(g, update_with_mean=0)
+    em = oracle.Em(cfg, rays)
+    assert em.mj().min() >= 65536, em.mj()
+    img = np.full(g.n_vox, 1e-7)
+    ref = img
+    for _ in range(3):
+        ref, _c, _ns = em.iterate(ref)
+    out = []
+    for brk in (1, 0):
+        plan = ctx.em_plan(cfg, rays=rays)
+        plan.set_option(M.MUTO_OPT_BRICK, M.brick_code((2, 2, 2, 256)))
+        plan.set_option(M.MUTO_OPT_MEDIAN_BRACKETS, brk)
+        if prec == "fp64_stream":
+            plan.set_option(M.MUTO_OPT_FP64_KERNEL, 2)
+        plan.set_image(img)
+        plan.run(3, M.MUTO_FP32C if prec == "fp32c" else M.MUTO_FP64)
+        assert plan.counters()["n_median_miss"] == 0
+        out.append(plan.image())
+        plan.close()
+    assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))
+    np.testing.assert_allclose(out[0], ref, rtol=1e-4 if prec == "fp32c" else 1e-10, atol=0)
+
+
 def test_map_fp32c_median_close_to_fp64(ctx, small_set):
     """MAP iterations whose ML step is the median (the reference's default
     update rule) in FP32C against FP64: images within 1e-4 relative after 8
@@ -1267,6 +1301,50 @@ def test_plan_without_lists_recomputes(ctx, small_set):
     b.close()
     full.close()
     lean.close()
+
+
+@pytest.mark.parametrize("straight", [0, 1])
+def test_map_recompute_tiles_match_per_entry_atomics(ctx, small_set, straight):
+    """The recomputing MAP backward with its sums per 16^3 tile in LDS (the
+    tile records fit the list budget of a plan without lists) against one
+    atomic per entry (a budget of 0 keeps no records): the same entry terms
+    from walk_path's voxels, lengths and tails, so the FP64 gradients agree
+    up to the order of the sums (1e-12 of the largest component) and the
+    FP32C ones within 1e-5 (FP32 against FP64 sums of the same FP32 terms);
+    kinked and straight paths on a 40^3 grid (partial tiles at the far
+    sides), and the FP32C gradient within 1e-4 of the FP64 one."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, beta=1.0, p=2.0, delta=0.1,
+                  straight_line=straight)
+    probe = ctx.em_plan(cfg, rays=rays)
+    nnz = probe.counters()["nnz"]
+    probe.close()
+    per_entry = ctx.em_plan(cfg, rays=rays, list_budget=0)
+    tiled = ctx.em_plan(cfg, rays=rays, list_budget=20 * nnz - 1)
+    lam = _lam_field(g.n_vox, 4)
+    grads = {}
+    for prec, tol in ((M.MUTO_FP64, 1e-12), (M.MUTO_FP32C, 1e-5)):
+        a = per_entry.map_forward(lam, prec)
+        b = tiled.map_forward(lam, prec)
+        assert a.phi == b.phi
+        ga, gb = a.backward(1.0), b.backward(1.0)
+        scale = np.abs(ga).max()
+        assert scale > 0
+        err = np.abs(gb - ga).max() / scale
+        print(f"straight {straight} precision {prec}: tiled against per-entry {err:.3e}", flush=True)
+        assert err <= tol, err
+        # a second call on the same records gives the same sums up to their order
+        np.testing.assert_allclose(b.backward(1.0), gb, rtol=0, atol=1e-13 * scale)
+        grads[prec] = gb
+        a.close()
+        b.close()
+    assert per_entry.counters()["n_map_tile_records"] == 0
+    n_rec = tiled.counters()["n_map_tile_records"]
+    assert 0 < n_rec < nnz, (n_rec, nnz)
+    g64 = grads[M.MUTO_FP64]
+    assert np.abs(grads[M.MUTO_FP32C] - g64).max() <= 1e-4 * np.abs(g64).max()
+    per_entry.close()
+    tiled.close()
 
 
 def _records_from_rays(rays, zl):

@@ -72,9 +72,10 @@ def test_em256_50_iterations_fp32c_and_fp64_stream_against_fp64(ctx):
 def test_map128_gradient_against_oracle(ctx):
     """The MAP gradient run: 1e6 muons at 128^3 x 15 mm, Phi and grad Phi at
     a seeded lambda.  FP64 from the lists and with the paths walked again
-    (a plan that keeps none): Phi within 1e-10, the gradient within 1e-9 of
-    its largest component; FP32C from the brick stream and FP32C with the
-    paths walked again: within 1e-4."""
+    (plans that keep none: the backward's sums per tile in LDS, or with a
+    budget of 0, one atomic per entry): Phi within 1e-10, the gradient within
+    1e-9 of its largest component; FP32C from the brick stream and FP32C
+    with the paths walked again: within 1e-4."""
     g, rays, _ = W.muons("map128")
     cfg = _cfg(g, beta=1.0, p=1.2, delta=0.1, all_neighbours=1)
     em = oracle.Em(cfg, rays)
@@ -84,11 +85,14 @@ def test_map128_gradient_against_oracle(ctx):
     scale = np.abs(g_ref).max()
     plan = ctx.em_plan(cfg, rays=rays)
     lean = ctx.em_plan(cfg, rays=rays, list_budget=0)
+    tiled = ctx.em_plan(cfg, rays=rays, list_budget=20 * em.nnz - 1)
     assert plan.counters()["nnz"] == em.nnz
     for name, pl, prec, rc, tol in (("fp64", plan, M.MUTO_FP64, False, 1e-9),
                                     ("fp64_recompute", lean, M.MUTO_FP64, True, 1e-9),
+                                    ("fp64_recompute_tiled", tiled, M.MUTO_FP64, True, 1e-9),
                                     ("fp32c", plan, M.MUTO_FP32C, False, 1e-4),
-                                    ("fp32c_recompute", lean, M.MUTO_FP32C, True, 1e-4)):
+                                    ("fp32c_recompute", lean, M.MUTO_FP32C, True, 1e-4),
+                                    ("fp32c_recompute_tiled", tiled, M.MUTO_FP32C, True, 1e-4)):
         sv = pl.map_forward(lam, prec, recompute=rc)
         gr = sv.backward(1.0)
         e = float(np.abs(gr - g_ref).max() / scale)
@@ -97,8 +101,11 @@ def test_map128_gradient_against_oracle(ctx):
         assert ephi <= (1e-10 if prec == M.MUTO_FP64 or rc else 1e-6)
         assert e <= tol
         sv.close()
+    assert lean.counters()["n_map_tile_records"] == 0
+    assert 0 < tiled.counters()["n_map_tile_records"] < em.nnz
     plan.close()
     lean.close()
+    tiled.close()
 
 
 def test_tiled_images_200_grid_against_oracle(ctx):
