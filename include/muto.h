@@ -122,9 +122,9 @@ typedef struct muto_counters {
   int64_t iterations_done;
   int64_t n_median_miss;    /* FP32C median by brackets: voxels whose median
                                left their bracket (last it)                   */
-  int64_t n_map_tile_records; /* MAP backward with the paths walked again: runs
+  int64_t n_map_tile_records; /* MAP operator with the paths walked again: runs
                                of entries recorded per 16^3 tile (0: none kept,
-                               one atomic per entry)                          */
+                               a lane per track, one atomic per entry)        */
   int64_t _reserved[3];
 } muto_counters;
 
@@ -302,9 +302,10 @@ int muto_map_plan_run(muto_em_plan* plan, int iters, int precision,
  * (20 bytes per track-voxel entry: u32 voxel, f64 length, f64 tail).  Above
  * it the plan keeps no lists: muto_em_plan_run refuses it, and the MAP
  * objective operator below walks every path again by Siddon (FP64).  Its
- * backward then sums per 16^3 tile in LDS from tile records (56 bytes per
- * run of a track's entries inside one tile, made at its first call) while
- * those fit the same budget, else it adds one atomic per entry
+ * passes then run per 16^3 tile from tile records (84 bytes per run of a
+ * track's entries inside one tile, made at its first call), the backward's
+ * sums in LDS, while those fit the same budget; else they walk one track
+ * per lane and the backward adds one atomic per entry
  * (muto_counters.n_map_tile_records says which).  No
  * counterpart in the reference, which always stores its paths
  * (MutoMLSD.cpp:382-415). */

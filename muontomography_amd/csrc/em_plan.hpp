@@ -244,9 +244,13 @@ struct muto_em_plan {
   muto::DBuf<muto::MapRec> mt_rec;
   muto::DBuf<muto::MapTask> mt_task;
   muto::DBuf<unsigned int> mt_work;  // [1] task counter
+  muto::DBuf<uint32_t> mt_orig;      // [n_rec] each sorted record's place in track order
+  muto::DBuf<uint64_t> mt_roff;      // [n_trk+1] each track's records in track order
+  muto::DBuf<double> mt_plen;        // [n_trk] path lengths (walk_path's)
+  muto::DBuf<double> mt_part;        // [3 n_rec] the forward's moments per record, track order
   uint64_t mt_n_rec = 0;
   uint32_t mt_n_task = 0;
-  int mt_grid = 0;
+  int mt_grid = 0, mt_grid_fwd = 0;
   // mj counted: by the FP32C stream build when that comes first (LDS counts
   // per brick), else by one atomic per entry (ensure_mj)
   bool mj_ready = false;

@@ -91,6 +91,9 @@ MapTrackArgs track_args(const muto_em_plan* pl) {
   return a;
 }
 
+__device__ __forceinline__ void track_finish(const MapTrackArgs& a, size_t i, Mat2 cov, double& cost,
+                                             bool& sing, double& G00, double& G01, double& G11);
+
 // Track i: Sigma_i from lambda (stored lists in the reference's order, or
 // the path walked again with moments about its start moved to the tail
 // basis), Eigen's 2x2 det/inverse, cost_i, and G_i (zero where cost_i is
@@ -126,6 +129,13 @@ __device__ __forceinline__ void map_track(const MapTrackArgs& a, size_t i, const
     cov.m01 = cov.m10 = P * a0 - a1;
     cov.m11 = P * (P * a0 - 2.0 * a1) + a2;
   }
+  track_finish(a, i, cov, cost, sing, G00, G01, G11);
+}
+
+// Sigma_i = p_i^2 cov + E (cov: the path's weighted moments in the tail
+// basis), Eigen's 2x2 det/inverse, cost_i, and G_i (zero where cost_i is 0)
+__device__ __forceinline__ void track_finish(const MapTrackArgs& a, size_t i, Mat2 cov, double& cost,
+                                             bool& sing, double& G00, double& G01, double& G11) {
   const double p = a.pr[i], pp = p * p;
   cov.m00 = pp * cov.m00 + a.E.e00;
   cov.m01 = pp * cov.m01 + a.E.e01;
@@ -275,9 +285,9 @@ __device__ __forceinline__ uint32_t mt_tile(const GridD& g, const MtGeo& T, uint
 // the entry and the state of the step that recorded the entry's voxel on
 // leg `leg` (0: the in-leg, or the whole straight path; 1: the out-leg).
 // The in-leg's last entry, whose length takes the out-leg's first length on
-// top, carries its own step's state.
+// top, carries its own step's state.  Returns the path length (walk_path's).
 template <class F>
-__device__ __forceinline__ void walk_entries(const GridD& g, const double* geo, int straight, F&& f) {
+__device__ __forceinline__ double walk_entries(const GridD& g, const double* geo, int straight, F&& f) {
   const double* pin = geo;
   const double* pout = geo + 3;
   const double* P = geo + 6;
@@ -285,14 +295,14 @@ __device__ __forceinline__ void walk_entries(const GridD& g, const double* geo, 
   double s = 0.0;
   if (straight) {
     SiddonCursor c;
-    if (!c.begin(g, pin, pout)) return;
+    if (!c.begin(g, pin, pout)) return s;
     c.run(0xFFFFFFFFu, [&](uint32_t x, uint32_t y, uint32_t z, double L) {
       double ap[3];
       c.snapshot(x, y, z, ap);
       f(x + y * nx + z * nxy, s, 0u, x, y, z, ap, c.alpha_last);
       s += L;
     });
-    return;
+    return s;
   }
   bool have = false;
   uint32_t px = 0, py = 0, pz = 0;
@@ -333,7 +343,11 @@ __device__ __forceinline__ void walk_entries(const GridD& g, const double* geo, 
       f(x + y * nx + z * nxy, s, 1u, x, y, z, ap, c2.alpha_last);
       s += L;
     });
-  if (have) f(px + py * nx + pz * nxy, s, 0u, px, py, pz, pap, pal);
+  if (have) {
+    f(px + py * nx + pz * nxy, s, 0u, px, py, pz, pap, pal);
+    s += pL;
+  }
+  return s;
 }
 
 // The entries of track geo's merged path from record R's entry on, in path
@@ -422,17 +436,19 @@ __device__ __forceinline__ uint32_t mt_key(uint32_t tile, uint32_t leg, uint32_t
 }
 
 // Step 1 per track: the runs of its entries (voxel below nv) in one tile.
-// COUNT: their number into cnt[i]; else the records at roff[i], each with
-// its sort key and its own index as the value.
+// COUNT: their number into cnt[i] and the path length into plen[i]; else
+// the records at roff[i], each with its sort key and its own index as the
+// value.
 template <bool COUNT>
 __global__ void __launch_bounds__(256)
 k_mt_split(MapTrackArgs a, MtGeo T, const uint64_t* __restrict__ roff, uint32_t* __restrict__ cnt,
-           MapRec* __restrict__ rec, uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
+           double* __restrict__ plen, MapRec* __restrict__ rec, uint32_t* __restrict__ key,
+           uint32_t* __restrict__ idx) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n_trk) return;
   const uint64_t b = COUNT ? 0 : roff[i];
   uint32_t cur = 0xFFFFFFFFu, k = 0, run = 0, run_leg = 0;
-  walk_entries(a.g, a.geo + 9 * i, a.straight,
+  const double P = walk_entries(a.g, a.geo + 9 * i, a.straight,
                [&](uint32_t j, double s, uint32_t leg, uint32_t x, uint32_t y, uint32_t z,
                    const double* ap, double al) {
                  if (j >= a.nv) return;
@@ -462,10 +478,12 @@ k_mt_split(MapTrackArgs a, MtGeo T, const uint64_t* __restrict__ roff, uint32_t*
                  }
                  ++k;
                });
-  if (COUNT)
+  if (COUNT) {
     cnt[i] = k;
-  else if (k > 0)
+    plen[i] = P;
+  } else if (k > 0) {
     key[b + k - 1] = mt_key(cur, run_leg, run);
+  }
 }
 
 __global__ void k_mt_gather(size_t n, const uint32_t* __restrict__ idx, const MapRec* __restrict__ src,
@@ -552,6 +570,89 @@ k_mt_bwd(MapTrackArgs a, MtGeo T, const MapTask* __restrict__ tasks, uint32_t n_
     }
     __syncthreads();
   }
+}
+
+// Forward per tile: each run's moments about its path's start,
+// sum lambda_j (L, L s + L^2/2, L s^2 + L^2 s + L^3/3) with the entry
+// arithmetic of map_track<1> and lambda from the tile in LDS, into
+// part[3 orig[q]] (the run's place in track order).
+__global__ void __launch_bounds__(MT_BS)
+k_mt_fwd(MapTrackArgs a, MtGeo T, const MapTask* __restrict__ tasks, uint32_t n_tasks,
+         unsigned int* __restrict__ work, const MapRec* __restrict__ rec,
+         const uint32_t* __restrict__ orig, const double* __restrict__ lam, double* __restrict__ part) {
+  __shared__ double lam_s[MT_V];
+  __shared__ uint32_t sh;
+  const uint32_t t = threadIdx.x;
+  for (;;) {
+    if (t == 0) sh = atomicAdd(work, 1u);
+    __syncthreads();
+    const uint32_t ti = sh;
+    if (ti >= n_tasks) break;
+    const MapTask tk = tasks[ti];
+    const uint32_t tyz = tk.tile / T.nt[0];
+    const uint32_t o0 = (tk.tile - tyz * T.nt[0]) * MT_E, o1 = (tyz % T.nt[1]) * MT_E,
+                   o2 = (tyz / T.nt[1]) * MT_E;
+    for (uint32_t v = t; v < MT_V; v += MT_BS) {
+      const uint32_t x = o0 + v % MT_E, y = o1 + (v / MT_E) % MT_E, z = o2 + v / (MT_E * MT_E);
+      lam_s[v] = (x < a.g.n[0] && y < a.g.n[1] && z < a.g.n[2])
+                     ? lam[(size_t)x + (size_t)a.g.n[0] * ((size_t)y + (size_t)a.g.n[1] * z)]
+                     : 0.0;
+    }
+    __syncthreads();
+    for (uint32_t q = tk.q0 + t; q < tk.q1; q += MT_BS) {
+      const MapRec R = rec[q];
+      const size_t i = R.tl >> 1;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+      resume_entries(a.g, a.geo + 9 * i, a.straight, R,
+                     [&](uint32_t j, uint32_t x, uint32_t y, uint32_t z, double L, double s) -> bool {
+                       if (j >= a.nv) return true;
+                       uint32_t lv;
+                       if (mt_tile(a.g, T, x, y, z, j, lv) != tk.tile) return false;
+                       const double lj = lam_s[lv];
+                       a0 += lj * L;
+                       a1 += lj * (L * s + 0.5 * L * L);
+                       a2 += lj * (L * s * s + L * L * s + L * L * L / 3.0);
+                       return true;
+                     });
+      double* o = part + 3 * (size_t)orig[q];
+      o[0] = a0;
+      o[1] = a1;
+      o[2] = a2;
+    }
+    __syncthreads();  // the tile is loaded again for the next task
+  }
+}
+
+// Per track: its runs' moments added in path order, moved to the tail
+// basis with the path length P as map_track<1> does, then cost_i and G_i.
+__global__ void __launch_bounds__(256)
+k_mt_track(MapTrackArgs a, const uint64_t* __restrict__ roff, const double* __restrict__ part,
+           const double* __restrict__ plen, double* __restrict__ gsave, double* __restrict__ cost_part,
+           unsigned long long* __restrict__ cnt) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double cost = 0.0;
+  bool sing = false;
+  if (i < a.n_trk) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (uint64_t r = roff[i]; r < roff[i + 1]; ++r) {
+      a0 += part[3 * r];
+      a1 += part[3 * r + 1];
+      a2 += part[3 * r + 2];
+    }
+    const double P = plen[i];
+    Mat2 cov;
+    cov.m00 = a0;
+    cov.m01 = cov.m10 = P * a0 - a1;
+    cov.m11 = P * (P * a0 - 2.0 * a1) + a2;
+    double G00, G01, G11;
+    track_finish(a, i, cov, cost, sing, G00, G01, G11);
+    gsave[4 * i] = G00;
+    gsave[4 * i + 1] = G01;
+    gsave[4 * i + 2] = G11;
+    gsave[4 * i + 3] = P;
+  }
+  block_sum_to(cost_part, cost);
+  wave_count(&cnt[C_SINGULAR], sing);
 }
 
 __global__ void k_zero_f32(size_t n, float* __restrict__ x) {
@@ -671,12 +772,15 @@ MtGeo mt_geo(const muto_em_plan* pl) {
   return T;
 }
 
-// The tile records and tasks of the recomputing backward, made at its
-// first call on the plan.  Returns false when they are not kept: more than
-// 2^31 records or tracks, records over the list budget of a plan that keeps
-// no lists (56 bytes each), or no device memory; the backward then makes
-// one atomic per entry.  Kept or not, the decision holds for the plan's
-// life.
+// The tile records and tasks of the passes that walk the paths again, made
+// at the first such MAP call on the plan, with each record's place in track
+// order, each track's records there, its path length and the forward's
+// moments per record (84 bytes per record, 16 per track).  Returns false
+// when they are not kept: more than 2^31 records or tracks, more than
+// 2^25 tiles, a plan that keeps no lists whose budget they exceed, or no
+// device memory; the passes then walk each track in one lane, the backward
+// with one atomic per entry.  Kept or not, the decision holds for the
+// plan's life.
 bool mt_build(muto_em_plan* pl) {
   if (pl->mt_state) return pl->mt_state > 0;
   pl->mt_state = -1;
@@ -694,35 +798,46 @@ bool mt_build(muto_em_plan* pl) {
     pl->mt_rec.release();
     pl->mt_task.release();
     pl->mt_work.release();
+    pl->mt_orig.release();
+    pl->mt_roff.release();
+    pl->mt_plen.release();
+    pl->mt_part.release();
     return false;
   };
   DBuf<uint32_t> cnt;
-  DBuf<uint64_t> roff;
-  if (cnt.alloc(nt + 1) != hipSuccess || roff.alloc(nt + 1) != hipSuccess) return fail();
+  if (cnt.alloc(nt + 1) != hipSuccess || pl->mt_roff.alloc(nt + 1) != hipSuccess ||
+      pl->mt_plen.alloc(nt) != hipSuccess)
+    return fail();
   if (hipMemsetAsync(cnt.p + nt, 0, sizeof(uint32_t), s) != hipSuccess) return fail();
-  k_mt_split<true><<<grid_for(nt, 256), 256, 0, s>>>(a, T, nullptr, cnt.p, nullptr, nullptr, nullptr);
+  k_mt_split<true><<<grid_for(nt, 256), 256, 0, s>>>(a, T, nullptr, cnt.p, pl->mt_plen.p, nullptr, nullptr,
+                                                     nullptr);
   if (hipGetLastError() != hipSuccess) return fail();
-  if (exclusive_scan_u32_to_u64(cnt.p, roff.p, nt + 1, s) != hipSuccess) return fail();
+  if (exclusive_scan_u32_to_u64(cnt.p, pl->mt_roff.p, nt + 1, s) != hipSuccess) return fail();
   uint64_t n_rec = 0;
-  if (copy_sync(ctx, &n_rec, roff.p + nt, sizeof n_rec, hipMemcpyDeviceToHost) != hipSuccess) return fail();
+  if (copy_sync(ctx, &n_rec, pl->mt_roff.p + nt, sizeof n_rec, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail();
   cnt.release();
   if (n_rec == 0 || n_rec >= (1ull << 31)) return fail();
-  if (!pl->has_lists && (double)n_rec * sizeof(MapRec) > (double)pl->list_budget) return fail();
+  constexpr double REC_BYTES = sizeof(MapRec) + sizeof(uint32_t) + 3 * sizeof(double);
+  if (!pl->has_lists && (double)n_rec * REC_BYTES + 16.0 * (double)nt > (double)pl->list_budget)
+    return fail();
   {
     DBuf<MapRec> rec0;
-    DBuf<uint32_t> key, idx;
-    if (rec0.alloc(n_rec) != hipSuccess || key.alloc(n_rec) != hipSuccess || idx.alloc(n_rec) != hipSuccess)
+    DBuf<uint32_t> key;
+    if (rec0.alloc(n_rec) != hipSuccess || key.alloc(n_rec) != hipSuccess ||
+        pl->mt_orig.alloc(n_rec) != hipSuccess)
       return fail();
-    k_mt_split<false><<<grid_for(nt, 256), 256, 0, s>>>(a, T, roff.p, nullptr, rec0.p, key.p, idx.p);
+    k_mt_split<false><<<grid_for(nt, 256), 256, 0, s>>>(a, T, pl->mt_roff.p, nullptr, nullptr, rec0.p, key.p,
+                                                        pl->mt_orig.p);
     if (hipGetLastError() != hipSuccess) return fail();
     int bits = 1;
     while ((1ull << bits) < n_tiles) ++bits;
-    if (sort_pairs_u32_u32(key.p, idx.p, n_rec, 0, bits + MT_SUB_BITS, s) != hipSuccess) return fail();
+    if (sort_pairs_u32_u32(key.p, pl->mt_orig.p, n_rec, 0, bits + MT_SUB_BITS, s) != hipSuccess) return fail();
     if (pl->mt_rec.alloc(n_rec) != hipSuccess) return fail();
-    k_mt_gather<<<grid_for(n_rec, 256), 256, 0, s>>>(n_rec, idx.p, rec0.p, pl->mt_rec.p);
+    k_mt_gather<<<grid_for(n_rec, 256), 256, 0, s>>>(n_rec, pl->mt_orig.p, rec0.p, pl->mt_rec.p);
     if (hipGetLastError() != hipSuccess) return fail();
     rec0.release();
-    idx.release();
+    if (pl->mt_part.alloc(3 * n_rec) != hipSuccess) return fail();
     DBuf<uint32_t> ts, te;
     if (ts.alloc(n_tiles) != hipSuccess || te.alloc(n_tiles) != hipSuccess) return fail();
     if (hipMemsetAsync(ts.p, 0, n_tiles * sizeof(uint32_t), s) != hipSuccess ||
@@ -749,10 +864,12 @@ bool mt_build(muto_em_plan* pl) {
       return fail();
     pl->mt_n_task = (uint32_t)tasks.size();
   }
-  int ob = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob, k_mt_bwd<false>, MT_BS, 0) != hipSuccess || ob < 1)
+  int ob = 0, of = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob, k_mt_bwd<false>, MT_BS, 0) != hipSuccess || ob < 1 ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&of, k_mt_fwd, MT_BS, 0) != hipSuccess || of < 1)
     return fail();
   pl->mt_grid = (int)std::min<uint64_t>((uint64_t)ob * ctx->n_cu, pl->mt_n_task);
+  pl->mt_grid_fwd = (int)std::min<uint64_t>((uint64_t)of * ctx->n_cu, pl->mt_n_task);
   pl->mt_n_rec = n_rec;
   pl->mt_state = 1;
   return true;
@@ -810,10 +927,20 @@ extern "C" int muto_map_forward(muto_em_plan* pl, const double* lambda, int prec
     if (precision == MUTO_FP64 || recompute) {
       if (sv->g64.alloc(4 * nt) != hipSuccess) { ctx->set_error("muto_map_forward: out of device memory"); return fail(MUTO_ERR_NOMEM); }
       const MapTrackArgs a = track_args(pl);
-      if (recompute)
+      if (recompute && mt_build(pl)) {
+        // paths walked again per tile: each run's moments, then per track
+        k_mt_reset<<<1, 64, 0, s>>>(pl->mt_work.p);
+        MUTO_CHECK_HIP(ctx, hipGetLastError());
+        k_mt_fwd<<<pl->mt_grid_fwd, MT_BS, 0, s>>>(a, mt_geo(pl), pl->mt_task.p, pl->mt_n_task, pl->mt_work.p,
+                                                   pl->mt_rec.p, pl->mt_orig.p, sv->lam.p, pl->mt_part.p);
+        MUTO_CHECK_HIP(ctx, hipGetLastError());
+        k_mt_track<<<grid_for(nt, 256), 256, 0, s>>>(a, pl->mt_roff.p, pl->mt_part.p, pl->mt_plen.p, sv->g64.p,
+                                                     sv->part.p, pl->dcount.p);
+      } else if (recompute) {
         k_map_fwd64<1><<<grid_for(nt, 256), 256, 0, s>>>(a, sv->lam.p, sv->g64.p, sv->part.p, pl->dcount.p);
-      else
+      } else {
         k_map_fwd64<0><<<grid_for(nt, 256), 256, 0, s>>>(a, sv->lam.p, sv->g64.p, sv->part.p, pl->dcount.p);
+      }
       MUTO_CHECK_HIP(ctx, hipGetLastError());
       k_sum_parts<<<1, 256, 0, s>>>(sv->part.p, grid_for(nt, 256), sv->sum.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());

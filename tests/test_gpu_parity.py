@@ -1305,14 +1305,16 @@ def test_plan_without_lists_recomputes(ctx, small_set):
 
 @pytest.mark.parametrize("straight", [0, 1])
 def test_map_recompute_tiles_match_per_entry_atomics(ctx, small_set, straight):
-    """The recomputing MAP backward with its sums per 16^3 tile in LDS (the
-    tile records fit the list budget of a plan without lists) against one
-    atomic per entry (a budget of 0 keeps no records): the same entry terms
-    from walk_path's voxels, lengths and tails, so the FP64 gradients agree
-    up to the order of the sums (1e-12 of the largest component) and the
-    FP32C ones within 1e-5 (FP32 against FP64 sums of the same FP32 terms);
-    kinked and straight paths on a 40^3 grid (partial tiles at the far
-    sides), and the FP32C gradient within 1e-4 of the FP64 one."""
+    """The MAP operator with the paths walked again per 16^3 tile (the tile
+    records fit the list budget of a plan without lists: moments per run in
+    the forward pass, sums in LDS in the backward) against one lane per
+    track and one atomic per entry (a budget of 0 keeps no records): the
+    same entry terms from walk_path's voxels, lengths and tails, so Phi
+    agrees to 1e-12 relative and the FP64 gradients up to the order of the
+    sums (1e-12 of the largest component), the FP32C ones within 1e-5 (FP32
+    against FP64 sums of the same FP32 terms); kinked and straight paths on
+    a 40^3 grid (partial tiles at the far sides), and the FP32C gradient
+    within 1e-4 of the FP64 one."""
     g, rays, _ = small_set
     cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, beta=1.0, p=2.0, delta=0.1,
                   straight_line=straight)
@@ -1326,7 +1328,10 @@ def test_map_recompute_tiles_match_per_entry_atomics(ctx, small_set, straight):
     for prec, tol in ((M.MUTO_FP64, 1e-12), (M.MUTO_FP32C, 1e-5)):
         a = per_entry.map_forward(lam, prec)
         b = tiled.map_forward(lam, prec)
-        assert a.phi == b.phi
+        assert abs(a.phi - b.phi) <= 1e-12 * abs(a.phi), (a.phi, b.phi)
+        c = tiled.map_forward(lam, prec)
+        assert c.phi == b.phi  # the same sums in the same order
+        c.close()
         ga, gb = a.backward(1.0), b.backward(1.0)
         scale = np.abs(ga).max()
         assert scale > 0

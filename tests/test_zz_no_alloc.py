@@ -1,10 +1,12 @@
 """muto_map_value_grad_noalloc (the MAP objective and its gradient in one
 call that allocates nothing, FP64) against muto_map_forward +
-muto_map_backward, through the C ABI: Phi bit for bit, the gradient up to
-the order of its FP64 atomic additions, on two calls in a row with the
-second on a smaller problem (fewer tracks and voxels, so stale partial sums
-of the first call would show), from stored lists and with the paths walked
-again; sizes above the static scratch are refused."""
+muto_map_backward, through the C ABI: from stored lists Phi bit for bit;
+with the paths walked again, where muto_map_forward sums each track's
+moments per tile run and the no-alloc call in one lane per track, Phi within
+1e-13 relative; the gradient up to the order of its FP64 additions; on two
+calls in a row with the second on a smaller problem (fewer tracks and
+voxels, so stale partial sums of the first call would show); sizes above
+the static scratch are refused."""
 import numpy as np
 import pytest
 
@@ -43,7 +45,10 @@ def test_noalloc_matches_forward_backward(ctx, recompute):
         dp = DeviceArray.from_numpy(np.array([np.nan]))
         plan.map_value_grad_noalloc(dl, dg, dp, recompute=recompute)
         phi = dp.numpy()[0]
-        assert phi == sv.phi, (phi, sv.phi)
+        if recompute:
+            assert abs(phi - sv.phi) <= 1e-13 * abs(sv.phi), (phi, sv.phi)
+        else:
+            assert phi == sv.phi, (phi, sv.phi)
         scale = np.abs(gref).max()
         assert scale > 0
         np.testing.assert_allclose(dg.numpy(), gref, rtol=0, atol=1e-13 * scale)
