@@ -1303,8 +1303,8 @@ def test_plan_without_lists_recomputes(ctx, small_set):
     lean.close()
 
 
-@pytest.mark.parametrize("straight", [0, 1])
-def test_map_recompute_tiles_match_per_entry_atomics(ctx, small_set, straight):
+@pytest.mark.parametrize("straight,shape", [(0, None), (1, None), (0, (7, 33, 18))])
+def test_map_recompute_tiles_match_per_entry_atomics(ctx, small_set, straight, shape):
     """The MAP operator with the paths walked again per 16^3 tile (the tile
     records fit the list budget of a plan without lists: moments per run in
     the forward pass, sums in LDS in the backward) against one lane per
@@ -1313,9 +1313,13 @@ def test_map_recompute_tiles_match_per_entry_atomics(ctx, small_set, straight):
     agrees to 1e-12 relative and the FP64 gradients up to the order of the
     sums (1e-12 of the largest component), the FP32C ones within 1e-5 (FP32
     against FP64 sums of the same FP32 terms); kinked and straight paths on
-    a 40^3 grid (partial tiles at the far sides), and the FP32C gradient
-    within 1e-4 of the FP64 one."""
+    a 40^3 grid (partial tiles at the far sides) and on a 7x33x18 grid over
+    the same cube (one axis shorter than a tile, unequal voxel sides), and
+    the FP32C gradient within 1e-4 of the FP64 one."""
     g, rays, _ = small_set
+    if shape is not None:
+        ext = g.dx * g.nx
+        g = make_grid(shape, tuple(ext / k for k in shape), g.x_min)
     cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, beta=1.0, p=2.0, delta=0.1,
                   straight_line=straight)
     probe = ctx.em_plan(cfg, rays=rays)
