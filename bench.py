@@ -531,6 +531,7 @@ def bench_map_grad(args):
     res = {}
     first = {}
     for name, pl, prec, rc in (("fp32c", plan, M.MUTO_FP32C, False), ("fp64", plan, M.MUTO_FP64, False),
+                               ("fp64_stream", plan, M.MUTO_FP64_STREAM, False),
                                ("fp64_recompute", lean, M.MUTO_FP64, True),
                                ("fp32c_recompute", lean, M.MUTO_FP32C, True),
                                ("fp64_recompute_per_entry", lean0, M.MUTO_FP64, True),
@@ -597,6 +598,8 @@ def bench_map_grad(args):
            "grad_err_fp32c_vs_fp64_norm_inf": float(np.abs(res["fp32c"][1] - g64).max() / np.abs(g64).max()),
            "grad_err_recompute_vs_fp64_norm_inf": float(np.abs(res["fp64_recompute"][1] - g64).max() /
                                                          np.abs(g64).max()),
+           "grad_err_fp64_stream_vs_fp64_norm_inf": float(np.abs(res["fp64_stream"][1] - g64).max() /
+                                                          np.abs(g64).max()),
            "grad_err_fp32c_recompute_vs_fp64_norm_inf": float(
                np.abs(res["fp32c_recompute"][1] - g64).max() / np.abs(g64).max()),
            "grad_err_recompute_tiled_vs_per_entry_norm_inf": float(

@@ -327,13 +327,15 @@ int muto_em_plan_create_ex(muto_ctx* ctx, const muto_mlsd_cfg* cfg, size_t n,
  * with MUTO_MAP_DEVICE_BUFFERS, device memory) on the
  * plan's tracks and returns a saved context; muto_map_backward writes
  * grad_lambda = grad_phi dPhi/dlambda (host, n_vox doubles) and may be
- * called several times.  precision: MUTO_FP64 (track-major, FP64) or
- * MUTO_FP32C (brick stream, FP32 entry terms).  flags:
- * MUTO_MAP_RECOMPUTE_SIDDON walks the paths again instead of reading the
- * stored lists or the brick stream (implied for plans without lists): the
- * walk and the forward sums are FP64; with MUTO_FP32C the backward adds FP32
- * entry terms with FP32 atomics.  A saved context must be destroyed before
- * its plan. */
+ * called several times.  precision: MUTO_FP64 (track-major lists, FP64;
+ * with MUTO_OPT_FP64_KERNEL 2 the FP64 brick stream), MUTO_FP64_STREAM (the
+ * FP64 brick stream: the lists' exact FP64 lengths and tails, sums per
+ * segment and per brick) or MUTO_FP32C (brick stream, FP32 entry terms).
+ * flags: MUTO_MAP_RECOMPUTE_SIDDON walks the paths again instead of reading
+ * the stored lists or a brick stream (implied for plans without lists;
+ * refused with MUTO_FP64_STREAM): the walk and the forward sums are FP64;
+ * with MUTO_FP32C the backward's entry terms are FP32.  A saved context
+ * must be destroyed before its plan. */
 typedef struct muto_map_saved muto_map_saved;
 #define MUTO_MAP_RECOMPUTE_SIDDON 1
 /* flag of muto_map_forward: lambda, and grad_lambda of every

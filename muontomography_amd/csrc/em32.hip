@@ -3907,6 +3907,63 @@ int fp32c_map_backward(muto_em_plan* pl, const float* g, const unsigned long lon
   return MUTO_OK;
 }
 
+// The MAP objective's passes on the FP64 stream: the forward pass with the
+// caller's lambda, then the track kernel in its gradient mode, p^2 G per
+// track in rank order into g_out (zero where cost_i is 0) and the cost sum
+// to the host...
+int fp64s_map_forward(muto_em_plan* pl, const double* lam, double* g_out, double* cost_sum) {
+  Fp32cData& f = pl->f32;
+  muto_ctx* ctx = pl->ctx;
+  hipStream_t s = ctx->stream;
+  if (int rc = fp64s_build(pl)) return rc;
+  const IterArgs a = iter_args(pl);
+  const Args64 b = args64(pl);
+  const ErrM E = error_matrix(pl->cfg);
+  const size_t lds = (size_t)f.bs.vol() * sizeof(double);
+  k_iter_begin64<<<1, 64, 0, s>>>(pl->cost_acc.p, pl->dcount.p, pl->work.p, f.qwork64.p, b.nq);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  if (f.n_tasks) {
+    MUTO_CHECK_HIP(ctx, by_block64(f.bs.block, [&](auto B) {
+      constexpr uint32_t BS = decltype(B)::value;
+      k_fwd64s<BS><<<f.grid_fwd64, BS, lds, s>>>(a, b, lam, reinterpret_cast<double4*>(f.seg_part64.p));
+      return hipGetLastError();
+    }));
+  }
+  const unsigned nblk = grid_for(pl->n_trk, 256);
+  k_track64s<<<nblk, 256, 0, s>>>(pl->n_trk, f.tseg_ptr.p, f.tseg_pos.p,
+                                  reinterpret_cast<const double4*>(f.seg_part64.p), f.trk_d.p, E, 1,
+                                  reinterpret_cast<double4*>(g_out), f.cost_part64.p, pl->dcount.p);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  k_sum_parts_f64<<<1, 256, 0, s>>>(f.cost_part64.p, nblk, pl->cost_acc.p);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  MUTO_CHECK_HIP(ctx, hipMemcpyAsync(cost_sum, pl->cost_acc.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  MUTO_CHECK_HIP(ctx, hipStreamSynchronize(s));
+  return MUTO_OK;
+}
+
+// ... and the backprojection of that p^2 G into acc (sum_i <W_ij, p^2 G_i>
+// with the exact FP64 W_ij, FP64 sums per brick in LDS; acc is cleared
+// first: bricks that no task covers are not written)
+int fp64s_map_backward(muto_em_plan* pl, const double* g, double* acc) {
+  Fp32cData& f = pl->f32;
+  muto_ctx* ctx = pl->ctx;
+  hipStream_t s = ctx->stream;
+  const IterArgs a = iter_args(pl);
+  const Args64 b = args64(pl);
+  const size_t lds = (size_t)f.bs.vol() * sizeof(double);
+  k_iter_begin64<<<1, 64, 0, s>>>(pl->cost_acc.p, pl->dcount.p, pl->work.p, f.qwork64.p, b.nq);
+  MUTO_CHECK_HIP(ctx, hipGetLastError());
+  MUTO_CHECK_HIP(ctx, hipMemsetAsync(acc, 0, pl->nv * sizeof(double), s));
+  if (f.n_tasks) {
+    MUTO_CHECK_HIP(ctx, by_block64(f.bs.block, [&](auto B) {
+      constexpr uint32_t BS = decltype(B)::value;
+      k_bwd64s<0, BS><<<f.grid_bwd64, BS, lds, s>>>(a, b, MedArgs{}, reinterpret_cast<const double4*>(g), acc);
+      return hipGetLastError();
+    }));
+  }
+  return MUTO_OK;
+}
+
 // the float mirror of the plan's FP64 image
 int fp32c_mirror(muto_em_plan* pl) {
   muto_ctx* ctx = pl->ctx;

@@ -36,6 +36,8 @@ int fp32c_map_forward(muto_em_plan* pl, const double* lam, float* g_out,
                       unsigned long long* cmax_out, double* cost_sum);
 int fp32c_map_backward(muto_em_plan* pl, const float* g, const unsigned long long* cmax,
                        double* acc);
+int fp64s_map_forward(muto_em_plan* pl, const double* lam, double* g_out, double* cost_sum);
+int fp64s_map_backward(muto_em_plan* pl, const double* g, double* acc);
 }  // namespace muto
 
 using namespace muto;
@@ -54,6 +56,7 @@ struct muto_map_saved {
   DBuf<double> part;               // per-block partial sums
   bool dev = false;                // MUTO_MAP_DEVICE_BUFFERS: caller buffers in device memory
   DBuf<float> acc32;               // FP32C with the paths walked again: [nv] float sums
+  bool stream64 = false;           // FP64 on the FP64 brick stream (g64: p^2 G in rank order)
 };
 
 namespace {
@@ -883,8 +886,8 @@ extern "C" int muto_map_forward(muto_em_plan* pl, const double* lambda, int prec
   muto_ctx* ctx = pl->ctx;
   if (!lambda || !phi || !saved) { ctx->set_error("muto_map_forward: null argument"); return MUTO_ERR_ARG; }
   *saved = nullptr;
-  if (precision != MUTO_FP64 && precision != MUTO_FP32C) {
-    ctx->set_error("muto_map_forward: precision must be MUTO_FP64 or MUTO_FP32C");
+  if (precision != MUTO_FP64 && precision != MUTO_FP32C && precision != MUTO_FP64_STREAM) {
+    ctx->set_error("muto_map_forward: precision must be MUTO_FP64, MUTO_FP64_STREAM or MUTO_FP32C");
     return MUTO_ERR_ARG;
   }
   if (flags & ~(MUTO_MAP_RECOMPUTE_SIDDON | MUTO_MAP_DEVICE_BUFFERS)) {
@@ -924,7 +927,16 @@ extern "C" int muto_map_forward(muto_em_plan* pl, const double* lambda, int prec
     // FP64, and FP32C with the paths walked again: the FP64 walk (index
     // exactness) with FP64 moment sums, G_i saved in FP64; FP32C's
     // recomputing backward then adds FP32 entry terms
-    if (precision == MUTO_FP64 || recompute) {
+    if (precision == MUTO_FP64_STREAM && recompute) {
+      ctx->set_error("muto_map_forward: MUTO_FP64_STREAM reads the plan's lists; it does not walk the paths again");
+      return fail(MUTO_ERR_ARG);
+    }
+    if (precision == MUTO_FP64_STREAM || (precision == MUTO_FP64 && !recompute && pl->fp64_kernel == 2)) {
+      // FP64 on the FP64 brick stream (MUTO_FP64_STREAM, or MUTO_OPT_FP64_KERNEL 2)
+      if (sv->g64.alloc(4 * nt) != hipSuccess) { ctx->set_error("muto_map_forward: out of device memory"); return fail(MUTO_ERR_NOMEM); }
+      if (int rc = fp64s_map_forward(pl, sv->lam.p, sv->g64.p, &cost_sum)) return fail(rc);
+      sv->stream64 = true;
+    } else if (precision == MUTO_FP64 || recompute) {
       if (sv->g64.alloc(4 * nt) != hipSuccess) { ctx->set_error("muto_map_forward: out of device memory"); return fail(MUTO_ERR_NOMEM); }
       const MapTrackArgs a = track_args(pl);
       if (recompute && mt_build(pl)) {
@@ -980,7 +992,11 @@ extern "C" int muto_map_backward(muto_map_saved* sv, double grad_phi, double* gr
   k_zero_f64<<<grid_for(nv, 256), 256, 0, s>>>(nv, grad);
   MUTO_CHECK_HIP(ctx, hipGetLastError());
   if (nt) {
-    if (sv->recompute && mt_build(pl)) {
+    if (sv->stream64) {
+      if (int rc = fp64s_map_backward(pl, sv->g64.p, grad)) return rc;
+      k_scale<<<grid_for(nv, 256), 256, 0, s>>>(nv, -0.5 * grad_phi, grad);
+      MUTO_CHECK_HIP(ctx, hipGetLastError());
+    } else if (sv->recompute && mt_build(pl)) {
       // paths walked again, summed per tile in LDS (FP64 or FP32 terms)
       k_mt_reset<<<1, 64, 0, s>>>(pl->mt_work.p);
       MUTO_CHECK_HIP(ctx, hipGetLastError());

@@ -1159,6 +1159,41 @@ def test_map_objective_fp64_matches_oracle(ctx, small_set, nb, obj, p):
     plan.close()
 
 
+@pytest.mark.parametrize("nb,p", [(0, 2.0), (1, 1.2)])
+def test_map_objective_fp64_stream_matches_oracle(ctx, small_set, nb, p):
+    """The MAP objective operator on the FP64 brick stream (MUTO_FP64_STREAM,
+    or MUTO_FP64 with MUTO_OPT_FP64_KERNEL 2): the lists' exact FP64 W_ij,
+    sums per segment and per brick, so Phi within 1e-10 of the oracle and
+    grad Phi within 1e-9; both selections give the same Phi bit for bit and
+    the same gradient up to the order of the shared bricks' additions; the
+    stream with the paths walked again is refused."""
+    g, rays, _ = small_set
+    cfg = _em_cfg(g, update_with_mean=1, mean_percentile=100, beta=1.0, p=p, delta=0.1,
+                  all_neighbours=nb, add_object_prior=1)
+    em = oracle.Em(cfg, rays)
+    lam = _lam_field(g.n_vox)
+    phi_ref = em.phi(lam, cfg)
+    g_ref = em.grad(lam, cfg)
+    plan = ctx.em_plan(cfg, rays=rays)
+    sv = plan.map_forward(lam, M.MUTO_FP64_STREAM)
+    assert abs(sv.phi - phi_ref) <= 1e-10 * abs(phi_ref), (sv.phi, phi_ref)
+    gr = sv.backward(1.0)
+    scale = np.abs(g_ref).max()
+    print(f"fp64 stream map: phi rel {abs(sv.phi - phi_ref) / abs(phi_ref):.3e} "
+          f"grad {np.abs(gr - g_ref).max() / scale:.3e}", flush=True)
+    np.testing.assert_allclose(gr, g_ref, rtol=1e-9, atol=1e-10 * scale)
+    np.testing.assert_allclose(sv.backward(-2.5), -2.5 * gr, rtol=1e-11, atol=1e-12 * scale)
+    plan.set_option(M.MUTO_OPT_FP64_KERNEL, 2)
+    s2 = plan.map_forward(lam, M.MUTO_FP64)
+    assert s2.phi == sv.phi
+    np.testing.assert_allclose(s2.backward(1.0), gr, rtol=0, atol=1e-13 * scale)
+    with pytest.raises(M.MutoError):
+        plan.map_forward(lam, M.MUTO_FP64_STREAM, recompute=True)
+    s2.close()
+    sv.close()
+    plan.close()
+
+
 @pytest.fixture(scope="module")
 def fd32_set():
     return W.muons("fd32")

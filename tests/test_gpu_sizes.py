@@ -71,7 +71,8 @@ def test_em256_50_iterations_fp32c_and_fp64_stream_against_fp64(ctx):
 
 def test_map128_gradient_against_oracle(ctx):
     """The MAP gradient run: 1e6 muons at 128^3 x 15 mm, Phi and grad Phi at
-    a seeded lambda.  FP64 from the lists and with the paths walked again
+    a seeded lambda.  FP64 from the lists, on the FP64 brick stream and with
+    the paths walked again
     (plans that keep none: the backward's sums per tile in LDS, or with a
     budget of 0, one atomic per entry): Phi within 1e-10, the gradient within
     1e-9 of its largest component; FP32C from the brick stream and FP32C
@@ -88,6 +89,7 @@ def test_map128_gradient_against_oracle(ctx):
     tiled = ctx.em_plan(cfg, rays=rays, list_budget=20 * em.nnz - 1)
     assert plan.counters()["nnz"] == em.nnz
     for name, pl, prec, rc, tol in (("fp64", plan, M.MUTO_FP64, False, 1e-9),
+                                    ("fp64_stream", plan, M.MUTO_FP64_STREAM, False, 1e-9),
                                     ("fp64_recompute", lean, M.MUTO_FP64, True, 1e-9),
                                     ("fp64_recompute_tiled", tiled, M.MUTO_FP64, True, 1e-9),
                                     ("fp32c", plan, M.MUTO_FP32C, False, 1e-4),
