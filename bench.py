@@ -558,6 +558,7 @@ def bench_map_grad(args):
     phi_d = DeviceArray(1)
     dev = {}
     for name, pl, prec, rc in (("fp32c", plan, M.MUTO_FP32C, False), ("fp64", plan, M.MUTO_FP64, False),
+                               ("fp64_stream", plan, M.MUTO_FP64_STREAM, False),
                                ("fp64_recompute", lean, M.MUTO_FP64, True)):
         for k in range(args.steps + 1):
             if k == 1:
